@@ -1,0 +1,292 @@
+"""Python host mirror of the reference's `NativeFdmBackend`
+(crates/fullmag-runner/src/native_fdm.rs:49-970): create from a plan, check
+`last_error`, step, read fields, upload magnetisation, destroy.
+
+All arithmetic happens in libfullmag_fdm.so through the C ABI; this module
+only marshals numpy arrays.  There is no CPU fallback: without the built
+library and a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as abi
+
+MU0 = 4.0 * np.pi * 1e-7
+
+
+class FdmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class Interrupted(FdmError):
+    pass
+
+
+@dataclass
+class FdmPlan:
+    """Plan in physical units; the field meanings are those of
+    `fullmag_fdm_plan_desc` (native/include/fullmag_fdm.h:100-269)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ms: float
+    a_ex: float
+    alpha: float
+    m0: np.ndarray                      # (n,3) float64
+    gamma: float = 2.211e5
+    precision: str = "double"           # "single" | "double"
+    integrator: str = "dp45"            # "heun" | "dp45" | "rk4"
+    exchange: bool = True
+    demag: bool = True
+    precession: bool = True
+    h_ext: tuple | None = None          # A/m
+    ku1: float = 0.0
+    ku2: float = 0.0
+    axis: tuple | None = None           # enables uniaxial anisotropy
+    mask: np.ndarray | None = None      # uint8 (n,)
+    regions: np.ndarray | None = None   # uint32 (n,)
+    exchange_lut: np.ndarray | None = None  # (256*256,) float64
+    spectra: dict | None = None         # {"xx": interleaved complex f64, ...}
+    atol: float = 0.0                   # 0 -> engine default 1e-5
+    dt_min: float = 0.0
+    dt_max: float = 0.0
+    headroom: float = 0.0
+    extra: dict = field(default_factory=dict)  # raw PlanDesc fields (for error tests)
+
+
+_INTEGRATORS = {"heun": abi.INTEGRATOR_HEUN, "dp45": abi.INTEGRATOR_DP45, "rk45": abi.INTEGRATOR_DP45,
+                "rk4": abi.INTEGRATOR_RK4, "abm3": abi.INTEGRATOR_ABM3, "rk23": abi.INTEGRATOR_RK23}
+_OBS = {"m": abi.OBSERVABLE_M, "h_ex": abi.OBSERVABLE_H_EX, "h_demag": abi.OBSERVABLE_H_DEMAG,
+        "h_ext": abi.OBSERVABLE_H_EXT, "h_eff": abi.OBSERVABLE_H_EFF}
+
+
+def _dp(a):
+    return a.ctypes.data_as(abi.c_double_p)
+
+
+def is_available() -> bool:
+    return bool(abi.load().fullmag_fdm_is_available())
+
+
+class FdmBackend:
+    """One engine handle (one host thread drives it)."""
+
+    def __init__(self, plan: FdmPlan, lib_path: str | None = None):
+        self.lib = abi.load(lib_path)
+        self.plan = plan
+        self.n = plan.nx * plan.ny * plan.nz
+        self._keep = []
+        desc = abi.PlanDesc()
+        desc.grid = abi.GridDesc(plan.nx, plan.ny, plan.nz, plan.dx, plan.dy, plan.dz)
+        desc.material = abi.MaterialDesc(plan.ms, plan.a_ex, plan.alpha, plan.gamma)
+        desc.precision = abi.PRECISION_DOUBLE if plan.precision == "double" else abi.PRECISION_SINGLE
+        desc.integrator = _INTEGRATORS[plan.integrator]
+        desc.disable_precession = 0 if plan.precession else 1
+        desc.enable_exchange = int(plan.exchange)
+        desc.enable_demag = int(plan.demag)
+        if plan.h_ext is not None:
+            desc.has_external_field = 1
+            desc.external_field_am[:] = list(plan.h_ext)
+        if plan.axis is not None:
+            desc.has_uniaxial_anisotropy = 1
+            desc.uniaxial_anisotropy_constant = plan.ku1
+            desc.uniaxial_anisotropy_k2 = plan.ku2
+            desc.anisotropy_axis[:] = list(plan.axis)
+        m0 = np.ascontiguousarray(np.asarray(plan.m0, dtype=np.float64).reshape(-1))
+        self._keep.append(m0)
+        desc.initial_magnetization_xyz = _dp(m0)
+        desc.initial_magnetization_len = m0.size
+        if plan.mask is not None:
+            mk = np.ascontiguousarray(np.asarray(plan.mask, dtype=np.uint8).reshape(-1))
+            self._keep.append(mk)
+            desc.active_mask = mk.ctypes.data_as(abi.c_u8_p)
+            desc.active_mask_len = mk.size
+        if plan.regions is not None:
+            rg = np.ascontiguousarray(np.asarray(plan.regions, dtype=np.uint32).reshape(-1))
+            self._keep.append(rg)
+            desc.region_mask = rg.ctypes.data_as(abi.c_u32_p)
+            desc.region_mask_len = rg.size
+        if plan.exchange_lut is not None:
+            lt = np.ascontiguousarray(np.asarray(plan.exchange_lut, dtype=np.float64).reshape(-1))
+            self._keep.append(lt)
+            desc.exchange_lut = _dp(lt)
+            desc.exchange_lut_len = lt.size
+        if plan.spectra is not None:
+            ln = None
+            for nm in ("xx", "yy", "zz", "xy", "xz", "yz"):
+                if nm in plan.spectra:
+                    a = np.ascontiguousarray(np.asarray(plan.spectra[nm], dtype=np.float64))
+                    self._keep.append(a)
+                    setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
+                    ln = a.size
+            desc.demag_kernel_spectrum_len = ln
+        desc.adaptive_max_error = plan.atol
+        desc.adaptive_dt_min = plan.dt_min
+        desc.adaptive_dt_max = plan.dt_max
+        desc.adaptive_headroom = plan.headroom
+        for k, v in plan.extra.items():
+            setattr(desc, k, v)
+        self._desc = desc
+        self.h = self.lib.fullmag_fdm_backend_create(ctypes.byref(desc))
+        if not self.h:
+            raise FdmError(abi.ERR_INTERNAL, "fullmag_fdm_backend_create returned NULL")
+        err = self.lib.fullmag_fdm_backend_last_error(self.h)
+        if err:
+            msg = err.decode()
+            self.lib.fullmag_fdm_backend_destroy(self.h)
+            self.h = None
+            raise FdmError(abi.ERR_INVALID, msg)
+        self._poll = None
+
+    # ---------------------------------------------------------------- core
+    def _check(self, rc):
+        if rc == abi.OK:
+            return
+        err = self.lib.fullmag_fdm_backend_last_error(self.h)
+        msg = err.decode() if err else "error"
+        if rc == abi.ERR_INTERRUPTED:
+            raise Interrupted(rc, msg)
+        raise FdmError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fullmag_fdm_backend_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, dt: float) -> dict:
+        st = abi.StepStats()
+        self._check(self.lib.fullmag_fdm_backend_step(self.h, dt, ctypes.byref(st)))
+        return st.as_dict()
+
+    def advance(self, t_end: float, max_steps: int, dt: float) -> dict:
+        st = abi.StepStats()
+        n = ctypes.c_uint64(0)
+        self._check(self.lib.fullmag_fdm_backend_advance(self.h, t_end, max_steps, dt, ctypes.byref(st),
+                                                         ctypes.byref(n)))
+        d = st.as_dict()
+        d["steps_taken"] = int(n.value)
+        return d
+
+    def set_adaptive_ext(self, rtol: float, growth: float = 0.0, shrink: float = 0.0):
+        self._check(self.lib.fullmag_fdm_backend_set_adaptive_ext(self.h, rtol, growth, shrink))
+
+    def counters(self):
+        a, r, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        self._check(self.lib.fullmag_fdm_backend_get_counters(self.h, ctypes.byref(a), ctypes.byref(r),
+                                                              ctypes.byref(t)))
+        return {"accepted": int(a.value), "rejected": int(r.value), "time": t.value}
+
+    def set_interrupt_poll(self, fn):
+        """fn() -> bool; called on the stepping thread between stages."""
+        if fn is None:
+            self._poll = None
+            self._check(self.lib.fullmag_fdm_backend_set_interrupt_poll(self.h, abi.POLL_FN(), None))
+            return
+        self._poll = abi.POLL_FN(lambda _u: 1 if fn() else 0)
+        self._check(self.lib.fullmag_fdm_backend_set_interrupt_poll(self.h, self._poll, None))
+
+    def field(self, name: str = "m", dtype=np.float64) -> np.ndarray:
+        if dtype == np.float32:
+            out = np.zeros((self.n, 3), dtype=np.float32)
+            self._check(self.lib.fullmag_fdm_backend_copy_field_f32(
+                self.h, _OBS[name], out.ctypes.data_as(abi.c_float_p), out.size))
+        else:
+            out = np.zeros((self.n, 3))
+            self._check(self.lib.fullmag_fdm_backend_copy_field_f64(self.h, _OBS[name], _dp(out), out.size))
+        return out
+
+    def preview(self, name, pnx, pny, pnz, z_origin=0, z_stride=1, dtype=np.float64):
+        cnt = pnx * pny * pnz
+        if dtype == np.float32:
+            out = np.zeros((cnt, 3), dtype=np.float32)
+            self._check(self.lib.fullmag_fdm_backend_copy_field_preview_f32(
+                self.h, _OBS[name], pnx, pny, pnz, z_origin, z_stride, out.ctypes.data_as(abi.c_float_p), out.size))
+        else:
+            out = np.zeros((cnt, 3))
+            self._check(self.lib.fullmag_fdm_backend_copy_field_preview_f64(
+                self.h, _OBS[name], pnx, pny, pnz, z_origin, z_stride, _dp(out), out.size))
+        return out
+
+    def field_snapshot(self, name: str = "m") -> np.ndarray:
+        """Asynchronous snapshot API; returns the SoA payload as (3, n)."""
+        s = self.lib.fullmag_fdm_backend_begin_field_snapshot(self.h, _OBS[name])
+        if not s:
+            self._check(abi.ERR_INTERNAL)
+        try:
+            p, ln, d = ctypes.c_void_p(), ctypes.c_uint64(), abi.SnapshotDesc()
+            rc = self.lib.fullmag_fdm_field_snapshot_wait(s, ctypes.byref(p), ctypes.byref(ln), ctypes.byref(d))
+            if rc != abi.OK:
+                raise FdmError(rc, "snapshot wait failed")
+            dt = np.float64 if d.scalar_type == abi.SNAPSHOT_F64 else np.float32
+            buf = (ctypes.c_char * ln.value).from_address(p.value)
+            return np.frombuffer(bytes(buf), dtype=dt).reshape(3, d.cell_count).copy()
+        finally:
+            self.lib.fullmag_fdm_field_snapshot_destroy(s)
+
+    def upload_m(self, m: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))
+        self._check(self.lib.fullmag_fdm_backend_upload_magnetization_f64(self.h, _dp(a), a.size))
+
+    def refresh(self):
+        self._check(self.lib.fullmag_fdm_backend_refresh_observables(self.h))
+
+    def stats(self) -> dict:
+        st = abi.StepStats()
+        self._check(self.lib.fullmag_fdm_backend_snapshot_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def device_info(self) -> dict:
+        d = abi.DeviceInfo()
+        self._check(self.lib.fullmag_fdm_backend_get_device_info(self.h, ctypes.byref(d)))
+        return {"name": d.name.decode(), "major": d.compute_capability_major,
+                "minor": d.compute_capability_minor, "driver": d.driver_version, "runtime": d.runtime_version}
+
+    def time_kernels(self, reps: int = 20) -> list:
+        out = np.zeros(8)
+        self._check(self.lib.fullmag_fdm_backend_time_kernels(self.h, reps, _dp(out), 8))
+        return out[:7].tolist()
+
+
+# ------------------------------------------------------------ input builders
+def random_unit_vectors(seed: int, count: int) -> np.ndarray:
+    """xorshift64 random unit vectors of the planner,
+    crates/fullmag-plan/src/util.rs:118-147 (seed 0 never terminates there
+    and is refused here); generated by the library's host code."""
+    if seed == 0:
+        raise ValueError("seed 0 is not allowed")
+    out = np.empty((count, 3))
+    rc = abi.load().fullmag_fdm_random_unit_vectors(seed, count, _dp(out))
+    if rc != abi.OK:
+        raise FdmError(rc, "random_unit_vectors failed")
+    return out
+
+
+def hole_mask(nx: int, ny: int, nz: int, dx: float, dy: float, radius: float) -> np.ndarray:
+    """Box minus a centred cylinder by cell centres,
+    crates/fullmag-plan/src/geometry.rs:159-231."""
+    x = (np.arange(nx) + 0.5) * dx - nx * dx * 0.5
+    y = (np.arange(ny) + 0.5) * dy - ny * dy * 0.5
+    r2 = x[None, :] ** 2 + y[:, None] ** 2
+    plane = (r2 > radius * radius).astype(np.uint8)
+    return np.ascontiguousarray(np.broadcast_to(plane, (nz, ny, nx)).reshape(-1))

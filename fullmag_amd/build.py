@@ -1,0 +1,66 @@
+"""Build libfullmag_fdm.so (SONAME libfullmag_fdm.so.0) for gfx950 in-tree.
+
+    python -m fullmag_amd.build
+
+compiles fullmag_amd/csrc/*.{hip,cpp} with hipcc into
+fullmag_amd/_build/libfullmag_fdm.so.  The library has no torch or Python
+dependency: it is the C-ABI drop-in for the reference's libfullmag_fdm.so.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_build")
+LIB_NAME = "libfullmag_fdm.so"
+LIB_PATH = os.path.join(OUT_DIR, LIB_NAME)
+SONAME = "libfullmag_fdm.so.0"
+SOURCES = ["kernels.hip", "api.cpp", "newell_host.cpp"]
+ARCH = os.environ.get("FULLMAG_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source and link the shared library; returns its path."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objs = []
+    common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+              "-Wno-unused-result", "-I" + os.path.join(HERE, "..", "include")]
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(OUT_DIR, src.replace(".", "_") + ".o")
+        cmd = [hipcc()] + common
+        if src.endswith(".hip"):
+            # packed FP32 instructions disabled: plain scalar FP32 only
+            cmd += ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", "-ffp-contract=fast"]
+        else:
+            # host-only translation units: HIP runtime API, no device code
+            cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wno-unused-value"]
+        cmd += ["-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + [
+        f"-Wl,-soname,{SONAME}", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

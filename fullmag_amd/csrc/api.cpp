@@ -1,0 +1,1082 @@
+// api.cpp -- C ABI of the MI355X FDM engine (include/fullmag_fdm.h and
+// include/fullmag_fdm_ext.h).  Host-side counterpart of the reference's
+// native/backends/fdm/src/api.cpp:151-1070 and context.cu: plan validation,
+// device allocation, uploads/downloads, step driving, snapshots.
+//
+// Step semantics follow the reference CPU engine (crates/fullmag-engine):
+// the trial dt is clamped to [dt_min, dt_max], rejected trials shrink dt and
+// retry until one is accepted, and the accepted step reports the
+// controller's suggested next dt.  The trial/accept loop runs on the device;
+// the host reads one small control block per accepted step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/fullmag_fdm.h"
+#include "../../include/fullmag_fdm_ext.h"
+#include "engine.hpp"
+
+namespace fm {
+hipError_t upload_methods();
+int method_nstage(int method);
+template <class T>
+const char *configure(Args<T> &A, size_t lds_limit);
+template <class T>
+long long num_partials(const Args<T> &A);
+template <class T>
+hipError_t enqueue_eval(const Args<T> &A, int s, int stats, hipStream_t st);
+template <class T>
+hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st);
+template <class T>
+hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
+std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
+bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz, std::vector<double> &out,
+                          std::string *err);
+}  // namespace fm
+
+namespace {
+constexpr double kMu0 = 4.0 * 3.14159265358979323846 * 1e-7;
+constexpr int kAttemptChunk = 8;  // trial steps enqueued per completion check in advance()
+}  // namespace
+
+struct fullmag_fdm_backend {
+    std::string err;
+    bool ok = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool fp64 = false;
+    uint32_t nx = 0, ny = 0, nz = 0;
+    uint64_t n = 0;
+    int nstage = 0;
+    bool adaptive = false;
+    fm::Args<float> af{};
+    fm::Args<double> ad{};
+    std::vector<void *> allocs;
+    fm::Ctl *ctl_d = nullptr;
+    fm::Ctl *ctl_h = nullptr;  // pinned mirror
+    std::vector<uint8_t> mask_h;
+    bool has_mask = false;
+    double hext[3] = {0, 0, 0};
+    bool has_ext = false;
+    fullmag_fdm_interrupt_poll_fn poll = nullptr;
+    void *poll_user = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_tried = false;
+    bool obs_fresh = false;
+    void *obs_base[3] = {nullptr, nullptr, nullptr};  // hex, hd, heff
+};
+
+struct fullmag_fdm_field_snapshot {
+    void *host = nullptr;  // pinned
+    uint64_t bytes = 0;
+    fullmag_fdm_snapshot_desc desc{};
+    hipEvent_t ev = nullptr;
+    bool waited = false;
+};
+
+struct fullmag_fdm_preview_snapshot {
+    std::vector<unsigned char> host;
+    fullmag_fdm_snapshot_desc desc{};
+};
+
+namespace {
+
+using Backend = fullmag_fdm_backend;
+
+template <class F>
+auto with_args(Backend *h, F &&f) {
+    return h->fp64 ? f(h->ad) : f(h->af);
+}
+
+bool hip_ok(Backend *h, hipError_t e, const char *what) {
+    if (e == hipSuccess) return true;
+    h->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+int fail(Backend *h, int code, const std::string &msg) {
+    h->err = msg;
+    return code;
+}
+
+void *dalloc(Backend *h, size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+    h->allocs.push_back(p);
+    return p;
+}
+
+bool plan_fft(int n, fm::FftDesc &fd, std::string *err) {
+    fd.n = n;
+    fd.npass = 0;
+    int m = n;
+    auto push = [&](int r) {
+        if (fd.npass >= fm::kMaxPasses) return false;
+        fd.radix[fd.npass++] = r;
+        return true;
+    };
+    while (m % 8 == 0) {
+        if (!push(8)) return false;
+        m /= 8;
+    }
+    while (m % 4 == 0) {
+        if (!push(4)) return false;
+        m /= 4;
+    }
+    while (m % 2 == 0) {
+        if (!push(2)) return false;
+        m /= 2;
+    }
+    for (int p = 3; m > 1; p += 2) {
+        while (m % p == 0) {
+            if (p > fm::kMaxRadix) {
+                *err = "padded FFT length " + std::to_string(n) + " has a prime factor " + std::to_string(p) +
+                       " > " + std::to_string(fm::kMaxRadix);
+                return false;
+            }
+            if (!push(p)) return false;
+            m /= p;
+        }
+    }
+    return true;
+}
+
+template <class T>
+void *upload_twiddles(Backend *h, int n) {
+    std::vector<T> tw(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        const double a = 2.0 * 3.14159265358979323846 * (double)k / (double)n;
+        tw[2 * k] = (T)std::cos(a);
+        tw[2 * k + 1] = (T)(-std::sin(a));
+    }
+    void *d = dalloc(h, tw.size() * sizeof(T));
+    if (d && hipMemcpy(d, tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+}
+
+int select_device(Backend *h) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(h, FULLMAG_FDM_ERR_CUDA, "no HIP device");
+    int dev = 0;
+    const char *e = std::getenv("FULLMAG_FDM_GPU_INDEX");
+    if (!e) e = std::getenv("FULLMAG_CUDA_DEVICE_INDEX");
+    if (e) dev = std::atoi(e);
+    if (dev < 0 || dev >= count) return fail(h, FULLMAG_FDM_ERR_INVALID, "GPU index out of range");
+    h->device = dev;
+    if (hipSetDevice(dev) != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, "hipSetDevice failed");
+    return FULLMAG_FDM_OK;
+}
+
+std::string unsupported_terms(const fullmag_fdm_plan_desc *p) {
+    std::vector<std::string> bad;
+    if (p->has_cubic_anisotropy) bad.push_back("cubic anisotropy");
+    if (p->kc1_field || p->kc2_field || p->kc3_field) bad.push_back("per-cell cubic constants");
+    if (p->ku1_field || p->ku2_field) bad.push_back("per-cell uniaxial constants");
+    if (p->has_interfacial_dmi && p->dmi_D_interfacial != 0.0) bad.push_back("interfacial DMI");
+    if (p->has_bulk_dmi && p->dmi_D_bulk != 0.0) bad.push_back("bulk DMI");
+    if (p->has_magnetoelastic) bad.push_back("magnetoelastic coupling");
+    if (p->temperature > 0.0) bad.push_back("thermal field");
+    if (p->current_density_x != 0.0 || p->current_density_y != 0.0 || p->current_density_z != 0.0)
+        bad.push_back("spin-transfer torque");
+    if (p->has_sot) bad.push_back("spin-orbit torque");
+    if (p->has_oersted_cylinder) bad.push_back("Oersted field");
+    if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE) bad.push_back("boundary correction tiers");
+    if (p->has_demag_boundary_corr) bad.push_back("sparse demag boundary correction");
+    std::string s;
+    for (size_t i = 0; i < bad.size(); ++i) s += (i ? ", " : "") + bad[i];
+    return s;
+}
+
+template <class T>
+bool upload_state(Backend *h, const double *aos_d, const float *aos_f, int slot) {
+    auto &A = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
+    const uint64_t n = h->n;
+    std::vector<T> soa(3 * n);
+    for (uint64_t i = 0; i < n; ++i) {
+        double v[3];
+        for (int c = 0; c < 3; ++c) v[c] = aos_d ? aos_d[3 * i + c] : (double)aos_f[3 * i + c];
+        const bool act = !h->has_mask || h->mask_h[i] != 0;
+        const double nrm = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int c = 0; c < 3; ++c) soa[c * n + i] = (act && nrm > 0.0) ? (T)(v[c] * (1.0 / nrm)) : (T)0;
+    }
+    return hip_ok(h, hipMemcpy(A.m_all + (size_t)slot * 3 * n, soa.data(), soa.size() * sizeof(T), hipMemcpyHostToDevice),
+                  "upload magnetisation");
+}
+
+bool push_ctl(Backend *h) {
+    return hip_ok(h, hipMemcpyAsync(h->ctl_d, h->ctl_h, sizeof(fm::Ctl), hipMemcpyHostToDevice, h->stream), "ctl upload");
+}
+
+bool pull_ctl(Backend *h) {
+    if (!hip_ok(h, hipMemcpyAsync(h->ctl_h, h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream), "ctl download"))
+        return false;
+    return hip_ok(h, hipStreamSynchronize(h->stream), "stream sync");
+}
+
+// k1 of the current state and its statistics (stage-1 evaluation).
+bool eval_current(Backend *h, bool observables) {
+    const bool ok = with_args(h, [&](auto &A) {
+        auto B = A;
+        if (observables) {
+            using T = typename std::remove_pointer<decltype(A.m_all)>::type;
+            B.obs_hex = reinterpret_cast<T *>(h->obs_base[0]);
+            B.obs_hd = reinterpret_cast<T *>(h->obs_base[1]);
+            B.obs_heff = reinterpret_cast<T *>(h->obs_base[2]);
+        }
+        if (!hip_ok(h, fm::enqueue_eval(B, 1, 1, h->stream), "launch (evaluation)")) return false;
+        return hip_ok(h, fm::enqueue_control(B, 1, h->stream), "launch (statistics)");
+    });
+    if (!ok || !pull_ctl(h)) return false;
+    if (observables) h->obs_fresh = true;
+    return true;
+}
+
+bool enqueue_attempt(Backend *h) {
+    return with_args(h, [&](auto &A) {
+        for (int s = 2; s <= h->nstage; ++s)
+            if (!hip_ok(h, fm::enqueue_eval(A, s, s == h->nstage ? 1 : 0, h->stream), "launch (stage)")) return false;
+        return hip_ok(h, fm::enqueue_control(A, 0, h->stream), "launch (control)");
+    });
+}
+
+// Captures one trial step as a HIP graph (falls back to direct launches).
+void ensure_graph(Backend *h) {
+    if (h->graph_tried) return;
+    h->graph_tried = true;
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+    const bool ok = enqueue_attempt(h);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (!ok || e != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        (void)hipGetLastError();
+        h->err.clear();
+        return;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return;
+    }
+    h->graph = g;
+    h->graph_exec = ex;
+}
+
+bool launch_attempt(Backend *h) {
+    if (h->graph_exec) return hip_ok(h, hipGraphLaunch(h->graph_exec, h->stream), "graph launch");
+    return enqueue_attempt(h);
+}
+
+void fill_stats(Backend *h, fullmag_fdm_step_stats *s, double dt, uint64_t wall_ns) {
+    if (!s) return;
+    const fm::Ctl &c = *h->ctl_h;
+    std::memset(s, 0, sizeof(*s));
+    s->step = c.step;
+    s->time_seconds = c.t;
+    s->dt_seconds = dt;
+    s->exchange_energy_joules = c.stats[fm::ST_EEX];
+    s->demag_energy_joules = c.stats[fm::ST_EDEMAG];
+    s->external_energy_joules = c.stats[fm::ST_EEXT];
+    s->anisotropy_energy_joules = c.stats[fm::ST_EANI];
+    s->total_energy_joules =
+        c.stats[fm::ST_EEX] + c.stats[fm::ST_EDEMAG] + c.stats[fm::ST_EEXT] + c.stats[fm::ST_EANI];
+    s->max_effective_field_amplitude = std::sqrt(c.stats[fm::ST_HEFF2]);
+    s->max_demag_field_amplitude = std::sqrt(c.stats[fm::ST_HD2]);
+    s->max_rhs_amplitude = std::sqrt(c.stats[fm::ST_RHS2]);
+    s->suggested_next_dt = h->adaptive ? c.dt_next : 0.0;
+    s->wall_time_ns = wall_ns;
+}
+
+template <class T>
+bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
+    fm::Args<T> &A = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
+    std::memset(&A, 0, sizeof(A));
+    const int nx = (int)h->nx, ny = (int)h->ny, nz = (int)h->nz;
+    const uint64_t n = h->n;
+    A.nx = nx;
+    A.ny = ny;
+    A.nz = nz;
+    A.ncell = (long long)n;
+    A.nrows = (long long)ny * nz;
+    A.nkx = nx + 1;
+    A.ly = nz == 1 ? ny : 2 * ny;
+    A.nzs = nz;
+    A.px = 2 * nx;
+    A.py = 2 * ny;
+    A.pz = nz == 1 ? 1 : 2 * nz;
+    A.inv_p = 1.0 / ((double)A.px * A.py * A.pz);
+    const double ms = p->material.saturation_magnetisation;
+    double a_ex = p->material.exchange_stiffness;
+    const double alpha = p->material.damping;
+    const double gamma = p->material.gyromagnetic_ratio > 0.0 ? p->material.gyromagnetic_ratio : 2.211e5;
+    A.ms = (T)ms;
+    A.demag = p->enable_demag ? 1 : 0;
+    A.precession = p->disable_precession ? 0 : 1;
+    A.gamma_bar = (T)(gamma / (1.0 + alpha * alpha));
+    A.alpha = (T)alpha;
+    A.idx2 = (T)(1.0 / (p->grid.dx * p->grid.dx));
+    A.idy2 = (T)(1.0 / (p->grid.dy * p->grid.dy));
+    A.idz2 = (T)(1.0 / (p->grid.dz * p->grid.dz));
+    A.e_pre = kMu0 * ms * p->grid.dx * p->grid.dy * p->grid.dz;
+    h->has_ext = p->has_external_field != 0;
+    A.has_ext = h->has_ext ? 1 : 0;
+    for (int c = 0; c < 3; ++c) {
+        h->hext[c] = h->has_ext ? p->external_field_am[c] : 0.0;
+        A.hext[c] = (T)h->hext[c];
+    }
+    A.has_uni = p->has_uniaxial_anisotropy ? 1 : 0;
+    if (A.has_uni) {
+        const double *u = p->anisotropy_axis;
+        const double nu = std::max(std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), 1e-30);
+        for (int c = 0; c < 3; ++c) A.axis[c] = (T)(u[c] / nu);
+        A.uni_c1 = (T)(2.0 * p->uniaxial_anisotropy_constant / (kMu0 * ms));
+        A.uni_c2 = (T)(4.0 * p->uniaxial_anisotropy_k2 / (kMu0 * ms));
+    }
+    A.atol = p->adaptive_max_error > 0 ? p->adaptive_max_error : 1e-5;
+    A.dt_min = p->adaptive_dt_min > 0 ? p->adaptive_dt_min : 1e-18;
+    A.dt_max = p->adaptive_dt_max > 0 ? p->adaptive_dt_max : 1e-10;
+    A.headroom = p->adaptive_headroom > 0 ? p->adaptive_headroom : 0.8;
+    A.rtol = 0.0;
+    A.growth = std::numeric_limits<double>::infinity();
+    A.shrink = 0.0;
+    A.method = p->integrator == FULLMAG_FDM_INTEGRATOR_DP45 ? fm::M_DP45
+               : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4 ? fm::M_RK4
+                                                               : fm::M_HEUN;
+    h->nstage = fm::method_nstage(A.method);
+    h->adaptive = A.method == fm::M_DP45;
+
+    // geometry
+    if (p->active_mask) {
+        if (p->active_mask_len != n) {
+            h->err = "active_mask_len mismatch: expected " + std::to_string(n) + ", got " +
+                     std::to_string(p->active_mask_len);
+            return false;
+        }
+        h->has_mask = true;
+        h->mask_h.assign(p->active_mask, p->active_mask + n);
+        for (auto &v : h->mask_h) v = v ? 1 : 0;
+        auto *d = static_cast<uint8_t *>(dalloc(h, n));
+        if (!d) return hip_ok(h, hipErrorOutOfMemory, "alloc mask");
+        if (!hip_ok(h, hipMemcpy(d, h->mask_h.data(), n, hipMemcpyHostToDevice), "upload mask")) return false;
+        A.mask = d;
+    }
+    // exchange: uniform stiffness unless the region map uses more than one id
+    if (p->region_mask) {
+        if (p->region_mask_len != n) {
+            h->err = "region_mask_len mismatch: expected " + std::to_string(n) + ", got " +
+                     std::to_string(p->region_mask_len);
+            return false;
+        }
+        const int R = FULLMAG_FDM_MAX_EXCHANGE_REGIONS;
+        if (p->exchange_lut && p->exchange_lut_len != (uint64_t)R * R) {
+            h->err = "exchange_lut_len mismatch: expected " + std::to_string(R * R);
+            return false;
+        }
+        std::vector<double> lut((size_t)R * R, 0.0);
+        if (p->exchange_lut) std::memcpy(lut.data(), p->exchange_lut, lut.size() * sizeof(double));
+        else
+            for (int r = 0; r < R; ++r) lut[(size_t)r * R + r] = a_ex;
+        bool uniform = true;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (p->region_mask[i] >= (uint32_t)R) {
+                h->err = "region id out of range (>= 256)";
+                return false;
+            }
+            if (p->region_mask[i] != p->region_mask[0]) uniform = false;
+        }
+        if (uniform) {
+            a_ex = lut[(size_t)p->region_mask[0] * R + p->region_mask[0]];
+        } else {
+            auto *rd = static_cast<uint32_t *>(dalloc(h, n * 4));
+            auto *ld = static_cast<T *>(dalloc(h, lut.size() * sizeof(T)));
+            if (!rd || !ld) return hip_ok(h, hipErrorOutOfMemory, "alloc regions");
+            std::vector<T> lt(lut.begin(), lut.end());
+            if (!hip_ok(h, hipMemcpy(rd, p->region_mask, n * 4, hipMemcpyHostToDevice), "upload regions")) return false;
+            if (!hip_ok(h, hipMemcpy(ld, lt.data(), lt.size() * sizeof(T), hipMemcpyHostToDevice), "upload lut"))
+                return false;
+            A.region = rd;
+            A.lut = ld;
+        }
+    }
+    A.ex_pre = p->enable_exchange ? (T)(2.0 * a_ex / (kMu0 * ms)) : (T)0;
+    A.lut_pre = p->enable_exchange ? (T)(2.0 / (kMu0 * ms)) : (T)0;
+
+    // state buffers
+    A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));
+    A.k_all = static_cast<T *>(dalloc(h, 7 * 3 * n * sizeof(T)));
+    A.ybuf = static_cast<T *>(dalloc(h, 3 * n * sizeof(T)));
+    for (int i = 0; i < 3; ++i) h->obs_base[i] = dalloc(h, 3 * n * sizeof(T));
+    h->ctl_d = static_cast<fm::Ctl *>(dalloc(h, sizeof(fm::Ctl)));
+    if (!A.m_all || !A.k_all || !A.ybuf || !h->obs_base[0] || !h->obs_base[1] || !h->obs_base[2] || !h->ctl_d)
+        return hip_ok(h, hipErrorOutOfMemory, "alloc state");
+    A.ctl = h->ctl_d;
+    if (!hip_ok(h, hipMemset(A.k_all, 0, 7 * 3 * n * sizeof(T)), "clear k")) return false;
+    if (!hip_ok(h, hipMemset(A.m_all, 0, 2 * 3 * n * sizeof(T)), "clear m")) return false;
+
+    // demag
+    if (A.demag) {
+        std::string e;
+        if (!plan_fft(A.px, A.fx, &e) || !plan_fft(A.py, A.fy, &e) || !plan_fft(A.pz, A.fz, &e)) {
+            h->err = e.empty() ? "FFT plan failed" : e;
+            return false;
+        }
+        std::vector<double> red;
+        const double *spec[6] = {p->demag_kernel_xx_spectrum, p->demag_kernel_yy_spectrum, p->demag_kernel_zz_spectrum,
+                                 p->demag_kernel_xy_spectrum, p->demag_kernel_xz_spectrum, p->demag_kernel_yz_spectrum};
+        if (spec[0]) {
+            if (!fm::reduce_given_spectra(spec, p->demag_kernel_spectrum_len, nx, ny, nz, red, &e)) {
+                h->err = e;
+                return false;
+            }
+        } else {
+            red = fm::newell_reduced_spectrum(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz);
+        }
+        std::vector<T> rt(red.begin(), red.end());
+        T *nt = static_cast<T *>(dalloc(h, rt.size() * sizeof(T)));
+        if (!nt) return hip_ok(h, hipErrorOutOfMemory, "alloc tensor");
+        if (!hip_ok(h, hipMemcpy(nt, rt.data(), rt.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))
+            return false;
+        A.ntens = nt;
+        const size_t spec_elems = 3ull * A.nzs * A.nkx * A.ly;
+        A.spec = dalloc(h, spec_elems * 2 * sizeof(T));
+        A.twx = upload_twiddles<T>(h, A.px);
+        A.twy = upload_twiddles<T>(h, A.py);
+        A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
+        if (!A.spec || !A.twx || !A.twy || !A.twz) return hip_ok(h, hipErrorOutOfMemory, "alloc spectrum");
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)
+            lds = 65536;
+        const char *ce = fm::configure(A, (size_t)lds - 1024);
+        if (ce && *ce) {
+            h->err = ce;
+            return false;
+        }
+    }
+    const long long nparts = fm::num_partials(A);
+    A.partials = static_cast<double *>(dalloc(h, (size_t)nparts * fm::kStatSlots * sizeof(double)));
+    if (!A.partials) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
+    return upload_state<T>(h, p->initial_magnetization_xyz, nullptr, 0);
+}
+
+bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p) {
+    if (select_device(h) != FULLMAG_FDM_OK) return false;
+    const auto &g = p->grid;
+    if (g.nx == 0 || g.ny == 0 || g.nz == 0) return (h->err = "grid dimensions must be positive", false);
+    if (!(g.dx > 0) || !(g.dy > 0) || !(g.dz > 0)) return (h->err = "cell sizes must be positive", false);
+    if (!(p->material.saturation_magnetisation > 0)) return (h->err = "saturation magnetisation must be positive", false);
+    if (p->precision != FULLMAG_FDM_PRECISION_SINGLE && p->precision != FULLMAG_FDM_PRECISION_DOUBLE)
+        return (h->err = "invalid precision", false);
+    if (p->integrator != FULLMAG_FDM_INTEGRATOR_DP45 && p->integrator != FULLMAG_FDM_INTEGRATOR_RK4 &&
+        p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN)
+        return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4)", false);
+    const std::string bad = unsupported_terms(p);
+    if (!bad.empty()) return (h->err = "terms not supported by fdm_gpu_native: " + bad, false);
+    h->nx = g.nx;
+    h->ny = g.ny;
+    h->nz = g.nz;
+    h->n = (uint64_t)g.nx * g.ny * g.nz;
+    if (!p->initial_magnetization_xyz || p->initial_magnetization_len != 3 * h->n)
+        return (h->err = "initial_magnetization_len mismatch: expected " + std::to_string(3 * h->n) + ", got " +
+                         std::to_string(p->initial_magnetization_len),
+                false);
+    h->fp64 = p->precision == FULLMAG_FDM_PRECISION_DOUBLE;
+    if (!hip_ok(h, fm::upload_methods(), "method table")) return false;
+    if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) return false;
+    if (!hip_ok(h, hipHostMalloc((void **)&h->ctl_h, sizeof(fm::Ctl), hipHostMallocDefault), "pinned ctl"))
+        return false;
+    const bool ok = h->fp64 ? create_typed<double>(h, p) : create_typed<float>(h, p);
+    if (!ok) return false;
+    std::memset(h->ctl_h, 0, sizeof(fm::Ctl));
+    h->ctl_h->dt = 1e-13;
+    if (!push_ctl(h)) return false;
+    if (!eval_current(h, true)) return false;
+    h->ok = true;
+    return true;
+}
+
+template <class T>
+bool download_soa(Backend *h, const void *dev, std::vector<T> &out) {
+    out.resize(3 * h->n);
+    return hip_ok(h, hipMemcpy(out.data(), dev, out.size() * sizeof(T), hipMemcpyDeviceToHost), "download");
+}
+
+// Field as AoS double on the host.
+bool field_aos(Backend *h, fullmag_fdm_observable obs, std::vector<double> &out) {
+    const uint64_t n = h->n;
+    out.assign(3 * n, 0.0);
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        for (uint64_t i = 0; i < n; ++i)
+            if (h->has_ext && (!h->has_mask || h->mask_h[i]))
+                for (int c = 0; c < 3; ++c) out[3 * i + c] = h->hext[c];
+        return true;
+    }
+    const void *src = nullptr;
+    if (obs == FULLMAG_FDM_OBSERVABLE_M) {
+        src = with_args(h, [&](auto &A) -> const void * { return A.m_all + (size_t)h->ctl_h->parity * 3 * n; });
+    } else {
+        if (!h->obs_fresh && !eval_current(h, true)) return false;
+        const int w = obs == FULLMAG_FDM_OBSERVABLE_H_EX ? 0 : obs == FULLMAG_FDM_OBSERVABLE_H_DEMAG ? 1 : 2;
+        src = h->obs_base[w];
+    }
+    if (h->fp64) {
+        std::vector<double> s;
+        if (!download_soa(h, src, s)) return false;
+        for (uint64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) out[3 * i + c] = s[c * n + i];
+    } else {
+        std::vector<float> s;
+        if (!download_soa(h, src, s)) return false;
+        for (uint64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) out[3 * i + c] = (double)s[c * n + i];
+    }
+    return true;
+}
+
+bool valid_observable(int o) { return o >= FULLMAG_FDM_OBSERVABLE_M && o <= FULLMAG_FDM_OBSERVABLE_H_EFF; }
+
+// Bin averages, reference context.cu:254-317 binning rule.
+void preview_bins(Backend *h, const std::vector<double> &f, uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0,
+                  uint32_t zs, std::vector<double> &out) {
+    const uint32_t fx = h->nx, fy = h->ny, fz = h->nz;
+    out.assign((size_t)pnx * pny * pnz * 3, 0.0);
+    for (uint32_t pz = 0; pz < pnz; ++pz)
+        for (uint32_t py = 0; py < pny; ++py)
+            for (uint32_t px = 0; px < pnx; ++px) {
+                uint32_t xs = (uint32_t)((uint64_t)px * fx / pnx), xe = (uint32_t)((uint64_t)(px + 1) * fx / pnx);
+                if (xe <= xs) xe = xs + 1;
+                if (xe > fx) xe = fx;
+                uint32_t ys = (uint32_t)((uint64_t)py * fy / pny), ye = (uint32_t)((uint64_t)(py + 1) * fy / pny);
+                if (ye <= ys) ye = ys + 1;
+                if (ye > fy) ye = fy;
+                uint32_t zs0 = z0 + pz * zs;
+                if (zs0 >= fz) zs0 = fz - 1;
+                uint32_t ze = z0 + (pz + 1) * zs;
+                if (ze <= zs0) ze = zs0 + 1;
+                if (ze > fz) ze = fz;
+                double a[3] = {0, 0, 0}, cnt = 0;
+                for (uint32_t z = zs0; z < ze; ++z)
+                    for (uint32_t y = ys; y < ye; ++y)
+                        for (uint32_t x = xs; x < xe; ++x) {
+                            const size_t i = ((size_t)z * fy + y) * fx + x;
+                            for (int c = 0; c < 3; ++c) a[c] += f[3 * i + c];
+                            cnt += 1.0;
+                        }
+                const size_t o = ((size_t)pz * pny + py) * pnx + px;
+                for (int c = 0; c < 3; ++c) out[3 * o + c] = cnt > 0 ? a[c] / cnt : 0.0;
+            }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fullmag_fdm_is_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx9", 4) == 0 ? 1 : 0;
+}
+
+fullmag_fdm_backend *fullmag_fdm_backend_create(const fullmag_fdm_plan_desc *plan) {
+    if (!plan) return nullptr;
+    Backend *h = new (std::nothrow) Backend();
+    if (!h) return nullptr;
+    try {
+        if (!create_impl(h, plan) && h->err.empty()) h->err = "create failed";
+    } catch (const std::bad_alloc &) {
+        h->err = "host allocation failed";
+    } catch (...) {
+        h->err = "internal error during create";
+    }
+    return h;
+}
+
+int fullmag_fdm_backend_step(fullmag_fdm_backend *h, double dt, fullmag_fdm_step_stats *out) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, h->err.empty() ? "backend not initialised" : h->err);
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(h, FULLMAG_FDM_ERR_INVALID, "dt must be positive and finite");
+    h->err.clear();
+    hipSetDevice(h->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    const double dmin = h->fp64 ? h->ad.dt_min : h->af.dt_min, dmax = h->fp64 ? h->ad.dt_max : h->af.dt_max;
+    h->ctl_h->dt = h->adaptive ? std::min(std::max(dt, dmin), dmax) : dt;
+    h->ctl_h->loop = 0;
+    h->ctl_h->done = 0;
+    if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    if (!h->poll) ensure_graph(h);
+    h->obs_fresh = false;
+    for (;;) {
+        if (h->poll && h->poll(h->poll_user)) {
+            fill_stats(h, out, 0.0, 0);
+            return fail(h, FULLMAG_FDM_ERR_INTERRUPTED, "interrupted");
+        }
+        if (h->poll) {
+            // stage-by-stage with a poll between stages; the state is only
+            // replaced by the controller, so an interrupt leaves m untouched
+            bool interrupted = false;
+            const bool ok = with_args(h, [&](auto &A) {
+                for (int s = 2; s <= h->nstage; ++s) {
+                    if (!hip_ok(h, fm::enqueue_eval(A, s, s == h->nstage ? 1 : 0, h->stream), "launch")) return false;
+                    if (h->poll(h->poll_user)) {
+                        interrupted = true;
+                        return true;
+                    }
+                }
+                return hip_ok(h, fm::enqueue_control(A, 0, h->stream), "launch (control)");
+            });
+            if (!ok) return FULLMAG_FDM_ERR_CUDA;
+            if (interrupted) {
+                hipStreamSynchronize(h->stream);
+                fill_stats(h, out, 0.0, 0);
+                return fail(h, FULLMAG_FDM_ERR_INTERRUPTED, "interrupted");
+            }
+        } else if (!launch_attempt(h)) {
+            return FULLMAG_FDM_ERR_CUDA;
+        }
+        if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+        if (h->ctl_h->accepted) break;
+    }
+    const uint64_t ns =
+        (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+    fill_stats(h, out, h->ctl_h->dt_used, ns);
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_set_interrupt_poll(fullmag_fdm_backend *h, fullmag_fdm_interrupt_poll_fn fn, void *user) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    h->poll = fn;
+    h->poll_user = user;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_copy_field_f64(fullmag_fdm_backend *h, fullmag_fdm_observable obs, double *out,
+                                       uint64_t len) {
+    if (!h || !out) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (!valid_observable(obs)) return fail(h, FULLMAG_FDM_ERR_INVALID, "unknown observable");
+    if (len != 3 * h->n) return fail(h, FULLMAG_FDM_ERR_INVALID, "out_len mismatch");
+    hipSetDevice(h->device);
+    std::vector<double> f;
+    if (!field_aos(h, obs, f)) return FULLMAG_FDM_ERR_CUDA;
+    std::memcpy(out, f.data(), f.size() * sizeof(double));
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_copy_field_f32(fullmag_fdm_backend *h, fullmag_fdm_observable obs, float *out, uint64_t len) {
+    if (!h || !out) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (!valid_observable(obs)) return fail(h, FULLMAG_FDM_ERR_INVALID, "unknown observable");
+    if (len != 3 * h->n) return fail(h, FULLMAG_FDM_ERR_INVALID, "out_len mismatch");
+    hipSetDevice(h->device);
+    std::vector<double> f;
+    if (!field_aos(h, obs, f)) return FULLMAG_FDM_ERR_CUDA;
+    for (size_t i = 0; i < f.size(); ++i) out[i] = (float)f[i];
+    return FULLMAG_FDM_OK;
+}
+
+static int preview_common(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny, uint32_t pnz,
+                          uint32_t z0, uint32_t zs, uint64_t len, std::vector<double> &out) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (!valid_observable(obs) || pnx == 0 || pny == 0 || pnz == 0 || zs == 0)
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "invalid preview request");
+    if (len != (uint64_t)pnx * pny * pnz * 3) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview out_len mismatch");
+    hipSetDevice(h->device);
+    std::vector<double> f;
+    if (!field_aos(h, obs, f)) return FULLMAG_FDM_ERR_CUDA;
+    preview_bins(h, f, pnx, pny, pnz, z0, zs, out);
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_copy_field_preview_f64(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx,
+                                               uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, double *out,
+                                               uint64_t len) {
+    if (!out) return FULLMAG_FDM_ERR_INVALID;
+    std::vector<double> v;
+    const int rc = preview_common(h, obs, pnx, pny, pnz, z0, zs, len, v);
+    if (rc == FULLMAG_FDM_OK) std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return rc;
+}
+
+int fullmag_fdm_backend_copy_field_preview_f32(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx,
+                                               uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, float *out,
+                                               uint64_t len) {
+    if (!out) return FULLMAG_FDM_ERR_INVALID;
+    std::vector<double> v;
+    const int rc = preview_common(h, obs, pnx, pny, pnz, z0, zs, len, v);
+    if (rc == FULLMAG_FDM_OK)
+        for (size_t i = 0; i < v.size(); ++i) out[i] = (float)v[i];
+    return rc;
+}
+
+fullmag_fdm_field_snapshot *fullmag_fdm_backend_begin_field_snapshot(fullmag_fdm_backend *h,
+                                                                     fullmag_fdm_observable obs) {
+    if (!h || !h->ok || !valid_observable(obs)) {
+        if (h) h->err = "invalid snapshot request";
+        return nullptr;
+    }
+    hipSetDevice(h->device);
+    auto *s = new (std::nothrow) fullmag_fdm_field_snapshot();
+    if (!s) return nullptr;
+    const size_t sb = h->fp64 ? 8 : 4;
+    s->bytes = 3 * h->n * sb;
+    s->desc.cell_count = h->n;
+    s->desc.component_count = 3;
+    s->desc.scalar_bytes = (uint32_t)sb;
+    s->desc.scalar_type = h->fp64 ? FULLMAG_FDM_SNAPSHOT_SCALAR_F64 : FULLMAG_FDM_SNAPSHOT_SCALAR_F32;
+    if (hipHostMalloc(&s->host, s->bytes, hipHostMallocDefault) != hipSuccess) {
+        delete s;
+        h->err = "pinned snapshot allocation failed";
+        return nullptr;
+    }
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        std::vector<double> f;
+        field_aos(h, obs, f);
+        for (uint64_t i = 0; i < h->n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                if (h->fp64) static_cast<double *>(s->host)[c * h->n + i] = f[3 * i + c];
+                else static_cast<float *>(s->host)[c * h->n + i] = (float)f[3 * i + c];
+            }
+        return s;
+    }
+    const void *src;
+    if (obs == FULLMAG_FDM_OBSERVABLE_M) {
+        src = with_args(h, [&](auto &A) -> const void * { return A.m_all + (size_t)h->ctl_h->parity * 3 * h->n; });
+    } else {
+        if (!h->obs_fresh && !eval_current(h, true)) {
+            hipHostFree(s->host);
+            delete s;
+            return nullptr;
+        }
+        src = h->obs_base[obs == FULLMAG_FDM_OBSERVABLE_H_EX ? 0 : obs == FULLMAG_FDM_OBSERVABLE_H_DEMAG ? 1 : 2];
+    }
+    if (hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess ||
+        hipMemcpyAsync(s->host, src, s->bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipEventRecord(s->ev, h->stream) != hipSuccess) {
+        h->err = "snapshot copy failed";
+        if (s->ev) hipEventDestroy(s->ev);
+        hipHostFree(s->host);
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+fullmag_fdm_preview_snapshot *fullmag_fdm_backend_begin_preview_snapshot(fullmag_fdm_backend *h,
+                                                                         fullmag_fdm_observable obs, uint32_t pnx,
+                                                                         uint32_t pny, uint32_t pnz, uint32_t z0,
+                                                                         uint32_t zs) {
+    if (!h) return nullptr;
+    std::vector<double> v;
+    if (preview_common(h, obs, pnx, pny, pnz, z0, zs, (uint64_t)pnx * pny * pnz * 3, v) != FULLMAG_FDM_OK)
+        return nullptr;
+    auto *s = new (std::nothrow) fullmag_fdm_preview_snapshot();
+    if (!s) return nullptr;
+    const uint64_t cells = (uint64_t)pnx * pny * pnz;
+    s->desc.cell_count = cells;
+    s->desc.component_count = 3;
+    if (h->fp64) {
+        s->desc.scalar_bytes = 8;
+        s->desc.scalar_type = FULLMAG_FDM_SNAPSHOT_SCALAR_F64;
+        s->host.resize(v.size() * 8);
+        std::memcpy(s->host.data(), v.data(), s->host.size());
+    } else {
+        s->desc.scalar_bytes = 4;
+        s->desc.scalar_type = FULLMAG_FDM_SNAPSHOT_SCALAR_F32;
+        s->host.resize(v.size() * 4);
+        float *f = reinterpret_cast<float *>(s->host.data());
+        for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+    }
+    return s;
+}
+
+int fullmag_fdm_field_snapshot_wait(fullmag_fdm_field_snapshot *s, const void **data, uint64_t *len,
+                                    fullmag_fdm_snapshot_desc *desc) {
+    if (!s) return FULLMAG_FDM_ERR_INVALID;
+    if (s->ev && !s->waited) {
+        if (hipEventSynchronize(s->ev) != hipSuccess) return FULLMAG_FDM_ERR_CUDA;
+        s->waited = true;
+    }
+    if (data) *data = s->host;
+    if (len) *len = s->bytes;
+    if (desc) *desc = s->desc;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_preview_snapshot_wait(fullmag_fdm_preview_snapshot *s, const void **data, uint64_t *len,
+                                      fullmag_fdm_snapshot_desc *desc) {
+    if (!s) return FULLMAG_FDM_ERR_INVALID;
+    if (data) *data = s->host.data();
+    if (len) *len = s->host.size();
+    if (desc) *desc = s->desc;
+    return FULLMAG_FDM_OK;
+}
+
+void fullmag_fdm_field_snapshot_destroy(fullmag_fdm_field_snapshot *s) {
+    if (!s) return;
+    if (s->ev) {
+        hipEventSynchronize(s->ev);
+        hipEventDestroy(s->ev);
+    }
+    if (s->host) hipHostFree(s->host);
+    delete s;
+}
+
+void fullmag_fdm_preview_snapshot_destroy(fullmag_fdm_preview_snapshot *s) { delete s; }
+
+static int upload_common(fullmag_fdm_backend *h, const double *d, const float *f, uint64_t len) {
+    if (!h || (!d && !f)) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (len != 3 * h->n) return fail(h, FULLMAG_FDM_ERR_INVALID, "magnetisation length mismatch");
+    hipSetDevice(h->device);
+    if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    const int slot = h->ctl_h->parity;
+    const bool ok = h->fp64 ? upload_state<double>(h, d, f, slot) : upload_state<float>(h, d, f, slot);
+    if (!ok) return FULLMAG_FDM_ERR_CUDA;
+    // the first stage of the next step needs k1 of the new state
+    if (!eval_current(h, true)) return FULLMAG_FDM_ERR_CUDA;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_upload_magnetization_f64(fullmag_fdm_backend *h, const double *m, uint64_t len) {
+    return upload_common(h, m, nullptr, len);
+}
+
+int fullmag_fdm_backend_upload_magnetization_f32(fullmag_fdm_backend *h, const float *m, uint64_t len) {
+    return upload_common(h, nullptr, m, len);
+}
+
+int fullmag_fdm_backend_refresh_observables(fullmag_fdm_backend *h) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    hipSetDevice(h->device);
+    return eval_current(h, true) ? FULLMAG_FDM_OK : FULLMAG_FDM_ERR_CUDA;
+}
+
+int fullmag_fdm_backend_refresh_demag_observable(fullmag_fdm_backend *h) {
+    return fullmag_fdm_backend_refresh_observables(h);
+}
+
+int fullmag_fdm_backend_snapshot_stats(fullmag_fdm_backend *h, fullmag_fdm_step_stats *out) {
+    if (!h || !out) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    hipSetDevice(h->device);
+    if (!eval_current(h, true)) return FULLMAG_FDM_ERR_CUDA;
+    fill_stats(h, out, h->ctl_h->dt_used, 0);
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_get_device_info(fullmag_fdm_backend *h, fullmag_fdm_device_info *info) {
+    if (!h || !info) return FULLMAG_FDM_ERR_INVALID;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, "device query");
+    std::memset(info, 0, sizeof(*info));
+    std::snprintf(info->name, sizeof(info->name), "%s (%s)", prop.name, prop.gcnArchName);
+    info->compute_capability_major = prop.major;
+    info->compute_capability_minor = prop.minor;
+    int v = 0;
+    if (hipDriverGetVersion(&v) == hipSuccess) info->driver_version = v;
+    if (hipRuntimeGetVersion(&v) == hipSuccess) info->runtime_version = v;
+    return FULLMAG_FDM_OK;
+}
+
+const char *fullmag_fdm_backend_last_error(fullmag_fdm_backend *h) {
+    if (!h || h->err.empty()) return nullptr;
+    return h->err.c_str();
+}
+
+void fullmag_fdm_backend_destroy(fullmag_fdm_backend *h) {
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    for (void *p : h->allocs) hipFree(p);
+    if (h->ctl_h) hipHostFree(h->ctl_h);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// ------------------------------------------------------------ extensions --
+
+int fullmag_fdm_ext_version(void) { return FULLMAG_FDM_EXT_VERSION; }
+
+int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out) {
+    if (seed == 0 || (!out && count)) return FULLMAG_FDM_ERR_INVALID;
+    uint64_t st = seed;
+    const double scale = (double)UINT64_MAX;
+    for (uint64_t i = 0; i < count; ++i) {
+        for (;;) {
+            double c[3];
+            for (int k = 0; k < 3; ++k) {
+                st ^= st << 13;
+                st ^= st >> 7;
+                st ^= st << 17;
+                c[k] = ((double)st / scale) * 2.0 - 1.0;
+            }
+            const double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (nrm > 1e-10) {
+                for (int k = 0; k < 3; ++k) out[3 * i + k] = c[k] / nrm;
+                break;
+            }
+        }
+    }
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_advance(fullmag_fdm_backend *h, double t_end, uint64_t max_steps, double dt,
+                                fullmag_fdm_step_stats *out, uint64_t *out_steps) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (!(dt > 0.0) || !std::isfinite(dt) || std::isnan(t_end))
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "dt must be positive and finite");
+    h->err.clear();
+    hipSetDevice(h->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t step0 = h->ctl_h->step;
+    fm::Ctl &c = *h->ctl_h;
+    const double dmin = h->fp64 ? h->ad.dt_min : h->af.dt_min, dmax = h->fp64 ? h->ad.dt_max : h->af.dt_max;
+    c.loop = 1;
+    c.t_end = t_end;
+    c.step_end = c.step + max_steps;
+    c.dt_carry = dt;
+    double d = std::min(dt, t_end - c.t);
+    if (h->adaptive) d = std::min(std::max(d, dmin), dmax);
+    c.dt = d;
+    c.done = (c.t >= t_end || max_steps == 0) ? 1 : 0;
+    if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    ensure_graph(h);
+    h->obs_fresh = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    fm::Ctl *flag_h = nullptr;
+    if (hipHostMalloc((void **)&flag_h, 2 * sizeof(fm::Ctl), hipHostMallocDefault) != hipSuccess)
+        return fail(h, FULLMAG_FDM_ERR_CUDA, "pinned alloc");
+    hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    int rc = FULLMAG_FDM_OK;
+    // double-buffered: chunk i+1 is queued before chunk i's flag is read
+    int cur = 0;
+    bool pending = false;
+    while (!c.done) {
+        for (int a = 0; a < kAttemptChunk; ++a)
+            if (!launch_attempt(h)) {
+                rc = FULLMAG_FDM_ERR_CUDA;
+                break;
+            }
+        if (rc != FULLMAG_FDM_OK) break;
+        hipMemcpyAsync(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream);
+        hipEventRecord(ev[cur], h->stream);
+        if (pending) {
+            if (hipEventSynchronize(ev[1 - cur]) != hipSuccess) {
+                rc = fail(h, FULLMAG_FDM_ERR_CUDA, "event sync");
+                break;
+            }
+            if (flag_h[1 - cur].done) break;
+        }
+        pending = true;
+        cur = 1 - cur;
+        if (h->poll && h->poll(h->poll_user)) {
+            rc = FULLMAG_FDM_ERR_INTERRUPTED;
+            break;
+        }
+    }
+    hipStreamSynchronize(h->stream);
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    hipHostFree(flag_h);
+    if (rc == FULLMAG_FDM_ERR_INTERRUPTED) {
+        // stop the device loop where it is: the state is the last accepted one
+        if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    } else if (rc != FULLMAG_FDM_OK) {
+        return rc;
+    } else if (!pull_ctl(h)) {
+        return FULLMAG_FDM_ERR_CUDA;
+    }
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, std::string("device error: ") + hipGetErrorString(le));
+    c.loop = 0;
+    c.done = 0;
+    if (!push_ctl(h) || !pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    const uint64_t ns =
+        (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+    fill_stats(h, out, c.dt_used, ns);
+    if (out_steps) *out_steps = c.step - step0;
+    if (rc == FULLMAG_FDM_ERR_INTERRUPTED) return fail(h, rc, "interrupted");
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_set_adaptive_ext(fullmag_fdm_backend *h, double rtol, double growth, double shrink) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (rtol < 0 || shrink < 0 || std::isnan(growth)) return fail(h, FULLMAG_FDM_ERR_INVALID, "invalid tolerances");
+    const double g = growth <= 0 ? std::numeric_limits<double>::infinity() : growth;
+    h->af.rtol = h->ad.rtol = rtol;
+    h->af.growth = h->ad.growth = g;
+    h->af.shrink = h->ad.shrink = shrink;
+    // the captured graph holds the old kernel arguments
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+    h->graph_tried = false;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_get_counters(fullmag_fdm_backend *h, uint64_t *acc, uint64_t *rej, double *t) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (acc) *acc = h->ctl_h->step;
+    if (rej) *rej = h->ctl_h->rejected;
+    if (t) *t = h->ctl_h->t;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *out_us, int n_out) {
+    if (!h || !out_us || n_out < 7 || reps < 1) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    hipSetDevice(h->device);
+    for (int i = 0; i < n_out; ++i) out_us[i] = 0.0;
+    if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    // The kernels of stage 2 only write trial buffers (stage state, stage-2
+    // RHS, spectrum, partials); the controller may change dt or parity, so the
+    // control block is restored afterwards.  m and k1 are never written.
+    const fm::Ctl saved = *h->ctl_h;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    int rc = FULLMAG_FDM_OK;
+    for (int kind = 0; kind < 6 && rc == FULLMAG_FDM_OK; ++kind) {
+        if (with_args(h, [&](auto &A) { return fm::enqueue_kind(A, kind, 2, h->stream); }) != hipSuccess) continue;
+        hipEventRecord(e0, h->stream);
+        for (int r = 0; r < reps; ++r) with_args(h, [&](auto &A) { return fm::enqueue_kind(A, kind, 2, h->stream); });
+        hipEventRecord(e1, h->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(h, FULLMAG_FDM_ERR_CUDA, "timing sync");
+            break;
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        out_us[kind] = 1000.0 * ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *h->ctl_h = saved;
+    if (!push_ctl(h) || !pull_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
+    h->obs_fresh = false;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// engine.hpp -- device state and kernel-argument layout of the MI355X FDM
+// engine (shared by the HIP kernels and the C-ABI host layer).
+//
+// Data layout in HBM (SoA, cell index = (z*ny + y)*nx + x, as the reference
+// exchange_fp64.cu:46-50):
+//   m_all  : 2 slots x 3 x ncell  (ping-pong state; slot `parity` is current)
+//   k_all  : 7 slots x 3 x ncell  (slots 0/1: k1 of the current state and
+//            the last-stage RHS of the trial state; slots 2..6: k2..k6)
+//   ybuf   : 3 x ncell            (intermediate stage state)
+//   spec   : 3 x nzs x nkx x ly complex (x-half-spectrum of Ms*m, y-lines
+//            contiguous; nkx = nx+1, ly = ny for nz == 1 else 2ny, nzs = nz)
+//   ntens  : real symmetric Newell spectrum, symmetry-reduced:
+//            thin film: 4 comps (xx,yy,zz,xy) x nkx x (ny+1)
+//            3-D:       6 comps (xx,yy,zz,xy,xz,yz) x nkx x (ny+1) x (nz+1)
+// The control block `Ctl` lives in device memory; every kernel reads dt,
+// parity and the done flag from it, so a whole adaptive step (including
+// rejections) runs without host round trips.
+#pragma once
+#include <cstdint>
+
+namespace fm {
+
+constexpr int kThreads = 256;       // work-group size of every kernel
+constexpr int kMaxPasses = 24;      // FFT passes per length
+constexpr int kMaxRadix = 64;       // largest prime factor of a padded length
+constexpr int kStatSlots = 8;       // per-block partial reductions
+constexpr int kMaxStages = 8;
+
+// Per-block partial layout (doubles).
+enum StatSlot {
+    ST_ERR = 0,       // max error norm (max)
+    ST_EEX = 1,       // exchange energy (sum)
+    ST_EDEMAG = 2,    // demag energy (sum)
+    ST_EEXT = 3,      // Zeeman energy (sum)
+    ST_EANI = 4,      // anisotropy energy (sum)
+    ST_HEFF2 = 5,     // max |H_eff|^2 (max)
+    ST_HD2 = 6,       // max |H_d|^2 (max)
+    ST_RHS2 = 7,      // max |dm/dt|^2 (max)
+};
+
+struct FftDesc {
+    int n;
+    int npass;
+    int radix[kMaxPasses];
+};
+
+// Explicit Runge-Kutta tableau as stage rows.  Stage s (2..nstage) evaluates
+// y_s = normalise(m0 + (dt*scale_s) * sum_q a_sq k_{j_sq}); stage `nstage`
+// is the new state.  k_j for j == 1 is the RHS of m0, for j == nstage the RHS
+// of the trial state.
+struct StageRow {
+    int nk;
+    int kidx[kMaxStages];
+    double a[kMaxStages];
+    double scale;
+};
+
+struct Method {
+    int nstage;         // index of the final stage (DP45: 7, RK4: 5, Heun: 3)
+    int adaptive;       // 1 = embedded error estimate + step control
+    int nerr;
+    int eidx[kMaxStages];
+    double e[kMaxStages];
+    StageRow row[kMaxStages + 1];  // row[s] for s = 2..nstage
+};
+
+// Device-resident control block.
+struct Ctl {
+    double t;          // simulated time of the current state
+    double dt;         // step size of the attempt in flight
+    double dt_carry;   // runner dt carried between steps (loop mode)
+    double t_end;      // loop mode: stop time
+    double dt_used;    // last accepted dt
+    double dt_next;    // controller proposal after the last accepted step
+    double err;        // error norm of the last attempt
+    double stats[kStatSlots];  // reduced statistics of the last evaluation
+    unsigned long long step;       // accepted steps since create
+    unsigned long long step_end;   // loop mode: stop after this many steps
+    unsigned long long rejected;   // rejected attempts since create
+    unsigned long long attempts;   // attempts since create
+    int parity;        // slot of the current state
+    int accepted;      // last attempt accepted
+    int done;          // loop mode finished: kernels return at once
+    int loop;          // 1 = loop mode, 0 = single step driven by the host
+};
+
+template <class T>
+struct Args {
+    // grid
+    int nx, ny, nz;
+    long long ncell;
+    long long nrows;        // ny*nz x-lines
+    int nkx, ly, nzs;       // spectrum extents
+    int px, py, pz;         // padded FFT lengths (pz = 1 for thin film)
+    int rows_per_wg;        // x-lines per work-group in the x passes
+    int ty;                 // y-lines per work-group in the z pass
+    int nseq_x;             // complex sequences per x work-group
+    double inv_p;           // 1 / (px*py*pz)
+    // buffers
+    T *m_all;
+    T *k_all;
+    T *ybuf;
+    void *spec;             // complex<T>
+    const T *ntens;
+    const void *twx, *twy, *twz;   // complex<T> forward twiddles
+    const uint8_t *mask;    // NULL = all active
+    const uint32_t *region; // NULL = uniform exchange
+    const T *lut;           // 256 x 256 A_ij when region != NULL
+    Ctl *ctl;
+    double *partials;       // [nblocks][kStatSlots]
+    // observables (written only when non-NULL)
+    T *obs_hex, *obs_hd, *obs_heff;
+    // physics
+    T ms;
+    T ex_pre;               // 2A/(mu0 Ms)
+    T lut_pre;              // 2/(mu0 Ms)
+    T idx2, idy2, idz2;     // 1/dx^2 ...
+    T hext[3];
+    int has_ext;
+    int has_uni;
+    T uni_c1, uni_c2;       // 2Ku1/(mu0 Ms), 4Ku2/(mu0 Ms)
+    T axis[3];
+    T gamma_bar, alpha;
+    int precession;
+    int demag;
+    double e_pre;           // mu0*Ms*V (energy prefactor)
+    // step control
+    double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
+    int method;             // index into the constant method table
+    FftDesc fx, fy, fz;
+};
+
+// Method indices in the constant table.
+enum MethodId { M_HEUN = 0, M_DP45 = 1, M_RK4 = 2, M_COUNT = 3 };
+
+}  // namespace fm

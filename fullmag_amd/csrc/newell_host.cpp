@@ -1,0 +1,297 @@
+// newell_host.cpp -- FP64 Newell demagnetisation tensor and its
+// symmetry-reduced real spectrum, computed once at create().
+//
+// Tensor: Newell, Williams & Dunlop (1993) f/g potentials, 27-point second
+// difference with Neumaier-compensated summation, point-dipole far field
+// beyond 40 cells, octant parity (xx,yy,zz even; xy odd in x,y; xz odd in
+// x,z; yz odd in y,z).  This is the algorithm of the reference's
+// crates/fullmag-fdm-demag/src/newell.rs:21-517; the formulas, the 40-cell
+// threshold and the stencil weights must be identical for the tensor to be
+// the same operator.
+//
+// Spectrum: the reference transforms the full zero-padded real tensor with
+// complex FFTs (crates/fullmag-engine/src/fdm_fft.rs:334-432).  Because each
+// component has definite parity along every axis its spectrum is real and
+// has the same parity in k, so only kx in [0,nx], ky in [0,ny], kz in [0,nz]
+// is kept: thin film 4 x (nx+1)(ny+1) values, 3-D 6 x (nx+1)(ny+1)(nz+1).
+// Transform: per axis, a complex FFT of each (parity-extended) line of the
+// first-octant data, keeping the lower half of the output.
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+#include <algorithm>
+#include <functional>
+
+namespace fm {
+namespace {
+
+using cd = std::complex<double>;
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kFar = 40;
+
+double pot_f(double x, double y, double z) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z, r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    const double r = std::sqrt(r2);
+    double v = (2.0 * x2 - y2 - z2) * r / 6.0;
+    if (x2 + z2 > 1e-300) {
+        const double u = 2.0 * y * (y + r) / (x2 + z2);
+        if (u > -1.0) v += y * (z2 - x2) / 4.0 * std::log1p(u);
+    }
+    if (x2 + y2 > 1e-300) {
+        const double u = 2.0 * z * (z + r) / (x2 + y2);
+        if (u > -1.0) v += z * (y2 - x2) / 4.0 * std::log1p(u);
+    }
+    if (std::fabs(x) > 1e-300) v -= x * y * z * std::atan(y * z / (x * r));
+    return v;
+}
+
+double pot_g(double x, double y, double z) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z, r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    const double r = std::sqrt(r2);
+    double v = -x * y * r / 3.0;
+    if (x2 + y2 > 1e-300) {
+        const double u = 2.0 * z * (z + r) / (x2 + y2);
+        if (u > -1.0) v += x * y * z * std::log1p(u) / 2.0;
+    }
+    if (y2 + z2 > 1e-300) {
+        const double u = 2.0 * x * (x + r) / (y2 + z2);
+        if (u > -1.0) v += y * (3.0 * z2 - y2) * std::log1p(u) / 12.0;
+    }
+    if (x2 + z2 > 1e-300) {
+        const double u = 2.0 * y * (y + r) / (x2 + z2);
+        if (u > -1.0) v += x * (3.0 * z2 - x2) * std::log1p(u) / 12.0;
+    }
+    if (std::fabs(z) > 1e-300) v -= z2 * z / 6.0 * std::atan(x * y / (z * r));
+    if (std::fabs(y) > 1e-300) v -= y2 * z / 2.0 * std::atan(x * z / (y * r));
+    if (std::fabs(x) > 1e-300) v -= x2 * z / 2.0 * std::atan(y * z / (x * r));
+    return v;
+}
+
+void parallel_for(long n, const std::function<void(long, long)> &body) {
+    unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    if (n < 64 || nt == 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const long chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const long a = t * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back(body, a, b);
+    }
+    for (auto &t : th) t.join();
+}
+
+// Compensated 27-point sum: weights 8 (centre), -4 (faces), 2 (edges),
+// -1 (corners) of the potential sampled at (i+a, j+b, k+c), a,b,c in {-1,0,1}.
+double second_difference(const std::vector<double> &pv, int sx, int sy, int i, int j, int k) {
+    double s = 0.0, comp = 0.0;
+    for (int c = -1; c <= 1; ++c)
+        for (int b = -1; b <= 1; ++b)
+            for (int a = -1; a <= 1; ++a) {
+                const int nzc = (a != 0) + (b != 0) + (c != 0);
+                const double w = nzc == 0 ? 8.0 : nzc == 1 ? -4.0 : nzc == 2 ? 2.0 : -1.0;
+                const double t = w * pv[(size_t)(k + c + 1) * sy * sx + (size_t)(j + b + 1) * sx + (i + a + 1)];
+                const double u = s + t;
+                comp += std::fabs(s) >= std::fabs(t) ? (s - u) + t : (t - u) + s;
+                s = u;
+            }
+    return s + comp;
+}
+
+// Mixed-radix Stockham FFT (forward, unnormalised) on one complex line.
+struct LineFft {
+    int n;
+    std::vector<int> rad;
+    std::vector<cd> w;
+    explicit LineFft(int n_) : n(n_) {
+        int m = n;
+        while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+        while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+        for (int p = 3; m > 1; p += 2)
+            while (m % p == 0) { rad.push_back(p); m /= p; }
+        w.resize(n);
+        for (int k = 0; k < n; ++k) w[k] = std::polar(1.0, -2.0 * kPi * (double)k / (double)n);
+    }
+    void run(cd *a, cd *b) const {  // result in a; b is scratch of n
+        int ns = 1;
+        std::vector<cd> v, o;
+        for (int R : rad) {
+            v.assign(R, 0.0);
+            o.assign(R, 0.0);
+            const int nr = n / R, ts = n / (ns * R);
+            for (int j = 0; j < nr; ++j) {
+                const int k = j % ns;
+                for (int r = 0; r < R; ++r) v[r] = a[j + r * nr] * (r && ns > 1 ? w[(size_t)k * r * ts] : cd(1.0));
+                for (int q = 0; q < R; ++q) {
+                    cd acc = 0.0;
+                    for (int r = 0; r < R; ++r) acc += v[r] * w[(size_t)((r * q) % R) * (n / R)];
+                    o[q] = acc;
+                }
+                const int base = (j / ns) * ns * R + k;
+                for (int r = 0; r < R; ++r) b[base + r * ns] = o[r];
+            }
+            std::copy(b, b + n, a);
+            ns *= R;
+        }
+    }
+};
+
+// Transform along one axis: `in` has extents (e0 = the axis, e1, e2) with
+// the axis fastest when stride==1, or generic strides; keeps outputs 0..n.
+// data layout: index(a, b, c) = a*sa + b*sb + c*sc over extents (n, nb, nc).
+void axis_transform(const std::vector<cd> &in, std::vector<cd> &out, int n, int nb, int nc, size_t sa, size_t sb,
+                    size_t sc, size_t oa, size_t ob, size_t oc, int sign) {
+    const int P = 2 * n;
+    LineFft f(P);
+    parallel_for((long)nb * nc, [&](long lo, long hi) {
+        std::vector<cd> a(P), b(P);
+        for (long l = lo; l < hi; ++l) {
+            const long bi = l % nb, ci = l / nb;
+            std::fill(a.begin(), a.end(), cd(0.0));
+            for (int i = 0; i < n; ++i) {
+                const cd v = in[(size_t)i * sa + (size_t)bi * sb + (size_t)ci * sc];
+                a[i] = v;
+                if (i > 0) a[P - i] = (double)sign * v;
+            }
+            f.run(a.data(), b.data());
+            for (int k = 0; k <= n; ++k) out[(size_t)k * oa + (size_t)bi * ob + (size_t)ci * oc] = a[k];
+        }
+    });
+}
+
+}  // namespace
+
+// First-octant real-space tensor, 6 arrays (xx,yy,zz,xy,xz,yz) of nx*ny*nz,
+// index (k*ny + j)*nx + i = displacement (i dx, j dy, k dz).
+std::vector<double> newell_octant(int nx, int ny, int nz, double dx, double dy, double dz) {
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<double> out(6 * n);
+    const int fx = std::min(nx, kFar) + 2, fy = std::min(ny, kFar) + 2, fz = std::min(nz, kFar) + 2;
+    const size_t fl = (size_t)fx * fy * fz;
+    std::vector<double> pv[6];
+    for (auto &v : pv) v.resize(fl);
+    parallel_for(fz, [&](long lo, long hi) {
+        for (long k = lo; k < hi; ++k)
+            for (int j = 0; j < fy; ++j)
+                for (int i = 0; i < fx; ++i) {
+                    const double x = (i - 1) * dx, y = (j - 1) * dy, z = (k - 1) * dz;
+                    const size_t id = (size_t)k * fy * fx + (size_t)j * fx + i;
+                    pv[0][id] = pot_f(x, y, z);
+                    pv[1][id] = pot_f(y, x, z);
+                    pv[2][id] = pot_f(z, y, x);
+                    pv[3][id] = pot_g(x, y, z);
+                    pv[4][id] = pot_g(x, z, y);
+                    pv[5][id] = pot_g(y, z, x);
+                }
+    });
+    const double vol = dx * dy * dz;
+    const double den = 4.0 * kPi * vol;
+    parallel_for(nz, [&](long lo, long hi) {
+        for (long k = lo; k < hi; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const size_t o = ((size_t)k * ny + j) * nx + i;
+                    const long d2 = (long)i * i + (long)j * j + k * k;
+                    if (i >= kFar || j >= kFar || k >= kFar || d2 >= (long)kFar * kFar) {
+                        const double x = i * dx, y = j * dy, z = k * dz;
+                        const double r2 = x * x + y * y + z * z, r = std::sqrt(r2);
+                        const double r3 = r2 * r, r5 = r2 * r2 * r;
+                        auto dia = [&](double a) { return (1.0 / r3 - 3.0 * a * a / (r3 * r2)) / (4.0 * kPi) * vol; };
+                        auto off = [&](double a, double b) { return -3.0 * a * b / (4.0 * kPi * r5) * vol; };
+                        out[o] = dia(x);
+                        out[n + o] = dia(y);
+                        out[2 * n + o] = dia(z);
+                        out[3 * n + o] = off(x, y);
+                        out[4 * n + o] = off(x, z);
+                        out[5 * n + o] = off(y, z);
+                    } else {
+                        for (int c = 0; c < 6; ++c) out[c * n + o] = second_difference(pv[c], fx, fy, i, j, (int)k) / den;
+                    }
+                }
+    });
+    return out;
+}
+
+// Symmetry-reduced real spectrum in the engine's layout:
+//   thin film (nz == 1): [c][kx][ky], c in xx,yy,zz,xy
+//   3-D:                [c][kx][ky][kz], c in xx,yy,zz,xy,xz,yz
+std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz) {
+    const std::vector<double> oct = newell_octant(nx, ny, nz, dx, dy, dz);
+    const size_t n = (size_t)nx * ny * nz;
+    const bool film = nz == 1;
+    const int ncomp = film ? 4 : 6;
+    const int nkx = nx + 1, nky = ny + 1, nkz = film ? 1 : nz + 1;
+    const size_t red = (size_t)nkx * nky * nkz;
+    std::vector<double> out((size_t)ncomp * red);
+    // parity (x,y,z) sign of the odd axes for each component
+    const int par[6][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}};
+    for (int c = 0; c < ncomp; ++c) {
+        // stage 0: octant (z, y, x) complex
+        std::vector<cd> a(n);
+        for (size_t i = 0; i < n; ++i) a[i] = oct[c * n + i];
+        // x: (x, y, z) with strides (1, nx, nx*ny) -> (kx, y, z) strides (1, nkx, nkx*ny)
+        std::vector<cd> b((size_t)nkx * ny * nz);
+        axis_transform(a, b, nx, ny, nz, 1, nx, (size_t)nx * ny, 1, nkx, (size_t)nkx * ny, par[c][0]);
+        // y: (y, kx, z) in b strides (nkx, 1, nkx*ny) -> (ky, kx, z) strides (nkx, 1, nkx*nky)
+        std::vector<cd> d((size_t)nkx * nky * nz);
+        axis_transform(b, d, ny, nkx, nz, nkx, 1, (size_t)nkx * ny, nkx, 1, (size_t)nkx * nky, par[c][1]);
+        if (film) {
+            for (int kx = 0; kx < nkx; ++kx)
+                for (int ky = 0; ky < nky; ++ky) out[c * red + (size_t)kx * nky + ky] = d[(size_t)ky * nkx + kx].real();
+        } else {
+            // z: (z, kx, ky) in d strides (nkx*nky, 1, nkx) -> (kz, kx, ky)
+            std::vector<cd> e((size_t)nkx * nky * nkz);
+            axis_transform(d, e, nz, nkx, nky, (size_t)nkx * nky, 1, nkx, (size_t)nkx * nky, 1, nkx, par[c][2]);
+            for (int kx = 0; kx < nkx; ++kx)
+                for (int ky = 0; ky < nky; ++ky)
+                    for (int kz = 0; kz < nkz; ++kz)
+                        out[c * red + ((size_t)kx * nky + ky) * nkz + kz] =
+                            e[((size_t)kz * nky + ky) * nkx + kx].real();
+        }
+    }
+    return out;
+}
+
+// Reduced spectrum from reference-format spectra (complex interleaved over the
+// padded grid, thin-film 2-D or 3-D form).  Real parts are sampled at the
+// kept frequencies.  Returns false (with *err) on a length mismatch.
+bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz,
+                          std::vector<double> &out, std::string *err) {
+    const uint64_t px = 2ull * nx, py = 2ull * ny, pz3 = 2ull * nz;
+    const bool film = nz == 1;
+    uint64_t pz;
+    if (film && len == 2 * px * py) pz = 1;
+    else if (len == 2 * px * py * pz3) pz = pz3;
+    else {
+        *err = "demag_kernel_spectrum_len mismatch: expected " + std::to_string(2 * px * py * pz3) +
+               (film ? " or " + std::to_string(2 * px * py) : std::string()) + ", got " + std::to_string(len);
+        return false;
+    }
+    const int ncomp = film ? 4 : 6;
+    for (int c = 0; c < ncomp; ++c)
+        if (!spec[c]) {
+            *err = "demag kernel spectra must be given for all components or none";
+            return false;
+        }
+    const int nkx = nx + 1, nky = ny + 1, nkz = film ? 1 : nz + 1;
+    const size_t red = (size_t)nkx * nky * nkz;
+    out.assign((size_t)ncomp * red, 0.0);
+    for (int c = 0; c < ncomp; ++c)
+        for (int kx = 0; kx < nkx; ++kx)
+            for (int ky = 0; ky < nky; ++ky)
+                for (int kz = 0; kz < nkz; ++kz) {
+                    const uint64_t q = ((uint64_t)kz * py + ky) * px + kx;
+                    out[c * red + ((size_t)kx * nky + ky) * nkz + kz] = spec[c][2 * q];
+                }
+    return true;
+}
+
+}  // namespace fm

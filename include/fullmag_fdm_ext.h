@@ -1,0 +1,75 @@
+/*
+ * fullmag_fdm_ext.h -- entry points of the MI355X FDM engine that have no
+ * counterpart in the reference ABI (native/include/fullmag_fdm.h).  They are
+ * additive: a caller that only knows fullmag_fdm.h never needs them.
+ */
+#ifndef FULLMAG_FDM_EXT_H
+#define FULLMAG_FDM_EXT_H
+
+#include "fullmag_fdm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI revision of this library (bumped when an entry point is added). */
+#define FULLMAG_FDM_EXT_VERSION 1
+int fullmag_fdm_ext_version(void);
+
+/*
+ * Device-resident runner loop: repeat accepted steps until time t_end or
+ * max_steps accepted steps, whichever comes first, with the CPU runner's
+ * semantics (crates/fullmag-runner/src/cpu_reference.rs:622-639):
+ * dt_step = min(dt, t_end - t); for DP45 the next dt is the controller's
+ * suggestion and every trial step is clamped to [dt_min, dt_max].  Step
+ * acceptance and dt control run on the device; the host only launches and
+ * checks a completion flag.  out_stats receives the last accepted step,
+ * *out_steps the number of accepted steps taken (either may be NULL).
+ */
+int fullmag_fdm_backend_advance(fullmag_fdm_backend *handle, double t_end, uint64_t max_steps,
+                                double dt_seconds, fullmag_fdm_step_stats *out_stats,
+                                uint64_t *out_steps);
+
+/*
+ * Mixed-tolerance step control of the CPU engine's AdaptiveStepConfig
+ * (crates/fullmag-engine/src/fdm_types.rs:182-209): per-cell error divided
+ * by (atol + rtol*|m0|) and compared with 1 when rtol > 0; step growth and
+ * shrink factors limited to growth_limit (<= 0: unlimited) and shrink_limit.
+ */
+int fullmag_fdm_backend_set_adaptive_ext(fullmag_fdm_backend *handle, double rtol,
+                                         double growth_limit, double shrink_limit);
+
+/* Counters since create: accepted steps, rejected trial steps, time. */
+int fullmag_fdm_backend_get_counters(fullmag_fdm_backend *handle, uint64_t *accepted,
+                                     uint64_t *rejected, double *time_seconds);
+
+/*
+ * Kernel timing with HIP events on the engine's stream.  Runs `reps`
+ * launches of each kernel of one RHS evaluation (stage 2 of the configured
+ * method) and of the step controller, and writes the mean duration in
+ * microseconds of kernel kind i to out_us[i]:
+ *   0 x-forward (stage combine + R2C; stage combine alone without demag),
+ *   1 y pass (thin film fused / 3-D forward), 2 z pass (3-D),
+ *   3 y inverse (3-D), 4 x-inverse + local pass, 5 controller;
+ *   out_us[6] is reserved (0).
+ * Kinds that do not exist for the grid are reported as 0.  The state is
+ * left consistent (a following step recomputes every stage).  n_out must be
+ * at least 7.
+ */
+int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *handle, int reps, double *out_us,
+                                     int n_out);
+
+/*
+ * Seeded random unit vectors of the planner's RandomSeeded initial state
+ * (crates/fullmag-plan/src/util.rs:118-147, xorshift64 13/7/17, components
+ * uniform in [-1,1], rejection of near-zero norms, then normalised).  Host
+ * code, no device needed.  out: 3*count doubles AoS.  Seed 0 is rejected
+ * (the generator would never leave its rejection loop).
+ */
+int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FULLMAG_FDM_EXT_H */

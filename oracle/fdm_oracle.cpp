@@ -1,0 +1,896 @@
+// fdm_oracle.cpp -- FP64 CPU restatement of fullmag's `fdm_cpu_reference` path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the MI355X engine in
+// fullmag_amd/: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.  The product library never links it.
+//
+// It follows the reference's CPU engine (Rust, not buildable in this image):
+//   Newell tensor ........ crates/fullmag-fdm-demag/src/newell.rs:21-517
+//   FFT workspace ........ crates/fullmag-engine/src/fdm_fft.rs:64-131,445-602
+//   exchange ............. crates/fullmag-engine/src/fdm_fields.rs:445-526
+//   demag ................ crates/fullmag-engine/src/fdm_fields.rs:528-609
+//   local terms .......... crates/fullmag-engine/src/fdm_fields.rs:910-1047
+//   effective field ...... crates/fullmag-engine/src/fdm_fields.rs:1153-1183
+//   observables/energies . crates/fullmag-engine/src/fdm_fields.rs:1683-1813,2168-2273,349-355
+//   LLG RHS .............. crates/fullmag-engine/src/fdm_fields.rs:2151-2162
+//   Heun ................. crates/fullmag-engine/src/fdm_integrators.rs:84-150
+//   RK4 .................. crates/fullmag-engine/src/fdm_integrators.rs:155-264
+//   RK45 (Dormand-Prince)  crates/fullmag-engine/src/fdm_integrators.rs:429-680
+//   error norm ........... crates/fullmag-engine/src/fdm_integrators.rs:828-870
+//   neighbour clamp ...... crates/fullmag-engine/src/fdm_types.rs:126-132
+//   runner time loop ..... crates/fullmag-runner/src/cpu_reference.rs:448-455,622-639
+//   random unit vectors .. crates/fullmag-plan/src/util.rs:118-147
+//   hole mask ............ crates/fullmag-plan/src/geometry.rs:159-231
+//
+// The FFT is its own mixed-radix Stockham (the reference uses rustfft 6.4.1,
+// whose exact rounding is not reproduced); it is pinned against numpy.fft and
+// against a brute-force real-space convolution in tests/.
+//
+// Build: see oracle/Makefile (g++ -O3 -fopenmp -shared).
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+namespace {
+
+using cplx = std::complex<double>;
+constexpr double PI = 3.14159265358979323846;
+constexpr double MU0 = 4.0 * PI * 1e-7;
+constexpr int ASYMPTOTIC_DISTANCE = 40;
+
+// ---------------------------------------------------------------- Newell ---
+// f and g base functions, newell.rs:21-122 (log1p form).
+double newell_f(double x, double y, double z) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    const double r = std::sqrt(r2);
+    double res = (2.0 * x2 - y2 - z2) * r / 6.0;
+    const double rxz2 = x2 + z2;
+    if (rxz2 > 1e-300) {
+        const double arg = 2.0 * y * (y + r) / rxz2;
+        if (arg > -1.0) res += y * (z2 - x2) / 4.0 * std::log1p(arg);
+    }
+    const double rxy2 = x2 + y2;
+    if (rxy2 > 1e-300) {
+        const double arg = 2.0 * z * (z + r) / rxy2;
+        if (arg > -1.0) res += z * (y2 - x2) / 4.0 * std::log1p(arg);
+    }
+    if (std::fabs(x) > 1e-300) res -= x * y * z * std::atan(y * z / (x * r));
+    return res;
+}
+
+double newell_g(double x, double y, double z) {
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    const double r = std::sqrt(r2);
+    double res = -x * y * r / 3.0;
+    const double rxy2 = x2 + y2;
+    if (rxy2 > 1e-300) {
+        const double arg = 2.0 * z * (z + r) / rxy2;
+        if (arg > -1.0) res += x * y * z * std::log1p(arg) / 2.0;
+    }
+    const double ryz2 = y2 + z2;
+    if (ryz2 > 1e-300) {
+        const double arg = 2.0 * x * (x + r) / ryz2;
+        if (arg > -1.0) res += y * (3.0 * z2 - y2) * std::log1p(arg) / 12.0;
+    }
+    const double rxz2 = x2 + z2;
+    if (rxz2 > 1e-300) {
+        const double arg = 2.0 * y * (y + r) / rxz2;
+        if (arg > -1.0) res += x * (3.0 * z2 - x2) * std::log1p(arg) / 12.0;
+    }
+    if (std::fabs(z) > 1e-300) res -= z2 * z / 6.0 * std::atan(x * y / (z * r));
+    if (std::fabs(y) > 1e-300) res -= y2 * z / 2.0 * std::atan(x * z / (y * r));
+    if (std::fabs(x) > 1e-300) res -= x2 * z / 2.0 * std::atan(y * z / (x * r));
+    return res;
+}
+
+// Kahan-Neumaier compensated sum, newell.rs:133-147.
+double neumaier(const double *t, int n) {
+    double s = 0.0, c = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double v = t[i];
+        const double u = s + v;
+        if (std::fabs(s) >= std::fabs(v)) c += (s - u) + v;
+        else c += (v - u) + s;
+        s = u;
+    }
+    return s + c;
+}
+
+// 27-point second-difference stencil of f (or g) at displacement (i,j,k),
+// newell.rs:150-207.  fv holds values at signed indices -1..dist, offset +1.
+double stencil27(int i, int j, int k, const std::vector<double> &fv, int sx, int sy, double hvol) {
+    auto F = [&](int a, int b, int c) { return fv[(size_t)(c + 1) * sy * sx + (size_t)(b + 1) * sx + (a + 1)]; };
+    const double t[27] = {
+        8.0 * F(i, j, k),
+        -4.0 * F(i + 1, j, k), -4.0 * F(i - 1, j, k),
+        -4.0 * F(i, j + 1, k), -4.0 * F(i, j - 1, k),
+        -4.0 * F(i, j, k + 1), -4.0 * F(i, j, k - 1),
+        2.0 * F(i - 1, j - 1, k), 2.0 * F(i - 1, j + 1, k), 2.0 * F(i + 1, j - 1, k), 2.0 * F(i + 1, j + 1, k),
+        2.0 * F(i - 1, j, k - 1), 2.0 * F(i - 1, j, k + 1), 2.0 * F(i + 1, j, k - 1), 2.0 * F(i + 1, j, k + 1),
+        2.0 * F(i, j - 1, k - 1), 2.0 * F(i, j - 1, k + 1), 2.0 * F(i, j + 1, k - 1), 2.0 * F(i, j + 1, k + 1),
+        -F(i - 1, j - 1, k - 1), -F(i - 1, j - 1, k + 1), -F(i - 1, j + 1, k - 1), -F(i + 1, j - 1, k - 1),
+        -F(i - 1, j + 1, k + 1), -F(i + 1, j - 1, k + 1), -F(i + 1, j + 1, k - 1), -F(i + 1, j + 1, k + 1),
+    };
+    return neumaier(t, 27) / (4.0 * PI * hvol);
+}
+
+// Point-dipole far field, newell.rs:212-229.
+double asym_diag(double x, double y, double z, double vol) {
+    const double r2 = x * x + y * y + z * z;
+    const double r = std::sqrt(r2);
+    const double r3 = r2 * r;
+    return (1.0 / r3 - 3.0 * x * x / (r3 * r2)) / (4.0 * PI) * vol;
+}
+double asym_off(double x, double y, double z, double vol) {
+    const double r2 = x * x + y * y + z * z;
+    const double r = std::sqrt(r2);
+    const double r5 = r2 * r2 * r;
+    return -3.0 * x * y / (4.0 * PI * r5) * vol;
+}
+
+// Real-space Newell tensor on the (2nx,2ny,2nz) padded grid with octant
+// parity, newell.rs:254-517.  out: 6 arrays [xx,yy,zz,xy,xz,yz] of P each.
+void newell_kernels(int nx, int ny, int nz, double dx, double dy, double dz, double *out) {
+    const int px = 2 * nx, py = 2 * ny, pz = 2 * nz;
+    const size_t P = (size_t)px * py * pz;
+    const int fsx = std::min(nx, ASYMPTOTIC_DISTANCE) + 2;
+    const int fsy = std::min(ny, ASYMPTOTIC_DISTANCE) + 2;
+    const int fsz = std::min(nz, ASYMPTOTIC_DISTANCE) + 2;
+    const size_t flen = (size_t)fsx * fsy * fsz;
+    std::vector<double> fxx(flen), fyy(flen), fzz(flen), gxy(flen), gxz(flen), gyz(flen);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < fsz; ++k)
+        for (int j = 0; j < fsy; ++j)
+            for (int i = 0; i < fsx; ++i) {
+                const double x = (i - 1) * dx, y = (j - 1) * dy, z = (k - 1) * dz;
+                const size_t id = (size_t)k * fsy * fsx + (size_t)j * fsx + i;
+                fxx[id] = newell_f(x, y, z);
+                fyy[id] = newell_f(y, x, z);
+                fzz[id] = newell_f(z, y, x);
+                gxy[id] = newell_g(x, y, z);
+                gxz[id] = newell_g(x, z, y);
+                gyz[id] = newell_g(y, z, x);
+            }
+    std::memset(out, 0, sizeof(double) * 6 * P);
+    double *nxx = out, *nyy = out + P, *nzz = out + 2 * P, *nxy = out + 3 * P, *nxz = out + 4 * P, *nyz = out + 5 * P;
+    const double hvol = dx * dy * dz;
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double vxx, vyy, vzz, vxy, vxz, vyz;
+                const long d2 = (long)i * i + (long)j * j + (long)k * k;
+                const bool asym = i >= ASYMPTOTIC_DISTANCE || j >= ASYMPTOTIC_DISTANCE || k >= ASYMPTOTIC_DISTANCE ||
+                                  d2 >= (long)ASYMPTOTIC_DISTANCE * ASYMPTOTIC_DISTANCE;
+                if (asym) {
+                    const double x = i * dx, y = j * dy, z = k * dz;
+                    vxx = asym_diag(x, y, z, hvol);
+                    vyy = asym_diag(y, x, z, hvol);
+                    vzz = asym_diag(z, y, x, hvol);
+                    vxy = asym_off(x, y, z, hvol);
+                    vxz = asym_off(x, z, y, hvol);
+                    vyz = asym_off(y, z, x, hvol);
+                } else {
+                    vxx = stencil27(i, j, k, fxx, fsx, fsy, hvol);
+                    vyy = stencil27(i, j, k, fyy, fsx, fsy, hvol);
+                    vzz = stencil27(i, j, k, fzz, fsx, fsy, hvol);
+                    vxy = stencil27(i, j, k, gxy, fsx, fsy, hvol);
+                    vxz = stencil27(i, j, k, gxz, fsx, fsy, hvol);
+                    vyz = stencil27(i, j, k, gyz, fsx, fsy, hvol);
+                }
+                const int xs_n = i == 0 ? 1 : 2, ys_n = j == 0 ? 1 : 2, zs_n = k == 0 ? 1 : 2;
+                const int xi[2] = {i, px - i}, yi[2] = {j, py - j}, zi[2] = {k, pz - k};
+                const double sg[2] = {1.0, -1.0};
+                for (int a = 0; a < xs_n; ++a)
+                    for (int b = 0; b < ys_n; ++b)
+                        for (int c = 0; c < zs_n; ++c) {
+                            const size_t p = (size_t)zi[c] * py * px + (size_t)yi[b] * px + xi[a];
+                            nxx[p] = vxx;
+                            nyy[p] = vyy;
+                            nzz[p] = vzz;
+                            nxy[p] = vxy * sg[a] * sg[b];
+                            nxz[p] = vxz * sg[a] * sg[c];
+                            nyz[p] = vyz * sg[b] * sg[c];
+                        }
+            }
+}
+
+// ------------------------------------------------------------------- FFT ---
+// Mixed-radix Stockham autosort FFT, unnormalised, sign -1 forward, +1 inverse.
+struct FftPlan {
+    int n = 0;
+    std::vector<int> radix;
+    std::vector<cplx> wf, wi;  // exp(-2 pi i k / n), exp(+2 pi i k / n)
+    explicit FftPlan(int n_ = 1) : n(n_) {
+        int m = n;
+        while (m % 4 == 0) { radix.push_back(4); m /= 4; }
+        while (m % 2 == 0) { radix.push_back(2); m /= 2; }
+        for (int p = 3; m > 1; p += 2)
+            while (m % p == 0) { radix.push_back(p); m /= p; }
+        wf.resize(n);
+        wi.resize(n);
+        for (int k = 0; k < n; ++k) {
+            const double a = 2.0 * PI * (double)k / (double)n;
+            wf[k] = cplx(std::cos(a), -std::sin(a));
+            wi[k] = std::conj(wf[k]);
+        }
+    }
+    // data: n contiguous elements with element stride `stride`; scratch: 2n.
+    void run(cplx *data, size_t stride, bool inverse, cplx *scratch) const {
+        if (n == 1) return;
+        cplx *a = scratch, *b = scratch + n;
+        for (int i = 0; i < n; ++i) a[i] = data[(size_t)i * stride];
+        const std::vector<cplx> &w = inverse ? wi : wf;
+        int ns = 1;
+        cplx v[64], o[64];
+        std::vector<cplx> vbig, obig;
+        for (int R : radix) {
+            cplx *vv = v, *oo = o;
+            if (R > 64) { vbig.resize(R); obig.resize(R); vv = vbig.data(); oo = obig.data(); }
+            const int nr = n / R;
+            const int tstep = n / (ns * R);
+            for (int j = 0; j < nr; ++j) {
+                const int k = j % ns;
+                for (int r = 0; r < R; ++r) vv[r] = a[j + r * nr];
+                if (ns > 1)
+                    for (int r = 1; r < R; ++r) vv[r] *= w[(size_t)k * r * tstep % n];
+                if (R == 2) {
+                    oo[0] = vv[0] + vv[1];
+                    oo[1] = vv[0] - vv[1];
+                } else if (R == 4) {
+                    const cplx s0 = vv[0] + vv[2], d0 = vv[0] - vv[2];
+                    const cplx s1 = vv[1] + vv[3], d1 = vv[1] - vv[3];
+                    // forward: -i * d1 ; inverse: +i * d1
+                    const cplx jd1 = inverse ? cplx(-d1.imag(), d1.real()) : cplx(d1.imag(), -d1.real());
+                    oo[0] = s0 + s1;
+                    oo[2] = s0 - s1;
+                    oo[1] = d0 + jd1;
+                    oo[3] = d0 - jd1;
+                } else {
+                    const int step = n / R;
+                    for (int s = 0; s < R; ++s) {
+                        cplx acc = 0.0;
+                        for (int q = 0; q < R; ++q) acc += vv[q] * w[(size_t)((q * s) % R) * step];
+                        oo[s] = acc;
+                    }
+                }
+                const int base = (j / ns) * ns * R + k;
+                for (int r = 0; r < R; ++r) b[base + r * ns] = oo[r];
+            }
+            std::swap(a, b);
+            ns *= R;
+        }
+        for (int i = 0; i < n; ++i) data[(size_t)i * stride] = a[i];
+    }
+};
+
+// In-place 3-D FFT on a (px,py,pz) complex array, x then y then z
+// (fdm_fft.rs:445-598 order).
+void fft3(cplx *buf, int px, int py, int pz, const FftPlan &fx, const FftPlan &fy, const FftPlan &fz, bool inverse) {
+    const int maxn = std::max(px, std::max(py, pz));
+#pragma omp parallel
+    {
+        std::vector<cplx> scr(2 * (size_t)maxn);
+#pragma omp for schedule(static)
+        for (long r = 0; r < (long)py * pz; ++r) fx.run(buf + (size_t)r * px, 1, inverse, scr.data());
+        if (py > 1) {
+#pragma omp for schedule(static)
+            for (long r = 0; r < (long)px * pz; ++r) {
+                const long z = r / px, x = r % px;
+                fy.run(buf + (size_t)z * px * py + x, px, inverse, scr.data());
+            }
+        }
+        if (pz > 1) {
+#pragma omp for schedule(static)
+            for (long r = 0; r < (long)px * py; ++r) fz.run(buf + r, (size_t)px * py, inverse, scr.data());
+        }
+    }
+}
+
+// ---------------------------------------------------------------- vector ---
+struct V3 {
+    double c[3];
+};
+inline V3 v_add(V3 a, V3 b) { return {{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}}; }
+inline V3 v_scale(V3 a, double s) { return {{a.c[0] * s, a.c[1] * s, a.c[2] * s}}; }
+inline double v_dot(V3 a, V3 b) { return a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2]; }
+inline double v_norm(V3 a) { return std::sqrt(a.c[0] * a.c[0] + a.c[1] * a.c[1] + a.c[2] * a.c[2]); }
+inline V3 v_cross(V3 a, V3 b) {
+    return {{a.c[1] * b.c[2] - a.c[2] * b.c[1], a.c[2] * b.c[0] - a.c[0] * b.c[2], a.c[0] * b.c[1] - a.c[1] * b.c[0]}};
+}
+// vector.rs:11-19: normalise(0) = 0.
+inline V3 v_normalized(V3 a) {
+    const double n = v_norm(a);
+    if (n <= 0.0) return {{0.0, 0.0, 0.0}};
+    return v_scale(a, 1.0 / n);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ public API ---
+extern "C" {
+
+typedef struct {
+    uint32_t nx, ny, nz;
+    double dx, dy, dz;
+    double ms, a_ex, alpha, gamma;
+    int precession;      // 0 = pure-damping RHS
+    int exchange, demag;
+    int has_ext;
+    double h_ext[3];     // A/m
+    int has_uni;
+    double ku1, ku2, axis[3];
+    const uint8_t *mask;  // NULL = all active
+    double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
+} orc_desc;
+
+typedef struct {
+    double time, dt_used, dt_next;
+    double e_ex, e_demag, e_ext, e_ani, e_total;
+    double max_heff, max_hdemag, max_rhs;
+    uint64_t rejected;
+} orc_report;
+
+}  // extern "C"
+
+namespace {
+
+struct Problem {
+    orc_desc d;
+    size_t n;
+    std::vector<uint8_t> mask;
+    bool has_mask;
+    // demag workspace (fdm_fft.rs:64-131)
+    int px = 0, py = 0, pz = 0;
+    size_t P = 0;
+    FftPlan fx, fy, fz;
+    std::vector<cplx> kxx, kyy, kzz, kxy, kxz, kyz;
+    std::vector<cplx> bmx, bmy, bmz, bhx, bhy, bhz;
+    // integrator buffers
+    std::vector<V3> m0, ms, heff, hs, rhs;
+    std::vector<V3> k[7];
+    bool fsal_valid = false;
+    std::vector<V3> kfsal;
+
+    bool active(size_t i) const { return !has_mask || mask[i] != 0; }
+};
+
+void demag_add(Problem &p, const V3 *m, V3 *h) {
+    const int nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    std::fill(p.bmx.begin(), p.bmx.end(), cplx(0.0));
+    std::fill(p.bmy.begin(), p.bmy.end(), cplx(0.0));
+    std::fill(p.bmz.begin(), p.bmz.end(), cplx(0.0));
+    const double ms = p.d.ms;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t s = ((size_t)z * ny + y) * nx + x;
+                const size_t t = ((size_t)z * p.py + y) * p.px + x;
+                if (!p.active(s)) continue;
+                p.bmx[t] = m[s].c[0] * ms;
+                p.bmy[t] = m[s].c[1] * ms;
+                p.bmz[t] = m[s].c[2] * ms;
+            }
+    fft3(p.bmx.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, false);
+    fft3(p.bmy.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, false);
+    fft3(p.bmz.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, false);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)p.P; ++i) {
+        const cplx mx = p.bmx[i], my = p.bmy[i], mz = p.bmz[i];
+        p.bhx[i] = -(p.kxx[i] * mx + p.kxy[i] * my + p.kxz[i] * mz);
+        p.bhy[i] = -(p.kxy[i] * mx + p.kyy[i] * my + p.kyz[i] * mz);
+        p.bhz[i] = -(p.kxz[i] * mx + p.kyz[i] * my + p.kzz[i] * mz);
+    }
+    fft3(p.bhx.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, true);
+    fft3(p.bhy.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, true);
+    fft3(p.bhz.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, true);
+    const double norm = 1.0 / (double)p.P;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const long x = s % nx, y = (s / nx) % ny, z = s / ((long)nx * ny);
+        const size_t t = ((size_t)z * p.py + y) * p.px + x;
+        h[s].c[0] += p.bhx[t].real() * norm;
+        h[s].c[1] += p.bhy[t].real() * norm;
+        h[s].c[2] += p.bhz[t].real() * norm;
+    }
+}
+
+void exchange_add(const Problem &p, const V3 *m, V3 *h) {
+    const double pre = 2.0 * p.d.a_ex / (MU0 * p.d.ms);
+    const double dx2 = p.d.dx * p.d.dx, dy2 = p.d.dy * p.d.dy, dz2 = p.d.dz * p.d.dz;
+    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
+        const V3 c = m[s];
+        auto sample = [&](long xi, long yi, long zi) {
+            const size_t q = ((size_t)zi * ny + yi) * nx + xi;
+            return p.active(q) ? m[q] : c;
+        };
+        // fdm_types.rs:126-132 non-periodic clamp
+        const V3 xm = sample(std::max(x - 1, 0L), y, z), xp = sample(std::min(x + 1, nx - 1), y, z);
+        const V3 ym = sample(x, std::max(y - 1, 0L), z), yp = sample(x, std::min(y + 1, ny - 1), z);
+        const V3 zm = sample(x, y, std::max(z - 1, 0L)), zp = sample(x, y, std::min(z + 1, nz - 1));
+        for (int k = 0; k < 3; ++k)
+            h[s].c[k] += pre * ((xp.c[k] - 2.0 * c.c[k] + xm.c[k]) / dx2 + (yp.c[k] - 2.0 * c.c[k] + ym.c[k]) / dy2 +
+                                (zp.c[k] - 2.0 * c.c[k] + zm.c[k]) / dz2);
+    }
+}
+
+void ext_add(const Problem &p, V3 *h) {
+    if (!p.d.has_ext) return;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s)
+        if (p.active(s))
+            for (int k = 0; k < 3; ++k) h[s].c[k] += p.d.h_ext[k];
+}
+
+void uni_add(const Problem &p, const V3 *m, V3 *h) {
+    if (!p.d.has_uni) return;
+    const double ms = std::max(p.d.ms, 1e-30);
+    V3 u = {{p.d.axis[0], p.d.axis[1], p.d.axis[2]}};
+    u = v_scale(u, 1.0 / std::max(v_norm(u), 1e-30));
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const double mu = v_dot(m[s], u);
+        const double coeff = 2.0 * p.d.ku1 / (MU0 * ms) * mu + 4.0 * p.d.ku2 / (MU0 * ms) * mu * mu * mu;
+        for (int k = 0; k < 3; ++k) h[s].c[k] += u.c[k] * coeff;
+    }
+}
+
+// fdm_fields.rs:1153-1183 (MEL/cubic/DMI/Oersted/thermal are off on this path)
+void effective_field(Problem &p, const V3 *m, V3 *h) {
+    std::fill(h, h + p.n, V3{{0.0, 0.0, 0.0}});
+    if (p.d.exchange) exchange_add(p, m, h);
+    if (p.d.demag) demag_add(p, m, h);
+    ext_add(p, h);
+    uni_add(p, m, h);
+}
+
+inline V3 llg(const Problem &p, V3 m, V3 h) {
+    const double a = p.d.alpha;
+    const double gb = p.d.gamma / (1.0 + a * a);
+    const V3 pr = v_cross(m, h);
+    const V3 dm = v_cross(m, pr);
+    const V3 pt = p.d.precession ? pr : V3{{0.0, 0.0, 0.0}};
+    return v_scale(v_add(pt, v_scale(dm, a)), -gb);
+}
+
+void rhs_eval(Problem &p, const V3 *m, V3 *k) {
+    effective_field(p, m, p.heff.data());
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) k[s] = llg(p, m[s], p.heff[s]);
+}
+
+double max_norm(const V3 *v, size_t n) {
+    double r = 0.0;
+#pragma omp parallel for reduction(max : r) schedule(static)
+    for (long s = 0; s < (long)n; ++s) r = std::max(r, v_norm(v[s]));
+    return r;
+}
+
+double energy_field_form(const Problem &p, const V3 *m, const V3 *h, double f) {
+    const double vol = p.d.dx * p.d.dy * p.d.dz;
+    double e = 0.0;
+    for (size_t s = 0; s < p.n; ++s) e += f * MU0 * p.d.ms * v_dot(m[s], h[s]) * vol;
+    return e;
+}
+
+// compute_step_observables_zero_alloc, fdm_fields.rs:1683-1813.
+void observables(Problem &p, const V3 *m, orc_report *r) {
+    const size_t n = p.n;
+    std::fill(p.heff.begin(), p.heff.end(), V3{{0.0, 0.0, 0.0}});
+    r->e_ex = r->e_demag = r->e_ext = r->e_ani = 0.0;
+    r->max_hdemag = 0.0;
+    if (p.d.exchange) {
+        std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+        exchange_add(p, m, p.hs.data());
+        r->e_ex = energy_field_form(p, m, p.hs.data(), -0.5);
+        for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    }
+    if (p.d.demag) {
+        std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+        demag_add(p, m, p.hs.data());
+        r->e_demag = energy_field_form(p, m, p.hs.data(), -0.5);
+        r->max_hdemag = max_norm(p.hs.data(), n);
+        for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    }
+    if (p.d.has_ext) {
+        std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+        ext_add(p, p.hs.data());
+        r->e_ext = energy_field_form(p, m, p.hs.data(), -1.0);
+        for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    }
+    uni_add(p, m, p.heff.data());
+    std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+    uni_add(p, m, p.hs.data());
+    r->e_ani = energy_field_form(p, m, p.hs.data(), -0.5);
+    r->max_heff = max_norm(p.heff.data(), n);
+    for (size_t s = 0; s < n; ++s) p.rhs[s] = llg(p, m[s], p.heff[s]);
+    r->max_rhs = max_norm(p.rhs.data(), n);
+    r->e_total = r->e_ex + r->e_demag + r->e_ext + r->e_ani;
+}
+
+}  // namespace
+
+extern "C" {
+
+double orc_newell_f(double x, double y, double z) { return newell_f(x, y, z); }
+double orc_newell_g(double x, double y, double z) { return newell_g(x, y, z); }
+
+// out: 6 * (2nx*2ny*2nz) doubles, order xx,yy,zz,xy,xz,yz.
+int orc_newell_kernels(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double dz, double *out) {
+    if (!nx || !ny || !nz || !out) return -1;
+    newell_kernels((int)nx, (int)ny, (int)nz, dx, dy, dz, out);
+    return 0;
+}
+
+// In-place complex FFT of `n` interleaved (re,im) doubles, unnormalised.
+int orc_fft(double *data, int n, int inverse) {
+    if (n < 1 || !data) return -1;
+    FftPlan plan(n);
+    std::vector<cplx> scr(2 * (size_t)n);
+    plan.run(reinterpret_cast<cplx *>(data), 1, inverse != 0, scr.data());
+    return 0;
+}
+
+// util.rs:118-147.  Seed 0 never leaves the rejection loop, so it is refused.
+int orc_random_unit_vectors(uint64_t seed, uint64_t count, double *out) {
+    if (seed == 0 || !out) return -1;
+    uint64_t st = seed;
+    for (uint64_t i = 0; i < count; ++i) {
+        double c[3];
+        for (;;) {
+            for (int k = 0; k < 3; ++k) {
+                st ^= st << 13;
+                st ^= st >> 7;
+                st ^= st << 17;
+                c[k] = ((double)st / (double)UINT64_MAX) * 2.0 - 1.0;
+            }
+            const double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (nrm > 1e-10) {
+                out[3 * i + 0] = c[0] / nrm;
+                out[3 * i + 1] = c[1] / nrm;
+                out[3 * i + 2] = c[2] / nrm;
+                break;
+            }
+        }
+    }
+    return 0;
+}
+
+// Box minus centred cylinder, geometry.rs:159-231 cell-centre rule.
+uint64_t orc_hole_mask(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double radius, uint8_t *mask) {
+    const double cx = nx * dx * 0.5, cy = ny * dy * 0.5, r2 = radius * radius;
+    uint64_t act = 0;
+    for (uint32_t z = 0; z < nz; ++z)
+        for (uint32_t y = 0; y < ny; ++y)
+            for (uint32_t x = 0; x < nx; ++x) {
+                const double px = (x + 0.5) * dx - cx, py = (y + 0.5) * dy - cy;
+                const size_t i = x + (size_t)nx * (y + (size_t)ny * z);
+                mask[i] = (px * px + py * py) <= r2 ? 0 : 1;
+                act += mask[i];
+            }
+    return act;
+}
+
+void *orc_create(const orc_desc *d) {
+    if (!d || !d->nx || !d->ny || !d->nz) return nullptr;
+    Problem *p = new Problem();
+    p->d = *d;
+    p->n = (size_t)d->nx * d->ny * d->nz;
+    p->has_mask = d->mask != nullptr;
+    if (p->has_mask) p->mask.assign(d->mask, d->mask + p->n);
+    p->d.mask = nullptr;
+    const size_t n = p->n;
+    p->m0.resize(n);
+    p->ms.resize(n);
+    p->heff.resize(n);
+    p->hs.resize(n);
+    p->rhs.resize(n);
+    for (auto &k : p->k) k.resize(n);
+    if (d->demag) {
+        p->px = 2 * d->nx;
+        p->py = 2 * d->ny;
+        p->pz = 2 * d->nz;
+        p->P = (size_t)p->px * p->py * p->pz;
+        p->fx = FftPlan(p->px);
+        p->fy = FftPlan(p->py);
+        p->fz = FftPlan(p->pz);
+        std::vector<double> nk(6 * p->P);
+        newell_kernels(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, nk.data());
+        std::vector<cplx> *dst[6] = {&p->kxx, &p->kyy, &p->kzz, &p->kxy, &p->kxz, &p->kyz};
+        for (int c = 0; c < 6; ++c) {
+            dst[c]->resize(p->P);
+            for (size_t i = 0; i < p->P; ++i) (*dst[c])[i] = cplx(nk[c * p->P + i], 0.0);
+            fft3(dst[c]->data(), p->px, p->py, p->pz, p->fx, p->fy, p->fz, false);
+        }
+        for (auto *b : {&p->bmx, &p->bmy, &p->bmz, &p->bhx, &p->bhy, &p->bhz}) b->assign(p->P, cplx(0.0));
+    }
+    return p;
+}
+
+void orc_destroy(void *h) { delete static_cast<Problem *>(h); }
+
+// Kernel spectra as used by the reference GPU ABI (interleaved re/im over the
+// full (2nx,2ny,2nz) grid), component c in xx,yy,zz,xy,xz,yz order.
+int orc_kernel_spectrum(void *h, int c, double *out) {
+    Problem *p = static_cast<Problem *>(h);
+    if (!p || !p->d.demag || c < 0 || c > 5) return -1;
+    const std::vector<cplx> *src[6] = {&p->kxx, &p->kyy, &p->kzz, &p->kxy, &p->kxz, &p->kyz};
+    std::memcpy(out, src[c]->data(), sizeof(cplx) * p->P);
+    return 0;
+}
+
+// Individual field terms at magnetisation m (AoS 3n).  Any output may be NULL.
+int orc_fields(void *h, const double *m, double *h_ex, double *h_d, double *h_ext, double *h_ani, double *h_eff) {
+    Problem *p = static_cast<Problem *>(h);
+    const V3 *mm = reinterpret_cast<const V3 *>(m);
+    std::vector<V3> t(p->n);
+    auto one = [&](double *out, int which) {
+        if (!out) return;
+        std::fill(t.begin(), t.end(), V3{{0.0, 0.0, 0.0}});
+        if (which == 0 && p->d.exchange) exchange_add(*p, mm, t.data());
+        if (which == 1 && p->d.demag) demag_add(*p, mm, t.data());
+        if (which == 2) ext_add(*p, t.data());
+        if (which == 3) uni_add(*p, mm, t.data());
+        std::memcpy(out, t.data(), sizeof(V3) * p->n);
+    };
+    one(h_ex, 0);
+    one(h_d, 1);
+    one(h_ext, 2);
+    one(h_ani, 3);
+    if (h_eff) {
+        effective_field(*p, mm, p->heff.data());
+        std::memcpy(h_eff, p->heff.data(), sizeof(V3) * p->n);
+    }
+    return 0;
+}
+
+// Brute-force real-space demag: H_i = -sum_j N(r_i - r_j) . Ms m_j, using the
+// same Newell tensor.  O(n^2); for pinning the FFT path on small grids.
+int orc_demag_direct(void *h, const double *m, double *h_d) {
+    Problem *p = static_cast<Problem *>(h);
+    const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
+    const int px = 2 * nx, py = 2 * ny, pz = 2 * nz;
+    const size_t P = (size_t)px * py * pz;
+    std::vector<double> nk(6 * P);
+    newell_kernels(nx, ny, nz, p->d.dx, p->d.dy, p->d.dz, nk.data());
+    const double *N = nk.data();
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)p->n; ++i) {
+        double hx = 0, hy = 0, hz = 0;
+        if (p->active(i)) {
+            const int xi = i % nx, yi = (i / nx) % ny, zi = i / (nx * ny);
+            for (size_t j = 0; j < p->n; ++j) {
+                if (!p->active(j)) continue;
+                const int xj = j % nx, yj = (j / nx) % ny, zj = j / (nx * ny);
+                const int ax = (xi - xj + px) % px, ay = (yi - yj + py) % py, az = (zi - zj + pz) % pz;
+                const size_t q = ((size_t)az * py + ay) * px + ax;
+                const double Mx = m[3 * j] * p->d.ms, My = m[3 * j + 1] * p->d.ms, Mz = m[3 * j + 2] * p->d.ms;
+                hx -= N[q] * Mx + N[3 * P + q] * My + N[4 * P + q] * Mz;
+                hy -= N[3 * P + q] * Mx + N[P + q] * My + N[5 * P + q] * Mz;
+                hz -= N[4 * P + q] * Mx + N[5 * P + q] * My + N[2 * P + q] * Mz;
+            }
+        }
+        h_d[3 * i] = hx;
+        h_d[3 * i + 1] = hy;
+        h_d[3 * i + 2] = hz;
+    }
+    return 0;
+}
+
+int orc_rhs(void *h, const double *m, double *k) {
+    Problem *p = static_cast<Problem *>(h);
+    rhs_eval(*p, reinterpret_cast<const V3 *>(m), reinterpret_cast<V3 *>(k));
+    return 0;
+}
+
+int orc_observables(void *h, const double *m, orc_report *r) {
+    Problem *p = static_cast<Problem *>(h);
+    observables(*p, reinterpret_cast<const V3 *>(m), r);
+    return 0;
+}
+
+// Normalise (ExchangeLlgState::new) and zero inactive cells (new_state).
+int orc_prepare_state(void *h, double *m) {
+    Problem *p = static_cast<Problem *>(h);
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    for (size_t s = 0; s < p->n; ++s) mm[s] = p->active(s) ? v_normalized(mm[s]) : V3{{0.0, 0.0, 0.0}};
+    p->fsal_valid = false;
+    return 0;
+}
+
+void orc_invalidate_fsal(void *h) { static_cast<Problem *>(h)->fsal_valid = false; }
+
+// fdm_integrators.rs:84-150
+int orc_heun_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    Problem *p = static_cast<Problem *>(h);
+    const size_t n = p->n;
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    std::copy(mm, mm + n, p->m0.begin());
+    rhs_eval(*p, p->m0.data(), p->k[0].data());
+    for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(p->k[0][s], dt)));
+    rhs_eval(*p, p->ms.data(), p->k[1].data());
+    for (size_t s = 0; s < n; ++s) mm[s] = v_normalized(v_add(p->m0[s], v_scale(v_add(p->k[0][s], p->k[1][s]), 0.5 * dt)));
+    *t += dt;
+    observables(*p, mm, rep);
+    rep->time = *t;
+    rep->dt_used = dt;
+    rep->dt_next = 0.0;
+    rep->rejected = 0;
+    p->fsal_valid = false;
+    return 0;
+}
+
+// fdm_integrators.rs:155-264
+int orc_rk4_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    Problem *p = static_cast<Problem *>(h);
+    const size_t n = p->n;
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    std::copy(mm, mm + n, p->m0.begin());
+    auto &K = p->k;
+    rhs_eval(*p, p->m0.data(), K[0].data());
+    for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[0][s], 0.5 * dt)));
+    rhs_eval(*p, p->ms.data(), K[1].data());
+    for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[1][s], 0.5 * dt)));
+    rhs_eval(*p, p->ms.data(), K[2].data());
+    for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[2][s], dt)));
+    rhs_eval(*p, p->ms.data(), K[3].data());
+    const double dt6 = dt / 6.0;
+    for (size_t s = 0; s < n; ++s)
+        mm[s] = v_normalized(v_add(
+            p->m0[s], v_scale(v_add(v_add(K[0][s], v_scale(K[1][s], 2.0)), v_add(v_scale(K[2][s], 2.0), K[3][s])), dt6)));
+    *t += dt;
+    observables(*p, mm, rep);
+    rep->time = *t;
+    rep->dt_used = dt;
+    rep->dt_next = 0.0;
+    rep->rejected = 0;
+    p->fsal_valid = false;
+    return 0;
+}
+
+// Dormand-Prince 5(4) with FSAL, fdm_integrators.rs:429-680,828-870.
+int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) {
+    Problem *p = static_cast<Problem *>(h);
+    const orc_desc &c = p->d;
+    const size_t n = p->n;
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    double dt = std::max(std::min(dt_in, c.dt_max), c.dt_min);
+    std::copy(mm, mm + n, p->m0.begin());
+    const double A21 = 1.0 / 5.0;
+    const double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+    const double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+    const double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0, A54 = -212.0 / 729.0;
+    const double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0, A64 = 49.0 / 176.0,
+                 A65 = -5103.0 / 18656.0;
+    const double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0, B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+    const double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0, E5 = -17253.0 / 339200.0,
+                 E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+    auto &K = p->k;
+    const V3 *m0 = p->m0.data();
+    V3 *st = p->ms.data();
+    uint64_t rejected = 0;
+    for (;;) {
+        if (p->fsal_valid) {
+            std::copy(p->kfsal.begin(), p->kfsal.end(), K[0].begin());
+            p->fsal_valid = false;
+        } else {
+            rhs_eval(*p, m0, K[0].data());
+        }
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s) st[s] = v_normalized(v_add(m0[s], v_scale(K[0][s], A21 * dt)));
+        rhs_eval(*p, st, K[1].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(m0[s], v_scale(v_add(v_scale(K[0][s], A31), v_scale(K[1][s], A32)), dt)));
+        rhs_eval(*p, st, K[2].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(
+                m0[s], v_scale(v_add(v_add(v_scale(K[0][s], A41), v_scale(K[1][s], A42)), v_scale(K[2][s], A43)), dt)));
+        rhs_eval(*p, st, K[3].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(m0[s], v_scale(v_add(v_add(v_scale(K[0][s], A51), v_scale(K[1][s], A52)),
+                                                            v_add(v_scale(K[2][s], A53), v_scale(K[3][s], A54))),
+                                                      dt)));
+        rhs_eval(*p, st, K[4].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(
+                m0[s],
+                v_scale(v_add(v_add(v_add(v_scale(K[0][s], A61), v_scale(K[1][s], A62)), v_scale(K[2][s], A63)),
+                              v_add(v_scale(K[3][s], A64), v_scale(K[4][s], A65))),
+                        dt)));
+        rhs_eval(*p, st, K[5].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(
+                m0[s], v_scale(v_add(v_add(v_add(v_scale(K[0][s], B1), v_scale(K[2][s], B3)), v_scale(K[3][s], B4)),
+                                     v_add(v_scale(K[4][s], B5), v_scale(K[5][s], B6))),
+                               dt)));
+        rhs_eval(*p, st, K[6].data());
+        // error norm
+        const bool use_rtol = c.rtol > 0.0;
+        double err = 0.0;
+        const int idx[6] = {0, 2, 3, 4, 5, 6};
+        const double w[6] = {E1, E3, E4, E5, E6, E7};
+#pragma omp parallel for reduction(max : err) schedule(static)
+        for (long s = 0; s < (long)n; ++s) {
+            double e[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < 6; ++q)
+                for (int a = 0; a < 3; ++a) e[a] += w[q] * K[idx[q]][s].c[a];
+            for (int a = 0; a < 3; ++a) e[a] *= dt;
+            double v = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (use_rtol) v /= (c.atol + c.rtol * std::max(v_norm(m0[s]), 1e-30));
+            err = std::max(err, v);
+        }
+        const double thr = use_rtol ? 1.0 : c.atol;
+        if (err <= thr || dt <= c.dt_min) {
+            std::copy(st, st + n, mm);
+            *t += dt;
+            p->kfsal = K[6];
+            p->fsal_valid = true;
+            const double ratio = std::max(std::min(c.headroom * std::pow(thr / std::max(err, 1e-30), 0.2), c.growth), c.shrink);
+            const double dt_next = std::min(std::max(dt * ratio, c.dt_min), c.dt_max);
+            observables(*p, mm, rep);
+            rep->time = *t;
+            rep->dt_used = dt;
+            rep->dt_next = dt_next;
+            rep->rejected = rejected;
+            return 0;
+        }
+        ++rejected;
+        const double ratio = std::max(std::min(c.headroom * std::pow(thr / err, 0.2), c.growth), c.shrink);
+        dt = std::min(std::max(dt * ratio, c.dt_min), c.dt_max);
+    }
+}
+
+// Runner loop, cpu_reference.rs:622-639: dt_step = min(dt, until - t); the
+// next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 4 RK4.
+// Records each accepted step's (time, dt) in `trace` (2 doubles per step) if
+// non-NULL.  Returns the number of steps taken.
+uint64_t orc_run(void *h, int integrator, double *m, double *t, double until, double dt0, uint64_t max_steps,
+                 orc_report *last, double *trace) {
+    Problem *p = static_cast<Problem *>(h);
+    double dt = dt0;
+    uint64_t steps = 0;
+    orc_report rep{};
+    while (*t < until && steps < max_steps) {
+        const double dt_step = std::min(dt, until - *t);
+        if (integrator == 2) {
+            orc_rk45_step(p, m, t, dt_step, &rep);
+            dt = rep.dt_next;
+        } else if (integrator == 4) {
+            orc_rk4_step(p, m, t, dt_step, &rep);
+        } else {
+            orc_heun_step(p, m, t, dt_step, &rep);
+        }
+        if (trace) {
+            trace[2 * steps] = rep.time;
+            trace[2 * steps + 1] = rep.dt_used;
+        }
+        ++steps;
+    }
+    if (last) *last = rep;
+    return steps;
+}
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_num_threads(int n) { omp_set_num_threads(n); }
+
+}  // extern "C"

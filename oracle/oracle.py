@@ -1,0 +1,271 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/fdm_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the product.
+The oracle restates the reference CPU engine `fdm_cpu_reference`
+(crates/fullmag-engine, crates/fullmag-fdm-demag, crates/fullmag-runner);
+see the header of fdm_oracle.cpp for the file:line map.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libfdm_oracle.so")
+_lib = None
+
+MU0 = 4.0 * np.pi * 1e-7
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ -fopenmp)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("ms", ctypes.c_double), ("a_ex", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("precession", ctypes.c_int), ("exchange", ctypes.c_int), ("demag", ctypes.c_int),
+        ("has_ext", ctypes.c_int), ("h_ext", ctypes.c_double * 3),
+        ("has_uni", ctypes.c_int), ("ku1", ctypes.c_double), ("ku2", ctypes.c_double),
+        ("axis", ctypes.c_double * 3),
+        ("mask", ctypes.POINTER(ctypes.c_uint8)),
+        ("atol", ctypes.c_double), ("rtol", ctypes.c_double), ("dt_min", ctypes.c_double),
+        ("dt_max", ctypes.c_double), ("headroom", ctypes.c_double), ("growth", ctypes.c_double),
+        ("shrink", ctypes.c_double),
+    ]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double), ("dt_used", ctypes.c_double), ("dt_next", ctypes.c_double),
+        ("e_ex", ctypes.c_double), ("e_demag", ctypes.c_double), ("e_ext", ctypes.c_double),
+        ("e_ani", ctypes.c_double), ("e_total", ctypes.c_double),
+        ("max_heff", ctypes.c_double), ("max_hdemag", ctypes.c_double), ("max_rhs", ctypes.c_double),
+        ("rejected", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_newell_f.restype = ctypes.c_double
+        L.orc_newell_f.argtypes = [ctypes.c_double] * 3
+        L.orc_newell_g.restype = ctypes.c_double
+        L.orc_newell_g.argtypes = [ctypes.c_double] * 3
+        L.orc_newell_kernels.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_double] * 3 + [_dp]
+        L.orc_fft.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.orc_random_unit_vectors.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _dp]
+        L.orc_hole_mask.restype = ctypes.c_uint64
+        L.orc_hole_mask.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_uint8)]
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.POINTER(_Desc)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_kernel_spectrum.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_fields.argtypes = [ctypes.c_void_p] + [_dp] * 6
+        L.orc_demag_direct.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_observables.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(Report)]
+        L.orc_prepare_state.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_invalidate_fsal.argtypes = [ctypes.c_void_p]
+        for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step"):
+            getattr(L, nm).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.POINTER(Report)]
+        L.orc_run.restype = ctypes.c_uint64
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_uint64, ctypes.POINTER(Report), _dp]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def newell_f(x, y, z):
+    return lib().orc_newell_f(x, y, z)
+
+
+def newell_g(x, y, z):
+    return lib().orc_newell_g(x, y, z)
+
+
+def newell_kernels(nx, ny, nz, dx, dy, dz):
+    """Real-space padded tensor; dict of 6 arrays shaped (2nz, 2ny, 2nx)."""
+    P = 8 * nx * ny * nz
+    out = np.zeros(6 * P)
+    assert lib().orc_newell_kernels(nx, ny, nz, dx, dy, dz, _ptr(out)) == 0
+    shp = (2 * nz, 2 * ny, 2 * nx)
+    names = ("xx", "yy", "zz", "xy", "xz", "yz")
+    return {nm: out[i * P:(i + 1) * P].reshape(shp) for i, nm in enumerate(names)}
+
+
+def fft(x, inverse=False):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+    assert lib().orc_fft(a.view(np.float64).ctypes.data_as(_dp), a.size, int(inverse)) == 0
+    return a
+
+
+def random_unit_vectors(seed, count):
+    out = np.zeros((count, 3))
+    assert lib().orc_random_unit_vectors(seed, count, _ptr(out)) == 0
+    return out
+
+
+def hole_mask(nx, ny, nz, dx, dy, radius):
+    m = np.zeros(nx * ny * nz, dtype=np.uint8)
+    lib().orc_hole_mask(nx, ny, nz, dx, dy, radius, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return m
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+@dataclass
+class Problem:
+    """Problem description; same physics fields as the ABI plan descriptor."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ms: float
+    a_ex: float
+    alpha: float
+    gamma: float = 2.211e5
+    precession: bool = True
+    exchange: bool = True
+    demag: bool = True
+    h_ext: tuple | None = None
+    ku1: float = 0.0
+    ku2: float = 0.0
+    axis: tuple | None = None
+    mask: np.ndarray | None = None
+    atol: float = 1e-5
+    rtol: float = 0.0
+    dt_min: float = 1e-18
+    dt_max: float = 1e-10
+    headroom: float = 0.8
+    growth: float = float("inf")
+    shrink: float = 0.0
+
+
+class Oracle:
+    def __init__(self, pb: Problem):
+        L = lib()
+        d = _Desc()
+        d.nx, d.ny, d.nz = pb.nx, pb.ny, pb.nz
+        d.dx, d.dy, d.dz = pb.dx, pb.dy, pb.dz
+        d.ms, d.a_ex, d.alpha, d.gamma = pb.ms, pb.a_ex, pb.alpha, pb.gamma
+        d.precession, d.exchange, d.demag = int(pb.precession), int(pb.exchange), int(pb.demag)
+        d.has_ext = int(pb.h_ext is not None)
+        if pb.h_ext is not None:
+            d.h_ext[:] = list(pb.h_ext)
+        d.has_uni = int(pb.axis is not None)
+        if pb.axis is not None:
+            d.ku1, d.ku2 = pb.ku1, pb.ku2
+            d.axis[:] = list(pb.axis)
+        self._mask = None
+        if pb.mask is not None:
+            self._mask = np.ascontiguousarray(pb.mask, dtype=np.uint8).ravel()
+            d.mask = self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d.atol, d.rtol, d.dt_min, d.dt_max = pb.atol, pb.rtol, pb.dt_min, pb.dt_max
+        d.headroom, d.growth, d.shrink = pb.headroom, pb.growth, pb.shrink
+        self.pb = pb
+        self.n = pb.nx * pb.ny * pb.nz
+        self.h = L.orc_create(ctypes.byref(d))
+        if not self.h:
+            raise ValueError("oracle create failed")
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.orc_destroy(h)
+            self.h = None
+
+    def _m(self, m):
+        a = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(self.n, 3))
+        return a
+
+    def prepare(self, m):
+        a = self._m(m).copy()
+        lib().orc_prepare_state(self.h, _ptr(a))
+        return a
+
+    def fields(self, m):
+        a = self._m(m)
+        outs = {k: np.zeros((self.n, 3)) for k in ("h_ex", "h_demag", "h_ext", "h_ani", "h_eff")}
+        lib().orc_fields(self.h, _ptr(a), *[_ptr(outs[k]) for k in ("h_ex", "h_demag", "h_ext", "h_ani", "h_eff")])
+        return outs
+
+    def demag_direct(self, m):
+        a = self._m(m)
+        out = np.zeros((self.n, 3))
+        lib().orc_demag_direct(self.h, _ptr(a), _ptr(out))
+        return out
+
+    def rhs(self, m):
+        a = self._m(m)
+        out = np.zeros((self.n, 3))
+        lib().orc_rhs(self.h, _ptr(a), _ptr(out))
+        return out
+
+    def observables(self, m):
+        a = self._m(m)
+        r = Report()
+        lib().orc_observables(self.h, _ptr(a), ctypes.byref(r))
+        return r.as_dict()
+
+    def kernel_spectrum(self, c):
+        P = 8 * self.n
+        out = np.zeros(2 * P)
+        assert lib().orc_kernel_spectrum(self.h, c, _ptr(out)) == 0
+        return out
+
+    def invalidate_fsal(self):
+        lib().orc_invalidate_fsal(self.h)
+
+    def step(self, integrator, m, t, dt):
+        """One step in place on `m` (n,3 float64 C-contiguous). Returns (t, report)."""
+        tt = ctypes.c_double(t)
+        r = Report()
+        fn = {"heun": lib().orc_heun_step, "rk4": lib().orc_rk4_step, "rk45": lib().orc_rk45_step}[integrator]
+        fn(self.h, _ptr(m), ctypes.byref(tt), dt, ctypes.byref(r))
+        return tt.value, r.as_dict()
+
+    def run(self, integrator, m, t, until, dt0, max_steps, trace=False):
+        code = {"heun": 1, "rk45": 2, "rk4": 4}[integrator]
+        tt = ctypes.c_double(t)
+        r = Report()
+        tr = np.zeros(2 * max_steps) if trace else None
+        n = lib().orc_run(self.h, code, _ptr(m), ctypes.byref(tt), until, dt0, max_steps, ctypes.byref(r),
+                          _ptr(tr) if trace else None)
+        res = {"t": tt.value, "steps": int(n), "last": r.as_dict()}
+        if trace:
+            res["trace"] = tr[:2 * n].reshape(n, 2)
+        return res

@@ -1,0 +1,111 @@
+"""Shared problem builders: one parameter set -> engine plan + oracle problem."""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+MU0 = 4.0 * np.pi * 1e-7
+
+
+@dataclass
+class Case:
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ms: float
+    a_ex: float
+    alpha: float
+    m0: np.ndarray
+    mask: np.ndarray | None = None
+    h_ext: tuple | None = None
+    ku1: float = 0.0
+    ku2: float = 0.0
+    axis: tuple | None = None
+    demag: bool = True
+    exchange: bool = True
+    precession: bool = True
+    gamma: float = 2.211e5
+    atol: float = 1e-5
+    rtol: float = 0.0
+    dt_min: float = 1e-18
+    dt_max: float = 1e-10
+    headroom: float = 0.8
+
+    @property
+    def n(self):
+        return self.nx * self.ny * self.nz
+
+    def plan(self, precision="double", integrator="dp45", **kw):
+        from fullmag_amd.backend import FdmPlan
+        return FdmPlan(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
+                       a_ex=self.a_ex, alpha=self.alpha, m0=self.m0, gamma=self.gamma, precision=precision,
+                       integrator=integrator, exchange=self.exchange, demag=self.demag,
+                       precession=self.precession, h_ext=self.h_ext, ku1=self.ku1, ku2=self.ku2, axis=self.axis,
+                       mask=self.mask, atol=self.atol, dt_min=self.dt_min, dt_max=self.dt_max,
+                       headroom=self.headroom, **kw)
+
+    def problem(self):
+        from oracle.oracle import Problem
+        return Problem(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
+                       a_ex=self.a_ex, alpha=self.alpha, gamma=self.gamma, precession=self.precession,
+                       exchange=self.exchange, demag=self.demag, h_ext=self.h_ext, ku1=self.ku1, ku2=self.ku2,
+                       axis=self.axis, mask=self.mask, atol=self.atol, rtol=self.rtol, dt_min=self.dt_min,
+                       dt_max=self.dt_max, headroom=self.headroom)
+
+
+def masked_3x3(demag: bool) -> Case:
+    """crates/fullmag-runner/src/native_fdm.rs:1278-1319 (masked 3x3x1 plan)."""
+    m0 = np.array([
+        [1.0, 0.0, 0.0],
+        [0.9950041652780258, 0.09983341664682815, 0.0],
+        [0.9800665778412416, 0.19866933079506122, 0.0],
+        [0.9992009587217894, 0.0, 0.03996803834887158],
+        [0.9937606691655043, 0.09970865087213879, 0.04972948160146045],
+        [0.9778332467629838, 0.19771314245924698, 0.06988589031642899],
+        [0.9968017063026194, -0.039904089712529575, 0.06972124896577284],
+        [0.9892364775387807, 0.05946310942269411, 0.1338082836649087],
+        [0.9711213242426827, 0.15730105252897553, 0.17902957342582418],
+    ])
+    mask = np.array([1, 1, 1, 1, 0, 1, 1, 1, 0], dtype=np.uint8)
+    return Case(3, 3, 1, 5e-9, 5e-9, 10e-9, 800e3, 13e-12, 0.1, m0, mask=mask, h_ext=(1.5e3, -2.0e3, 7.5e2),
+                demag=demag)
+
+
+def thin_film_8x6() -> Case:
+    """crates/fullmag-runner/src/native_fdm.rs:1321-1365 (thin film with demag)."""
+    nx, ny = 8, 6
+    m0 = []
+    for y in range(ny):
+        for x in range(nx):
+            th, ph = 0.11 * x, 0.07 * y
+            v = np.array([np.cos(th) * np.cos(ph), np.sin(th) * np.cos(ph), 0.2 * np.sin(ph)])
+            m0.append(v / np.linalg.norm(v))
+    return Case(nx, ny, 1, 4e-9, 4e-9, 10e-9, 800e3, 13e-12, 0.1, np.array(m0), h_ext=(2.0e3, -1.0e3, 5.0e2))
+
+
+def random_case(nx, ny, nz, seed=7, hole_radius_cells=None, dx=2e-9, dy=2e-9, dz=5e-9, ms=8.6e5, a_ex=1.3e-11,
+                alpha=0.02, h_ext=(0.01 / MU0, 0.0, 0.0), **kw) -> Case:
+    from fullmag_amd.backend import hole_mask, random_unit_vectors
+    m0 = random_unit_vectors(seed, nx * ny * nz)
+    mask = None
+    if hole_radius_cells:
+        mask = hole_mask(nx, ny, nz, dx, dy, hole_radius_cells * dx)
+        m0[mask == 0] = 0.0
+    return Case(nx, ny, nz, dx, dy, dz, ms, a_ex, alpha, m0, mask=mask, h_ext=h_ext, **kw)
+
+
+def thin_film_512(seed=42) -> Case:
+    """512x512x1 film of 2x2x5 nm cells with a 64-cell hole, Ms 8.6e5 A/m,
+    A 1.3e-11 J/m, alpha 0.02, B = 10 mT along x, seeded random m0."""
+    return random_case(512, 512, 1, seed=seed, hole_radius_cells=64)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
