@@ -282,35 +282,39 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
         v[2] = m0[2 * N + cell];
         return;
     }
+    // Accumulated in FP64 for both storage precisions: the Dormand-Prince
+    // rows have coefficients up to ~12 that cancel to O(1), which costs
+    // FP32 several bits; the kernel is memory-bound, so the FP64 ALU work is
+    // free.
     const StageRow &R = M.row[s];
-    T a0 = T(0), a1 = T(0), a2 = T(0);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int q = 0; q < R.nk; ++q) {
         const T *k = A.k_all + kslot(M, R.kidx[q], p) * 3 * N;
-        const T w = T(R.a[q]);
-        a0 += w * k[cell];
-        a1 += w * k[N + cell];
-        a2 += w * k[2 * N + cell];
+        const double w = R.a[q];
+        a0 += w * (double)k[cell];
+        a1 += w * (double)k[N + cell];
+        a2 += w * (double)k[2 * N + cell];
     }
-    const T f = T(dt * R.scale);
-    T x = m0[cell] + a0 * f;
-    T y = m0[N + cell] + a1 * f;
-    T z = m0[2 * N + cell] + a2 * f;
-    const T n2 = x * x + y * y + z * z;
-    if (n2 > T(0)) {
-        const T inv = T(1) / sqrt(n2);
+    const double f = dt * R.scale;
+    double x = (double)m0[cell] + a0 * f;
+    double y = (double)m0[N + cell] + a1 * f;
+    double z = (double)m0[2 * N + cell] + a2 * f;
+    const double n2 = x * x + y * y + z * z;
+    if (n2 > 0.0) {
+        const double inv = 1.0 / sqrt(n2);
         x *= inv;
         y *= inv;
         z *= inv;
     } else {
-        x = y = z = T(0);
+        x = y = z = 0.0;
     }
     T *out = s == M.nstage ? A.m_all + (1 - p) * 3 * N : A.ybuf;
-    out[cell] = x;
-    out[N + cell] = y;
-    out[2 * N + cell] = z;
-    v[0] = x;
-    v[1] = y;
-    v[2] = z;
+    out[cell] = (T)x;
+    out[N + cell] = (T)y;
+    out[2 * N + cell] = (T)z;
+    v[0] = (T)x;
+    v[1] = (T)y;
+    v[2] = (T)z;
 }
 
 struct Acc {
