@@ -967,16 +967,36 @@ int fullmag_fdm_backend_advance(fullmag_fdm_backend *h, double t_end, uint64_t m
     hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
     hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
     int rc = FULLMAG_FDM_OK;
-    // double-buffered: chunk i+1 is queued before chunk i's flag is read
+    // Trial steps are queued in chunks; chunk i+1 is queued before chunk i's
+    // control block is read back.  Without rejections exactly max_steps
+    // trials are queued (no early-exit launches); once every planned trial is
+    // queued the host waits and tops up by the number of steps still missing.
+    uint64_t launched = 0, planned = max_steps;
     int cur = 0;
     bool pending = false;
     while (!c.done) {
-        for (int a = 0; a < kAttemptChunk; ++a)
+        uint64_t nl = kAttemptChunk;
+        if (launched < planned) {
+            nl = std::min<uint64_t>(nl, planned - launched);
+        } else {
+            if (hipStreamSynchronize(h->stream) != hipSuccess ||
+                hipMemcpy(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = fail(h, FULLMAG_FDM_ERR_CUDA, "control readback");
+                break;
+            }
+            pending = false;
+            if (flag_h[cur].done) break;
+            const uint64_t missing = flag_h[cur].step_end > flag_h[cur].step ? flag_h[cur].step_end - flag_h[cur].step : 1;
+            nl = std::min<uint64_t>(kAttemptChunk, missing);
+            planned = launched + nl;
+        }
+        for (uint64_t a = 0; a < nl; ++a)
             if (!launch_attempt(h)) {
                 rc = FULLMAG_FDM_ERR_CUDA;
                 break;
             }
         if (rc != FULLMAG_FDM_OK) break;
+        launched += nl;
         hipMemcpyAsync(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream);
         hipEventRecord(ev[cur], h->stream);
         if (pending) {
