@@ -196,6 +196,27 @@ class SnapshotDesc(ctypes.Structure):
                 ("scalar_bytes", ctypes.c_uint32), ("scalar_type", ctypes.c_int)]
 
 
+class PlanExt(ctypes.Structure):
+    """fullmag_fdm_plan_ext (include/fullmag_fdm_ext.h)."""
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("ms_field", c_double_p),
+        ("anisotropy_axis_field", c_double_p),
+        ("field_waveform", ctypes.c_uint32),
+        ("field_amplitude_am", ctypes.c_double * 3),
+        ("field_t0", ctypes.c_double),
+        ("field_t1", ctypes.c_double),
+        ("field_frequency", ctypes.c_double),
+        ("field_phase", ctypes.c_double),
+        ("field_mask", c_u8_p),
+        ("adaptive_rtol", ctypes.c_double),
+        ("adaptive_growth_limit", ctypes.c_double),
+        ("adaptive_shrink_limit", ctypes.c_double),
+    ]
+
+
+WAVE_NONE, WAVE_RAMP, WAVE_SINC, WAVE_SINE = 0, 1, 2, 3
+
 POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
 # name -> (restype, argtypes); every symbol of include/fullmag_fdm.h first,
@@ -242,6 +263,7 @@ EXT_SYMBOLS = {
                        ctypes.POINTER(ctypes.c_double)]),
     "fullmag_fdm_backend_time_kernels": (ctypes.c_int, [_H, ctypes.c_int, c_double_p, ctypes.c_int]),
     "fullmag_fdm_random_unit_vectors": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, c_double_p]),
+    "fullmag_fdm_backend_create_ext": (_H, [ctypes.POINTER(PlanDesc), ctypes.POINTER(PlanExt)]),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -60,7 +60,26 @@ class FdmPlan:
     dt_min: float = 0.0
     dt_max: float = 0.0
     headroom: float = 0.0
+    ku1_field: np.ndarray | None = None      # per-cell Ku1 (reference ABI ku1_field)
+    ku2_field: np.ndarray | None = None
+    # plan extension (include/fullmag_fdm_ext.h)
+    ms_field: np.ndarray | None = None       # per-cell Ms
+    axis_field: np.ndarray | None = None     # (n,3) per-cell uniaxial axis
+    wave: int = 0                            # abi.WAVE_*
+    wave_amp: tuple = (0.0, 0.0, 0.0)
+    wave_t0: float = 0.0
+    wave_t1: float = 0.0
+    wave_freq: float = 0.0
+    wave_phase: float = 0.0
+    wave_mask: np.ndarray | None = None
+    rtol: float = 0.0
+    growth: float = 0.0
+    shrink: float = 0.0
     extra: dict = field(default_factory=dict)  # raw PlanDesc fields (for error tests)
+
+    def uses_ext(self) -> bool:
+        return (self.ms_field is not None or self.axis_field is not None or self.wave != 0
+                or self.rtol > 0 or self.growth > 0 or self.shrink > 0)
 
 
 _INTEGRATORS = {"heun": abi.INTEGRATOR_HEUN, "dp45": abi.INTEGRATOR_DP45, "rk45": abi.INTEGRATOR_DP45,
@@ -129,6 +148,12 @@ class FdmBackend:
                     setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
                     ln = a.size
             desc.demag_kernel_spectrum_len = ln
+        for nm in ("ku1_field", "ku2_field"):
+            a = getattr(plan, nm)
+            if a is not None:
+                a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+                self._keep.append(a)
+                setattr(desc, nm, _dp(a))
         desc.adaptive_max_error = plan.atol
         desc.adaptive_dt_min = plan.dt_min
         desc.adaptive_dt_max = plan.dt_max
@@ -136,7 +161,31 @@ class FdmBackend:
         for k, v in plan.extra.items():
             setattr(desc, k, v)
         self._desc = desc
-        self.h = self.lib.fullmag_fdm_backend_create(ctypes.byref(desc))
+        if plan.uses_ext():
+            ext = abi.PlanExt()
+            ext.struct_size = ctypes.sizeof(abi.PlanExt)
+            if plan.ms_field is not None:
+                a = np.ascontiguousarray(np.asarray(plan.ms_field, dtype=np.float64).reshape(-1))
+                self._keep.append(a)
+                ext.ms_field = _dp(a)
+            if plan.axis_field is not None:
+                a = np.ascontiguousarray(np.asarray(plan.axis_field, dtype=np.float64).reshape(-1))
+                self._keep.append(a)
+                ext.anisotropy_axis_field = _dp(a)
+            ext.field_waveform = plan.wave
+            ext.field_amplitude_am[:] = list(plan.wave_amp)
+            ext.field_t0, ext.field_t1 = plan.wave_t0, plan.wave_t1
+            ext.field_frequency, ext.field_phase = plan.wave_freq, plan.wave_phase
+            if plan.wave_mask is not None:
+                wm = np.ascontiguousarray(np.asarray(plan.wave_mask, dtype=np.uint8).reshape(-1))
+                self._keep.append(wm)
+                ext.field_mask = wm.ctypes.data_as(abi.c_u8_p)
+            ext.adaptive_rtol, ext.adaptive_growth_limit = plan.rtol, plan.growth
+            ext.adaptive_shrink_limit = plan.shrink
+            self._ext = ext
+            self.h = self.lib.fullmag_fdm_backend_create_ext(ctypes.byref(desc), ctypes.byref(ext))
+        else:
+            self.h = self.lib.fullmag_fdm_backend_create(ctypes.byref(desc))
         if not self.h:
             raise FdmError(abi.ERR_INTERNAL, "fullmag_fdm_backend_create returned NULL")
         err = self.lib.fullmag_fdm_backend_last_error(self.h)

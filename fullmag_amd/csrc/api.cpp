@@ -138,6 +138,12 @@ bool plan_fft(int n, fm::FftDesc &fd, std::string *err) {
         if (!push(2)) return false;
         m /= 2;
     }
+    fd.lg = -1;
+    if (m == 1) {
+        int lg = 0;
+        while ((1 << lg) < n) ++lg;
+        fd.lg = lg;
+    }
     for (int p = 3; m > 1; p += 2) {
         while (m % p == 0) {
             if (p > fm::kMaxRadix) {
@@ -182,7 +188,6 @@ std::string unsupported_terms(const fullmag_fdm_plan_desc *p) {
     std::vector<std::string> bad;
     if (p->has_cubic_anisotropy) bad.push_back("cubic anisotropy");
     if (p->kc1_field || p->kc2_field || p->kc3_field) bad.push_back("per-cell cubic constants");
-    if (p->ku1_field || p->ku2_field) bad.push_back("per-cell uniaxial constants");
     if (p->has_interfacial_dmi && p->dmi_D_interfacial != 0.0) bad.push_back("interfacial DMI");
     if (p->has_bulk_dmi && p->dmi_D_bulk != 0.0) bad.push_back("bulk DMI");
     if (p->has_magnetoelastic) bad.push_back("magnetoelastic coupling");
@@ -234,8 +239,7 @@ bool eval_current(Backend *h, bool observables) {
             B.obs_hd = reinterpret_cast<T *>(h->obs_base[1]);
             B.obs_heff = reinterpret_cast<T *>(h->obs_base[2]);
         }
-        if (!hip_ok(h, fm::enqueue_eval(B, 1, 1, h->stream), "launch (evaluation)")) return false;
-        return hip_ok(h, fm::enqueue_control(B, 1, h->stream), "launch (statistics)");
+        return hip_ok(h, fm::enqueue_eval(B, 1, 2, h->stream), "launch (evaluation)");
     });
     if (!ok || !pull_ctl(h)) return false;
     if (observables) h->obs_fresh = true;
@@ -246,7 +250,7 @@ bool enqueue_attempt(Backend *h) {
     return with_args(h, [&](auto &A) {
         for (int s = 2; s <= h->nstage; ++s)
             if (!hip_ok(h, fm::enqueue_eval(A, s, s == h->nstage ? 1 : 0, h->stream), "launch (stage)")) return false;
-        return hip_ok(h, fm::enqueue_control(A, 0, h->stream), "launch (control)");
+        return true;
     });
 }
 
@@ -300,7 +304,7 @@ void fill_stats(Backend *h, fullmag_fdm_step_stats *s, double dt, uint64_t wall_
 }
 
 template <class T>
-bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
+bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_plan_ext *x) {
     fm::Args<T> &A = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
     std::memset(&A, 0, sizeof(A));
     const int nx = (int)h->nx, ny = (int)h->ny, nz = (int)h->nz;
@@ -343,6 +347,8 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
         for (int c = 0; c < 3; ++c) A.axis[c] = (T)(u[c] / nu);
         A.uni_c1 = (T)(2.0 * p->uniaxial_anisotropy_constant / (kMu0 * ms));
         A.uni_c2 = (T)(4.0 * p->uniaxial_anisotropy_k2 / (kMu0 * ms));
+        A.ku1_u = (T)p->uniaxial_anisotropy_constant;
+        A.ku2_u = (T)p->uniaxial_anisotropy_k2;
     }
     A.atol = p->adaptive_max_error > 0 ? p->adaptive_max_error : 1e-5;
     A.dt_min = p->adaptive_dt_min > 0 ? p->adaptive_dt_min : 1e-18;
@@ -351,6 +357,31 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
     A.rtol = 0.0;
     A.growth = std::numeric_limits<double>::infinity();
     A.shrink = 0.0;
+    A.mu0_vol = kMu0 * p->grid.dx * p->grid.dy * p->grid.dz;
+    A.two_mu0inv = (T)(2.0 / kMu0);
+    if (x) {
+        if (x->adaptive_rtol < 0 || x->adaptive_shrink_limit < 0) {
+            h->err = "invalid adaptive tolerances in plan extension";
+            return false;
+        }
+        A.rtol = x->adaptive_rtol;
+        A.growth = x->adaptive_growth_limit > 0 ? x->adaptive_growth_limit : std::numeric_limits<double>::infinity();
+        A.shrink = x->adaptive_shrink_limit;
+        if (x->field_waveform > fm::WAVE_SINE) {
+            h->err = "unknown field_waveform";
+            return false;
+        }
+        A.wave_kind = (int)x->field_waveform;
+        for (int c = 0; c < 3; ++c) A.wave_amp[c] = x->field_amplitude_am[c];
+        A.wave_t0 = x->field_t0;
+        A.wave_t1 = x->field_t1;
+        A.wave_freq = x->field_frequency;
+        A.wave_phase = x->field_phase;
+        if (A.wave_kind == fm::WAVE_RAMP && !(x->field_t1 > x->field_t0)) {
+            h->err = "ramp waveform needs field_t1 > field_t0";
+            return false;
+        }
+    }
     A.method = p->integrator == FULLMAG_FDM_INTEGRATOR_DP45 ? fm::M_DP45
                : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4 ? fm::M_RK4
                                                                : fm::M_HEUN;
@@ -411,6 +442,49 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
         }
     }
     A.ex_pre = p->enable_exchange ? (T)(2.0 * a_ex / (kMu0 * ms)) : (T)0;
+    A.two_a_mu0 = p->enable_exchange ? (T)(2.0 * a_ex / kMu0) : (T)0;
+    // per-cell maps (reference ku1_field/ku2_field; extension Ms and axis)
+    auto upload_cell = [&](const double *src, size_t count, const char *what) -> const T * {
+        std::vector<T> v(src, src + count);
+        T *d = static_cast<T *>(dalloc(h, count * sizeof(T)));
+        if (!d || hipMemcpy(d, v.data(), count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+            h->err = std::string("upload ") + what;
+            return nullptr;
+        }
+        return d;
+    };
+    if (p->ku1_field && !(A.ku1_cell = upload_cell(p->ku1_field, n, "ku1_field"))) return false;
+    if (p->ku2_field && !(A.ku2_cell = upload_cell(p->ku2_field, n, "ku2_field"))) return false;
+    if (x && x->ms_field) {
+        for (uint64_t i = 0; i < n; ++i)
+            if ((!h->has_mask || h->mask_h[i]) && !(x->ms_field[i] > 0.0)) {
+                h->err = "ms_field must be positive on magnetic cells";
+                return false;
+            }
+        std::vector<double> msv(x->ms_field, x->ms_field + n);
+        for (uint64_t i = 0; i < n; ++i)
+            if (!(msv[i] > 0.0)) msv[i] = ms;  // empty cells: any positive value (m = 0 there)
+        if (!(A.ms_cell = upload_cell(msv.data(), n, "ms_field"))) return false;
+    }
+    if (x && x->anisotropy_axis_field) {
+        std::vector<double> ax(3 * n);
+        for (uint64_t i = 0; i < n; ++i) {
+            const double *u = x->anisotropy_axis_field + 3 * i;
+            const double nu = std::max(std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), 1e-30);
+            for (int c = 0; c < 3; ++c) ax[c * n + i] = u[c] / nu;
+        }
+        if (!(A.axis_cell = upload_cell(ax.data(), 3 * n, "anisotropy_axis_field"))) return false;
+    }
+    if (x && x->field_mask) {
+        std::vector<uint8_t> wm(x->field_mask, x->field_mask + n);
+        for (auto &v : wm) v = v ? 1 : 0;
+        auto *d = static_cast<uint8_t *>(dalloc(h, n));
+        if (!d || hipMemcpy(d, wm.data(), n, hipMemcpyHostToDevice) != hipSuccess) {
+            h->err = "upload field_mask";
+            return false;
+        }
+        A.wave_mask = d;
+    }
     A.lut_pre = p->enable_exchange ? (T)(2.0 / (kMu0 * ms)) : (T)0;
 
     // state buffers
@@ -466,11 +540,13 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p) {
     }
     const long long nparts = fm::num_partials(A);
     A.partials = static_cast<double *>(dalloc(h, (size_t)nparts * fm::kStatSlots * sizeof(double)));
-    if (!A.partials) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
+    A.counter = static_cast<unsigned *>(dalloc(h, 64));
+    if (!A.partials || !A.counter) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
+    if (!hip_ok(h, hipMemset(A.counter, 0, 64), "clear counter")) return false;
     return upload_state<T>(h, p->initial_magnetization_xyz, nullptr, 0);
 }
 
-bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p) {
+bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_plan_ext *x) {
     if (select_device(h) != FULLMAG_FDM_OK) return false;
     const auto &g = p->grid;
     if (g.nx == 0 || g.ny == 0 || g.nz == 0) return (h->err = "grid dimensions must be positive", false);
@@ -496,7 +572,9 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p) {
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) return false;
     if (!hip_ok(h, hipHostMalloc((void **)&h->ctl_h, sizeof(fm::Ctl), hipHostMallocDefault), "pinned ctl"))
         return false;
-    const bool ok = h->fp64 ? create_typed<double>(h, p) : create_typed<float>(h, p);
+    if (x && x->struct_size < sizeof(fullmag_fdm_plan_ext))
+        return (h->err = "fullmag_fdm_plan_ext.struct_size too small", false);
+    const bool ok = h->fp64 ? create_typed<double>(h, p, x) : create_typed<float>(h, p, x);
     if (!ok) return false;
     std::memset(h->ctl_h, 0, sizeof(fm::Ctl));
     h->ctl_h->dt = 1e-13;
@@ -591,11 +669,16 @@ int fullmag_fdm_is_available(void) {
 }
 
 fullmag_fdm_backend *fullmag_fdm_backend_create(const fullmag_fdm_plan_desc *plan) {
+    return fullmag_fdm_backend_create_ext(plan, nullptr);
+}
+
+fullmag_fdm_backend *fullmag_fdm_backend_create_ext(const fullmag_fdm_plan_desc *plan,
+                                                    const fullmag_fdm_plan_ext *ext) {
     if (!plan) return nullptr;
     Backend *h = new (std::nothrow) Backend();
     if (!h) return nullptr;
     try {
-        if (!create_impl(h, plan) && h->err.empty()) h->err = "create failed";
+        if (!create_impl(h, plan, ext) && h->err.empty()) h->err = "create failed";
     } catch (const std::bad_alloc &) {
         h->err = "host allocation failed";
     } catch (...) {
@@ -628,14 +711,14 @@ int fullmag_fdm_backend_step(fullmag_fdm_backend *h, double dt, fullmag_fdm_step
             // replaced by the controller, so an interrupt leaves m untouched
             bool interrupted = false;
             const bool ok = with_args(h, [&](auto &A) {
-                for (int s = 2; s <= h->nstage; ++s) {
-                    if (!hip_ok(h, fm::enqueue_eval(A, s, s == h->nstage ? 1 : 0, h->stream), "launch")) return false;
+                for (int s = 2; s < h->nstage; ++s) {
+                    if (!hip_ok(h, fm::enqueue_eval(A, s, 0, h->stream), "launch")) return false;
                     if (h->poll(h->poll_user)) {
                         interrupted = true;
                         return true;
                     }
                 }
-                return hip_ok(h, fm::enqueue_control(A, 0, h->stream), "launch (control)");
+                return hip_ok(h, fm::enqueue_eval(A, h->nstage, 1, h->stream), "launch (final stage)");
             });
             if (!ok) return FULLMAG_FDM_ERR_CUDA;
             if (interrupted) {

@@ -40,6 +40,7 @@ enum StatSlot {
 
 struct FftDesc {
     int n;
+    int lg;       // log2(n) when n is a power of two (radices 2/4/8 only), else -1
     int npass;
     int radix[kMaxPasses];
 };
@@ -61,8 +62,12 @@ struct Method {
     int nerr;
     int eidx[kMaxStages];
     double e[kMaxStages];
+    double node[kMaxStages + 1];   // stage s is evaluated at t + node[s]*dt
     StageRow row[kMaxStages + 1];  // row[s] for s = 2..nstage
 };
+
+// Time-dependent Zeeman waveforms f(t) (the field is H0 + f(t) H1).
+enum Wave { WAVE_NONE = 0, WAVE_RAMP = 1, WAVE_SINC = 2, WAVE_SINE = 3 };
 
 // Device-resident control block.
 struct Ctl {
@@ -108,6 +113,7 @@ struct Args {
     const T *lut;           // 256 x 256 A_ij when region != NULL
     Ctl *ctl;
     double *partials;       // [nblocks][kStatSlots]
+    unsigned *counter;      // finished work-groups of the current partial-producing launch
     // observables (written only when non-NULL)
     T *obs_hex, *obs_hd, *obs_heff;
     // physics
@@ -119,11 +125,23 @@ struct Args {
     int has_ext;
     int has_uni;
     T uni_c1, uni_c2;       // 2Ku1/(mu0 Ms), 4Ku2/(mu0 Ms)
+    T ku1_u, ku2_u;         // uniform Ku1, Ku2 (J/m^3)
     T axis[3];
     T gamma_bar, alpha;
     int precession;
     int demag;
     double e_pre;           // mu0*Ms*V (energy prefactor)
+    // per-cell maps (NULL = the uniform value above)
+    const T *ms_cell;       // Ms per cell
+    const T *ku1_cell, *ku2_cell;
+    const T *axis_cell;     // 3 x ncell SoA, unit vectors
+    T two_a_mu0;            // 2A/mu0 (exchange prefactor numerator)
+    T two_mu0inv;           // 2/mu0
+    double mu0_vol;         // mu0*V
+    // time-dependent Zeeman term f(t) H1 on the cells of wave_mask
+    int wave_kind;
+    double wave_amp[3], wave_t0, wave_t1, wave_freq, wave_phase;
+    const uint8_t *wave_mask;
     // step control
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
     int method;             // index into the constant method table

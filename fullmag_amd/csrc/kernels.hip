@@ -146,18 +146,32 @@ struct Bfly<T, 5, INV> {
 };
 
 // One Stockham pass (autosort, out of place) with a compile-time radix.
-// Sequence sq occupies [sq*n, (sq+1)*n) of src and dst.
-template <class T, int R, bool INV>
-__device__ __forceinline__ void fft_pass(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst, int n, int ns,
-                                         int nseq, const C2<T> *__restrict__ tw, bool first, int n_in, bool last,
-                                         int n_out) {
-    const int nr = n / R;
-    const int tstep = n / (ns * R);
+// Sequence sq occupies [sq*n, (sq+1)*n) of src and dst; tw is the LDS copy
+// of the forward twiddle table exp(-2 pi i k/n), k < n.  With POW2 every
+// index split is a shift/mask (lg_* are the base-2 logarithms).
+template <class T, int R, bool INV, bool POW2>
+__device__ __forceinline__ void fft_pass(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst, int n, int lg_n,
+                                         int ns, int lg_ns, int nseq, const C2<T> *__restrict__ tw, bool first,
+                                         int n_in, bool last, int n_out) {
+    constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : 0;
+    const int nr = POW2 ? n >> LR : n / R;
+    const int lg_nr = lg_n - LR;
+    const int tstep = POW2 ? n >> (lg_ns + LR) : n / (ns * R);
     const int total = nr * nseq;
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
-        const int sq = b / nr;
-        const int j = b - sq * nr;
-        const int k = j % ns;
+        int sq, j, k, base;
+        if (POW2) {
+            sq = b >> lg_nr;
+            j = b & (nr - 1);
+            k = j & (ns - 1);
+            base = ((j >> lg_ns) << (lg_ns + LR)) + k;
+        } else {
+            sq = b / nr;
+            j = b - sq * nr;
+            const int q = j / ns;
+            k = j - q * ns;
+            base = q * ns * R + k;
+        }
         const C2<T> *s = src + sq * n;
         C2<T> v[R];
 #pragma unroll
@@ -166,15 +180,15 @@ __device__ __forceinline__ void fft_pass(const C2<T> *__restrict__ src, C2<T> *_
             v[r] = (first && idx >= n_in) ? C2<T>{T(0), T(0)} : s[idx];
         }
         if (ns > 1) {
+            const int kt = k * tstep;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                C2<T> w = tw[k * r * tstep];
+                C2<T> w = tw[kt * r];
                 if (INV) w.y = -w.y;
                 v[r] = cmul(v[r], w);
             }
         }
         Bfly<T, R, INV>::run(v);
-        const int base = (j / ns) * ns * R + k;
         C2<T> *d = dst + sq * n;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -222,31 +236,50 @@ __device__ void fft_pass_generic(const C2<T> *__restrict__ src, C2<T> *__restric
     }
 }
 
+// Copies the forward twiddle table of length n into LDS (no barrier: the
+// caller's next __syncthreads publishes it).
+template <class T>
+__device__ __forceinline__ void load_twiddles(C2<T> *lds, const void *gtw, int n) {
+    const C2<T> *g = reinterpret_cast<const C2<T> *>(gtw);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = g[i];
+}
+
 // Batched FFT of `nseq` sequences of length fd.n held in LDS buffer `a`
 // (ping-pong with `b`).  Only the first n_in inputs are read (the rest are
 // taken as zero) and only the first n_out outputs are written.  Returns the
-// buffer that holds the result.  All threads of the block must call it.
+// buffer that holds the result.  All threads of the block must call it and
+// `tw` (LDS) must have been published by a barrier.
 template <class T, bool INV>
 __device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, const FftDesc &fd, const C2<T> *__restrict__ tw, int nseq, int n_in,
                           int n_out) {
     const int n = fd.n;
-    int ns = 1;
+    const bool pow2 = fd.lg >= 0;
+    int ns = 1, lg_ns = 0;
     for (int p = 0; p < fd.npass; ++p) {
         const int R = fd.radix[p];
         const bool first = p == 0, last = p == fd.npass - 1;
-        switch (R) {
-            case 2: fft_pass<T, 2, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out); break;
-            case 3: fft_pass<T, 3, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out); break;
-            case 4: fft_pass<T, 4, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out); break;
-            case 5: fft_pass<T, 5, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out); break;
-            case 8: fft_pass<T, 8, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out); break;
-            default: fft_pass_generic<T, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out, R); break;
+        if (pow2) {
+            switch (R) {
+                case 2: fft_pass<T, 2, INV, true>(a, b, n, fd.lg, ns, lg_ns, nseq, tw, first, n_in, last, n_out); break;
+                case 4: fft_pass<T, 4, INV, true>(a, b, n, fd.lg, ns, lg_ns, nseq, tw, first, n_in, last, n_out); break;
+                default: fft_pass<T, 8, INV, true>(a, b, n, fd.lg, ns, lg_ns, nseq, tw, first, n_in, last, n_out); break;
+            }
+        } else {
+            switch (R) {
+                case 2: fft_pass<T, 2, INV, false>(a, b, n, 0, ns, 0, nseq, tw, first, n_in, last, n_out); break;
+                case 3: fft_pass<T, 3, INV, false>(a, b, n, 0, ns, 0, nseq, tw, first, n_in, last, n_out); break;
+                case 4: fft_pass<T, 4, INV, false>(a, b, n, 0, ns, 0, nseq, tw, first, n_in, last, n_out); break;
+                case 5: fft_pass<T, 5, INV, false>(a, b, n, 0, ns, 0, nseq, tw, first, n_in, last, n_out); break;
+                case 8: fft_pass<T, 8, INV, false>(a, b, n, 0, ns, 0, nseq, tw, first, n_in, last, n_out); break;
+                default: fft_pass_generic<T, INV>(a, b, n, ns, nseq, tw, first, n_in, last, n_out, R); break;
+            }
         }
         __syncthreads();
         C2<T> *t = a;
         a = b;
         b = t;
         ns *= R;
+        lg_ns += R == 2 ? 1 : R == 4 ? 2 : 3;
     }
     return a;
 }
@@ -352,11 +385,34 @@ __device__ void block_reduce_store(Acc &a, double *out) {
     }
 }
 
+// Waveform value f(t) of the time-dependent Zeeman term.
+template <class T>
+__device__ __forceinline__ T wave_factor(const Args<T> &A, double t) {
+    switch (A.wave_kind) {
+        case WAVE_RAMP: {
+            const double u = (t - A.wave_t0) / (A.wave_t1 - A.wave_t0);
+            return T(u < 0.0 ? 0.0 : u > 1.0 ? 1.0 : u);
+        }
+        case WAVE_SINC: {
+            const double x = 2.0 * 3.14159265358979323846 * A.wave_freq * (t - A.wave_t0);
+            return T(x == 0.0 ? 1.0 : sin(x) / x);
+        }
+        case WAVE_SINE: return T(sin(2.0 * 3.14159265358979323846 * A.wave_freq * t + A.wave_phase));
+        default: return T(0);
+    }
+}
+
+// Evaluation time of stage s of the trial step from the control block.
+template <class T>
+__device__ __forceinline__ double stage_time(const Args<T> &A, const Method &M, int s, const Ctl *ctl, double dt) {
+    return s <= 1 ? ctl->t : ctl->t + M.node[s] * dt;
+}
+
 // Local field terms, LLG torque, and (optionally) statistics for one cell.
 template <class T, bool LUT>
 __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, int s, int p, double dt, long long cell,
                                            int x, int y, int z, const T *__restrict__ yin, T *__restrict__ kout,
-                                           const T hd[3], bool stats, bool err, Acc &acc) {
+                                           const T hd[3], T wf, bool stats, bool err, Acc &acc) {
     const long long N = A.ncell;
     const bool act = A.mask ? A.mask[cell] != 0 : true;
     if (!act) {
@@ -371,6 +427,7 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
         return;
     }
     const T m[3] = {yin[cell], yin[N + cell], yin[2 * N + cell]};
+    const T ms_i = A.ms_cell ? A.ms_cell[cell] : A.ms;
     // neighbours: Neumann clamp at the box, inactive neighbour -> centre
     const long long sx = 1, sy = A.nx, sz = (long long)A.nx * A.ny;
     long long nb[6] = {x > 0 ? cell - sx : cell, x < A.nx - 1 ? cell + sx : cell,
@@ -389,7 +446,7 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
             const T mc = m[c];
             const T lap = (f[nb[1]] - T(2) * mc + f[nb[0]]) * A.idx2 + (f[nb[3]] - T(2) * mc + f[nb[2]]) * A.idy2 +
                           (f[nb[5]] - T(2) * mc + f[nb[4]]) * A.idz2;
-            hex[c] = A.ex_pre * lap;
+            hex[c] = (A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre) * lap;
         }
     } else {
         const uint32_t rc = A.region[cell];
@@ -403,7 +460,7 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
             T acc2 = T(0);
 #pragma unroll
             for (int q = 0; q < 6; ++q) acc2 += aij[q] * (f[nb[q]] - m[c]) * w[q];
-            hex[c] = A.lut_pre * acc2;
+            hex[c] = (A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre) * acc2;
         }
     }
     T hext[3] = {T(0), T(0), T(0)};
@@ -412,13 +469,31 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
         hext[1] = A.hext[1];
         hext[2] = A.hext[2];
     }
+    if (A.wave_kind && (!A.wave_mask || A.wave_mask[cell])) {
+        hext[0] += wf * T(A.wave_amp[0]);
+        hext[1] += wf * T(A.wave_amp[1]);
+        hext[2] += wf * T(A.wave_amp[2]);
+    }
     T hani[3] = {T(0), T(0), T(0)};
     if (A.has_uni) {
-        const T mu = m[0] * A.axis[0] + m[1] * A.axis[1] + m[2] * A.axis[2];
-        const T coef = A.uni_c1 * mu + A.uni_c2 * mu * mu * mu;
-        hani[0] = A.axis[0] * coef;
-        hani[1] = A.axis[1] * coef;
-        hani[2] = A.axis[2] * coef;
+        T u[3] = {A.axis[0], A.axis[1], A.axis[2]};
+        if (A.axis_cell) {
+            u[0] = A.axis_cell[cell];
+            u[1] = A.axis_cell[N + cell];
+            u[2] = A.axis_cell[2 * N + cell];
+        }
+        T c1 = A.uni_c1, c2 = A.uni_c2;
+        if (A.ku1_cell || A.ku2_cell || A.ms_cell) {
+            const T k1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
+            const T k2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
+            c1 = A.two_mu0inv * k1 / ms_i;
+            c2 = T(2) * A.two_mu0inv * k2 / ms_i;
+        }
+        const T mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+        const T coef = c1 * mu + c2 * mu * mu * mu;
+        hani[0] = u[0] * coef;
+        hani[1] = u[1] * coef;
+        hani[2] = u[2] * coef;
     }
     T h[3];
 #pragma unroll
@@ -446,10 +521,11 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
         auto sq = [](const T *f) {
             return (double)f[0] * (double)f[0] + (double)f[1] * (double)f[1] + (double)f[2] * (double)f[2];
         };
-        acc.v[ST_EEX] += -0.5 * A.e_pre * dotd(hex);
-        acc.v[ST_EDEMAG] += -0.5 * A.e_pre * dotd(hd);
-        acc.v[ST_EEXT] += -1.0 * A.e_pre * dotd(hext);
-        acc.v[ST_EANI] += -0.5 * A.e_pre * dotd(hani);
+        const double ep = A.ms_cell ? A.mu0_vol * (double)ms_i : A.e_pre;
+        acc.v[ST_EEX] += -0.5 * ep * dotd(hex);
+        acc.v[ST_EDEMAG] += -0.5 * ep * dotd(hd);
+        acc.v[ST_EEXT] += -1.0 * ep * dotd(hext);
+        acc.v[ST_EANI] += -0.5 * ep * dotd(hani);
         acc.v[ST_HEFF2] = fmax(acc.v[ST_HEFF2], sq(h));
         acc.v[ST_HD2] = fmax(acc.v[ST_HD2], sq(hd));
         acc.v[ST_RHS2] = fmax(acc.v[ST_RHS2], sq(k));
@@ -493,8 +569,10 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
     const Method &M = c_methods[A.method];
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
     const int rows = A.rows_per_wg, nx = A.nx, px = A.px, nl = 3 * rows;
     const long long r0 = (long long)blockIdx.x * rows;
+    load_twiddles<T>(tw, A.twx, px);
     for (int i = threadIdx.x; i < rows * nx; i += blockDim.x) {
         const int rr = i / nx, x = i - rr * nx;
         const long long row = r0 + rr;
@@ -504,12 +582,12 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
             T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
-            e[l & 1] = A.ms * v[c];
+            e[l & 1] = (A.ms_cell && row < A.nrows ? A.ms_cell[row * nx + x] : A.ms) * v[c];
         }
         if ((nl & 1) && rr == rows - 1) b0[(nl >> 1) * px + x].y = T(0);
     }
     __syncthreads();
-    const C2<T> *res = lds_fft<T, false>(b0, b1, A.fx, reinterpret_cast<const C2<T> *>(A.twx), A.nseq_x, nx, px);
+    const C2<T> *res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
     const int nkx = A.nkx;
     for (int i = threadIdx.x; i < nl * nkx; i += blockDim.x) {
@@ -537,10 +615,11 @@ __global__ void __launch_bounds__(kThreads) k_y(Args<T> A) {
     const int py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + 3 * py;
+    C2<T> *tw = b1 + 3 * py;
     const int kx = MODE == 0 ? blockIdx.x : blockIdx.x % nkx;
     const int zz = MODE == 0 ? 0 : blockIdx.x / nkx;
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
-    const C2<T> *tw = reinterpret_cast<const C2<T> *>(A.twy);
+    load_twiddles<T>(tw, A.twy, py);
     const int nload = MODE == 2 ? py : ny;
     for (int i = threadIdx.x; i < 3 * nload; i += blockDim.x) {
         const int c = i / nload, yy = i - c * nload;
@@ -588,8 +667,9 @@ __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
     const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x % ntile) * TY;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + 3 * TY * pz;
+    C2<T> *tw = b1 + 3 * TY * pz;
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
-    const C2<T> *tw = reinterpret_cast<const C2<T> *>(A.twz);
+    load_twiddles<T>(tw, A.twz, pz);
     for (int i = threadIdx.x; i < 3 * nz * TY; i += blockDim.x) {
         const int t = i % TY, rest = i / TY;
         const int zz = rest % nz, c = rest / nz;
@@ -630,7 +710,7 @@ __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
 }
 
 template <class T, bool LUT>
-__global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats) {
+__global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats, int control) {
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -638,8 +718,10 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats)
     const Method &M = c_methods[A.method];
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
     const int rows = A.rows_per_wg, nx = A.nx, px = A.px, nl = 3 * rows, nkx = A.nkx;
     const long long r0 = (long long)blockIdx.x * rows;
+    load_twiddles<T>(tw, A.twx, px);
     const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
     auto line = [&](int l, int kk) -> C2<T> {
         if (l >= nl) return C2<T>{T(0), T(0)};
@@ -661,11 +743,12 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats)
         b0[i] = {a.x - b.y, a.y + b.x};
     }
     __syncthreads();
-    const C2<T> *res = lds_fft<T, true>(b0, b1, A.fx, reinterpret_cast<const C2<T> *>(A.twx), A.nseq_x, px, nx);
+    const C2<T> *res = lds_fft<T, true>(b0, b1, A.fx, tw, A.nseq_x, px, nx);
     const T *yin = stage_input(A, M, s, p);
     T *kout = stage_rhs(A, M, s, p);
     const bool err = M.adaptive && s == M.nstage;
     const T scale = T(A.inv_p);
+    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
     Acc acc;
     acc.init();
     for (int i = threadIdx.x; i < rows * nx; i += blockDim.x) {
@@ -680,9 +763,10 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats)
             hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
         }
         const int yy = (int)(row % A.ny), zz = (int)(row / A.ny);
-        local_cell<T, LUT>(A, M, s, p, dt, row * nx + x, x, yy, zz, yin, kout, hd, stats != 0, err, acc);
+        local_cell<T, LUT>(A, M, s, p, dt, row * nx + x, x, yy, zz, yin, kout, hd, wf, stats != 0, err, acc);
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
+    if (control) finish_last_block(A, control);
 }
 
 template <class T>
@@ -697,7 +781,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
 }
 
 template <class T, bool LUT>
-__global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats) {
+__global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats, int control) {
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -712,23 +796,29 @@ __global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats)
         const int y = (int)((cell / A.nx) % A.ny);
         const int z = (int)(cell / ((long long)A.nx * A.ny));
         const T hd[3] = {T(0), T(0), T(0)};
-        local_cell<T, LUT>(A, M, s, p, dt, cell, x, y, z, stage_input(A, M, s, p), stage_rhs(A, M, s, p), hd,
+        const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+        local_cell<T, LUT>(A, M, s, p, dt, cell, x, y, z, stage_input(A, M, s, p), stage_rhs(A, M, s, p), hd, wf,
                            stats != 0, err, acc);
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
+    if (control) finish_last_block(A, control);
 }
 
-// observe != 0: only reduce statistics (no step control).
+// Reduction of all per-block partials and the step controller, executed by
+// one work-group.  observe != 0: only store the statistics.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_control(Args<T> A, int nparts, int observe) {
+__device__ void control_block(const Args<T> &A, int nparts, int observe) {
     Ctl *c = A.ctl;
-    if (c->done) return;
     Acc acc;
     acc.init();
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
         const double *q = A.partials + (long long)i * kStatSlots;
 #pragma unroll
-        for (int k = 0; k < kStatSlots; ++k) acc.v[k] = slot_is_max(k) ? fmax(acc.v[k], q[k]) : acc.v[k] + q[k];
+        for (int k = 0; k < kStatSlots; ++k) {
+            // agent-scope loads: the partials were written by other CUs
+            const double v = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc.v[k] = slot_is_max(k) ? fmax(acc.v[k], v) : acc.v[k] + v;
+        }
     }
     __shared__ double tot[kStatSlots];
     block_reduce_store(acc, tot);
@@ -778,6 +868,34 @@ __global__ void __launch_bounds__(kThreads) k_control(Args<T> A, int nparts, int
     }
 }
 
+// Stand-alone controller launch (used for timing and by callers that keep
+// the partial-producing kernel and the controller apart).
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_control(Args<T> A, int nparts, int observe) {
+    if (A.ctl->done) return;
+    control_block(A, nparts, observe);
+}
+
+// End of a partial-producing kernel: the last work-group to finish reduces
+// the partials of the whole grid and runs the controller (control 1) or
+// stores the statistics (control 2), then re-arms the counter for the next
+// launch.  Every work-group of the grid must reach this point.
+template <class T>
+__device__ void finish_last_block(const Args<T> &A, int control) {
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(A.counter, 1u);
+        is_last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    control_block(A, (int)gridDim.x, control == 2);
+    if (threadIdx.x == 0) atomicExch(A.counter, 0u);
+}
+
 // ------------------------------------------------------------ host side --
 static int g_methods_ready = -1;
 
@@ -799,12 +917,16 @@ static void build_methods(Method *out) {
     Method &h = out[M_HEUN];
     h.nstage = 3;
     h.adaptive = 0;
+    h.node[2] = 1.0;
+    h.node[3] = 1.0;
     row(h, 2, {1}, {1.0}, 1.0);
     row(h, 3, {1, 2}, {1.0, 1.0}, 0.5);
     // Dormand-Prince 5(4) (fdm_integrators.rs:441-645)
     Method &d = out[M_DP45];
     d.nstage = 7;
     d.adaptive = 1;
+    const double dnode[8] = {0.0, 0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+    for (int i = 0; i < 8; ++i) d.node[i] = dnode[i];
     row(d, 2, {1}, {1.0}, 1.0 / 5.0);
     row(d, 3, {1, 2}, {3.0 / 40.0, 9.0 / 40.0}, 1.0);
     row(d, 4, {1, 2, 3}, {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0}, 1.0);
@@ -824,6 +946,10 @@ static void build_methods(Method *out) {
     Method &r = out[M_RK4];
     r.nstage = 5;
     r.adaptive = 0;
+    r.node[2] = 0.5;
+    r.node[3] = 0.5;
+    r.node[4] = 1.0;
+    r.node[5] = 1.0;
     row(r, 2, {1}, {1.0}, 0.5);
     row(r, 3, {2}, {1.0}, 0.5);
     row(r, 4, {3}, {1.0}, 1.0);
@@ -849,11 +975,11 @@ int method_nstage(int method) {
 }
 
 template <class T>
-static size_t lds_x(const Args<T> &A) { return 2ull * A.nseq_x * A.px * sizeof(C2<T>); }
+static size_t lds_x(const Args<T> &A) { return (2ull * A.nseq_x + 1) * A.px * sizeof(C2<T>); }
 template <class T>
-static size_t lds_y(const Args<T> &A) { return 2ull * 3 * A.py * sizeof(C2<T>); }
+static size_t lds_y(const Args<T> &A) { return (2ull * 3 + 1) * A.py * sizeof(C2<T>); }
 template <class T>
-static size_t lds_z(const Args<T> &A) { return 2ull * 3 * A.ty * A.pz * sizeof(C2<T>); }
+static size_t lds_z(const Args<T> &A) { return (2ull * 3 * A.ty + 1) * A.pz * sizeof(C2<T>); }
 
 template <class T>
 static hipError_t set_lds(const void *fn, size_t bytes) {
@@ -904,9 +1030,12 @@ long long num_partials(const Args<T> &A) {
     return (A.ncell + kThreads - 1) / kThreads;
 }
 
-// One RHS evaluation of stage s (1 = the current state m0).
+// One RHS evaluation of stage s (1 = the current state m0).  control:
+// 0 none, 1 step controller, 2 statistics only (fused into the last
+// work-group of the final kernel; implies per-block statistics).
 template <class T>
-hipError_t enqueue_eval(const Args<T> &A, int s, int stats, hipStream_t st) {
+hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
+    const int stats = control != 0;
     const unsigned gpart = (unsigned)num_partials(A);
     if (A.demag) {
         const size_t lx = lds_x(A), ly = lds_y(A);
@@ -919,12 +1048,12 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int stats, hipStream_t st) {
             hipLaunchKernelGGL(k_z<T>, dim3(gz), dim3(kThreads), lds_z(A), st, A);
             hipLaunchKernelGGL((k_y<T, 2>), dim3(A.nkx * A.nz), dim3(kThreads), ly, st, A);
         }
-        if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats);
-        else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats);
+        if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats, control);
+        else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats, control);
     } else {
         if (s >= 2) hipLaunchKernelGGL(k_stage<T>, dim3(gpart), dim3(kThreads), 0, st, A, s);
-        if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats);
-        else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats);
+        if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
+        else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
     }
     return hipGetLastError();
 }
@@ -976,11 +1105,11 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             break;
         case 4:
             if (A.demag) {
-                if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1);
-                else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1);
+                if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1, 0);
+                else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1, 0);
             } else {
-                if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1);
-                else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1);
+                if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
+                else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
             }
             break;
         case 5:

@@ -13,8 +13,40 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 1
+#define FULLMAG_FDM_EXT_VERSION 2
 int fullmag_fdm_ext_version(void);
+
+/* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
+ * selected by field_mask (all magnetic cells when NULL). */
+#define FULLMAG_FDM_WAVE_NONE 0
+#define FULLMAG_FDM_WAVE_RAMP 1  /* f = clamp((t - t0)/(t1 - t0), 0, 1)            */
+#define FULLMAG_FDM_WAVE_SINC 2  /* f = sin(x)/x, x = 2 pi frequency (t - t0)       */
+#define FULLMAG_FDM_WAVE_SINE 3  /* f = sin(2 pi frequency t + phase)               */
+
+/*
+ * Plan extension (no counterpart in the reference ABI).  struct_size must be
+ * sizeof(fullmag_fdm_plan_ext); fields added later go at the end.  Every
+ * pointer is optional (NULL = off) and only read during create.
+ */
+typedef struct {
+    uint32_t      struct_size;
+    const double *ms_field;               /* f64[cell_count]: per-cell Ms (A/m)          */
+    const double *anisotropy_axis_field;  /* f64[3*cell_count] AoS: per-cell uniaxial axis */
+    uint32_t      field_waveform;         /* FULLMAG_FDM_WAVE_*                          */
+    double        field_amplitude_am[3];  /* H1 (A/m)                                    */
+    double        field_t0;               /* s */
+    double        field_t1;               /* s (ramp end)                                */
+    double        field_frequency;        /* Hz */
+    double        field_phase;            /* rad */
+    const uint8_t *field_mask;            /* u8[cell_count]                              */
+    double        adaptive_rtol;          /* as fullmag_fdm_backend_set_adaptive_ext     */
+    double        adaptive_growth_limit;
+    double        adaptive_shrink_limit;
+} fullmag_fdm_plan_ext;
+
+/* create() with the extension struct (ext may be NULL: same as create()). */
+fullmag_fdm_backend *fullmag_fdm_backend_create_ext(const fullmag_fdm_plan_desc *plan,
+                                                    const fullmag_fdm_plan_ext *ext);
 
 /*
  * Device-resident runner loop: repeat accepted steps until time t_end or

@@ -331,6 +331,16 @@ typedef struct {
     double ku1, ku2, axis[3];
     const uint8_t *mask;  // NULL = all active
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
+    // Extensions without a reference counterpart (engine's plan extension):
+    // per-cell Ms, Ku1, Ku2, anisotropy axis (NULL = uniform value above) and
+    // a time-dependent Zeeman term H1 f(t) on the cells of wave_mask
+    // (NULL = all active cells); f: 0 none, 1 ramp clamp((t-t0)/(t1-t0),0,1),
+    // 2 sinc sin(x)/x with x = 2 pi freq (t-t0), 3 sin(2 pi freq t + phase).
+    const double *ms_cell, *ku1_cell, *ku2_cell, *axis_cell;
+    int wave_kind;
+    double wave_amp[3];
+    double wave_t0, wave_t1, wave_freq, wave_phase;
+    const uint8_t *wave_mask;
 } orc_desc;
 
 typedef struct {
@@ -349,6 +359,9 @@ struct Problem {
     size_t n;
     std::vector<uint8_t> mask;
     bool has_mask;
+    std::vector<double> ms_c, ku1_c, ku2_c, axis_c;
+    std::vector<uint8_t> wmask;
+    double t_eval = 0.0;  // time at which the Zeeman waveform is evaluated
     // demag workspace (fdm_fft.rs:64-131)
     int px = 0, py = 0, pz = 0;
     size_t P = 0;
@@ -362,6 +375,21 @@ struct Problem {
     std::vector<V3> kfsal;
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
+    double ms_at(size_t i) const { return ms_c.empty() ? d.ms : ms_c[i]; }
+    double wave(double t) const {
+        switch (d.wave_kind) {
+            case 1: {
+                const double u = (t - d.wave_t0) / (d.wave_t1 - d.wave_t0);
+                return u < 0.0 ? 0.0 : u > 1.0 ? 1.0 : u;
+            }
+            case 2: {
+                const double x = 2.0 * PI * d.wave_freq * (t - d.wave_t0);
+                return x == 0.0 ? 1.0 : std::sin(x) / x;
+            }
+            case 3: return std::sin(2.0 * PI * d.wave_freq * t + d.wave_phase);
+            default: return 0.0;
+        }
+    }
 };
 
 void demag_add(Problem &p, const V3 *m, V3 *h) {
@@ -369,13 +397,13 @@ void demag_add(Problem &p, const V3 *m, V3 *h) {
     std::fill(p.bmx.begin(), p.bmx.end(), cplx(0.0));
     std::fill(p.bmy.begin(), p.bmy.end(), cplx(0.0));
     std::fill(p.bmz.begin(), p.bmz.end(), cplx(0.0));
-    const double ms = p.d.ms;
     for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 const size_t s = ((size_t)z * ny + y) * nx + x;
                 const size_t t = ((size_t)z * p.py + y) * p.px + x;
                 if (!p.active(s)) continue;
+                const double ms = p.ms_at(s);
                 p.bmx[t] = m[s].c[0] * ms;
                 p.bmy[t] = m[s].c[1] * ms;
                 p.bmz[t] = m[s].c[2] * ms;
@@ -406,12 +434,12 @@ void demag_add(Problem &p, const V3 *m, V3 *h) {
 }
 
 void exchange_add(const Problem &p, const V3 *m, V3 *h) {
-    const double pre = 2.0 * p.d.a_ex / (MU0 * p.d.ms);
     const double dx2 = p.d.dx * p.d.dx, dy2 = p.d.dy * p.d.dy, dz2 = p.d.dz * p.d.dz;
     const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) {
         if (!p.active(s)) continue;
+        const double pre = 2.0 * p.d.a_ex / (MU0 * p.ms_at(s));
         const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
         const V3 c = m[s];
         auto sample = [&](long xi, long yi, long zi) {
@@ -429,23 +457,34 @@ void exchange_add(const Problem &p, const V3 *m, V3 *h) {
 }
 
 void ext_add(const Problem &p, V3 *h) {
-    if (!p.d.has_ext) return;
+    const double f = p.d.wave_kind ? p.wave(p.t_eval) : 0.0;
+    if (!p.d.has_ext && !p.d.wave_kind) return;
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s)
         if (p.active(s))
-            for (int k = 0; k < 3; ++k) h[s].c[k] += p.d.h_ext[k];
+            for (int k = 0; k < 3; ++k) {
+                if (p.d.has_ext) h[s].c[k] += p.d.h_ext[k];
+                if (p.d.wave_kind && (p.wmask.empty() || p.wmask[s])) h[s].c[k] += f * p.d.wave_amp[k];
+            }
 }
 
 void uni_add(const Problem &p, const V3 *m, V3 *h) {
     if (!p.d.has_uni) return;
-    const double ms = std::max(p.d.ms, 1e-30);
-    V3 u = {{p.d.axis[0], p.d.axis[1], p.d.axis[2]}};
-    u = v_scale(u, 1.0 / std::max(v_norm(u), 1e-30));
+    V3 u0 = {{p.d.axis[0], p.d.axis[1], p.d.axis[2]}};
+    u0 = v_scale(u0, 1.0 / std::max(v_norm(u0), 1e-30));
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) {
         if (!p.active(s)) continue;
+        const double ms = std::max(p.ms_at(s), 1e-30);
+        V3 u = u0;
+        if (!p.axis_c.empty()) {
+            u = {{p.axis_c[3 * s], p.axis_c[3 * s + 1], p.axis_c[3 * s + 2]}};
+            u = v_scale(u, 1.0 / std::max(v_norm(u), 1e-30));
+        }
+        const double k1 = p.ku1_c.empty() ? p.d.ku1 : p.ku1_c[s];
+        const double k2 = p.ku2_c.empty() ? p.d.ku2 : p.ku2_c[s];
         const double mu = v_dot(m[s], u);
-        const double coeff = 2.0 * p.d.ku1 / (MU0 * ms) * mu + 4.0 * p.d.ku2 / (MU0 * ms) * mu * mu * mu;
+        const double coeff = 2.0 * k1 / (MU0 * ms) * mu + 4.0 * k2 / (MU0 * ms) * mu * mu * mu;
         for (int k = 0; k < 3; ++k) h[s].c[k] += u.c[k] * coeff;
     }
 }
@@ -484,7 +523,7 @@ double max_norm(const V3 *v, size_t n) {
 double energy_field_form(const Problem &p, const V3 *m, const V3 *h, double f) {
     const double vol = p.d.dx * p.d.dy * p.d.dz;
     double e = 0.0;
-    for (size_t s = 0; s < p.n; ++s) e += f * MU0 * p.d.ms * v_dot(m[s], h[s]) * vol;
+    for (size_t s = 0; s < p.n; ++s) e += f * MU0 * p.ms_at(s) * v_dot(m[s], h[s]) * vol;
     return e;
 }
 
@@ -507,7 +546,7 @@ void observables(Problem &p, const V3 *m, orc_report *r) {
         r->max_hdemag = max_norm(p.hs.data(), n);
         for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
     }
-    if (p.d.has_ext) {
+    if (p.d.has_ext || p.d.wave_kind) {
         std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
         ext_add(p, p.hs.data());
         r->e_ext = energy_field_form(p, m, p.hs.data(), -1.0);
@@ -594,6 +633,13 @@ void *orc_create(const orc_desc *d) {
     p->has_mask = d->mask != nullptr;
     if (p->has_mask) p->mask.assign(d->mask, d->mask + p->n);
     p->d.mask = nullptr;
+    if (d->ms_cell) p->ms_c.assign(d->ms_cell, d->ms_cell + p->n);
+    if (d->ku1_cell) p->ku1_c.assign(d->ku1_cell, d->ku1_cell + p->n);
+    if (d->ku2_cell) p->ku2_c.assign(d->ku2_cell, d->ku2_cell + p->n);
+    if (d->axis_cell) p->axis_c.assign(d->axis_cell, d->axis_cell + 3 * p->n);
+    if (d->wave_mask) p->wmask.assign(d->wave_mask, d->wave_mask + p->n);
+    p->d.ms_cell = p->d.ku1_cell = p->d.ku2_cell = p->d.axis_cell = nullptr;
+    p->d.wave_mask = nullptr;
     const size_t n = p->n;
     p->m0.resize(n);
     p->ms.resize(n);
@@ -715,17 +761,24 @@ int orc_prepare_state(void *h, double *m) {
 
 void orc_invalidate_fsal(void *h) { static_cast<Problem *>(h)->fsal_valid = false; }
 
+// Time at which orc_fields / orc_rhs / orc_observables evaluate the waveform.
+void orc_set_time(void *h, double t) { static_cast<Problem *>(h)->t_eval = t; }
+
 // fdm_integrators.rs:84-150
 int orc_heun_step(void *h, double *m, double *t, double dt, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
     const size_t n = p->n;
     V3 *mm = reinterpret_cast<V3 *>(m);
     std::copy(mm, mm + n, p->m0.begin());
+    const double t0 = *t;
+    p->t_eval = t0;
     rhs_eval(*p, p->m0.data(), p->k[0].data());
     for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(p->k[0][s], dt)));
+    p->t_eval = t0 + dt;
     rhs_eval(*p, p->ms.data(), p->k[1].data());
     for (size_t s = 0; s < n; ++s) mm[s] = v_normalized(v_add(p->m0[s], v_scale(v_add(p->k[0][s], p->k[1][s]), 0.5 * dt)));
     *t += dt;
+    p->t_eval = *t;
     observables(*p, mm, rep);
     rep->time = *t;
     rep->dt_used = dt;
@@ -742,12 +795,16 @@ int orc_rk4_step(void *h, double *m, double *t, double dt, orc_report *rep) {
     V3 *mm = reinterpret_cast<V3 *>(m);
     std::copy(mm, mm + n, p->m0.begin());
     auto &K = p->k;
+    const double t0 = *t;
+    p->t_eval = t0;
     rhs_eval(*p, p->m0.data(), K[0].data());
     for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[0][s], 0.5 * dt)));
+    p->t_eval = t0 + 0.5 * dt;
     rhs_eval(*p, p->ms.data(), K[1].data());
     for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[1][s], 0.5 * dt)));
     rhs_eval(*p, p->ms.data(), K[2].data());
     for (size_t s = 0; s < n; ++s) p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(K[2][s], dt)));
+    p->t_eval = t0 + dt;
     rhs_eval(*p, p->ms.data(), K[3].data());
     const double dt6 = dt / 6.0;
     for (size_t s = 0; s < n; ++s)
@@ -784,7 +841,9 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
     const V3 *m0 = p->m0.data();
     V3 *st = p->ms.data();
     uint64_t rejected = 0;
+    const double t0 = *t;
     for (;;) {
+        p->t_eval = t0;
         if (p->fsal_valid) {
             std::copy(p->kfsal.begin(), p->kfsal.end(), K[0].begin());
             p->fsal_valid = false;
@@ -793,21 +852,25 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
         }
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s) st[s] = v_normalized(v_add(m0[s], v_scale(K[0][s], A21 * dt)));
+        p->t_eval = t0 + (1.0 / 5.0) * dt;
         rhs_eval(*p, st, K[1].data());
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s)
             st[s] = v_normalized(v_add(m0[s], v_scale(v_add(v_scale(K[0][s], A31), v_scale(K[1][s], A32)), dt)));
+        p->t_eval = t0 + (3.0 / 10.0) * dt;
         rhs_eval(*p, st, K[2].data());
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s)
             st[s] = v_normalized(v_add(
                 m0[s], v_scale(v_add(v_add(v_scale(K[0][s], A41), v_scale(K[1][s], A42)), v_scale(K[2][s], A43)), dt)));
+        p->t_eval = t0 + (4.0 / 5.0) * dt;
         rhs_eval(*p, st, K[3].data());
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s)
             st[s] = v_normalized(v_add(m0[s], v_scale(v_add(v_add(v_scale(K[0][s], A51), v_scale(K[1][s], A52)),
                                                             v_add(v_scale(K[2][s], A53), v_scale(K[3][s], A54))),
                                                       dt)));
+        p->t_eval = t0 + (8.0 / 9.0) * dt;
         rhs_eval(*p, st, K[4].data());
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s)
@@ -816,6 +879,7 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
                 v_scale(v_add(v_add(v_add(v_scale(K[0][s], A61), v_scale(K[1][s], A62)), v_scale(K[2][s], A63)),
                               v_add(v_scale(K[3][s], A64), v_scale(K[4][s], A65))),
                         dt)));
+        p->t_eval = t0 + 1.0 * dt;
         rhs_eval(*p, st, K[5].data());
 #pragma omp parallel for schedule(static)
         for (long s = 0; s < (long)n; ++s)
@@ -823,6 +887,7 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
                 m0[s], v_scale(v_add(v_add(v_add(v_scale(K[0][s], B1), v_scale(K[2][s], B3)), v_scale(K[3][s], B4)),
                                      v_add(v_scale(K[4][s], B5), v_scale(K[5][s], B6))),
                                dt)));
+        p->t_eval = t0 + 1.0 * dt;
         rhs_eval(*p, st, K[6].data());
         // error norm
         const bool use_rtol = c.rtol > 0.0;
@@ -847,6 +912,7 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
             p->fsal_valid = true;
             const double ratio = std::max(std::min(c.headroom * std::pow(thr / std::max(err, 1e-30), 0.2), c.growth), c.shrink);
             const double dt_next = std::min(std::max(dt * ratio, c.dt_min), c.dt_max);
+            p->t_eval = *t;
             observables(*p, mm, rep);
             rep->time = *t;
             rep->dt_used = dt;

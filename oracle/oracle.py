@@ -43,6 +43,11 @@ class _Desc(ctypes.Structure):
         ("atol", ctypes.c_double), ("rtol", ctypes.c_double), ("dt_min", ctypes.c_double),
         ("dt_max", ctypes.c_double), ("headroom", ctypes.c_double), ("growth", ctypes.c_double),
         ("shrink", ctypes.c_double),
+        ("ms_cell", ctypes.POINTER(ctypes.c_double)), ("ku1_cell", ctypes.POINTER(ctypes.c_double)),
+        ("ku2_cell", ctypes.POINTER(ctypes.c_double)), ("axis_cell", ctypes.POINTER(ctypes.c_double)),
+        ("wave_kind", ctypes.c_int), ("wave_amp", ctypes.c_double * 3),
+        ("wave_t0", ctypes.c_double), ("wave_t1", ctypes.c_double), ("wave_freq", ctypes.c_double),
+        ("wave_phase", ctypes.c_double), ("wave_mask", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 
@@ -86,6 +91,7 @@ def lib():
         L.orc_observables.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(Report)]
         L.orc_prepare_state.argtypes = [ctypes.c_void_p, _dp]
         L.orc_invalidate_fsal.argtypes = [ctypes.c_void_p]
+        L.orc_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.POINTER(Report)]
         L.orc_run.restype = ctypes.c_uint64
@@ -173,6 +179,18 @@ class Problem:
     headroom: float = 0.8
     growth: float = float("inf")
     shrink: float = 0.0
+    # extensions (no reference counterpart): per-cell maps and a Zeeman waveform
+    ms_cell: np.ndarray | None = None
+    ku1_cell: np.ndarray | None = None
+    ku2_cell: np.ndarray | None = None
+    axis_cell: np.ndarray | None = None
+    wave_kind: int = 0
+    wave_amp: tuple = (0.0, 0.0, 0.0)
+    wave_t0: float = 0.0
+    wave_t1: float = 0.0
+    wave_freq: float = 0.0
+    wave_phase: float = 0.0
+    wave_mask: np.ndarray | None = None
 
 
 class Oracle:
@@ -196,6 +214,20 @@ class Oracle:
             d.mask = self._mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         d.atol, d.rtol, d.dt_min, d.dt_max = pb.atol, pb.rtol, pb.dt_min, pb.dt_max
         d.headroom, d.growth, d.shrink = pb.headroom, pb.growth, pb.shrink
+        self._keep = []
+        for nm in ("ms_cell", "ku1_cell", "ku2_cell", "axis_cell"):
+            a = getattr(pb, nm)
+            if a is not None:
+                a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+                self._keep.append(a)
+                setattr(d, nm, _ptr(a))
+        d.wave_kind = pb.wave_kind
+        d.wave_amp[:] = list(pb.wave_amp)
+        d.wave_t0, d.wave_t1, d.wave_freq, d.wave_phase = pb.wave_t0, pb.wave_t1, pb.wave_freq, pb.wave_phase
+        if pb.wave_mask is not None:
+            wm = np.ascontiguousarray(pb.wave_mask, dtype=np.uint8).ravel()
+            self._keep.append(wm)
+            d.wave_mask = wm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
         self.pb = pb
         self.n = pb.nx * pb.ny * pb.nz
         self.h = L.orc_create(ctypes.byref(d))
@@ -249,6 +281,10 @@ class Oracle:
 
     def invalidate_fsal(self):
         lib().orc_invalidate_fsal(self.h)
+
+    def set_time(self, t):
+        """Time at which fields()/rhs()/observables() evaluate the waveform."""
+        lib().orc_set_time(self.h, t)
 
     def step(self, integrator, m, t, dt):
         """One step in place on `m` (n,3 float64 C-contiguous). Returns (t, report)."""

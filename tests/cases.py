@@ -34,6 +34,17 @@ class Case:
     dt_min: float = 1e-18
     dt_max: float = 1e-10
     headroom: float = 0.8
+    ms_cell: np.ndarray | None = None
+    ku1_cell: np.ndarray | None = None
+    ku2_cell: np.ndarray | None = None
+    axis_cell: np.ndarray | None = None
+    wave_kind: int = 0
+    wave_amp: tuple = (0.0, 0.0, 0.0)
+    wave_t0: float = 0.0
+    wave_t1: float = 0.0
+    wave_freq: float = 0.0
+    wave_phase: float = 0.0
+    wave_mask: np.ndarray | None = None
 
     @property
     def n(self):
@@ -46,7 +57,10 @@ class Case:
                        integrator=integrator, exchange=self.exchange, demag=self.demag,
                        precession=self.precession, h_ext=self.h_ext, ku1=self.ku1, ku2=self.ku2, axis=self.axis,
                        mask=self.mask, atol=self.atol, dt_min=self.dt_min, dt_max=self.dt_max,
-                       headroom=self.headroom, **kw)
+                       headroom=self.headroom, ku1_field=self.ku1_cell, ku2_field=self.ku2_cell,
+                       ms_field=self.ms_cell, axis_field=self.axis_cell, wave=self.wave_kind, wave_amp=self.wave_amp,
+                       wave_t0=self.wave_t0, wave_t1=self.wave_t1, wave_freq=self.wave_freq,
+                       wave_phase=self.wave_phase, wave_mask=self.wave_mask, rtol=self.rtol, **kw)
 
     def problem(self):
         from oracle.oracle import Problem
@@ -54,7 +68,10 @@ class Case:
                        a_ex=self.a_ex, alpha=self.alpha, gamma=self.gamma, precession=self.precession,
                        exchange=self.exchange, demag=self.demag, h_ext=self.h_ext, ku1=self.ku1, ku2=self.ku2,
                        axis=self.axis, mask=self.mask, atol=self.atol, rtol=self.rtol, dt_min=self.dt_min,
-                       dt_max=self.dt_max, headroom=self.headroom)
+                       dt_max=self.dt_max, headroom=self.headroom, ms_cell=self.ms_cell, ku1_cell=self.ku1_cell,
+                       ku2_cell=self.ku2_cell, axis_cell=self.axis_cell, wave_kind=self.wave_kind,
+                       wave_amp=self.wave_amp, wave_t0=self.wave_t0, wave_t1=self.wave_t1, wave_freq=self.wave_freq,
+                       wave_phase=self.wave_phase, wave_mask=self.wave_mask)
 
 
 def masked_3x3(demag: bool) -> Case:

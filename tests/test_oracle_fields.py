@@ -152,3 +152,35 @@ def test_pure_damping_has_no_precession():
     k = o.rhs(m)[0]
     assert abs(k[1]) < 1e-20            # no azimuthal component
     assert k[2] > 0 and k[0] < 0        # relaxes towards +z
+
+
+def test_extension_maps_reduce_to_scalar_problem():
+    # per-cell Ms/Ku/axis maps filled with the uniform values and a waveform
+    # that is identically 1 reproduce the scalar problem exactly
+    n = 5 * 4 * 3
+    kw = dict(dx=2e-9, dy=3e-9, dz=4e-9, ms=8e5, a_ex=1.3e-11, alpha=0.1, ku1=3e5, ku2=2e4, axis=(0.0, 0.6, 0.8))
+    base = O.Oracle(O.Problem(5, 4, 3, h_ext=(1e3, 2e3, -5e3), **kw))
+    mapped = O.Oracle(O.Problem(5, 4, 3, ms_cell=np.full(n, 8e5), ku1_cell=np.full(n, 3e5),
+                                ku2_cell=np.full(n, 2e4), axis_cell=np.tile([0.0, 0.6, 0.8], (n, 1)),
+                                wave_kind=1, wave_amp=(1e3, 2e3, -5e3), wave_t0=-2.0, wave_t1=-1.0, **kw))
+    m = base.prepare(O.random_unit_vectors(8, n))
+    fa, fb = base.fields(m), mapped.fields(m)
+    for k in ("h_ex", "h_demag", "h_ext", "h_ani", "h_eff"):
+        assert np.abs(fa[k] - fb[k]).max() <= 1e-12 * max(1.0, np.abs(fa[k]).max())
+    assert abs(base.observables(m)["e_total"] - mapped.observables(m)["e_total"]) < 1e-12 * abs(
+        base.observables(m)["e_total"])
+
+
+def test_extension_waveforms_closed_forms():
+    # single cell, no internal fields: H_ext = f(t) * amp with f from the
+    # waveform definitions (ramp clamp, sinc(0) = 1, sine phase)
+    amp = (0.0, 0.0, 1e4)
+    cases = [(1, dict(wave_t0=1e-12, wave_t1=3e-12), [(0.0, 0.0), (2e-12, 0.5), (5e-12, 1.0)]),
+             (2, dict(wave_t0=1e-12, wave_freq=2e10), [(1e-12, 1.0), (1e-12 + 0.25 / 2e10, np.sin(np.pi / 2) / (np.pi / 2))]),
+             (3, dict(wave_freq=1e9, wave_phase=0.5), [(0.0, np.sin(0.5)), (2.5e-10, np.sin(np.pi / 2 + 0.5))])]
+    for kind, kw, pts in cases:
+        o = O.Oracle(O.Problem(1, 1, 1, 1e-9, 1e-9, 1e-9, 8e5, 1.3e-11, 0.1, exchange=False, demag=False,
+                               wave_kind=kind, wave_amp=amp, **kw))
+        for t, f in pts:
+            o.set_time(t)
+            assert abs(o.fields(np.array([[1.0, 0.0, 0.0]]))["h_ext"][0, 2] - f * 1e4) < 1e-9
