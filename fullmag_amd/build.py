@@ -19,7 +19,14 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB_NAME = "libfullmag_fdm.so"
 LIB_PATH = os.path.join(OUT_DIR, LIB_NAME)
 SONAME = "libfullmag_fdm.so.0"
-SOURCES = ["kernels.hip", "api.cpp", "newell_host.cpp"]
+# (source, object stem, extra flags): the device code is compiled once per
+# storage precision; the FP32 unit also holds the precision-free host helpers
+SOURCES = [
+    ("kernels.hip", "kernels_f32", ["-DFM_REAL=float", "-DFM_PRIMARY"]),
+    ("kernels.hip", "kernels_f64", ["-DFM_REAL=double"]),
+    ("api.cpp", "api", []),
+    ("newell_host.cpp", "newell_host", []),
+]
 ARCH = os.environ.get("FULLMAG_OFFLOAD_ARCH", "gfx950")
 
 
@@ -36,10 +43,11 @@ def build(verbose: bool = False) -> str:
     objs = []
     common = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
               "-Wno-unused-result", "-I" + os.path.join(HERE, "..", "include")]
-    for src in SOURCES:
+    procs = []
+    for src, stem, extra in SOURCES:
         path = os.path.join(CSRC, src)
-        obj = os.path.join(OUT_DIR, src.replace(".", "_") + ".o")
-        cmd = [hipcc()] + common
+        obj = os.path.join(OUT_DIR, stem + ".o")
+        cmd = [hipcc()] + common + extra
         if src.endswith(".hip"):
             # packed FP32 instructions disabled: plain scalar FP32 only
             cmd += ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
@@ -50,8 +58,11 @@ def build(verbose: bool = False) -> str:
         cmd += ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((subprocess.Popen(cmd), cmd))
         objs.append(obj)
+    failed = [cmd for p, cmd in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB_PATH + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + [
         f"-Wl,-soname,{SONAME}", "-lpthread"]

@@ -27,8 +27,10 @@
 #include "engine.hpp"
 
 namespace fm {
+template <class T>
 hipError_t upload_methods();
 int method_nstage(int method);
+int method_stage_nk(int method, int s);
 template <class T>
 const char *configure(Args<T> &A, size_t lds_limit);
 template <class T>
@@ -36,9 +38,12 @@ long long num_partials(const Args<T> &A);
 template <class T>
 hipError_t enqueue_eval(const Args<T> &A, int s, int stats, hipStream_t st);
 template <class T>
+hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st);
+template <class T>
 hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st);
 template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
+bool yfast_factors(int py, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
 bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz, std::vector<double> &out,
                           std::string *err);
@@ -248,9 +253,7 @@ bool eval_current(Backend *h, bool observables) {
 
 bool enqueue_attempt(Backend *h) {
     return with_args(h, [&](auto &A) {
-        for (int s = 2; s <= h->nstage; ++s)
-            if (!hip_ok(h, fm::enqueue_eval(A, s, s == h->nstage ? 1 : 0, h->stream), "launch (stage)")) return false;
-        return true;
+        return hip_ok(h, fm::enqueue_trial(A, h->nstage, h->stream), "launch (trial step)");
     });
 }
 
@@ -386,6 +389,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
                : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4 ? fm::M_RK4
                                                                : fm::M_HEUN;
     h->nstage = fm::method_nstage(A.method);
+    for (int s = 0; s <= fm::kMaxStages; ++s) A.stage_nk[s] = fm::method_stage_nk(A.method, s);
     h->adaptive = A.method == fm::M_DP45;
 
     // geometry
@@ -490,7 +494,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     // state buffers
     A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));
     A.k_all = static_cast<T *>(dalloc(h, 7 * 3 * n * sizeof(T)));
-    A.ybuf = static_cast<T *>(dalloc(h, 3 * n * sizeof(T)));
+    A.ybuf = static_cast<T *>(dalloc(h, 6 * n * sizeof(T)));  // two stage-state buffers
     for (int i = 0; i < 3; ++i) h->obs_base[i] = dalloc(h, 3 * n * sizeof(T));
     h->ctl_d = static_cast<fm::Ctl *>(dalloc(h, sizeof(fm::Ctl)));
     if (!A.m_all || !A.k_all || !A.ybuf || !h->obs_base[0] || !h->obs_base[1] || !h->obs_base[2] || !h->ctl_d)
@@ -528,6 +532,25 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.twx = upload_twiddles<T>(h, A.px);
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
+        auto four_step_twiddles = [&](int n, const void **dst) -> bool {
+            int r1 = 0, r2 = 0;
+            if (!fm::yfast_factors(n, &r1, &r2)) return true;
+            // W_n^{t1 k2} stored at [k2*r1 + t1]
+            std::vector<T> t4(2 * (size_t)n);
+            for (int k2 = 0; k2 < r2; ++k2)
+                for (int t1 = 0; t1 < r1; ++t1) {
+                    const double a = 2.0 * 3.14159265358979323846 * (double)((long)t1 * k2) / (double)n;
+                    t4[2 * ((size_t)k2 * r1 + t1)] = (T)std::cos(a);
+                    t4[2 * ((size_t)k2 * r1 + t1) + 1] = (T)(-std::sin(a));
+                }
+            void *d = dalloc(h, t4.size() * sizeof(T));
+            if (!d || hipMemcpy(d, t4.data(), t4.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+                return hip_ok(h, hipErrorOutOfMemory, "four-step twiddles");
+            *dst = d;
+            return true;
+        };
+        if (nz == 1 && !four_step_twiddles(A.py, &A.tw4y)) return false;
+        if (!four_step_twiddles(A.px, &A.tw4x)) return false;
         if (!A.spec || !A.twx || !A.twy || !A.twz) return hip_ok(h, hipErrorOutOfMemory, "alloc spectrum");
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)
@@ -563,12 +586,16 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
     h->ny = g.ny;
     h->nz = g.nz;
     h->n = (uint64_t)g.nx * g.ny * g.nz;
+    // per-vector offsets (3 n) and half-spectrum offsets (~6 n complex) are 32-bit in the kernels
+    if ((uint64_t)(g.nx + 1) * g.ny * g.nz * 8 >= (1ull << 31))
+        return (h->err = "grid too large: (nx+1)*ny*nz must stay below 2^28 cells", false);
     if (!p->initial_magnetization_xyz || p->initial_magnetization_len != 3 * h->n)
         return (h->err = "initial_magnetization_len mismatch: expected " + std::to_string(3 * h->n) + ", got " +
                          std::to_string(p->initial_magnetization_len),
                 false);
     h->fp64 = p->precision == FULLMAG_FDM_PRECISION_DOUBLE;
-    if (!hip_ok(h, fm::upload_methods(), "method table")) return false;
+    if (!hip_ok(h, h->fp64 ? fm::upload_methods<double>() : fm::upload_methods<float>(), "method table"))
+        return false;
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) return false;
     if (!hip_ok(h, hipHostMalloc((void **)&h->ctl_h, sizeof(fm::Ctl), hipHostMallocDefault), "pinned ctl"))
         return false;
@@ -1161,7 +1188,7 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     int rc = FULLMAG_FDM_OK;
-    for (int kind = 0; kind < 6 && rc == FULLMAG_FDM_OK; ++kind) {
+    for (int kind = 0; kind < 7 && rc == FULLMAG_FDM_OK; ++kind) {
         if (with_args(h, [&](auto &A) { return fm::enqueue_kind(A, kind, 2, h->stream); }) != hipSuccess) continue;
         hipEventRecord(e0, h->stream);
         for (int r = 0; r < reps; ++r) with_args(h, [&](auto &A) { return fm::enqueue_kind(A, kind, 2, h->stream); });

@@ -21,10 +21,13 @@
 namespace fm {
 
 constexpr int kThreads = 256;       // work-group size of every kernel
+constexpr int kMaxRowsWg = 4;       // x-rows owned by one x-pass work-group
 constexpr int kMaxPasses = 24;      // FFT passes per length
 constexpr int kMaxRadix = 64;       // largest prime factor of a padded length
 constexpr int kStatSlots = 8;       // per-block partial reductions
 constexpr int kMaxStages = 8;
+constexpr int kMaxK = 5;            // k-vectors in one stage combination (DP45 rows 6, 7)
+constexpr int kMaxErr = 6;          // terms of the embedded error estimate (DP45)
 
 // Per-block partial layout (doubles).
 enum StatSlot {
@@ -108,6 +111,10 @@ struct Args {
     void *spec;             // complex<T>
     const T *ntens;
     const void *twx, *twy, *twz;   // complex<T> forward twiddles
+    const void *tw4y;       // four-step twiddles W_py^{t1 k2} at [k2*R1 + t1] (thin-film fast y pass)
+    int yfast;              // 0 = Stockham y pass, else py of the register four-step y pass
+    const void *tw4x;       // four-step twiddles of the x passes (W_px^{t1 k2} at [k2*R1 + t1])
+    int xfast;              // 0 = Stockham x passes, else px of the register four-step x passes
     const uint8_t *mask;    // NULL = all active
     const uint32_t *region; // NULL = uniform exchange
     const T *lut;           // 256 x 256 A_ij when region != NULL
@@ -145,6 +152,7 @@ struct Args {
     // step control
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
     int method;             // index into the constant method table
+    int stage_nk[kMaxStages + 1];  // k-vectors combined by stage s (0 for s <= 1)
     FftDesc fx, fy, fz;
 };
 

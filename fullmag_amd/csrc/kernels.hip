@@ -33,6 +33,10 @@
 #include <cstdio>
 #include "engine.hpp"
 
+#ifndef FM_REAL
+#error "compile once per storage precision with -DFM_REAL=float or -DFM_REAL=double"
+#endif
+
 namespace fm {
 
 template <class T>
@@ -54,7 +58,9 @@ __device__ __forceinline__ C2<T> mul_mi(C2<T> a) {
     return INV ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x};
 }
 
-__constant__ Method c_methods[M_COUNT];
+// Each storage precision is its own translation unit (FM_REAL = float or
+// double) with its own copy of the method table.
+static __constant__ Method c_methods[M_COUNT];
 
 extern __shared__ __align__(16) unsigned char fm_smem[];
 
@@ -145,6 +151,10 @@ struct Bfly<T, 5, INV> {
     }
 };
 
+}  // namespace fm
+#include "regfft.hpp"
+namespace fm {
+
 // One Stockham pass (autosort, out of place) with a compile-time radix.
 // Sequence sq occupies [sq*n, (sq+1)*n) of src and dst; tw is the LDS copy
 // of the forward twiddle table exp(-2 pi i k/n), k < n.  With POW2 every
@@ -198,7 +208,9 @@ __device__ __forceinline__ void fft_pass(const C2<T> *__restrict__ src, C2<T> *_
     }
 }
 
-// Generic odd prime radix (R <= kMaxRadix), O(R^2) DFT per butterfly.
+// Generic odd prime radix (R <= kMaxRadix), O(R^2) DFT per butterfly.  The
+// inputs are re-read from LDS for every output (no per-thread array, hence no
+// scratch memory in any kernel that links the FFT).
 template <class T, bool INV>
 __device__ void fft_pass_generic(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst, int n, int ns, int nseq,
                                  const C2<T> *__restrict__ tw, bool first, int n_in, bool last, int n_out, int R) {
@@ -210,25 +222,22 @@ __device__ void fft_pass_generic(const C2<T> *__restrict__ src, C2<T> *__restric
         const int sq = b / nr;
         const int j = b - sq * nr;
         const int k = j % ns;
-        const C2<T> *s = src + sq * n;
-        C2<T> v[kMaxRadix];
-        for (int r = 0; r < R; ++r) {
-            const int idx = j + r * nr;
-            v[r] = (first && idx >= n_in) ? C2<T>{T(0), T(0)} : s[idx];
-            if (ns > 1 && r > 0) {
-                C2<T> w = tw[k * r * tstep];
-                if (INV) w.y = -w.y;
-                v[r] = cmul(v[r], w);
-            }
-        }
+        const C2<T> *sp = src + sq * n;
         const int base = (j / ns) * ns * R + k;
         C2<T> *d = dst + sq * n;
         for (int o = 0; o < R; ++o) {
             C2<T> acc = {T(0), T(0)};
             for (int q = 0; q < R; ++q) {
+                const int idx = j + q * nr;
+                C2<T> v = (first && idx >= n_in) ? C2<T>{T(0), T(0)} : sp[idx];
+                if (ns > 1 && q > 0) {
+                    C2<T> w = tw[k * q * tstep];
+                    if (INV) w.y = -w.y;
+                    v = cmul(v, w);
+                }
                 C2<T> w = tw[((q * o) % R) * wstep];
                 if (INV) w.y = -w.y;
-                acc = cadd(acc, cmul(v[q], w));
+                acc = cadd(acc, cmul(v, w));
             }
             const int oi = base + o * ns;
             if (!last || oi < n_out) d[oi] = acc;
@@ -289,18 +298,31 @@ __device__ __forceinline__ int kslot(const Method &M, int j, int p) {
     return j == 1 ? p : (j == M.nstage ? 1 - p : j);
 }
 
+// Intermediate stage states alternate between two buffers, so that a fused
+// kernel can write y_{s+1} while other work-groups still read y_s.
+template <class T>
+__device__ __forceinline__ T *stage_buf(const Args<T> &A, int s) {
+    return A.ybuf + (size_t)(s & 1) * 3 * (size_t)A.ncell;
+}
+
 template <class T>
 __device__ __forceinline__ const T *stage_input(const Args<T> &A, const Method &M, int s, int p) {
-    const long long n3 = 3 * A.ncell;
+    const size_t n3 = 3 * (size_t)A.ncell;
     if (s <= 1) return A.m_all + p * n3;
     if (s == M.nstage) return A.m_all + (1 - p) * n3;
-    return A.ybuf;
+    return stage_buf(A, s);
+}
+
+// Where stage s writes its state (s >= 2).
+template <class T>
+__device__ __forceinline__ T *stage_output(const Args<T> &A, const Method &M, int s, int p) {
+    return s == M.nstage ? A.m_all + (size_t)(1 - p) * 3 * (size_t)A.ncell : stage_buf(A, s);
 }
 
 template <class T>
 __device__ __forceinline__ T *stage_rhs(const Args<T> &A, const Method &M, int s, int p) {
     const int slot = s <= 1 ? p : kslot(M, s, p);
-    return A.k_all + slot * 3 * A.ncell;
+    return A.k_all + (size_t)slot * 3 * (size_t)A.ncell;
 }
 
 // y_s at `cell` (s >= 2 combines and stores; s <= 1 reads m0).
@@ -341,7 +363,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     } else {
         x = y = z = 0.0;
     }
-    T *out = s == M.nstage ? A.m_all + (1 - p) * 3 * N : A.ybuf;
+    T *out = stage_output(A, M, s, p);
     out[cell] = (T)x;
     out[N + cell] = (T)y;
     out[2 * N + cell] = (T)z;
@@ -408,14 +430,74 @@ __device__ __forceinline__ double stage_time(const Args<T> &A, const Method &M, 
     return s <= 1 ? ctl->t : ctl->t + M.node[s] * dt;
 }
 
+// Everything the local pass reads for one cell.  The pass is bound by memory
+// latency, so the loads of a thread's cells are issued together (gather) and
+// only then consumed (compute); no load depends on another load's value
+// except the optional region-table lookup.
+template <class T>
+struct CellIn {
+    T m[3];           // state at the cell
+    T v[6][3];        // box-clamped neighbour states (-x,+x,-y,+y,-z,+z)
+    T ms, ku1, ku2, ax[3];
+    T ke[kMaxErr][3]; // k-vectors of the error estimate (final adaptive stage)
+    T m0[3];          // state at the start of the step (mixed tolerance)
+    unsigned char act, nact[6], wm;
+    uint32_t rc, rn[6];
+};
+
+template <class T, bool LUT>
+__device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, int p, int cell, int x, int y, int z,
+                                            const T *__restrict__ yin, bool err, CellIn<T> &g) {
+    const int N = (int)A.ncell;
+    const int sy = A.nx, sz = A.nx * A.ny;
+    const int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
+                             y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
+                             z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
+    g.act = A.mask ? A.mask[cell] : 1;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g.nact[q] = A.mask ? A.mask[nb[q]] : 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        g.m[c] = yin[c * N + cell];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) g.v[q][c] = yin[c * N + nb[q]];
+    }
+    g.ms = A.ms_cell ? A.ms_cell[cell] : A.ms;
+    if (A.has_uni) {
+        g.ku1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
+        g.ku2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.ax[c] = A.axis_cell ? A.axis_cell[c * N + cell] : A.axis[c];
+    }
+    g.wm = A.wave_mask ? A.wave_mask[cell] : 1;
+    if (LUT) {
+        g.rc = A.region[cell];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) g.rn[q] = A.region[nb[q]];
+    }
+    if (err) {
+#pragma unroll
+        for (int q = 0; q < kMaxErr; ++q) {
+            if (q >= M.nerr || M.eidx[q] == M.nstage) continue;
+            const T *kk = A.k_all + (size_t)kslot(M, M.eidx[q], p) * 3 * N;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g.ke[q][c] = kk[c * N + cell];
+        }
+        if (A.rtol > 0.0) {
+            const T *m0 = A.m_all + (size_t)p * 3 * N;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g.m0[c] = m0[c * N + cell];
+        }
+    }
+}
+
 // Local field terms, LLG torque, and (optionally) statistics for one cell.
 template <class T, bool LUT>
-__device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, int s, int p, double dt, long long cell,
-                                           int x, int y, int z, const T *__restrict__ yin, T *__restrict__ kout,
-                                           const T hd[3], T wf, bool stats, bool err, Acc &acc) {
-    const long long N = A.ncell;
-    const bool act = A.mask ? A.mask[cell] != 0 : true;
-    if (!act) {
+__device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
+                                             const CellIn<T> &g, T *__restrict__ kout, const T hd[3], T wf,
+                                             bool stats, bool err, Acc &acc) {
+    const int N = (int)A.ncell;
+    if (!g.act) {
         kout[cell] = T(0);
         kout[N + cell] = T(0);
         kout[2 * N + cell] = T(0);
@@ -426,41 +508,33 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
         }
         return;
     }
-    const T m[3] = {yin[cell], yin[N + cell], yin[2 * N + cell]};
-    const T ms_i = A.ms_cell ? A.ms_cell[cell] : A.ms;
-    // neighbours: Neumann clamp at the box, inactive neighbour -> centre
-    const long long sx = 1, sy = A.nx, sz = (long long)A.nx * A.ny;
-    long long nb[6] = {x > 0 ? cell - sx : cell, x < A.nx - 1 ? cell + sx : cell,
-                       y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
-                       z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
-    if (A.mask) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (A.mask[nb[q]] == 0) nb[q] = cell;
-    }
+    const T *m = g.m;
+    const T ms_i = g.ms;
     T hex[3];
     if (!LUT) {
+        const T pre = A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T *f = yin + c * N;
+            T w[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) w[q] = g.nact[q] ? g.v[q][c] : m[c];
             const T mc = m[c];
-            const T lap = (f[nb[1]] - T(2) * mc + f[nb[0]]) * A.idx2 + (f[nb[3]] - T(2) * mc + f[nb[2]]) * A.idy2 +
-                          (f[nb[5]] - T(2) * mc + f[nb[4]]) * A.idz2;
-            hex[c] = (A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre) * lap;
+            const T lap = (w[1] - T(2) * mc + w[0]) * A.idx2 + (w[3] - T(2) * mc + w[2]) * A.idy2 +
+                          (w[5] - T(2) * mc + w[4]) * A.idz2;
+            hex[c] = pre * lap;
         }
     } else {
-        const uint32_t rc = A.region[cell];
         T aij[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) aij[q] = A.lut[rc * 256u + A.region[nb[q]]];
-        const T w[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
+        for (int q = 0; q < 6; ++q) aij[q] = A.lut[g.rc * 256u + g.rn[q]];
+        const T ww[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
+        const T pre = A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const T *f = yin + c * N;
             T acc2 = T(0);
 #pragma unroll
-            for (int q = 0; q < 6; ++q) acc2 += aij[q] * (f[nb[q]] - m[c]) * w[q];
-            hex[c] = (A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre) * acc2;
+            for (int q = 0; q < 6; ++q) acc2 += aij[q] * ((g.nact[q] ? g.v[q][c] : m[c]) - m[c]) * ww[q];
+            hex[c] = pre * acc2;
         }
     }
     T hext[3] = {T(0), T(0), T(0)};
@@ -469,31 +543,23 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
         hext[1] = A.hext[1];
         hext[2] = A.hext[2];
     }
-    if (A.wave_kind && (!A.wave_mask || A.wave_mask[cell])) {
+    if (A.wave_kind && g.wm) {
         hext[0] += wf * T(A.wave_amp[0]);
         hext[1] += wf * T(A.wave_amp[1]);
         hext[2] += wf * T(A.wave_amp[2]);
     }
     T hani[3] = {T(0), T(0), T(0)};
     if (A.has_uni) {
-        T u[3] = {A.axis[0], A.axis[1], A.axis[2]};
-        if (A.axis_cell) {
-            u[0] = A.axis_cell[cell];
-            u[1] = A.axis_cell[N + cell];
-            u[2] = A.axis_cell[2 * N + cell];
-        }
         T c1 = A.uni_c1, c2 = A.uni_c2;
         if (A.ku1_cell || A.ku2_cell || A.ms_cell) {
-            const T k1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
-            const T k2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
-            c1 = A.two_mu0inv * k1 / ms_i;
-            c2 = T(2) * A.two_mu0inv * k2 / ms_i;
+            c1 = A.two_mu0inv * g.ku1 / ms_i;
+            c2 = T(2) * A.two_mu0inv * g.ku2 / ms_i;
         }
-        const T mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+        const T mu = m[0] * g.ax[0] + m[1] * g.ax[1] + m[2] * g.ax[2];
         const T coef = c1 * mu + c2 * mu * mu * mu;
-        hani[0] = u[0] * coef;
-        hani[1] = u[1] * coef;
-        hani[2] = u[2] * coef;
+        hani[0] = g.ax[0] * coef;
+        hani[1] = g.ax[1] * coef;
+        hani[2] = g.ax[2] * coef;
     }
     T h[3];
 #pragma unroll
@@ -532,35 +598,393 @@ __device__ __forceinline__ void local_cell(const Args<T> &A, const Method &M, in
     }
     if (err) {
         double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-        for (int q = 0; q < M.nerr; ++q) {
-            const int j = M.eidx[q];
+#pragma unroll
+        for (int q = 0; q < kMaxErr; ++q) {
+            if (q >= M.nerr) continue;
             const double w = M.e[q];
-            if (j == M.nstage) {
-                e0 += w * (double)k[0];
-                e1 += w * (double)k[1];
-                e2 += w * (double)k[2];
-            } else {
-                const T *kk = A.k_all + kslot(M, j, p) * 3 * N;
-                e0 += w * (double)kk[cell];
-                e1 += w * (double)kk[N + cell];
-                e2 += w * (double)kk[2 * N + cell];
-            }
+            const T *kq = M.eidx[q] == M.nstage ? k : g.ke[q];
+            e0 += w * (double)kq[0];
+            e1 += w * (double)kq[1];
+            e2 += w * (double)kq[2];
         }
         e0 *= dt;
         e1 *= dt;
         e2 *= dt;
         double v = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
         if (A.rtol > 0.0) {
-            const T *m0 = A.m_all + p * 3 * N;
-            const double a = (double)m0[cell], b = (double)m0[N + cell], c = (double)m0[2 * N + cell];
+            const double a = (double)g.m0[0], b = (double)g.m0[1], c = (double)g.m0[2];
             v /= (A.atol + A.rtol * fmax(sqrt(a * a + b * b + c * c), 1e-30));
         }
         acc.v[ST_ERR] = fmax(acc.v[ST_ERR], v);
     }
 }
 
-// ---------------------------------------------------------------- kernels --
+// Cells per thread whose loads are in flight together.
 template <class T>
+struct Chunk {
+    static constexpr int value = sizeof(T) == 4 ? 4 : 2;
+};
+
+// Logical work-group index that keeps consecutive logical groups on one XCD:
+// the hardware deals work-groups round-robin over the 8 XCDs (each with its
+// own L2), so neighbouring x-lines, which share spectrum cache lines, would
+// otherwise be fetched by 8 different L2s.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned g) {
+    const unsigned x = b & 7u, j = b >> 3;
+    const unsigned q = g >> 3, r = g & 7u;
+    return x * q + (x < r ? x : r) + j;
+}
+
+// Register four-step FFT of `nseq` sequences of length N = R1*R2 held in
+// natural order in LDS `buf` (stride N), in place.  Thread t1 < R1 of a
+// sequence holds x[t1 + R1 n2] (inputs n >= n_in are zero), does the
+// R2-point DFT in VGPRs, applies W_N^{t1 k2} (tw4[k2*R1 + t1]) and writes a
+// padded transpose to `tb`; thread k2 < R2 then does the R1-point DFT and
+// writes X[k2 + R2 k1] back to `buf`.  `tb` holds nseq*TB elements.  All
+// threads of the block must call it (it contains barriers).
+template <int R1, int R2>
+struct FourStep {
+    static constexpr int TS = R1 > R2 ? R1 : R2;
+    static constexpr int N = R1 * R2;
+    static constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    static constexpr int TB = R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2;
+};
+
+template <class T, int R1, int R2>
+__device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in) {
+    using F = FourStep<R1, R2>;
+    const int spb = blockDim.x / F::TS;  // sequences per round
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + threadIdx.x / F::TS, t = threadIdx.x % F::TS;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * F::TS;
+        C2<T> *b = buf + sq * F::N;
+        C2<T> *tt = tb + (sq - base) * F::TB;
+        if (on && t < R1) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) {
+                const int n = t + R1 * n2;
+                v[n2] = n < n_in ? b[n] : C2<T>{T(0), T(0)};
+            }
+            DftReg<T, R2, false>::run(v);
+#pragma unroll
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) tt[k2 * F::P1 + t] = v[k2];
+        }
+        __syncthreads();
+        if (on && t < R2) {
+            C2<T> u[R1];
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) u[t1] = tt[t * F::P1 + t1];
+            DftReg<T, R1, false>::run(u);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) b[t + R2 * k1] = u[k1];
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse of fs_forward (unnormalised); only outputs n < n_out are written.
+template <class T, int R1, int R2>
+__device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out) {
+    using F = FourStep<R1, R2>;
+    const int spb = blockDim.x / F::TS;
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + threadIdx.x / F::TS, t = threadIdx.x % F::TS;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * F::TS;
+        C2<T> *b = buf + sq * F::N;
+        C2<T> *tt = tb + (sq - base) * F::TB;
+        if (on && t < R2) {
+            C2<T> u[R1];
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) u[k1] = b[t + R2 * k1];
+            DftReg<T, R1, true>::run(u);
+#pragma unroll
+            for (int t1 = 1; t1 < R1; ++t1) {
+                C2<T> w = tw4[t * R1 + t1];
+                w.y = -w.y;
+                u[t1] = cmul(u[t1], w);
+            }
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) tt[t1 * F::P2 + t] = u[t1];
+        }
+        __syncthreads();
+        if (on && t < R1) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) v[k2] = tt[t * F::P2 + k2];
+            DftReg<T, R2, true>::run(v);
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) {
+                const int n = t + R1 * n2;
+                if (n < n_out) b[n] = v[n2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- x passes --
+// An x-pass work-group owns `rows_per_wg` consecutive x-rows of the grid
+// (row = y + ny z) and every cell and half-spectrum element of those rows.
+
+// i / n for 0 <= i < 2^22: a compile-time n (four-step shapes) folds to a
+// multiply-shift; otherwise a float reciprocal with one correction each way.
+template <bool CONST_N>
+__device__ __forceinline__ int small_div(int i, int n, float rn) {
+    if constexpr (CONST_N) {
+        return i / n;
+    } else {
+        int q = (int)((float)i * rn);
+        q -= q * n > i ? 1 : 0;
+        q += (q + 1) * n <= i ? 1 : 0;
+        return q;
+    }
+}
+
+// Row table of one work-group, filled once per launch: first cell of local
+// row rr (-1 past the grid), its y and z, and the spectrum offset of packed
+// line l = 3 rr + c at kx = 0 (-1 past the grid).
+struct XRows {
+    int cell0[kMaxRowsWg];
+    int y[kMaxRowsWg], z[kMaxRowsWg];
+    int spec[3 * kMaxRowsWg];
+};
+
+template <class T>
+__device__ __forceinline__ void x_rows(const Args<T> &A, int r0, XRows &xr) {
+    const int rr = threadIdx.x;
+    if (rr < A.rows_per_wg) {
+        const int row = r0 + rr;
+        const bool ok = row < (int)A.nrows;
+        const int z = ok ? row / A.ny : 0, y = ok ? row - z * A.ny : 0;
+        xr.cell0[rr] = ok ? row * A.nx : -1;
+        xr.y[rr] = y;
+        xr.z[rr] = z;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xr.spec[3 * rr + c] = ok ? ((c * A.nzs + z) * A.nkx) * A.ly + y : -1;
+    }
+}
+
+// Stage combination y_s = normalise(m0 + dt*scale*sum_q a_q k_q) over the
+// work-group's rows (accumulated in FP64 for both storage precisions: the
+// Dormand-Prince rows have coefficients up to ~12 that cancel to O(1)),
+// y_s stored for s >= 2, Ms*y packed two real x-lines per complex sequence
+// into b0.  NK = k-vectors of the combination (0: stage 1, y = m0).  The
+// loads of a chunk of cells are all issued before any is consumed.
+template <class T, int R1, int R2, int NK>
+__device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, int s, int p, double dt,
+                                             const XRows &xr, C2<T> *b0) {
+    constexpr bool FAST = R1 > 0;
+    constexpr int KM = NK > 0 ? NK : 1;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int rows = A.rows_per_wg, ncl = rows * nx, nl = 3 * rows;
+    const float rnx = 1.0f / (float)nx;
+    constexpr int CH = Chunk<T>::value;
+    const int N = (int)A.ncell, bd = blockDim.x;
+    const T *__restrict__ m0 = A.m_all + (size_t)p * 3 * N;
+    const StageRow &R = M.row[NK > 0 ? s : 2];
+    const T *kp[KM];
+    double wa[KM];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+        kp[q] = A.k_all + (size_t)kslot(M, R.kidx[q], p) * 3 * N;
+        wa[q] = R.a[q];
+    }
+    const double f = dt * R.scale;
+    T *__restrict__ yout = NK > 0 ? stage_output(A, M, s, p) : nullptr;
+    for (int base = 0; base < ncl; base += CH * bd) {
+        T mv[CH][3], kv[CH][KM][3], msv[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            const int ic = i < ncl ? i : ncl - 1;
+            const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
+            const int c0 = xr.cell0[rr];
+            const int cell = (c0 < 0 ? 0 : c0) + x;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[j][c] = m0[c * N + cell];
+#pragma unroll
+            for (int q = 0; q < NK; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) kv[j][q][c] = kp[q][c * N + cell];
+            msv[j] = A.ms_cell ? A.ms_cell[cell] : A.ms;
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            if (i >= ncl) continue;
+            const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+            const int c0 = xr.cell0[rr];
+            T v[3];
+            if constexpr (NK == 0) {
+                v[0] = mv[j][0];
+                v[1] = mv[j][1];
+                v[2] = mv[j][2];
+            } else {
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < NK; ++q) {
+                    a0 += wa[q] * (double)kv[j][q][0];
+                    a1 += wa[q] * (double)kv[j][q][1];
+                    a2 += wa[q] * (double)kv[j][q][2];
+                }
+                const double xx = (double)mv[j][0] + a0 * f, yy = (double)mv[j][1] + a1 * f,
+                             zz = (double)mv[j][2] + a2 * f;
+                const double n2 = xx * xx + yy * yy + zz * zz;
+                const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+                v[0] = (T)(xx * inv);
+                v[1] = (T)(yy * inv);
+                v[2] = (T)(zz * inv);
+                if (c0 >= 0) {
+                    const int cell = c0 + x;
+                    yout[cell] = v[0];
+                    yout[N + cell] = v[1];
+                    yout[2 * N + cell] = v[2];
+                }
+            }
+            const T w = c0 >= 0 ? msv[j] : T(0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int l = 3 * rr + c;
+                T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
+                e[l & 1] = w * v[c];
+            }
+        }
+    }
+    if (nl & 1)
+        for (int x = threadIdx.x; x < nx; x += bd) b0[(nl >> 1) * px + x].y = T(0);
+}
+
+// Zero-padded forward FFT of the packed sequences in b0 and the unpacking of
+// each sequence into the half spectra of its two real lines,
+// X = (Z[k] + conj Z[-k])/2 and (Z[k] - conj Z[-k])/(2i), y-lines contiguous.
+template <class T, int R1, int R2>
+__device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
+                                                const C2<T> *tw) {
+    constexpr bool FAST = R1 > 0;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1, nkx = nx + 1;
+    __syncthreads();
+    const C2<T> *res;
+    if constexpr (FAST) {
+        fs_forward<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx);
+        res = b0;
+    } else {
+        res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
+    }
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
+    const int ly = A.ly, tot = 3 * A.rows_per_wg * nkx;
+    const float rnk = 1.0f / (float)nkx;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int l = small_div<FAST>(i, nkx, rnk), kx = i - l * nkx;
+        const int off = xr.spec[l];
+        if (off < 0) continue;
+        const C2<T> *zq = res + (l >> 1) * px;
+        const C2<T> zk = zq[kx], zm = zq[kx == 0 ? 0 : px - kx];
+        const C2<T> X = (l & 1) ? C2<T>{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)}
+                                : C2<T>{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
+        S[off + kx * ly] = X;
+    }
+}
+
+// Hermitian rebuild of the packed sequences from the half spectra (two real
+// output lines per complex sequence) and the C2R inverse FFT along x, only
+// the first nx outputs kept.  Returns the LDS buffer holding the result.
+template <class T, int R1, int R2>
+__device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
+                                                       const C2<T> *tw) {
+    constexpr bool FAST = R1 > 0;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int nl = 3 * A.rows_per_wg, ly = A.ly, bd = blockDim.x;
+    const float rpx = 1.0f / (float)px;
+    const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
+    constexpr int CZ = 8;
+    const int nz_tot = A.nseq_x * px;
+    for (int base = 0; base < nz_tot; base += CZ * bd) {
+        C2<T> av[CZ], bv[CZ];
+        bool ma[CZ], mb[CZ];
+#pragma unroll
+        for (int j = 0; j < CZ; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            const int ic = i < nz_tot ? i : nz_tot - 1;
+            const int sq = small_div<FAST>(ic, px, rpx), k = ic - sq * px;
+            const int kk = k > nx ? px - k : k;
+            const int la = 2 * sq, lb = la + 1;
+            const int oa = xr.spec[la], ob = lb < nl ? xr.spec[lb] : -1;
+            ma[j] = oa >= 0;
+            mb[j] = ob >= 0;
+            av[j] = S[(oa < 0 ? 0 : oa) + kk * ly];
+            bv[j] = S[(ob < 0 ? 0 : ob) + kk * ly];
+        }
+#pragma unroll
+        for (int j = 0; j < CZ; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            if (i >= nz_tot) continue;
+            const int sq = small_div<FAST>(i, px, rpx), k = i - sq * px;
+            C2<T> a = ma[j] ? av[j] : C2<T>{T(0), T(0)};
+            C2<T> b = mb[j] ? bv[j] : C2<T>{T(0), T(0)};
+            if (k > nx) {
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            b0[i] = {a.x - b.y, a.y + b.x};
+        }
+    }
+    __syncthreads();
+    if constexpr (FAST) {
+        fs_inverse<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx);
+        return b0;
+    } else {
+        return lds_fft<T, true>(b0, b1, A.fx, tw, A.nseq_x, px, nx);
+    }
+}
+
+// Local pass of stage s over the work-group's rows: demag field from the
+// inverse x FFT (1/P scale), exchange, Zeeman, anisotropy, LLG torque,
+// optional statistics and error norm.
+template <class T, bool LUT, int R1, int R2>
+__device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
+                                        const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
+    constexpr bool FAST = R1 > 0;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
+    const float rnx = 1.0f / (float)nx;
+    const T *yin = stage_input(A, M, s, p);
+    T *kout = stage_rhs(A, M, s, p);
+    const T scale = T(A.inv_p);
+    constexpr int CH = Chunk<T>::value;
+    for (int base = 0; base < ncl; base += CH * bd) {
+        CellIn<T> g[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            const int ic = i < ncl ? i : ncl - 1;
+            const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
+            const int c0 = xr.cell0[rr];
+            gather_cell<T, LUT>(A, M, p, (c0 < 0 ? 0 : c0) + x, x, xr.y[rr], xr.z[rr], yin, err, g[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            if (i >= ncl) continue;
+            const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+            const int c0 = xr.cell0[rr];
+            if (c0 < 0) continue;
+            T hd[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int l = 3 * rr + c;
+                const C2<T> zv = res[(l >> 1) * px + x];
+                hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
+            }
+            compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc);
+        }
+    }
+}
+
+// Forward x pass of stage s: combination, Ms*y packing, R2C FFT along x.
+template <class T, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
@@ -570,41 +994,68 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
     C2<T> *tw = b1 + A.nseq_x * A.px;
-    const int rows = A.rows_per_wg, nx = A.nx, px = A.px, nl = 3 * rows;
-    const long long r0 = (long long)blockIdx.x * rows;
-    load_twiddles<T>(tw, A.twx, px);
-    for (int i = threadIdx.x; i < rows * nx; i += blockDim.x) {
-        const int rr = i / nx, x = i - rr * nx;
-        const long long row = r0 + rr;
-        T v[3] = {T(0), T(0), T(0)};
-        if (row < A.nrows) stage_state(A, M, s, p, dt, row * nx + x, v);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int l = 3 * rr + c;
-            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
-            e[l & 1] = (A.ms_cell && row < A.nrows ? A.ms_cell[row * nx + x] : A.ms) * v[c];
-        }
-        if ((nl & 1) && rr == rows - 1) b0[(nl >> 1) * px + x].y = T(0);
-    }
+    __shared__ XRows xr;
+    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
-    const C2<T> *res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
-    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
-    const int nkx = A.nkx;
-    for (int i = threadIdx.x; i < nl * nkx; i += blockDim.x) {
-        const int rr = i % rows;
-        const int rest = i / rows;
-        const int kx = rest % nkx, c = rest / nkx;
-        const long long row = r0 + rr;
-        if (row >= A.nrows) continue;
-        const int l = 3 * rr + c;
-        const C2<T> *zq = res + (l >> 1) * px;
-        const C2<T> zk = zq[kx], zm = zq[kx == 0 ? 0 : px - kx];
-        C2<T> X;
-        if (!(l & 1)) X = {(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
-        else X = {(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
-        const long long yy = row % A.ny, zz = row / A.ny;
-        S[(((long long)c * A.nzs + zz) * nkx + kx) * A.ly + yy] = X;
-    }
+    x_pack_stage<T, R1, R2, NK>(A, M, s, p, dt, xr, b0);
+    x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
+}
+
+// Inverse x pass and local pass of stage s; the last work-group reduces the
+// partials and runs the controller (control 1) or stores statistics (2).
+template <class T, bool LUT, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats, int control) {
+    Ctl *ctl = A.ctl;
+    if (ctl->done) return;
+    const int p = ctl->parity;
+    const double dt = ctl->dt;
+    const Method &M = c_methods[A.method];
+    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
+    __shared__ XRows xr;
+    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    __syncthreads();
+    const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
+    const bool err = M.adaptive && s == M.nstage;
+    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+    Acc acc;
+    acc.init();
+    x_local<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, res, stats != 0, err, acc);
+    if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
+    if (control) finish_last_block(A, control);
+}
+
+// Stage s (not the last of the step) ends and stage s+1 begins in one
+// launch: the combination of stage s+1 is pointwise in k_1..k_s, and the
+// work-group owns all cells of its rows, so the inverse x pass, local pass
+// and k_s of stage s feed the forward x pass of stage s+1 directly.  NK =
+// k-vectors of stage s+1.  Results are those of the separate launches.
+template <class T, bool LUT, int R1, int R2, int NK>
+__global__ void __launch_bounds__(kThreads) k_xx(Args<T> A, int s) {
+    Ctl *ctl = A.ctl;
+    if (ctl->done) return;
+    const int p = ctl->parity;
+    const double dt = ctl->dt;
+    const Method &M = c_methods[A.method];
+    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
+    __shared__ XRows xr;
+    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    __syncthreads();
+    const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
+    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+    Acc acc;
+    x_local<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, res, false, false, acc);
+    // k_s of these rows is written by this work-group; the barrier publishes
+    // it (work-group scope) and frees the LDS result buffer
+    __syncthreads();
+    x_pack_stage<T, R1, R2, NK>(A, M, s + 1, p, dt, xr, b0);
+    x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
 }
 
 // MODE 0: thin-film fused forward FFT + tensor multiply + inverse FFT.
@@ -659,6 +1110,106 @@ __global__ void __launch_bounds__(kThreads) k_y(Args<T> A) {
     }
 }
 
+// Thin-film y pass with a register four-step FFT of length py = R1*R2:
+// one work-group per kx column, TS = max(R1,R2) threads per component.
+// Forward: thread t1 holds x[t1 + R1 n2] (only n < ny loaded), R2-point DFT in
+// registers, twiddle W_py^{t1 k2}, one padded LDS transpose, R1-point DFT ->
+// thread k2 holds X[k2 + R2 k1].  The three components meet in LDS for the
+// tensor multiply H = -N M; the inverse runs the same steps backwards and
+// keeps the first ny outputs.
+template <class T, int R1, int R2>
+__global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(Args<T> A) {
+    if (A.ctl->done) return;
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    constexpr int N = R1 * R2;
+    const int ny = A.ny, ly = A.ly, nkx = A.nkx;
+    const int kx = blockIdx.x;
+    const int c = threadIdx.x / TS, t = threadIdx.x - c * TS;
+    C2<T> *lds = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + ((long long)c * A.nzs * nkx + kx) * ly;
+    const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4y);  // tw4[k2*R1 + t1] = W_N^{t1 k2}
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;                    // padded LDS rows
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);  // transpose block per component
+    C2<T> *tb = lds + c * TB;
+
+    // ---- forward step 1: thread t1 < R1
+    C2<T> v[R2];
+    if (t < R1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            const int n = t + R1 * n2;
+            v[n2] = n < ny ? S[n] : C2<T>{T(0), T(0)};
+        }
+        DftReg<T, R2, false>::run(v);
+#pragma unroll
+        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
+    }
+    __syncthreads();
+    // ---- forward step 2: thread k2 < R2
+    C2<T> u[R1];
+    if (t < R2) {
+#pragma unroll
+        for (int t1 = 0; t1 < R1; ++t1) u[t1] = tb[t * P1 + t1];
+        DftReg<T, R1, false>::run(u);
+    }
+    __syncthreads();
+    // ---- tensor multiply: components exchange through LDS in natural order
+    C2<T> *ex = lds;  // 3 x N
+    if (t < R2) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) ex[c * N + t + R2 * k1] = u[k1];
+    }
+    __syncthreads();
+    if (t < R2) {
+        const long long cs = (long long)nkx * (ny + 1);
+        const T *NT = A.ntens + (long long)kx * (ny + 1);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            const int k = t + R2 * k1;
+            const bool lo = k <= ny;
+            const int kr = lo ? k : N - k;
+            const C2<T> mx = ex[k], my = ex[N + k], mz = ex[2 * N + k];
+            if (c == 0) {
+                const T nxx = NT[kr], nxy = lo ? NT[3 * cs + kr] : -NT[3 * cs + kr];
+                u[k1] = {-(nxx * mx.x + nxy * my.x), -(nxx * mx.y + nxy * my.y)};
+            } else if (c == 1) {
+                const T nyy = NT[cs + kr], nxy = lo ? NT[3 * cs + kr] : -NT[3 * cs + kr];
+                u[k1] = {-(nxy * mx.x + nyy * my.x), -(nxy * mx.y + nyy * my.y)};
+            } else {
+                const T nzz = NT[2 * cs + kr];
+                u[k1] = {-(nzz * mz.x), -(nzz * mz.y)};
+            }
+        }
+    }
+    __syncthreads();
+    // ---- inverse step A: thread k2 < R2 transforms over k1, twiddle conj
+    if (t < R2) {
+        DftReg<T, R1, true>::run(u);
+#pragma unroll
+        for (int t1 = 1; t1 < R1; ++t1) {
+            C2<T> w = tw4[t * R1 + t1];
+            w.y = -w.y;
+            u[t1] = cmul(u[t1], w);
+        }
+#pragma unroll
+        for (int t1 = 0; t1 < R1; ++t1) tb[t1 * P2 + t] = u[t1];
+    }
+    __syncthreads();
+    // ---- inverse step B: thread t1 < R1 transforms over k2, keeps n < ny
+    if (t < R1) {
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k2] = tb[t * P2 + k2];
+        DftReg<T, R2, true>::run(v);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) {
+            const int n = t + R1 * n2;
+            if (n < ny) S[n] = v[n2];
+        }
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
     if (A.ctl->done) return;
@@ -709,66 +1260,6 @@ __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
     }
 }
 
-template <class T, bool LUT>
-__global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats, int control) {
-    Ctl *ctl = A.ctl;
-    if (ctl->done) return;
-    const int p = ctl->parity;
-    const double dt = ctl->dt;
-    const Method &M = c_methods[A.method];
-    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
-    const int rows = A.rows_per_wg, nx = A.nx, px = A.px, nl = 3 * rows, nkx = A.nkx;
-    const long long r0 = (long long)blockIdx.x * rows;
-    load_twiddles<T>(tw, A.twx, px);
-    const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
-    auto line = [&](int l, int kk) -> C2<T> {
-        if (l >= nl) return C2<T>{T(0), T(0)};
-        const int rr = l / 3, c = l - 3 * (l / 3);
-        const long long row = r0 + rr;
-        if (row >= A.nrows) return C2<T>{T(0), T(0)};
-        const long long yy = row % A.ny, zz = row / A.ny;
-        return S[(((long long)c * A.nzs + zz) * nkx + kk) * A.ly + yy];
-    };
-    for (int i = threadIdx.x; i < A.nseq_x * px; i += blockDim.x) {
-        const int sq = i / px, k = i - sq * px;
-        const bool cj = k > nx;
-        const int kk = cj ? px - k : k;
-        C2<T> a = line(2 * sq, kk), b = line(2 * sq + 1, kk);
-        if (cj) {
-            a.y = -a.y;
-            b.y = -b.y;
-        }
-        b0[i] = {a.x - b.y, a.y + b.x};
-    }
-    __syncthreads();
-    const C2<T> *res = lds_fft<T, true>(b0, b1, A.fx, tw, A.nseq_x, px, nx);
-    const T *yin = stage_input(A, M, s, p);
-    T *kout = stage_rhs(A, M, s, p);
-    const bool err = M.adaptive && s == M.nstage;
-    const T scale = T(A.inv_p);
-    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
-    Acc acc;
-    acc.init();
-    for (int i = threadIdx.x; i < rows * nx; i += blockDim.x) {
-        const int rr = i / nx, x = i - rr * nx;
-        const long long row = r0 + rr;
-        if (row >= A.nrows) continue;
-        T hd[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int l = 3 * rr + c;
-            const C2<T> zv = res[(l >> 1) * px + x];
-            hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
-        }
-        const int yy = (int)(row % A.ny), zz = (int)(row / A.ny);
-        local_cell<T, LUT>(A, M, s, p, dt, row * nx + x, x, yy, zz, yin, kout, hd, wf, stats != 0, err, acc);
-    }
-    if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
-    if (control) finish_last_block(A, control);
-}
-
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
     Ctl *ctl = A.ctl;
@@ -797,8 +1288,9 @@ __global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats,
         const int z = (int)(cell / ((long long)A.nx * A.ny));
         const T hd[3] = {T(0), T(0), T(0)};
         const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
-        local_cell<T, LUT>(A, M, s, p, dt, cell, x, y, z, stage_input(A, M, s, p), stage_rhs(A, M, s, p), hd, wf,
-                           stats != 0, err, acc);
+        CellIn<T> g;
+        gather_cell<T, LUT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
+        compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc);
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
     if (control) finish_last_block(A, control);
@@ -956,6 +1448,7 @@ static void build_methods(Method *out) {
     row(r, 5, {1, 2, 3, 4}, {1.0, 2.0, 2.0, 1.0}, 1.0 / 6.0);
 }
 
+template <class T>
 hipError_t upload_methods() {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -968,14 +1461,45 @@ hipError_t upload_methods() {
     return e;
 }
 
+#ifdef FM_PRIMARY
 int method_nstage(int method) {
     Method tab[M_COUNT];
     build_methods(tab);
     return tab[method].nstage;
 }
 
+int method_stage_nk(int method, int s) {
+    Method tab[M_COUNT];
+    build_methods(tab);
+    return s >= 2 && s <= tab[method].nstage ? tab[method].row[s].nk : 0;
+}
+#endif
+
+// Four-step factorisation py = R1*R2 of the thin-film fast y pass.
+static bool yfast_shape(int py, int &r1, int &r2) {
+    switch (py) {
+        case 64: r1 = 8; r2 = 8; return true;
+        case 128: r1 = 8; r2 = 16; return true;
+        case 256: r1 = 16; r2 = 16; return true;
+        case 512: r1 = 16; r2 = 32; return true;
+        case 1024: r1 = 32; r2 = 32; return true;
+        case 2048: r1 = 64; r2 = 32; return true;
+        default: return false;
+    }
+}
+
 template <class T>
-static size_t lds_x(const Args<T> &A) { return (2ull * A.nseq_x + 1) * A.px * sizeof(C2<T>); }
+static size_t lds_x(const Args<T> &A) {
+    int r1 = 0, r2 = 0;
+    if (A.xfast && yfast_shape(A.xfast, r1, r2)) {
+        // sequences + one round of padded transpose blocks (no twiddle table)
+        const int ts = r1 > r2 ? r1 : r2;
+        const long long tb = (long long)(r2 * (r1 + 1) > r1 * (r2 + 1) ? r2 * (r1 + 1) : r1 * (r2 + 1));
+        const long long rounds = A.nseq_x < kThreads / ts ? A.nseq_x : kThreads / ts;
+        return ((long long)A.nseq_x * A.px + rounds * tb) * sizeof(C2<T>);
+    }
+    return (2ull * A.nseq_x + 1) * A.px * sizeof(C2<T>);
+}
 template <class T>
 static size_t lds_y(const Args<T> &A) { return (2ull * 3 + 1) * A.py * sizeof(C2<T>); }
 template <class T>
@@ -989,9 +1513,58 @@ static hipError_t set_lds(const void *fn, size_t bytes) {
 
 // Chooses work-group tiling for the given LDS limit and sets the kernel
 // attributes.  Returns an error string (empty on success).
+template <class T, int R1, int R2>
+static size_t lds_yf() {
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
+    return 3ull * TB * sizeof(C2<T>);
+}
+
+template <class T, bool LUT, int R1, int R2>
+static hipError_t set_lds_xx(size_t bytes) {
+    hipError_t e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 1>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 3>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 4>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 5>, bytes);
+    return e;
+}
+
+template <class T, int R1, int R2>
+static hipError_t set_lds_x_shape(size_t bytes) {
+    hipError_t e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 0>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 1>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 3>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 4>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 5>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, false, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, true, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, false, R1, R2>(bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, true, R1, R2>(bytes);
+    return e;
+}
+template <class T>
+static hipError_t set_lds_x(int xfast, size_t bytes) {
+    switch (xfast) {
+        case 64: return set_lds_x_shape<T, 8, 8>(bytes);
+        case 128: return set_lds_x_shape<T, 8, 16>(bytes);
+        case 256: return set_lds_x_shape<T, 16, 16>(bytes);
+        case 512: return set_lds_x_shape<T, 16, 32>(bytes);
+        case 1024: return set_lds_x_shape<T, 32, 32>(bytes);
+        case 2048: return set_lds_x_shape<T, 64, 32>(bytes);
+        default: return set_lds_x_shape<T, 0, 0>(bytes);
+    }
+}
+
 template <class T>
 const char *configure(Args<T> &A, size_t lds_limit) {
     if (!A.demag) return "";
+    A.xfast = 0;
+    {
+        int r1 = 0, r2 = 0;
+        if (A.tw4x && yfast_shape(A.px, r1, r2)) A.xfast = A.px;
+    }
     auto fits_x = [&](int rows) {
         A.rows_per_wg = rows;
         A.nseq_x = (3 * rows + 1) / 2;
@@ -1001,7 +1574,12 @@ const char *configure(Args<T> &A, size_t lds_limit) {
     if (A.nrows >= 4096 && fits_x(4)) rows = 4;
     else if (!fits_x(2)) rows = 1;
     if (!fits_x(rows)) return "padded x length does not fit the LDS of one work-group";
-    if (lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
+    A.yfast = 0;
+    {
+        int r1 = 0, r2 = 0;
+        if (A.nz == 1 && A.tw4y && yfast_shape(A.py, r1, r2)) A.yfast = A.py;
+    }
+    if (!A.yfast && lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
     if (A.nz > 1) {
         int ty = 16;
         while (ty > 1) {
@@ -1014,20 +1592,131 @@ const char *configure(Args<T> &A, size_t lds_limit) {
     } else {
         A.ty = 1;
     }
-    if (set_lds<T>((const void *)k_fwd_x<T>, lds_x(A)) != hipSuccess) return "cannot raise LDS limit (x fwd)";
-    if (set_lds<T>((const void *)k_inv_x<T, false>, lds_x(A)) != hipSuccess) return "cannot raise LDS limit (x inv)";
-    if (set_lds<T>((const void *)k_inv_x<T, true>, lds_x(A)) != hipSuccess) return "cannot raise LDS limit (x inv)";
-    if (set_lds<T>((const void *)k_y<T, 0>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
-    if (set_lds<T>((const void *)k_y<T, 1>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
-    if (set_lds<T>((const void *)k_y<T, 2>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
+    if (set_lds_x<T>(A.xfast, lds_x(A)) != hipSuccess) return "cannot raise LDS limit (x passes)";
+    if (!A.yfast && set_lds<T>((const void *)k_y<T, 0>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
+    if (A.yfast == 2048 && (lds_yf<T, 64, 32>() > lds_limit ||
+                            set_lds<T>((const void *)k_yf<T, 64, 32>, lds_yf<T, 64, 32>()) != hipSuccess))
+        return "cannot raise LDS limit (four-step y)";
+    if (A.yfast == 1024 && set_lds<T>((const void *)k_yf<T, 32, 32>, lds_yf<T, 32, 32>()) != hipSuccess)
+        return "cannot raise LDS limit (four-step y)";
+    if (A.nz > 1 && set_lds<T>((const void *)k_y<T, 1>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
+    if (A.nz > 1 && set_lds<T>((const void *)k_y<T, 2>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
     if (A.nz > 1 && set_lds<T>((const void *)k_z<T>, lds_z(A)) != hipSuccess) return "cannot raise LDS limit (z)";
     return "";
+}
+
+
+template <class T, int R1, int R2>
+static void launch_yf_shape(const Args<T> &A, hipStream_t st) {
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    k_yf<T, R1, R2><<<dim3(A.nkx), dim3(3 * TS), lds_yf<T, R1, R2>(), st>>>(A);
+}
+
+template <class T>
+static void launch_yf(const Args<T> &A, hipStream_t st) {
+    switch (A.yfast) {
+        case 64: launch_yf_shape<T, 8, 8>(A, st); break;
+        case 128: launch_yf_shape<T, 8, 16>(A, st); break;
+        case 256: launch_yf_shape<T, 16, 16>(A, st); break;
+        case 512: launch_yf_shape<T, 16, 32>(A, st); break;
+        case 1024: launch_yf_shape<T, 32, 32>(A, st); break;
+        case 2048: launch_yf_shape<T, 64, 32>(A, st); break;
+        default: break;
+    }
 }
 
 template <class T>
 long long num_partials(const Args<T> &A) {
     if (A.demag) return (A.nrows + A.rows_per_wg - 1) / A.rows_per_wg;
     return (A.ncell + kThreads - 1) / kThreads;
+}
+
+template <class T, int R1, int R2>
+static void launch_fwd_x_shape(const Args<T> &A, int s, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    const size_t l = lds_x(A);
+    switch (A.stage_nk[s]) {
+        case 0: k_fwd_x<T, R1, R2, 0><<<g, b, l, st>>>(A, s); break;
+        case 1: k_fwd_x<T, R1, R2, 1><<<g, b, l, st>>>(A, s); break;
+        case 2: k_fwd_x<T, R1, R2, 2><<<g, b, l, st>>>(A, s); break;
+        case 3: k_fwd_x<T, R1, R2, 3><<<g, b, l, st>>>(A, s); break;
+        case 4: k_fwd_x<T, R1, R2, 4><<<g, b, l, st>>>(A, s); break;
+        default: k_fwd_x<T, R1, R2, 5><<<g, b, l, st>>>(A, s); break;
+    }
+}
+template <class T, int R1, int R2>
+static void launch_inv_x_shape(const Args<T> &A, int s, int stats, int control, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    if (A.region) k_inv_x<T, true, R1, R2><<<g, b, lds_x(A), st>>>(A, s, stats, control);
+    else k_inv_x<T, false, R1, R2><<<g, b, lds_x(A), st>>>(A, s, stats, control);
+}
+template <class T, bool LUT, int R1, int R2>
+static void launch_xx_lut(const Args<T> &A, int s, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    const size_t l = lds_x(A);
+    switch (A.stage_nk[s + 1]) {
+        case 1: k_xx<T, LUT, R1, R2, 1><<<g, b, l, st>>>(A, s); break;
+        case 2: k_xx<T, LUT, R1, R2, 2><<<g, b, l, st>>>(A, s); break;
+        case 3: k_xx<T, LUT, R1, R2, 3><<<g, b, l, st>>>(A, s); break;
+        case 4: k_xx<T, LUT, R1, R2, 4><<<g, b, l, st>>>(A, s); break;
+        default: k_xx<T, LUT, R1, R2, 5><<<g, b, l, st>>>(A, s); break;
+    }
+}
+template <class T, int R1, int R2>
+static void launch_xx_shape(const Args<T> &A, int s, hipStream_t st) {
+    if (A.region) launch_xx_lut<T, true, R1, R2>(A, s, st);
+    else launch_xx_lut<T, false, R1, R2>(A, s, st);
+}
+// Fused end of stage s and start of stage s+1 (2 <= s < nstage).
+template <class T>
+static void launch_xx(const Args<T> &A, int s, hipStream_t st) {
+    switch (A.xfast) {
+        case 64: launch_xx_shape<T, 8, 8>(A, s, st); break;
+        case 128: launch_xx_shape<T, 8, 16>(A, s, st); break;
+        case 256: launch_xx_shape<T, 16, 16>(A, s, st); break;
+        case 512: launch_xx_shape<T, 16, 32>(A, s, st); break;
+        case 1024: launch_xx_shape<T, 32, 32>(A, s, st); break;
+        case 2048: launch_xx_shape<T, 64, 32>(A, s, st); break;
+        default: launch_xx_shape<T, 0, 0>(A, s, st); break;
+    }
+}
+template <class T>
+static void launch_fwd_x(const Args<T> &A, int s, hipStream_t st) {
+    switch (A.xfast) {
+        case 64: launch_fwd_x_shape<T, 8, 8>(A, s, st); break;
+        case 128: launch_fwd_x_shape<T, 8, 16>(A, s, st); break;
+        case 256: launch_fwd_x_shape<T, 16, 16>(A, s, st); break;
+        case 512: launch_fwd_x_shape<T, 16, 32>(A, s, st); break;
+        case 1024: launch_fwd_x_shape<T, 32, 32>(A, s, st); break;
+        case 2048: launch_fwd_x_shape<T, 64, 32>(A, s, st); break;
+        default: launch_fwd_x_shape<T, 0, 0>(A, s, st); break;
+    }
+}
+template <class T>
+static void launch_inv_x(const Args<T> &A, int s, int stats, int control, hipStream_t st) {
+    switch (A.xfast) {
+        case 64: launch_inv_x_shape<T, 8, 8>(A, s, stats, control, st); break;
+        case 128: launch_inv_x_shape<T, 8, 16>(A, s, stats, control, st); break;
+        case 256: launch_inv_x_shape<T, 16, 16>(A, s, stats, control, st); break;
+        case 512: launch_inv_x_shape<T, 16, 32>(A, s, stats, control, st); break;
+        case 1024: launch_inv_x_shape<T, 32, 32>(A, s, stats, control, st); break;
+        case 2048: launch_inv_x_shape<T, 64, 32>(A, s, stats, control, st); break;
+        default: launch_inv_x_shape<T, 0, 0>(A, s, stats, control, st); break;
+    }
+}
+// The y (and for 3-D z) passes between the forward and inverse x passes.
+template <class T>
+static void launch_yz(const Args<T> &A, hipStream_t st) {
+    const size_t ly = lds_y(A);
+    if (A.nz == 1) {
+        if (A.yfast) launch_yf(A, st);
+        else k_y<T, 0><<<dim3(A.nkx), dim3(kThreads), ly, st>>>(A);
+    } else {
+        k_y<T, 1><<<dim3(A.nkx * A.nz), dim3(kThreads), ly, st>>>(A);
+        const unsigned gz = (unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty));
+        k_z<T><<<dim3(gz), dim3(kThreads), lds_z(A), st>>>(A);
+        k_y<T, 2><<<dim3(A.nkx * A.nz), dim3(kThreads), ly, st>>>(A);
+    }
 }
 
 // One RHS evaluation of stage s (1 = the current state m0).  control:
@@ -1038,22 +1727,34 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
     const int stats = control != 0;
     const unsigned gpart = (unsigned)num_partials(A);
     if (A.demag) {
-        const size_t lx = lds_x(A), ly = lds_y(A);
-        hipLaunchKernelGGL(k_fwd_x<T>, dim3(gpart), dim3(kThreads), lx, st, A, s);
-        if (A.nz == 1) {
-            hipLaunchKernelGGL((k_y<T, 0>), dim3(A.nkx), dim3(kThreads), ly, st, A);
-        } else {
-            hipLaunchKernelGGL((k_y<T, 1>), dim3(A.nkx * A.nz), dim3(kThreads), ly, st, A);
-            const unsigned gz = (unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty));
-            hipLaunchKernelGGL(k_z<T>, dim3(gz), dim3(kThreads), lds_z(A), st, A);
-            hipLaunchKernelGGL((k_y<T, 2>), dim3(A.nkx * A.nz), dim3(kThreads), ly, st, A);
-        }
-        if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats, control);
-        else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lx, st, A, s, stats, control);
+        launch_fwd_x(A, s, st);
+        launch_yz(A, st);
+        launch_inv_x(A, s, stats, control, st);
     } else {
         if (s >= 2) hipLaunchKernelGGL(k_stage<T>, dim3(gpart), dim3(kThreads), 0, st, A, s);
         if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
         else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
+    }
+    return hipGetLastError();
+}
+
+// Stages 2..nstage of one trial step, the controller fused into the last
+// launch.  With demag, the inverse x pass of stage s and the forward x pass
+// of stage s+1 share one launch.
+template <class T>
+hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
+    if (!A.demag) {
+        for (int s = 2; s <= nstage; ++s) {
+            const hipError_t e = enqueue_eval(A, s, s == nstage ? 1 : 0, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    launch_fwd_x(A, 2, st);
+    for (int s = 2; s <= nstage; ++s) {
+        launch_yz(A, st);
+        if (s < nstage) launch_xx(A, s, st);
+        else launch_inv_x(A, s, 1, 1, st);
     }
     return hipGetLastError();
 }
@@ -1064,14 +1765,16 @@ hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st) {
     return hipGetLastError();
 }
 
-template const char *configure<float>(Args<float> &, size_t);
-template const char *configure<double>(Args<double> &, size_t);
-template long long num_partials<float>(const Args<float> &);
-template long long num_partials<double>(const Args<double> &);
-template hipError_t enqueue_eval<float>(const Args<float> &, int, int, hipStream_t);
-template hipError_t enqueue_eval<double>(const Args<double> &, int, int, hipStream_t);
-template hipError_t enqueue_control<float>(const Args<float> &, int, hipStream_t);
-template hipError_t enqueue_control<double>(const Args<double> &, int, hipStream_t);
+#ifdef FM_PRIMARY
+bool yfast_factors(int py, int *r1, int *r2) { return yfast_shape(py, *r1, *r2); }
+#endif
+
+template hipError_t upload_methods<FM_REAL>();
+template const char *configure<FM_REAL>(Args<FM_REAL> &, size_t);
+template long long num_partials<FM_REAL>(const Args<FM_REAL> &);
+template hipError_t enqueue_eval<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
+template hipError_t enqueue_trial<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
+template hipError_t enqueue_control<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
 
 }  // namespace fm
 
@@ -1079,19 +1782,20 @@ namespace fm {
 
 // One launch of a single kernel kind (for event timing): 0 x-forward or
 // stage combine, 1 y pass, 2 z pass, 3 y inverse, 4 x-inverse/local pass,
-// 5 controller.  Returns hipErrorInvalidValue for a kind the grid lacks.
+// 5 controller, 6 fused x-inverse/local of stage s + x-forward of s+1.  Returns hipErrorInvalidValue for a kind the grid lacks.
 template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
     const unsigned gpart = (unsigned)num_partials(A);
     const bool film = A.nz == 1;
     switch (kind) {
         case 0:
-            if (A.demag) hipLaunchKernelGGL(k_fwd_x<T>, dim3(gpart), dim3(kThreads), lds_x(A), st, A, s);
+            if (A.demag) launch_fwd_x(A, s, st);
             else hipLaunchKernelGGL(k_stage<T>, dim3(gpart), dim3(kThreads), 0, st, A, s);
             break;
         case 1:
             if (!A.demag) return hipErrorInvalidValue;
-            if (film) hipLaunchKernelGGL((k_y<T, 0>), dim3(A.nkx), dim3(kThreads), lds_y(A), st, A);
+            if (film && A.yfast) launch_yf(A, st);
+            else if (film) hipLaunchKernelGGL((k_y<T, 0>), dim3(A.nkx), dim3(kThreads), lds_y(A), st, A);
             else hipLaunchKernelGGL((k_y<T, 1>), dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st, A);
             break;
         case 2:
@@ -1105,8 +1809,7 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             break;
         case 4:
             if (A.demag) {
-                if (A.region) hipLaunchKernelGGL((k_inv_x<T, true>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1, 0);
-                else hipLaunchKernelGGL((k_inv_x<T, false>), dim3(gpart), dim3(kThreads), lds_x(A), st, A, s, 1, 0);
+                launch_inv_x(A, s, 1, 0, st);
             } else {
                 if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
                 else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
@@ -1115,13 +1818,16 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
         case 5:
             hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(kThreads), 0, st, A, (int)gpart, 0);
             break;
+        case 6:
+            if (!A.demag) return hipErrorInvalidValue;
+            launch_xx(A, s, st);
+            break;
         default:
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template hipError_t enqueue_kind<float>(const Args<float> &, int, int, hipStream_t);
-template hipError_t enqueue_kind<double>(const Args<double> &, int, int, hipStream_t);
+template hipError_t enqueue_kind<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
 
 }  // namespace fm

@@ -82,8 +82,9 @@ int fullmag_fdm_backend_get_counters(fullmag_fdm_backend *handle, uint64_t *acce
  * microseconds of kernel kind i to out_us[i]:
  *   0 x-forward (stage combine + R2C; stage combine alone without demag),
  *   1 y pass (thin film fused / 3-D forward), 2 z pass (3-D),
- *   3 y inverse (3-D), 4 x-inverse + local pass, 5 controller;
- *   out_us[6] is reserved (0).
+ *   3 y inverse (3-D), 4 x-inverse + local pass, 5 controller,
+ *   6 fused x-inverse + local pass of stage 2 and x-forward of stage 3
+ *   (the launch that joins consecutive stages inside a trial step).
  * Kinds that do not exist for the grid are reported as 0.  The state is
  * left consistent (a following step recomputes every stage).  n_out must be
  * at least 7.

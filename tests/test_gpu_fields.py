@@ -82,3 +82,23 @@ def test_fields_mixed_radix_200x50(precision):
 def test_fields_odd_sizes_7x5x3(precision):
     # padded 14 x 10 x 6: radix 7, 5, 3 passes and a 3-D z pass
     _compare(random_case(7, 5, 3, seed=9, dx=2e-9, dy=3e-9, dz=4e-9), precision)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("nx,ny", [(40, 32), (24, 64), (16, 128), (20, 256), (12, 512), (8, 1024)])
+def test_fields_register_four_step_y_shapes(precision, nx, ny):
+    # padded y lengths 64..2048 take the register four-step y pass
+    _compare(random_case(nx, ny, 1, seed=ny, hole_radius_cells=min(nx, ny) // 4), precision)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_fields_full_size_film_512(precision):
+    from tests.cases import thin_film_512
+    _compare(thin_film_512(), precision)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("nx,ny", [(32, 20), (64, 12), (128, 8), (256, 6), (512, 5), (1024, 3)])
+def test_fields_register_four_step_x_shapes(precision, nx, ny):
+    # padded x lengths 64..2048 take the register four-step x passes
+    _compare(random_case(nx, ny, 1, seed=nx + 1, hole_radius_cells=min(nx, ny) // 3), precision)
