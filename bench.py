@@ -69,6 +69,12 @@ def kernel_bytes(s):
     }
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# on this workload (FP32, stage 2; FETCH_SIZE scaled by 2.0, WRITE_SIZE by
+# 1.0, the factors a plain 256 MiB device copy gives); the passes and their
+# commands are recorded in profiles/film512_dp45_hbm_traffic.md
+MEASURED_TRAFFIC_FP32 = {0: 16.01e6, 1: 17.53e6, 4: 13.71e6, 6: 29.87e6}
+
 # launches of each kind in one DP45 trial step (stages 2..7)
 LAUNCHES_PER_STEP = {0: 1, 1: 6, 4: 1, 6: 5}
 NAMES = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xx"}
@@ -148,6 +154,19 @@ def cpu_baseline(mask, m0, seconds):
                       f"{el:.1f} s wall"}
 
 
+def copy_calibration(nbytes=256 << 20, reps=4):
+    """Plain device-to-device copies of a known size: the hardware counters
+    of these dispatches (KiB fetched / written per launch) against the known
+    byte counts give the scale applied to the engine kernels' counters."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,7 +175,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 plain device copies of 256 MiB (scale for FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported by this engine "
               "(a single 512x512x1 system runs on one device)", file=sys.stderr)
@@ -174,7 +198,7 @@ def main():
     us = kt[dom]
     achieved = kb[dom] / (us * 1e-6) / 1e9
     roof = {"bound": "hbm", "kernel": NAMES[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC_FP32.get(dom),
             "bytes_per_launch": kb[dom], "avg_launch_us": round(us, 3),
             "per_kernel_us": {NAMES[k]: round(kt[k], 3) for k in LAUNCHES_PER_STEP},
             "launches_per_step": {NAMES[k]: v for k, v in LAUNCHES_PER_STEP.items()},
