@@ -44,6 +44,7 @@ hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st);
 template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
 bool yfast_factors(int py, int *r1, int *r2);
+bool zfast_factors(int pz, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
 bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz, std::vector<double> &out,
                           std::string *err);
@@ -532,9 +533,9 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.twx = upload_twiddles<T>(h, A.px);
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
-        auto four_step_twiddles = [&](int n, const void **dst) -> bool {
+        auto four_step_twiddles = [&](int n, const void **dst, bool zshape = false) -> bool {
             int r1 = 0, r2 = 0;
-            if (!fm::yfast_factors(n, &r1, &r2)) return true;
+            if (!(zshape ? fm::zfast_factors(n, &r1, &r2) : fm::yfast_factors(n, &r1, &r2))) return true;
             // W_n^{t1 k2} stored at [k2*r1 + t1]
             std::vector<T> t4(2 * (size_t)n);
             for (int k2 = 0; k2 < r2; ++k2)
@@ -549,8 +550,9 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             *dst = d;
             return true;
         };
-        if (nz == 1 && !four_step_twiddles(A.py, &A.tw4y)) return false;
+        if (!four_step_twiddles(A.py, &A.tw4y)) return false;
         if (!four_step_twiddles(A.px, &A.tw4x)) return false;
+        if (nz > 1 && !four_step_twiddles(A.pz, &A.tw4z, true)) return false;
         if (!A.spec || !A.twx || !A.twy || !A.twz) return hip_ok(h, hipErrorOutOfMemory, "alloc spectrum");
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)

@@ -1260,6 +1260,178 @@ __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
     }
 }
 
+// 3-D y passes with a register four-step FFT of length py = R1*R2: each
+// group of TS = max(R1,R2) threads owns one (component, z, kx) y-line of the
+// spectrum.  Forward: the first ny = py/2 points are read (the rest is the
+// zero padding), the full padded spectrum is written.  Inverse: py points
+// are read, the first ny written.
+template <class T, int R1, int R2, bool INV>
+__global__ void __launch_bounds__(kThreads) k_yl(Args<T> A) {
+    if (A.ctl->done) return;
+    constexpr int TS = R1 > R2 ? R1 : R2, LPB = kThreads / TS;
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
+    const int nlines = 3 * A.nzs * A.nkx;
+    const int g = threadIdx.x / TS, t = threadIdx.x - g * TS;
+    const int line = blockIdx.x * LPB + g;
+    const bool on = line < nlines;
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + (size_t)(on ? line : 0) * A.ly;
+    const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4y);
+    C2<T> *tb = reinterpret_cast<C2<T> *>(fm_smem) + g * TB;
+    if constexpr (!INV) {
+        if (t < R1) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? S[t + R1 * n2] : C2<T>{T(0), T(0)};
+            DftReg<T, R2, false>::run(v);
+#pragma unroll
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
+        }
+        __syncthreads();
+        if (t < R2) {
+            C2<T> u[R1];
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) u[t1] = tb[t * P1 + t1];
+            DftReg<T, R1, false>::run(u);
+            if (on) {
+#pragma unroll
+                for (int k1 = 0; k1 < R1; ++k1) S[t + R2 * k1] = u[k1];
+            }
+        }
+    } else {
+        if (t < R2) {
+            C2<T> u[R1];
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) u[k1] = S[t + R2 * k1];
+            DftReg<T, R1, true>::run(u);
+#pragma unroll
+            for (int t1 = 1; t1 < R1; ++t1) {
+                C2<T> w = tw4[t * R1 + t1];
+                w.y = -w.y;
+                u[t1] = cmul(u[t1], w);
+            }
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) tb[t1 * P2 + t] = u[t1];
+        }
+        __syncthreads();
+        if (t < R1) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) v[k2] = tb[t * P2 + k2];
+            DftReg<T, R2, true>::run(v);
+            if (on) {
+#pragma unroll
+                for (int n2 = 0; n2 < R2 / 2; ++n2) S[t + R1 * n2] = v[n2];
+            }
+        }
+    }
+}
+
+// 3-D z pass with a register four-step FFT of length pz = R1*R2: a
+// work-group takes UB consecutive ky of one kx; the TS = max(R1,R2) threads
+// of each ky hold all three components, so the symmetric 6-component real
+// tensor multiply H = -N M needs no exchange.  The first nz of the pz
+// z-points are read (the rest is the zero padding) and, after the inverse,
+// the first nz are written back in place.  LDS holds the padded transposes
+// as [component][element][ky] so that consecutive lanes (consecutive ky) hit
+// consecutive addresses.
+template <class T, int R1, int R2, int UB>
+__global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(Args<T> A) {
+    if (A.ctl->done) return;
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
+    const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
+    const int ntile = (py + UB - 1) / UB;
+    const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x - kx * ntile) * UB;
+    const int u = threadIdx.x % UB, t = threadIdx.x / UB;
+    const int ky = ky0 + u;
+    const bool on = ky < py;
+    const size_t zs = (size_t)nkx * ly;  // stride between z planes (complex elements)
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + (size_t)kx * ly + (on ? ky : py - 1);
+    const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4z);
+    C2<T> *tb = reinterpret_cast<C2<T> *>(fm_smem);
+    auto L = [&](int c, int i) -> C2<T> & { return tb[(c * TB + i) * UB + u]; };
+    // forward step 1: thread t < R1 holds z = t + R1 n2 (n2 < R2/2 non-zero)
+    if (t < R1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; ++n2)
+                v[n2] = n2 < R2 / 2 ? S[(size_t)(c * nzs + t + R1 * n2) * zs] : C2<T>{T(0), T(0)};
+            DftReg<T, R2, false>::run(v);
+#pragma unroll
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) L(c, k2 * P1 + t) = v[k2];
+        }
+    }
+    __syncthreads();
+    // forward step 2, multiply, inverse step A: thread t < R2 holds kz = t + R2 k1
+    C2<T> x[3][R1];
+    if (t < R2) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) x[c][t1] = L(c, t * P1 + t1);
+            DftReg<T, R1, false>::run(x[c]);
+        }
+        const bool ylo = ky <= ny;
+        const int kyr = ylo ? ky : (on ? py - ky : 0);
+        const T sy = ylo ? T(1) : T(-1);
+        const size_t cs = (size_t)nkx * (ny + 1) * (nz + 1);
+        const T *NT = A.ntens + ((size_t)kx * (ny + 1) + kyr) * (nz + 1);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            const int kz = t + R2 * k1;
+            const bool zlo = kz <= nz;
+            const int kzr = zlo ? kz : 2 * nz - kz;
+            const T sz = zlo ? T(1) : T(-1);
+            const T *N = NT + kzr;
+            const T nxx = N[0], nyy = N[cs], nzz = N[2 * cs];
+            const T nxy = sy * N[3 * cs], nxz = sz * N[4 * cs], nyz = sy * sz * N[5 * cs];
+            const C2<T> mx = x[0][k1], my = x[1][k1], mz = x[2][k1];
+            x[0][k1] = {-(nxx * mx.x + nxy * my.x + nxz * mz.x), -(nxx * mx.y + nxy * my.y + nxz * mz.y)};
+            x[1][k1] = {-(nxy * mx.x + nyy * my.x + nyz * mz.x), -(nxy * mx.y + nyy * my.y + nyz * mz.y)};
+            x[2][k1] = {-(nxz * mx.x + nyz * my.x + nzz * mz.x), -(nxz * mx.y + nyz * my.y + nzz * mz.y)};
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            DftReg<T, R1, true>::run(x[c]);
+#pragma unroll
+            for (int t1 = 1; t1 < R1; ++t1) {
+                C2<T> w = tw4[t * R1 + t1];
+                w.y = -w.y;
+                x[c][t1] = cmul(x[c][t1], w);
+            }
+        }
+    }
+    __syncthreads();  // every read of the forward transpose is done
+    if (t < R2) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) L(c, t1 * P2 + t) = x[c][t1];
+    }
+    __syncthreads();
+    // inverse step B: thread t < R1 holds z = t + R1 n2, the first nz kept
+    if (t < R1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            C2<T> v[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) v[k2] = L(c, t * P2 + k2);
+            DftReg<T, R2, true>::run(v);
+            if (on) {
+#pragma unroll
+                for (int n2 = 0; n2 < R2 / 2; ++n2) S[(size_t)(c * nzs + t + R1 * n2) * zs] = v[n2];
+            }
+        }
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
     Ctl *ctl = A.ctl;
@@ -1488,6 +1660,33 @@ static bool yfast_shape(int py, int &r1, int &r2) {
     }
 }
 
+// Four-step factorisation pz = R1*R2 of the 3-D fast z pass.
+static bool zfast_shape(int pz, int &r1, int &r2) {
+    switch (pz) {
+        case 16: r1 = 4; r2 = 4; return true;
+        case 32: r1 = 4; r2 = 8; return true;
+        case 64: r1 = 8; r2 = 8; return true;
+        case 128: r1 = 8; r2 = 16; return true;
+        case 256: r1 = 16; r2 = 16; return true;
+        default: return false;
+    }
+}
+constexpr int kZfUnits = 16;  // ky lines per work-group of the fast z pass
+
+template <class T, int R1, int R2>
+static size_t lds_yl() {
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
+    return (size_t)(kThreads / TS) * TB * sizeof(C2<T>);
+}
+template <class T, int R1, int R2>
+static size_t lds_zf() {
+    constexpr int P1 = R1 + 1, P2 = R2 + 1;
+    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
+    return 3ull * TB * kZfUnits * sizeof(C2<T>);
+}
+
 template <class T>
 static size_t lds_x(const Args<T> &A) {
     int r1 = 0, r2 = 0;
@@ -1557,6 +1756,44 @@ static hipError_t set_lds_x(int xfast, size_t bytes) {
     }
 }
 
+template <class T, int R1, int R2>
+static hipError_t set_lds_yl_shape(size_t limit) {
+    const size_t b = lds_yl<T, R1, R2>();
+    if (b > limit) return hipErrorInvalidValue;
+    hipError_t e = set_lds<T>((const void *)k_yl<T, R1, R2, false>, b);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_yl<T, R1, R2, true>, b);
+    return e;
+}
+template <class T>
+static hipError_t set_lds_yl(int py, size_t limit) {
+    switch (py) {
+        case 64: return set_lds_yl_shape<T, 8, 8>(limit);
+        case 128: return set_lds_yl_shape<T, 8, 16>(limit);
+        case 256: return set_lds_yl_shape<T, 16, 16>(limit);
+        case 512: return set_lds_yl_shape<T, 16, 32>(limit);
+        case 1024: return set_lds_yl_shape<T, 32, 32>(limit);
+        case 2048: return set_lds_yl_shape<T, 64, 32>(limit);
+        default: return hipErrorInvalidValue;
+    }
+}
+template <class T, int R1, int R2>
+static hipError_t set_lds_zf_shape(size_t limit) {
+    const size_t b = lds_zf<T, R1, R2>();
+    if (b > limit) return hipErrorInvalidValue;
+    return set_lds<T>((const void *)k_zf<T, R1, R2, kZfUnits>, b);
+}
+template <class T>
+static hipError_t set_lds_zf(int pz, size_t limit) {
+    switch (pz) {
+        case 16: return set_lds_zf_shape<T, 4, 4>(limit);
+        case 32: return set_lds_zf_shape<T, 4, 8>(limit);
+        case 64: return set_lds_zf_shape<T, 8, 8>(limit);
+        case 128: return set_lds_zf_shape<T, 8, 16>(limit);
+        case 256: return set_lds_zf_shape<T, 16, 16>(limit);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <class T>
 const char *configure(Args<T> &A, size_t lds_limit) {
     if (!A.demag) return "";
@@ -1579,8 +1816,16 @@ const char *configure(Args<T> &A, size_t lds_limit) {
         int r1 = 0, r2 = 0;
         if (A.nz == 1 && A.tw4y && yfast_shape(A.py, r1, r2)) A.yfast = A.py;
     }
-    if (!A.yfast && lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
+    A.ylfast = 0;
+    A.zfast = 0;
     if (A.nz > 1) {
+        // register four-step passes where the shape has one and its LDS fits
+        int r1 = 0, r2 = 0;
+        if (A.tw4y && yfast_shape(A.py, r1, r2) && set_lds_yl<T>(A.py, lds_limit) == hipSuccess) A.ylfast = A.py;
+        if (A.tw4z && zfast_shape(A.pz, r1, r2) && set_lds_zf<T>(A.pz, lds_limit) == hipSuccess) A.zfast = A.pz;
+    }
+    if (!A.yfast && !A.ylfast && lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
+    if (A.nz > 1 && !A.zfast) {
         int ty = 16;
         while (ty > 1) {
             A.ty = ty;
@@ -1599,9 +1844,12 @@ const char *configure(Args<T> &A, size_t lds_limit) {
         return "cannot raise LDS limit (four-step y)";
     if (A.yfast == 1024 && set_lds<T>((const void *)k_yf<T, 32, 32>, lds_yf<T, 32, 32>()) != hipSuccess)
         return "cannot raise LDS limit (four-step y)";
-    if (A.nz > 1 && set_lds<T>((const void *)k_y<T, 1>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
-    if (A.nz > 1 && set_lds<T>((const void *)k_y<T, 2>, lds_y(A)) != hipSuccess) return "cannot raise LDS limit (y)";
-    if (A.nz > 1 && set_lds<T>((const void *)k_z<T>, lds_z(A)) != hipSuccess) return "cannot raise LDS limit (z)";
+    if (A.nz > 1 && !A.ylfast && set_lds<T>((const void *)k_y<T, 1>, lds_y(A)) != hipSuccess)
+        return "cannot raise LDS limit (y)";
+    if (A.nz > 1 && !A.ylfast && set_lds<T>((const void *)k_y<T, 2>, lds_y(A)) != hipSuccess)
+        return "cannot raise LDS limit (y)";
+    if (A.nz > 1 && !A.zfast && set_lds<T>((const void *)k_z<T>, lds_z(A)) != hipSuccess)
+        return "cannot raise LDS limit (z)";
     return "";
 }
 
@@ -1704,18 +1952,59 @@ static void launch_inv_x(const Args<T> &A, int s, int stats, int control, hipStr
         default: launch_inv_x_shape<T, 0, 0>(A, s, stats, control, st); break;
     }
 }
+template <class T, int R1, int R2>
+static void launch_yl_shape(const Args<T> &A, bool inv, hipStream_t st) {
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    const unsigned g = (unsigned)((3ll * A.nzs * A.nkx + kThreads / TS - 1) / (kThreads / TS));
+    if (inv) k_yl<T, R1, R2, true><<<dim3(g), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A);
+    else k_yl<T, R1, R2, false><<<dim3(g), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A);
+}
+// 3-D y pass (forward: inv = false; inverse: inv = true).
+template <class T>
+static void launch_y3(const Args<T> &A, bool inv, hipStream_t st) {
+    switch (A.ylfast) {
+        case 64: launch_yl_shape<T, 8, 8>(A, inv, st); return;
+        case 128: launch_yl_shape<T, 8, 16>(A, inv, st); return;
+        case 256: launch_yl_shape<T, 16, 16>(A, inv, st); return;
+        case 512: launch_yl_shape<T, 16, 32>(A, inv, st); return;
+        case 1024: launch_yl_shape<T, 32, 32>(A, inv, st); return;
+        case 2048: launch_yl_shape<T, 64, 32>(A, inv, st); return;
+        default: break;
+    }
+    if (inv) k_y<T, 2><<<dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st>>>(A);
+    else k_y<T, 1><<<dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st>>>(A);
+}
+template <class T, int R1, int R2>
+static void launch_zf_shape(const Args<T> &A, hipStream_t st) {
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    const unsigned g = (unsigned)(A.nkx * ((A.py + kZfUnits - 1) / kZfUnits));
+    k_zf<T, R1, R2, kZfUnits><<<dim3(g), dim3(kZfUnits * TS), lds_zf<T, R1, R2>(), st>>>(A);
+}
+// 3-D z pass (forward FFT, tensor multiply, inverse FFT).
+template <class T>
+static void launch_z3(const Args<T> &A, hipStream_t st) {
+    switch (A.zfast) {
+        case 16: launch_zf_shape<T, 4, 4>(A, st); return;
+        case 32: launch_zf_shape<T, 4, 8>(A, st); return;
+        case 64: launch_zf_shape<T, 8, 8>(A, st); return;
+        case 128: launch_zf_shape<T, 8, 16>(A, st); return;
+        case 256: launch_zf_shape<T, 16, 16>(A, st); return;
+        default: break;
+    }
+    const unsigned gz = (unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty));
+    k_z<T><<<dim3(gz), dim3(kThreads), lds_z(A), st>>>(A);
+}
+
 // The y (and for 3-D z) passes between the forward and inverse x passes.
 template <class T>
 static void launch_yz(const Args<T> &A, hipStream_t st) {
-    const size_t ly = lds_y(A);
     if (A.nz == 1) {
         if (A.yfast) launch_yf(A, st);
-        else k_y<T, 0><<<dim3(A.nkx), dim3(kThreads), ly, st>>>(A);
+        else k_y<T, 0><<<dim3(A.nkx), dim3(kThreads), lds_y(A), st>>>(A);
     } else {
-        k_y<T, 1><<<dim3(A.nkx * A.nz), dim3(kThreads), ly, st>>>(A);
-        const unsigned gz = (unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty));
-        k_z<T><<<dim3(gz), dim3(kThreads), lds_z(A), st>>>(A);
-        k_y<T, 2><<<dim3(A.nkx * A.nz), dim3(kThreads), ly, st>>>(A);
+        launch_y3(A, false, st);
+        launch_z3(A, st);
+        launch_y3(A, true, st);
     }
 }
 
@@ -1767,6 +2056,7 @@ hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st) {
 
 #ifdef FM_PRIMARY
 bool yfast_factors(int py, int *r1, int *r2) { return yfast_shape(py, *r1, *r2); }
+bool zfast_factors(int pz, int *r1, int *r2) { return zfast_shape(pz, *r1, *r2); }
 #endif
 
 template hipError_t upload_methods<FM_REAL>();
@@ -1796,16 +2086,15 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             if (!A.demag) return hipErrorInvalidValue;
             if (film && A.yfast) launch_yf(A, st);
             else if (film) hipLaunchKernelGGL((k_y<T, 0>), dim3(A.nkx), dim3(kThreads), lds_y(A), st, A);
-            else hipLaunchKernelGGL((k_y<T, 1>), dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st, A);
+            else launch_y3(A, false, st);
             break;
         case 2:
             if (!A.demag || film) return hipErrorInvalidValue;
-            hipLaunchKernelGGL(k_z<T>, dim3((unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty))), dim3(kThreads), lds_z(A),
-                               st, A);
+            launch_z3(A, st);
             break;
         case 3:
             if (!A.demag || film) return hipErrorInvalidValue;
-            hipLaunchKernelGGL((k_y<T, 2>), dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st, A);
+            launch_y3(A, true, st);
             break;
         case 4:
             if (A.demag) {

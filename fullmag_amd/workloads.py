@@ -1,0 +1,59 @@
+"""Synthetic problem builders for the benchmark and test workloads.
+
+Each builder returns the keyword arguments of `FdmPlan` (physical units,
+SoA-free host arrays) for one named workload.  Random inputs are seeded so
+every run sees the same problem.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MU0 = 4e-7 * np.pi
+
+
+def film_with_hole(seed: int = 42, n: int = 512) -> dict:
+    """n x n x 1 film of 2x2x5 nm cells with a centred circular hole of
+    radius n/8 cells (geometry mask by cell centres), Ms 8.6e5 A/m,
+    A 1.3e-11 J/m, alpha 0.02, B = (10 mT, 0, 0), seeded random m0 zeroed on
+    the hole, DP45 atol 1e-5."""
+    from .backend import hole_mask, random_unit_vectors
+    dx = 2e-9
+    mask = hole_mask(n, n, 1, dx, dx, (n // 8) * dx)
+    m0 = random_unit_vectors(seed, n * n)
+    m0[mask == 0] = 0.0
+    return dict(nx=n, ny=n, nz=1, dx=dx, dy=dx, dz=5e-9, ms=8.6e5, a_ex=1.3e-11, alpha=0.02, m0=m0,
+                integrator="dp45", h_ext=(0.01 / MU0, 0.0, 0.0), mask=mask, atol=1e-5, dt_min=1e-18,
+                dt_max=1e-10, headroom=0.8)
+
+
+def voronoi_axes(nx: int, ny: int, nz: int, cell: float, grains: int, seed: int):
+    """Per-cell easy axes of a seeded Voronoi grain structure: `grains`
+    sites uniform in the box, each cell takes the axis of the nearest site
+    (cell centres, Euclidean, non-periodic), axes uniform on the sphere
+    (normalised Gaussian triples).  Returns (axes (n,3), grain id (n,))."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    size = np.array([nx, ny, nz], dtype=float) * cell
+    sites = rng.random((grains, 3)) * size
+    ax = rng.standard_normal((grains, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    centres = np.stack([(x.ravel() + 0.5) * cell, (y.ravel() + 0.5) * cell, (z.ravel() + 0.5) * cell], axis=1)
+    _, gid = cKDTree(sites).query(centres, k=1)
+    return np.ascontiguousarray(ax[gid]), gid.astype(np.int64)
+
+
+def granular_block(nx: int = 256, ny: int = 256, nz: int = 64, grains: int = 512, seed: int = 7) -> dict:
+    """3-D granular block of 3 nm cubes: Ms 1.0e6 A/m, A 1.5e-11 J/m,
+    uniaxial Ku 5e5 J/m^3 with per-grain easy axes (seeded Voronoi, axes
+    uniform on the sphere), alpha 0.1, B ramp 0 -> 1 T along z over 1 ns,
+    seeded random m0, DP45 atol 1e-5."""
+    from .backend import random_unit_vectors
+    from . import _abi as abi
+    cell = 3e-9
+    axes, _ = voronoi_axes(nx, ny, nz, cell, grains, seed)
+    m0 = random_unit_vectors(seed, nx * ny * nz)
+    return dict(nx=nx, ny=ny, nz=nz, dx=cell, dy=cell, dz=cell, ms=1.0e6, a_ex=1.5e-11, alpha=0.1, m0=m0,
+                integrator="dp45", ku1=5e5, axis=(0.0, 0.0, 1.0), axis_field=axes, wave=abi.WAVE_RAMP,
+                wave_amp=(0.0, 0.0, 1.0 / MU0), wave_t0=0.0, wave_t1=1e-9, atol=1e-5, dt_min=1e-18,
+                dt_max=1e-10, headroom=0.8)

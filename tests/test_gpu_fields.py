@@ -102,3 +102,11 @@ def test_fields_full_size_film_512(precision):
 def test_fields_register_four_step_x_shapes(precision, nx, ny):
     # padded x lengths 64..2048 take the register four-step x passes
     _compare(random_case(nx, ny, 1, seed=nx + 1, hole_radius_cells=min(nx, ny) // 3), precision)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("nx,ny,nz", [(6, 32, 8), (5, 64, 16), (4, 128, 32), (3, 256, 64), (4, 32, 128)])
+def test_fields_register_four_step_3d_shapes(precision, nx, ny, nz):
+    # 3-D grids whose padded y (64..512) and z (16..256) lengths take the
+    # register four-step y-line and z passes
+    _compare(random_case(nx, ny, nz, seed=ny + nz, dx=3e-9, dy=3e-9, dz=3e-9), precision)

@@ -188,7 +188,7 @@ def test_fixed_step_advance_rk4_matches_oracle():
 
 @pytest.mark.parametrize("precision", ["double", "single"])
 @pytest.mark.parametrize("integrator", ["dp45", "rk4", "heun"])
-@pytest.mark.parametrize("shape", [(64, 48, 1), (50, 30, 1), (12, 10, 4)])
+@pytest.mark.parametrize("shape", [(64, 48, 1), (50, 30, 1), (12, 10, 4), (8, 32, 16)])
 def test_joined_stage_launches_equal_separate(precision, integrator, shape):
     # A trial step joins the inverse x pass of stage s with the forward x pass
     # of stage s+1 in one launch; with a poll callback the engine runs one
