@@ -219,6 +219,21 @@ WAVE_NONE, WAVE_RAMP, WAVE_SINC, WAVE_SINE = 0, 1, 2, 3
 
 POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
+class AdjointDesc(ctypes.Structure):
+    """fullmag_fdm_adjoint_desc (include/fullmag_fdm_ext.h)."""
+    _fields_ = [
+        ("struct_size", ctypes.c_uint64),
+        ("steps", ctypes.c_uint64),
+        ("dt", ctypes.c_double),
+        ("objective_first_step", ctypes.c_uint64),
+        ("objective_component", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("objective_weight", c_double_p),
+        ("objective_weight_len", ctypes.c_uint64),
+        ("checkpoint_interval", ctypes.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol of include/fullmag_fdm.h first,
 # then include/fullmag_fdm_ext.h.
 _H = ctypes.c_void_p
@@ -264,6 +279,8 @@ EXT_SYMBOLS = {
     "fullmag_fdm_backend_time_kernels": (ctypes.c_int, [_H, ctypes.c_int, c_double_p, ctypes.c_int]),
     "fullmag_fdm_random_unit_vectors": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, c_double_p]),
     "fullmag_fdm_backend_create_ext": (_H, [ctypes.POINTER(PlanDesc), ctypes.POINTER(PlanExt)]),
+    "fullmag_fdm_adjoint_forward": (ctypes.c_int, [_H, ctypes.POINTER(AdjointDesc), c_double_p]),
+    "fullmag_fdm_adjoint_backward": (ctypes.c_int, [_H, c_double_p, c_double_p]),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

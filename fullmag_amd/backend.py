@@ -237,6 +237,32 @@ class FdmBackend:
         d["steps_taken"] = int(n.value)
         return d
 
+    def adjoint_forward(self, steps: int, dt: float, weight: np.ndarray, first_step: int = 1, component: int = 2,
+                        checkpoint_interval: int = 0) -> float:
+        """Fixed-step run of `steps` steps with checkpoints; returns
+        J = sum_{n>=first_step} sum_i w_i m_{c,i}(t_n)^2 / (steps - first_step + 1)."""
+        w = np.ascontiguousarray(weight, dtype=np.float64).reshape(-1)
+        d = abi.AdjointDesc()
+        d.struct_size = ctypes.sizeof(abi.AdjointDesc)
+        d.steps = steps
+        d.dt = dt
+        d.objective_first_step = first_step
+        d.objective_component = component
+        d.objective_weight = _dp(w)
+        d.objective_weight_len = w.size
+        d.checkpoint_interval = checkpoint_interval
+        j = ctypes.c_double(0.0)
+        self._check(self.lib.fullmag_fdm_adjoint_forward(self.h, ctypes.byref(d), ctypes.byref(j)))
+        return j.value
+
+    def adjoint_backward(self):
+        """Reverse sweep after adjoint_forward: (dJ/dMs per cell (n,),
+        dJ/d waveform amplitude in A/m (3,))."""
+        g = np.zeros(self.n)
+        ga = np.zeros(3)
+        self._check(self.lib.fullmag_fdm_adjoint_backward(self.h, _dp(g), _dp(ga)))
+        return g, ga
+
     def set_adaptive_ext(self, rtol: float, growth: float = 0.0, shrink: float = 0.0):
         self._check(self.lib.fullmag_fdm_backend_set_adaptive_ext(self.h, rtol, growth, shrink))
 

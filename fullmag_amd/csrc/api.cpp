@@ -42,6 +42,16 @@ hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st);
 template <class T>
 hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st);
 template <class T>
+long long adj_blocks(const Args<T> &A);
+template <class T>
+hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipStream_t st);
+template <class T>
+hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, int with_obj, hipStream_t st);
+template <class T>
+hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st);
+template <class T>
+hipError_t adj_configure(const Args<T> &A);
+template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
 bool yfast_factors(int py, int *r1, int *r2);
 bool zfast_factors(int pz, int *r1, int *r2);
@@ -81,6 +91,21 @@ struct fullmag_fdm_backend {
     bool graph_tried = false;
     bool obs_fresh = false;
     void *obs_base[3] = {nullptr, nullptr, nullptr};  // hex, hd, heff
+    // adjoint sweep (fullmag_fdm_adjoint_*)
+    struct Adj {
+        std::vector<void *> bufs;
+        void *cp = nullptr;                       // ncp x 3N checkpoints
+        void *rec_y = nullptr, *rec_h = nullptr, *rec_vn = nullptr;
+        void *cot_m = nullptr, *cot_k = nullptr, *beta = nullptr, *obj_w = nullptr, *m_end = nullptr;
+        double *grad_ms = nullptr, *grad_amp = nullptr, *obj_part = nullptr;
+        uint64_t steps = 0, seg = 0, ncp = 0, first = 0;
+        uint32_t comp = 0;
+        double dt = 0.0, t0 = 0.0, obj_scale = 0.0;
+        long long blocks = 0;
+        int parity0 = 0;
+        fm::Ctl ctl_end{};
+        bool ready = false;
+    } adj;
 };
 
 struct fullmag_fdm_field_snapshot {
@@ -1017,6 +1042,7 @@ void fullmag_fdm_backend_destroy(fullmag_fdm_backend *h) {
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->graph) hipGraphDestroy(h->graph);
     for (void *p : h->allocs) hipFree(p);
+    for (void *p : h->adj.bufs) hipFree(p);
     if (h->ctl_h) hipHostFree(h->ctl_h);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -1209,6 +1235,246 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
     if (!push_ctl(h) || !pull_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
     h->obs_fresh = false;
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ adjoint --
+namespace {
+
+void adj_free(Backend *h) {
+    for (void *p : h->adj.bufs) hipFree(p);
+    h->adj = Backend::Adj{};
+}
+
+void *adj_alloc(Backend *h, size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+    h->adj.bufs.push_back(p);
+    return p;
+}
+
+template <class T>
+fm::AdjArgs<T> adj_args(Backend *h) {
+    auto &D = h->adj;
+    fm::AdjArgs<T> J{};
+    J.cot_m = static_cast<T *>(D.cot_m);
+    J.cot_k = static_cast<T *>(D.cot_k);
+    J.beta = static_cast<T *>(D.beta);
+    J.grad_ms = D.grad_ms;
+    J.grad_amp = D.grad_amp;
+    J.obj_part = D.obj_part;
+    J.obj_w = static_cast<const T *>(D.obj_w);
+    J.obj_comp = (int)D.comp;
+    J.obj_scale = D.obj_scale;
+    J.dt = D.dt;
+    return J;
+}
+
+template <class T>
+int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc *d, double *out_j) {
+    adj_free(h);
+    auto &D = h->adj;
+    const uint64_t n = h->n, steps = d->steps;
+    const int ns = h->nstage;
+    D.steps = steps;
+    D.dt = d->dt;
+    D.first = d->objective_first_step;
+    D.comp = d->objective_component;
+    D.obj_scale = 1.0 / (double)(steps - D.first + 1);
+    D.seg = d->checkpoint_interval ? d->checkpoint_interval : (uint64_t)std::ceil(std::sqrt((double)steps));
+    D.seg = std::min<uint64_t>(std::max<uint64_t>(D.seg, 1), steps);
+    D.ncp = (steps + D.seg - 1) / D.seg;
+    D.blocks = fm::adj_blocks(A);
+    const size_t v3 = 3 * n * sizeof(T), slots = (size_t)D.seg * (ns - 1) + 1;
+    D.cp = adj_alloc(h, D.ncp * v3);
+    D.rec_y = adj_alloc(h, slots * v3);
+    D.rec_h = adj_alloc(h, slots * v3);
+    D.rec_vn = adj_alloc(h, slots * n * sizeof(T));
+    D.cot_m = adj_alloc(h, v3);
+    D.cot_k = adj_alloc(h, fm::kMaxStages * v3);
+    D.beta = adj_alloc(h, v3);
+    D.obj_w = adj_alloc(h, n * sizeof(T));
+    D.m_end = adj_alloc(h, v3);
+    D.grad_ms = static_cast<double *>(adj_alloc(h, n * sizeof(double)));
+    D.grad_amp = static_cast<double *>(adj_alloc(h, (size_t)D.blocks * 3 * sizeof(double)));
+    D.obj_part = static_cast<double *>(adj_alloc(h, (size_t)D.blocks * 3 * sizeof(double)));
+    if (!D.cp || !D.rec_y || !D.rec_h || !D.rec_vn || !D.cot_m || !D.cot_k || !D.beta || !D.obj_w || !D.m_end ||
+        !D.grad_ms || !D.grad_amp || !D.obj_part) {
+        adj_free(h);
+        return fail(h, FULLMAG_FDM_ERR_CUDA, "adjoint: device allocation failed");
+    }
+    std::vector<T> w(n);
+    for (uint64_t i = 0; i < n; ++i) w[i] = (T)d->objective_weight[i];
+    if (!hip_ok(h, hipMemcpy(D.obj_w, w.data(), n * sizeof(T), hipMemcpyHostToDevice), "adjoint weights") ||
+        !hip_ok(h, fm::adj_configure(A), "adjoint LDS limits") || !pull_ctl(h))
+        return FULLMAG_FDM_ERR_CUDA;
+    fm::Ctl &c = *h->ctl_h;
+    D.t0 = c.t;
+    D.parity0 = c.parity;
+    const size_t n3 = 3 * n;
+    hipMemcpyAsync(D.cp, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream);
+    hipMemsetAsync(D.obj_part, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
+    c.loop = 1;
+    c.t_end = std::numeric_limits<double>::infinity();
+    c.step_end = c.step + steps;
+    c.dt_carry = D.dt;
+    c.dt = D.dt;
+    c.done = 0;
+    if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    ensure_graph(h);
+    h->obs_fresh = false;
+    const fm::AdjArgs<T> J = adj_args<T>(h);
+    int par = D.parity0;
+    for (uint64_t s = 1; s <= steps; ++s) {
+        if (!launch_attempt(h)) return FULLMAG_FDM_ERR_CUDA;
+        par ^= 1;
+        if (s >= D.first && !hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective"))
+            return FULLMAG_FDM_ERR_CUDA;
+        if (s % D.seg == 0 && s < steps)
+            hipMemcpyAsync(static_cast<char *>(D.cp) + (s / D.seg) * v3, A.m_all + (size_t)par * n3, v3,
+                           hipMemcpyDeviceToDevice, h->stream);
+    }
+    std::vector<double> part((size_t)D.blocks * 3);
+    if (!hip_ok(h, hipMemcpyAsync(part.data(), D.obj_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream),
+                "adjoint objective readback") ||
+        !pull_ctl(h))
+        return FULLMAG_FDM_ERR_CUDA;
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, std::string("device error: ") + hipGetErrorString(le));
+    double jsum = 0.0;
+    for (long long b = 0; b < D.blocks; ++b) jsum += part[(size_t)b * 3];
+    c.loop = 0;
+    c.done = 0;
+    if (!push_ctl(h) || !pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    hipMemcpyAsync(D.m_end, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream);
+    D.ctl_end = c;
+    D.ready = true;
+    if (out_j) *out_j = jsum * D.obj_scale;
+    return FULLMAG_FDM_OK;
+}
+
+template <class T>
+int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad_amp) {
+    auto &D = h->adj;
+    const uint64_t n = h->n;
+    const int ns = h->nstage;
+    const size_t n3 = 3 * n, v3 = n3 * sizeof(T);
+    hipMemsetAsync(D.grad_ms, 0, n * sizeof(double), h->stream);
+    hipMemsetAsync(D.grad_amp, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
+    hipMemsetAsync(D.cot_m, 0, v3, h->stream);
+    // host replica of the device time sequence t_n = t_{n-1} + dt
+    std::vector<double> tn(D.steps + 1);
+    tn[0] = D.t0;
+    for (uint64_t s = 1; s <= D.steps; ++s) tn[s] = tn[s - 1] + D.dt;
+    fm::AdjArgs<T> J = adj_args<T>(h);
+    fm::Ctl &c = *h->ctl_h;
+    h->obs_fresh = false;
+    T *rec_y = static_cast<T *>(D.rec_y), *rec_h = static_cast<T *>(D.rec_h), *rec_vn = static_cast<T *>(D.rec_vn);
+    const size_t step_v = (size_t)(ns - 1) * n3, step_s = (size_t)(ns - 1) * n;
+    for (uint64_t k = D.ncp; k-- > 0;) {
+        const uint64_t n0 = k * D.seg, n1 = std::min<uint64_t>(n0 + D.seg, D.steps);
+        // the pinned control block is read by the queued upload only when the
+        // stream reaches it: drain the stream before rewriting it
+        if (!hip_ok(h, hipStreamSynchronize(h->stream), "adjoint sync")) return FULLMAG_FDM_ERR_CUDA;
+        // recompute the segment from its checkpoint, recording every stage
+        c = D.ctl_end;
+        c.t = tn[n0];
+        c.parity = 0;
+        c.loop = 1;
+        c.t_end = std::numeric_limits<double>::infinity();
+        c.step_end = c.step + (n1 - n0);
+        c.dt_carry = D.dt;
+        c.dt = D.dt;
+        c.done = 0;
+        const char *cpk = static_cast<const char *>(D.cp) + k * v3;
+        hipMemcpyAsync(A.m_all, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
+        hipMemcpyAsync(rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
+        if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+        fm::Args<T> B = A;
+        B.rec_h = rec_h;
+        if (!hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint recompute (k1)")) return FULLMAG_FDM_ERR_CUDA;
+        for (uint64_t i = 0; i < n1 - n0; ++i) {
+            B.rec_y = rec_y + i * step_v;
+            B.rec_h = rec_h + i * step_v;
+            B.rec_vn = rec_vn + i * step_s;
+            if (!hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint recompute")) return FULLMAG_FDM_ERR_CUDA;
+        }
+        // the recompute ends with the device loop marked done; the y/z passes
+        // that the reverse sweep shares with the forward step return early
+        // while it is set
+        hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream);
+        // reverse sweep through the segment
+        for (uint64_t s = n1; s-- > n0;) {
+            const uint64_t i = s - n0;
+            J.rec_y = rec_y + i * step_v;
+            J.rec_h = rec_h + i * step_v;
+            J.rec_vn = rec_vn + i * step_s;
+            J.t_n = tn[s];
+            if (!hip_ok(h, fm::enqueue_adj_init(A, J, ns, s + 1 >= D.first ? 1 : 0, h->stream), "adjoint init"))
+                return FULLMAG_FDM_ERR_CUDA;
+            for (int j = ns - 1; j >= 1; --j)
+                if (!hip_ok(h, fm::enqueue_adj_stage(A, J, j, h->stream), "adjoint stage"))
+                    return FULLMAG_FDM_ERR_CUDA;
+        }
+    }
+    std::vector<double> gms(n), gamp((size_t)D.blocks * 3);
+    if (!hip_ok(h, hipMemcpyAsync(gms.data(), D.grad_ms, n * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                "gradient readback") ||
+        !hip_ok(h, hipMemcpyAsync(gamp.data(), D.grad_amp, gamp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream),
+                "gradient readback") ||
+        !hip_ok(h, hipStreamSynchronize(h->stream), "adjoint sync"))
+        return FULLMAG_FDM_ERR_CUDA;
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, std::string("device error: ") + hipGetErrorString(le));
+    // restore the final state of the forward run and its k1
+    c = D.ctl_end;
+    hipMemcpyAsync(A.m_all + (size_t)c.parity * n3, D.m_end, v3, hipMemcpyDeviceToDevice, h->stream);
+    if (!push_ctl(h) || !eval_current(h, false)) return FULLMAG_FDM_ERR_CUDA;
+    if (grad_ms)
+        for (uint64_t i = 0; i < n; ++i) grad_ms[i] = gms[i];
+    if (grad_amp) {
+        grad_amp[0] = grad_amp[1] = grad_amp[2] = 0.0;
+        for (long long b = 0; b < D.blocks; ++b)
+            for (int q = 0; q < 3; ++q) grad_amp[q] += gamp[(size_t)b * 3 + q];
+    }
+    return FULLMAG_FDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *h, const fullmag_fdm_adjoint_desc *d, double *out_objective) {
+    if (!h || !d) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (d->struct_size < sizeof(fullmag_fdm_adjoint_desc))
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "fullmag_fdm_adjoint_desc.struct_size too small");
+    if (h->adaptive) return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint needs a fixed-step integrator (rk4 or heun)");
+    if (d->steps == 0 || !(d->dt > 0.0) || !std::isfinite(d->dt))
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: steps must be >= 1 and dt positive and finite");
+    if (d->objective_first_step < 1 || d->objective_first_step > d->steps)
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: objective_first_step must be in 1..steps");
+    if (d->objective_component > 2) return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: objective_component must be 0..2");
+    if (!d->objective_weight || d->objective_weight_len != h->n)
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: objective_weight_len must equal the cell count");
+    const bool lut = h->fp64 ? h->ad.region != nullptr : h->af.region != nullptr;
+    if (lut) return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: region exchange tables are not supported");
+    h->err.clear();
+    hipSetDevice(h->device);
+    h->adj.ready = false;
+    return with_args(h, [&](auto &A) { return adj_forward_typed(h, A, d, out_objective); });
+}
+
+int fullmag_fdm_adjoint_backward(fullmag_fdm_backend *h, double *grad_ms, double *grad_field_amplitude) {
+    if (!h) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    if (!h->adj.ready) return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint_backward needs a preceding adjoint_forward");
+    h->err.clear();
+    hipSetDevice(h->device);
+    return with_args(h, [&](auto &A) { return adj_backward_typed(h, A, grad_ms, grad_field_amplitude); });
 }
 
 }  // extern "C"

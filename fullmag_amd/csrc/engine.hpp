@@ -126,6 +126,10 @@ struct Args {
     unsigned *counter;      // finished work-groups of the current partial-producing launch
     // observables (written only when non-NULL)
     T *obs_hex, *obs_hd, *obs_heff;
+    // trajectory record of one step for the adjoint sweep (written only when
+    // non-NULL): stage state y_s at rec_y[(s-1)*3N], effective field at y_s
+    // at rec_h[(s-1)*3N], pre-normalisation norm |v_s| at rec_vn[(s-2)*N]
+    T *rec_y, *rec_h, *rec_vn;
     // physics
     T ms;
     T ex_pre;               // 2A/(mu0 Ms)
@@ -157,6 +161,24 @@ struct Args {
     int method;             // index into the constant method table
     int stage_nk[kMaxStages + 1];  // k-vectors combined by stage s (0 for s <= 1)
     FftDesc fx, fy, fz;
+};
+
+// Reverse (adjoint) sweep of one fixed step: cotangents and parameter
+// gradients (see the adjoint kernels for the formulas).
+template <class T>
+struct AdjArgs {
+    const T *rec_y, *rec_h, *rec_vn;  // record of the step being reversed (layout as in Args)
+    T *cot_m;                         // 3N: cotangent of the step's start state
+    T *cot_k;                         // kMaxStages x 3N: cotangent of k_j at [j*3N]
+    T *beta;                          // 3N: field cotangent of the stage being reversed
+    double *grad_ms;                  // N: dJ/dMs_i, accumulated
+    double *grad_amp;                 // [work-groups][3]: dJ/d(wave amplitude), accumulated
+    double *obj_part;                 // [work-groups]: objective partial sums, accumulated
+    const T *obj_w;                   // N objective weights
+    int obj_comp;                     // objective component (0 x, 1 y, 2 z)
+    double obj_scale;                 // 1 / (number of objective samples)
+    double t_n;                       // time at the start of the step
+    double dt;                        // step size
 };
 
 // Method indices in the constant table.

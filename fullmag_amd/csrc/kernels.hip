@@ -29,6 +29,7 @@
 //
 // Without demag the chain is k_stage (combination) + k_local.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include "engine.hpp"
@@ -367,6 +368,13 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     out[cell] = (T)x;
     out[N + cell] = (T)y;
     out[2 * N + cell] = (T)z;
+    if (A.rec_y) {
+        T *ry = A.rec_y + (size_t)(s - 1) * 3 * N;
+        ry[cell] = (T)x;
+        ry[N + cell] = (T)y;
+        ry[2 * N + cell] = (T)z;
+        A.rec_vn[(size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+    }
     v[0] = (T)x;
     v[1] = (T)y;
     v[2] = (T)z;
@@ -495,12 +503,13 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 template <class T, bool LUT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
                                              const CellIn<T> &g, T *__restrict__ kout, const T hd[3], T wf,
-                                             bool stats, bool err, Acc &acc) {
+                                             bool stats, bool err, Acc &acc, T *__restrict__ hrec) {
     const int N = (int)A.ncell;
     if (!g.act) {
         kout[cell] = T(0);
         kout[N + cell] = T(0);
         kout[2 * N + cell] = T(0);
+        if (hrec) hrec[cell] = hrec[N + cell] = hrec[2 * N + cell] = T(0);
         if (A.obs_hex) {
             A.obs_hex[cell] = A.obs_hex[N + cell] = A.obs_hex[2 * N + cell] = T(0);
             A.obs_hd[cell] = A.obs_hd[N + cell] = A.obs_hd[2 * N + cell] = T(0);
@@ -573,6 +582,11 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     kout[cell] = k[0];
     kout[N + cell] = k[1];
     kout[2 * N + cell] = k[2];
+    if (hrec) {
+        hrec[cell] = h[0];
+        hrec[N + cell] = h[1];
+        hrec[2 * N + cell] = h[2];
+    }
     if (A.obs_hex) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -822,6 +836,12 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
                 v[0] = mv[j][0];
                 v[1] = mv[j][1];
                 v[2] = mv[j][2];
+                if (A.rec_y && c0 >= 0) {
+                    const int cell = c0 + x;
+                    A.rec_y[cell] = v[0];
+                    A.rec_y[N + cell] = v[1];
+                    A.rec_y[2 * N + cell] = v[2];
+                }
             } else {
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
@@ -842,6 +862,13 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
                     yout[cell] = v[0];
                     yout[N + cell] = v[1];
                     yout[2 * N + cell] = v[2];
+                    if (A.rec_y) {
+                        T *ry = A.rec_y + (size_t)(s - 1) * 3 * N;
+                        ry[cell] = v[0];
+                        ry[N + cell] = v[1];
+                        ry[2 * N + cell] = v[2];
+                        A.rec_vn[(size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+                    }
                 }
             }
             const T w = c0 >= 0 ? msv[j] : T(0);
@@ -952,6 +979,7 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
     const float rnx = 1.0f / (float)nx;
     const T *yin = stage_input(A, M, s, p);
     T *kout = stage_rhs(A, M, s, p);
+    T *hrec = A.rec_h ? A.rec_h + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
     constexpr int CH = Chunk<T>::value;
     for (int base = 0; base < ncl; base += CH * bd) {
@@ -978,7 +1006,7 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
                 const C2<T> zv = res[(l >> 1) * px + x];
                 hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
             }
-            compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc);
+            compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec);
         }
     }
 }
@@ -1432,6 +1460,317 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(Args<T> A) {
     }
 }
 
+// ------------------------------------------------------------ adjoint --
+// Reverse sweep of one fixed step of an explicit normalised Runge-Kutta
+// method (RK4, Heun; the forward rows are the method table's).  Forward:
+// y_1 = m_n, k_j = f(y_j, t_n + c_j dt), y_s = N(v_s) with
+// v_s = m_n + dt*scale_s*sum_q a_sq k_q and N(v) = v/|v|, m_{n+1} = y_nstage;
+// f(m) = -gb (P m x H + alpha m x (m x H)), H = H_ex + H_d + H_ani + H_ext.
+//
+// Cotangents, for a cotangent r of y_s and a of k_j:
+//   N:  c = (r - y (y.r)) / |v|   ->  cot m_n += c,  cot k_q += dt scale_s a_sq c
+//   f:  beta = -gb (P a x m + alpha (a x m) x m)              (= (df/dH)^T a)
+//       cot y_j = -gb (P H x a + alpha ((m x H) x a + H x (a x m)))
+//               + (2A/mu0) sum_nbr (beta_j/Ms_j - beta_i/Ms_i)/d^2   (exchange Laplacian, symmetric)
+//               + Ms_i (-K beta)_i                                  (demag, K symmetric)
+//               + (c1 + 3 c2 (m.u)^2)(beta.u) u                     (uniaxial anisotropy)
+//   parameters: dJ/dMs_i += -beta_i.(H_ex,i + H_ani,i)/Ms_i + m_i.(-K beta)_i
+//               dJ/dH1   += sum_i beta_i f(t) w_i   (time-dependent Zeeman amplitude)
+// (-K beta) is the demag convolution of beta with unit Ms: the forward FFT
+// passes applied to beta.
+
+template <class T>
+__device__ __forceinline__ void cross3(const T a[3], const T b[3], T o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// beta of stage j at `cell` from the recorded state and the cotangent of k_j.
+template <class T>
+__device__ __forceinline__ void adj_beta_cell(const Args<T> &A, const AdjArgs<T> &J, int j, int cell, T b[3]) {
+    const int N = (int)A.ncell;
+    const T *y = J.rec_y + (size_t)(j - 1) * 3 * N;
+    const T *a = J.cot_k + (size_t)j * 3 * N;
+    const T m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
+    const T av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
+    T axm[3], axmxm[3];
+    cross3(av, m, axm);
+    cross3(axm, m, axmxm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[c] = -A.gamma_bar * ((A.precession ? axm[c] : T(0)) + A.alpha * axmxm[c]);
+}
+
+// Cotangent of y_j at `cell` (hd = (-K beta)_i, zero without demag),
+// parameter gradients, then the normalisation of stage j and the
+// accumulation into cot m_n and cot k_q.
+template <class T>
+__device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T> &J, const Method &M, int j,
+                                               int cell, int x, int y, int z, const T hd[3], T wf, double amp[3]) {
+    const int N = (int)A.ncell;
+    if (A.mask && !A.mask[cell]) return;
+    const T *yv = J.rec_y + (size_t)(j - 1) * 3 * N;
+    const T *Hv = J.rec_h + (size_t)(j - 1) * 3 * N;
+    const T *a = J.cot_k + (size_t)j * 3 * N;
+    const T m[3] = {yv[cell], yv[N + cell], yv[2 * N + cell]};
+    const T H[3] = {Hv[cell], Hv[N + cell], Hv[2 * N + cell]};
+    const T av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
+    const T bi[3] = {J.beta[cell], J.beta[N + cell], J.beta[2 * N + cell]};
+    const T ms_i = A.ms_cell ? A.ms_cell[cell] : A.ms;
+    T Hxa[3], mxH[3], mxHxa[3], axm[3], Hxaxm[3];
+    cross3(H, av, Hxa);
+    cross3(m, H, mxH);
+    cross3(mxH, av, mxHxa);
+    cross3(av, m, axm);
+    cross3(H, axm, Hxaxm);
+    T r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = -A.gamma_bar * ((A.precession ? Hxa[c] : T(0)) + A.alpha * (mxHxa[c] + Hxaxm[c]));
+    // exchange: adjoint stencil on beta/Ms, and H_ex at y_j for the Ms gradient
+    const int sy = A.nx, sz = A.nx * A.ny;
+    const int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
+                       y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
+                       z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
+    const T wq[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
+    T hex[3] = {T(0), T(0), T(0)};
+    const T inv_ms_i = T(1) / ms_i;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int o = nb[q];
+        const bool use = o != cell && (!A.mask || A.mask[o]);
+        if (!use) continue;
+        const T inv_ms_o = A.ms_cell ? T(1) / A.ms_cell[o] : inv_ms_i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r[c] += A.two_a_mu0 * wq[q] * (J.beta[c * N + o] * inv_ms_o - bi[c] * inv_ms_i);
+            hex[c] += wq[q] * (yv[c * N + o] - m[c]);
+        }
+    }
+    const T pre = A.ms_cell ? A.two_a_mu0 * inv_ms_i : A.ex_pre;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        hex[c] *= pre;
+        r[c] += ms_i * hd[c];
+    }
+    T hani[3] = {T(0), T(0), T(0)};
+    if (A.has_uni) {
+        T u[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c] = A.axis_cell ? A.axis_cell[c * N + cell] : A.axis[c];
+        T c1 = A.uni_c1, c2 = A.uni_c2;
+        if (A.ku1_cell || A.ku2_cell || A.ms_cell) {
+            const T k1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
+            const T k2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
+            c1 = A.two_mu0inv * k1 * inv_ms_i;
+            c2 = T(2) * A.two_mu0inv * k2 * inv_ms_i;
+        }
+        const T mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+        const T bu = bi[0] * u[0] + bi[1] * u[1] + bi[2] * u[2];
+        const T dcoef = (c1 + T(3) * c2 * mu * mu) * bu;
+        const T coef = c1 * mu + c2 * mu * mu * mu;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            r[c] += dcoef * u[c];
+            hani[c] = coef * u[c];
+        }
+    }
+    // parameter gradients (FP64 accumulation)
+    {
+        double g = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            g += -(double)bi[c] * ((double)hex[c] + (double)hani[c]) / (double)ms_i + (double)m[c] * (double)hd[c];
+        J.grad_ms[cell] += g;
+        if (A.wave_kind && (!A.wave_mask || A.wave_mask[cell])) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) amp[c] += (double)bi[c] * (double)wf;
+        }
+    }
+    if (j >= 2) {
+        const T vn = J.rec_vn[(size_t)(j - 2) * N + cell];
+        const T yr = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
+        T cv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cv[c] = vn > T(0) ? (r[c] - m[c] * yr) / vn : T(0);
+        const StageRow &R = M.row[j];
+        const T f = T(J.dt * R.scale);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J.cot_m[c * N + cell] += cv[c];
+        for (int q = 0; q < R.nk; ++q) {
+            T *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
+            const T w = f * T(R.a[q]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ck[c * N + cell] += w * cv[c];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J.cot_m[c * N + cell] += r[c];
+    }
+}
+
+// Deterministic block sum of 3 doubles added to out[0..3) by thread 0.
+__device__ void block_add3(double v[3], double *out) {
+    __shared__ double red3[3][kThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double x = v[i];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) red3[i][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double x = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) x += red3[threadIdx.x][w];
+        out[threadIdx.x] += x;
+    }
+}
+
+// Start of the reverse sweep of step n: lambda = cot m_{n+1} (+ objective
+// derivative at m_{n+1}), normalisation of the final stage, k cotangents
+// of the final row; cot m_n restarts from there.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J, int nstage, int with_obj) {
+    const int N = (int)A.ncell;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= N) return;
+    const Method &M = c_methods[A.method];
+    const T *y = J.rec_y + (size_t)(nstage - 1) * 3 * N;
+    const T m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
+    T lam[3] = {J.cot_m[cell], J.cot_m[N + cell], J.cot_m[2 * N + cell]};
+    if (with_obj) lam[J.obj_comp] += T(2.0 * J.obj_scale) * J.obj_w[cell] * m[J.obj_comp];
+    const T vn = J.rec_vn[(size_t)(nstage - 2) * N + cell];
+    const T yl = m[0] * lam[0] + m[1] * lam[1] + m[2] * lam[2];
+    T cv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cv[c] = vn > T(0) ? (lam[c] - m[c] * yl) / vn : T(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) J.cot_m[c * N + cell] = cv[c];
+    for (int j = 1; j < nstage; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) J.cot_k[((size_t)j * 3 + c) * N + cell] = T(0);
+    const StageRow &R = M.row[nstage];
+    const T f = T(J.dt * R.scale);
+    for (int q = 0; q < R.nk; ++q) {
+        T *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
+        const T w = f * T(R.a[q]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ck[c * N + cell] += w * cv[c];
+    }
+}
+
+// Objective partial sums of the current state: obj_part[block] += sum w_i m_c,i^2.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_adj_objective(Args<T> A, AdjArgs<T> J) {
+    const int N = (int)A.ncell;
+    const T *m = A.m_all + (size_t)A.ctl->parity * 3 * N + (size_t)J.obj_comp * N;
+    double s = 0.0;
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < N; cell += gridDim.x * blockDim.x)
+        s += (double)J.obj_w[cell] * (double)m[cell] * (double)m[cell];
+    double v[3] = {s, 0.0, 0.0};
+    block_add3(v, J.obj_part + (size_t)blockIdx.x * 3);
+}
+
+// Without demag: beta of stage j, one thread per cell.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J, int j) {
+    const int N = (int)A.ncell;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= N) return;
+    T b[3];
+    adj_beta_cell(A, J, j, cell, b);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) J.beta[c * N + cell] = b[c];
+}
+
+// Without demag: cotangent of y_j, one thread per cell.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J, int j) {
+    const int N = (int)A.ncell;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    const Method &M = c_methods[A.method];
+    double amp[3] = {0.0, 0.0, 0.0};
+    if (cell < N) {
+        const int x = cell % A.nx, y = (cell / A.nx) % A.ny, z = cell / (A.nx * A.ny);
+        const T hd[3] = {T(0), T(0), T(0)};
+        const T wf = A.wave_kind ? wave_factor(A, J.t_n + M.node[j] * J.dt) : T(0);
+        adj_local_cell(A, J, M, j, cell, x, y, z, hd, wf, amp);
+    }
+    block_add3(amp, J.grad_amp + (size_t)blockIdx.x * 3);
+}
+
+// With demag: beta of stage j for the work-group's x-rows, stored and
+// packed (unit Ms) two real rows per complex sequence, forward x FFT.
+template <class T, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J, int j) {
+    constexpr bool FAST = R1 > 0;
+    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
+    __shared__ XRows xr;
+    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    __syncthreads();
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int rows = A.rows_per_wg, ncl = rows * nx, nl = 3 * rows, N = (int)A.ncell;
+    const float rnx = 1.0f / (float)nx;
+    for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
+        const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+        const int c0 = xr.cell0[rr];
+        T b[3] = {T(0), T(0), T(0)};
+        if (c0 >= 0) {
+            adj_beta_cell(A, J, j, c0 + x, b);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) J.beta[c * N + c0 + x] = b[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int l = 3 * rr + c;
+            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
+            e[l & 1] = b[c];
+        }
+    }
+    if (nl & 1)
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * px + x].y = T(0);
+    x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
+}
+
+// With demag: inverse x FFT of the convolved beta spectrum (1/P), then the
+// cotangent of y_j for the work-group's cells.
+template <class T, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_adj_bx(Args<T> A, AdjArgs<T> J, int j) {
+    constexpr bool FAST = R1 > 0;
+    const Method &M = c_methods[A.method];
+    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *b1 = b0 + A.nseq_x * A.px;
+    C2<T> *tw = b1 + A.nseq_x * A.px;
+    __shared__ XRows xr;
+    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    __syncthreads();
+    const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int ncl = A.rows_per_wg * nx;
+    const float rnx = 1.0f / (float)nx;
+    const T scale = T(A.inv_p);
+    const T wf = A.wave_kind ? wave_factor(A, J.t_n + M.node[j] * J.dt) : T(0);
+    double amp[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
+        const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+        const int c0 = xr.cell0[rr];
+        if (c0 < 0) continue;
+        T hd[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int l = 3 * rr + c;
+            const C2<T> zv = res[(l >> 1) * px + x];
+            hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
+        }
+        adj_local_cell(A, J, M, j, c0 + x, x, xr.y[rr], xr.z[rr], hd, wf, amp);
+    }
+    block_add3(amp, J.grad_amp + (size_t)blockIdx.x * 3);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
     Ctl *ctl = A.ctl;
@@ -1462,7 +1801,8 @@ __global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats,
         const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
         CellIn<T> g;
         gather_cell<T, LUT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
-        compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc);
+        compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc,
+                             A.rec_h ? A.rec_h + (size_t)(s - 1) * 3 * A.ncell : nullptr);
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
     if (control) finish_last_block(A, control);
@@ -2048,6 +2388,84 @@ hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
     return hipGetLastError();
 }
 
+// ---- adjoint launches
+template <class T, int R1, int R2>
+static void launch_adj_x_shape(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    if (inverse) k_adj_bx<T, R1, R2><<<g, b, lds_x(A), st>>>(A, J, j);
+    else k_adj_fx<T, R1, R2><<<g, b, lds_x(A), st>>>(A, J, j);
+}
+template <class T>
+static void launch_adj_x(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
+    switch (A.xfast) {
+        case 64: launch_adj_x_shape<T, 8, 8>(A, J, j, inverse, st); break;
+        case 128: launch_adj_x_shape<T, 8, 16>(A, J, j, inverse, st); break;
+        case 256: launch_adj_x_shape<T, 16, 16>(A, J, j, inverse, st); break;
+        case 512: launch_adj_x_shape<T, 16, 32>(A, J, j, inverse, st); break;
+        case 1024: launch_adj_x_shape<T, 32, 32>(A, J, j, inverse, st); break;
+        case 2048: launch_adj_x_shape<T, 64, 32>(A, J, j, inverse, st); break;
+        default: launch_adj_x_shape<T, 0, 0>(A, J, j, inverse, st); break;
+    }
+}
+template <class T, int R1, int R2>
+static hipError_t set_lds_adj_shape(size_t bytes) {
+    hipError_t e = set_lds<T>((const void *)k_adj_fx<T, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_adj_bx<T, R1, R2>, bytes);
+    return e;
+}
+template <class T>
+static hipError_t set_lds_adj(int xfast, size_t bytes) {
+    switch (xfast) {
+        case 64: return set_lds_adj_shape<T, 8, 8>(bytes);
+        case 128: return set_lds_adj_shape<T, 8, 16>(bytes);
+        case 256: return set_lds_adj_shape<T, 16, 16>(bytes);
+        case 512: return set_lds_adj_shape<T, 16, 32>(bytes);
+        case 1024: return set_lds_adj_shape<T, 32, 32>(bytes);
+        case 2048: return set_lds_adj_shape<T, 64, 32>(bytes);
+        default: return set_lds_adj_shape<T, 0, 0>(bytes);
+    }
+}
+
+// Work-groups of the adjoint per-block partial arrays (3 doubles each).
+template <class T>
+long long adj_blocks(const Args<T> &A) {
+    const long long cells = (A.ncell + kThreads - 1) / kThreads;
+    const long long xs = A.demag ? num_partials(A) : 0;
+    return std::max(cells, xs);
+}
+
+// Reverse sweep of stage j (1 <= j < nstage): beta, its demag convolution,
+// cotangent of y_j with the normalisation and accumulation.
+template <class T>
+hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipStream_t st) {
+    const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
+    if (A.demag) {
+        launch_adj_x(A, J, j, false, st);
+        launch_yz(A, st);
+        launch_adj_x(A, J, j, true, st);
+    } else {
+        k_adj_beta<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, j);
+        k_adj_local<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, j);
+    }
+    return hipGetLastError();
+}
+template <class T>
+hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, int with_obj, hipStream_t st) {
+    const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
+    k_adj_init<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, nstage, with_obj);
+    return hipGetLastError();
+}
+template <class T>
+hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st) {
+    const unsigned gc = (unsigned)std::min<long long>((A.ncell + kThreads - 1) / kThreads, 1024);
+    k_adj_objective<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J);
+    return hipGetLastError();
+}
+template <class T>
+hipError_t adj_configure(const Args<T> &A) {
+    return A.demag ? set_lds_adj<T>(A.xfast, lds_x(A)) : hipSuccess;
+}
+
 template <class T>
 hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st) {
     hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(kThreads), 0, st, A, (int)num_partials(A), observe);
@@ -2065,6 +2483,11 @@ template long long num_partials<FM_REAL>(const Args<FM_REAL> &);
 template hipError_t enqueue_eval<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
 template hipError_t enqueue_trial<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
 template hipError_t enqueue_control<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
+template long long adj_blocks<FM_REAL>(const Args<FM_REAL> &);
+template hipError_t enqueue_adj_stage<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, hipStream_t);
+template hipError_t enqueue_adj_init<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, int, hipStream_t);
+template hipError_t enqueue_adj_objective<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, hipStream_t);
+template hipError_t adj_configure<FM_REAL>(const Args<FM_REAL> &);
 
 }  // namespace fm
 

@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 2
+#define FULLMAG_FDM_EXT_VERSION 3
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -100,6 +100,42 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *handle, int reps, doub
  * (the generator would never leave its rejection loop).
  */
 int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out);
+
+/*
+ * Adjoint sensitivities of a fixed-step run (no reference counterpart; the
+ * reference has no differentiable path).  The handle's plan must use a
+ * fixed-step integrator (RK4 or Heun) and uniform exchange (no region
+ * table).  Objective over the states m_n after steps n = first..steps:
+ *   J = sum_{n=first}^{steps} sum_i w_i m_{c,i}(t_n)^2 / (steps - first + 1)
+ * with c = objective_component and w = objective_weight.
+ *
+ * adjoint_forward runs `steps` steps of size dt from the current state and
+ * time, keeping the state every checkpoint_interval steps (0: ceil(sqrt(
+ * steps))), and returns J.  adjoint_backward then sweeps the steps in
+ * reverse (each checkpoint segment is recomputed once with its stage
+ * states and fields recorded) and returns dJ/dMs_i for every cell
+ * (grad_ms, n values, 0 on inactive cells) and dJ/d field_amplitude_am of
+ * the plan's waveform (grad_field_amplitude, 3 values).  Either output may
+ * be NULL.  After either call the handle holds the final state m_steps at
+ * t0 + steps*dt.  Memory: the checkpoints plus one segment's record,
+ * about (3 + 7 (nstage-1)) x 3N values of the state precision per step of
+ * a segment.
+ */
+typedef struct fullmag_fdm_adjoint_desc {
+    uint64_t struct_size;             /* sizeof(fullmag_fdm_adjoint_desc) */
+    uint64_t steps;                   /* fixed steps of the run (>= 1) */
+    double dt;                        /* step size [s] */
+    uint64_t objective_first_step;    /* first step whose state enters J (1..steps) */
+    uint32_t objective_component;     /* 0 x, 1 y, 2 z */
+    uint32_t reserved0;
+    const double *objective_weight;   /* n per-cell weights */
+    uint64_t objective_weight_len;
+    uint64_t checkpoint_interval;     /* 0 = ceil(sqrt(steps)) */
+} fullmag_fdm_adjoint_desc;
+
+int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *handle, const fullmag_fdm_adjoint_desc *desc,
+                                double *out_objective);
+int fullmag_fdm_adjoint_backward(fullmag_fdm_backend *handle, double *grad_ms, double *grad_field_amplitude);
 
 #ifdef __cplusplus
 }

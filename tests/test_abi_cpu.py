@@ -49,11 +49,13 @@ _PROBE = r"""
 #include <stddef.h>
 #include <stdio.h>
 #include "fullmag_fdm.h"
+#include "fullmag_fdm_ext.h"
 #define F(s, f) printf("%s.%s %zu\n", #s, #f, offsetof(s, f));
 int main(void) {
   printf("sizeof.plan %zu\nsizeof.stats %zu\nsizeof.info %zu\nsizeof.snap %zu\n",
          sizeof(fullmag_fdm_plan_desc), sizeof(fullmag_fdm_step_stats),
          sizeof(fullmag_fdm_device_info), sizeof(fullmag_fdm_snapshot_desc));
+  printf("sizeof.ext %zu\nsizeof.adj %zu\n", sizeof(fullmag_fdm_plan_ext), sizeof(fullmag_fdm_adjoint_desc));
   FIELDS
   return 0;
 }
@@ -63,8 +65,10 @@ int main(void) {
 def test_struct_layout_matches_c_header(tmp_path):
     from fullmag_amd import _abi
     lines = []
-    for st, cls in (("fullmag_fdm_plan_desc", _abi.PlanDesc), ("fullmag_fdm_step_stats", _abi.StepStats),
-                    ("fullmag_fdm_device_info", _abi.DeviceInfo)):
+    structs = (("fullmag_fdm_plan_desc", _abi.PlanDesc), ("fullmag_fdm_step_stats", _abi.StepStats),
+               ("fullmag_fdm_device_info", _abi.DeviceInfo), ("fullmag_fdm_plan_ext", _abi.PlanExt),
+               ("fullmag_fdm_adjoint_desc", _abi.AdjointDesc))
+    for st, cls in structs:
         for f, _ in cls._fields_:
             lines.append(f"F({st}, {f})")
     src = tmp_path / "probe.c"
@@ -77,8 +81,9 @@ def test_struct_layout_matches_c_header(tmp_path):
     assert int(out["sizeof.stats"]) == ctypes.sizeof(_abi.StepStats)
     assert int(out["sizeof.info"]) == ctypes.sizeof(_abi.DeviceInfo)
     assert int(out["sizeof.snap"]) == ctypes.sizeof(_abi.SnapshotDesc)
-    for st, cls in (("fullmag_fdm_plan_desc", _abi.PlanDesc), ("fullmag_fdm_step_stats", _abi.StepStats),
-                    ("fullmag_fdm_device_info", _abi.DeviceInfo)):
+    assert int(out["sizeof.ext"]) == ctypes.sizeof(_abi.PlanExt)
+    assert int(out["sizeof.adj"]) == ctypes.sizeof(_abi.AdjointDesc)
+    for st, cls in structs:
         for f, _ in cls._fields_:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
 
