@@ -219,6 +219,73 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}), flush=True)
 
 
+def adjoint_bench(steps, fd_params, fd_h=1e-3):
+    """Config-5 waveguide: forward cost, gradient cost (forward + reverse
+    sweep) and, optionally, central differences of the FP64 forward on
+    seeded parameters (fd_params - 1 design cells and the pulse amplitude)."""
+    from fullmag_amd.adjoint import WaveguideProblem
+    rng = np.random.default_rng(2024)
+    base = WaveguideProblem(steps=steps, window=min(0.5e-9, steps * 50e-15 / 2))
+    design = rng.uniform(6e5, 1e6, size=base.design_shape())
+    amp = 5e-3
+    out = {"workload": "waveguide 256x64x2 of 5 nm cells, 64x64 design Ms map U[6e5,1e6], sinc pulse 5 mT "
+                       f"20 GHz on 8 columns, RK4 dt 50 fs x {steps} steps, J = mean m_z^2 over the last 8 "
+                       "columns in the last 0.5 ns",
+           "cells": base.n}
+    grads = {}
+    for prec in ("single", "double"):
+        pr = WaveguideProblem(steps=steps, window=base.window, precision=prec)
+        warm = WaveguideProblem(steps=2, window=base.dt, precision=prec)
+        warm.value_and_grad(design, amp)  # module load, LDS limits, graph capture
+        t0 = time.perf_counter()
+        j_f, be = pr.run(design, amp)
+        t_fwd = time.perf_counter() - t0
+        be.close()
+        t0 = time.perf_counter()
+        j, gd, ga = pr.value_and_grad(design, amp)
+        t_grad = time.perf_counter() - t0
+        grads[prec] = (j, gd, ga)
+        out[prec] = {"objective": j, "forward_s": round(t_fwd, 3), "gradient_s": round(t_grad, 3),
+                     "gradient_over_forward": round(t_grad / t_fwd, 3),
+                     "forward_Mcell_steps_per_s": round(pr.n * steps / t_fwd / 1e6, 2),
+                     "checkpoint_interval": int(np.ceil(np.sqrt(steps)))}
+    if fd_params:
+        pr = WaveguideProblem(steps=steps, window=base.window, precision="double")
+        dy, dx = pr.design_shape()
+        cells = rng.choice(dy * dx, size=fd_params - 1, replace=False)
+        fd, adj32, adj64 = [], [], []
+        for f in cells:
+            iy, ix = int(f // dx), int(f % dx)
+            p, m = design.copy(), design.copy()
+            p[iy, ix] *= 1 + fd_h
+            m[iy, ix] *= 1 - fd_h
+            jp, b = pr.run(p, amp)
+            b.close()
+            jm, b = pr.run(m, amp)
+            b.close()
+            fd.append((jp - jm) / (2 * fd_h * design[iy, ix]))
+            adj32.append(grads["single"][1][iy, ix])
+            adj64.append(grads["double"][1][iy, ix])
+        jp, b = pr.run(design, amp * (1 + fd_h))
+        b.close()
+        jm, b = pr.run(design, amp * (1 - fd_h))
+        b.close()
+        fd.append((jp - jm) / (2 * fd_h * amp))
+        adj32.append(grads["single"][2])
+        adj64.append(grads["double"][2])
+        fd, adj32, adj64 = map(np.array, (fd, adj32, adj64))
+        sc = np.abs(fd).max()
+        out["finite_differences"] = {
+            "params": fd_params, "h_rel": fd_h, "fd_from": "FP64 engine forward",
+            "max_rel_err_fp64_adjoint": float(np.abs(adj64 - fd).max() / sc),
+            "max_rel_err_fp32_adjoint": float(np.abs(adj32 - fd).max() / sc),
+            "grad_scale": float(sc),
+            "per_param_rel_err_fp32": [float(v) for v in np.abs(adj32 - fd) / np.abs(fd)],
+            "per_param_rel_err_fp64": [float(v) for v in np.abs(adj64 - fd) / np.abs(fd)],
+            "fd": [float(v) for v in fd]}
+    return out
+
+
 def roofline(wl, r32):
     kb = wl.kernel_bytes(4)
     kt = r32["kernel_us"]
@@ -242,7 +309,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="film512", choices=["film512", "granular3d"])
+    ap.add_argument("--workload", default="film512", choices=["film512", "granular3d", "adjoint"])
+    ap.add_argument("--adj-steps", type=int, default=20000)
+    ap.add_argument("--fd-params", type=int, default=16, help="adjoint workload: parameters checked by FD (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-accuracy", action="store_true")
@@ -259,6 +328,16 @@ def main():
     from fullmag_amd.backend import is_available
     if not is_available():
         raise SystemExit("no gfx9 GPU visible to libfullmag_fdm.so")
+    if a.workload == "adjoint":
+        r = adjoint_bench(a.adj_steps, a.fd_params)
+        out = {"metric": "adjoint gradient cost / forward cost (RK4, 256x64x2, fp32)",
+               "value": r["single"]["gradient_over_forward"], "unit": "x", "n_gpus": 1, "steps": a.adj_steps,
+               "warmup": 0, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+               "data": "synthetic", "config": {"workload": r.pop("workload"), "cells": r.pop("cells"),
+                                               "parallelism": "single GPU"}}
+        out.update(r)
+        print(json.dumps(out), flush=True)
+        return
     wl = Workload(a.workload)
     r32 = run_gpu(wl, "single", a.steps, a.warmup)
     r64 = run_gpu(wl, "double", a.steps, a.warmup)

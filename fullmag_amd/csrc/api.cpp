@@ -1258,9 +1258,9 @@ template <class T>
 fm::AdjArgs<T> adj_args(Backend *h) {
     auto &D = h->adj;
     fm::AdjArgs<T> J{};
-    J.cot_m = static_cast<T *>(D.cot_m);
-    J.cot_k = static_cast<T *>(D.cot_k);
-    J.beta = static_cast<T *>(D.beta);
+    J.cot_m = static_cast<double *>(D.cot_m);
+    J.cot_k = static_cast<double *>(D.cot_k);
+    J.beta = static_cast<double *>(D.beta);
     J.grad_ms = D.grad_ms;
     J.grad_amp = D.grad_amp;
     J.obj_part = D.obj_part;
@@ -1291,9 +1291,9 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     D.rec_y = adj_alloc(h, slots * v3);
     D.rec_h = adj_alloc(h, slots * v3);
     D.rec_vn = adj_alloc(h, slots * n * sizeof(T));
-    D.cot_m = adj_alloc(h, v3);
-    D.cot_k = adj_alloc(h, fm::kMaxStages * v3);
-    D.beta = adj_alloc(h, v3);
+    D.cot_m = adj_alloc(h, 3 * n * sizeof(double));
+    D.cot_k = adj_alloc(h, fm::kMaxStages * 3 * n * sizeof(double));
+    D.beta = adj_alloc(h, 3 * n * sizeof(double));
     D.obj_w = adj_alloc(h, n * sizeof(T));
     D.m_end = adj_alloc(h, v3);
     D.grad_ms = static_cast<double *>(adj_alloc(h, n * sizeof(double)));
@@ -1363,7 +1363,7 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
     const size_t n3 = 3 * n, v3 = n3 * sizeof(T);
     hipMemsetAsync(D.grad_ms, 0, n * sizeof(double), h->stream);
     hipMemsetAsync(D.grad_amp, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
-    hipMemsetAsync(D.cot_m, 0, v3, h->stream);
+    hipMemsetAsync(D.cot_m, 0, 3 * n * sizeof(double), h->stream);
     // host replica of the device time sequence t_n = t_{n-1} + dt
     std::vector<double> tn(D.steps + 1);
     tn[0] = D.t0;

@@ -168,9 +168,9 @@ struct Args {
 template <class T>
 struct AdjArgs {
     const T *rec_y, *rec_h, *rec_vn;  // record of the step being reversed (layout as in Args)
-    T *cot_m;                         // 3N: cotangent of the step's start state
-    T *cot_k;                         // kMaxStages x 3N: cotangent of k_j at [j*3N]
-    T *beta;                          // 3N: field cotangent of the stage being reversed
+    double *cot_m;                    // 3N: cotangent of the step's start state (FP64: it accumulates over steps)
+    double *cot_k;                    // kMaxStages x 3N: cotangent of k_j at [j*3N]
+    double *beta;                     // 3N: field cotangent of the stage being reversed
     double *grad_ms;                  // N: dJ/dMs_i, accumulated
     double *grad_amp;                 // [work-groups][3]: dJ/d(wave amplitude), accumulated
     double *obj_part;                 // [work-groups]: objective partial sums, accumulated

@@ -1486,119 +1486,126 @@ __device__ __forceinline__ void cross3(const T a[3], const T b[3], T o[3]) {
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// beta of stage j at `cell` from the recorded state and the cotangent of k_j.
+// beta of stage j at `cell` from the recorded state and the cotangent of k_j
+// (FP64 arithmetic for both storage precisions: the reverse sweep sums
+// small differences of cotangents over many steps).
 template <class T>
-__device__ __forceinline__ void adj_beta_cell(const Args<T> &A, const AdjArgs<T> &J, int j, int cell, T b[3]) {
+__device__ __forceinline__ void adj_beta_cell(const Args<T> &A, const AdjArgs<T> &J, int j, int cell, double b[3]) {
     const int N = (int)A.ncell;
     const T *y = J.rec_y + (size_t)(j - 1) * 3 * N;
-    const T *a = J.cot_k + (size_t)j * 3 * N;
-    const T m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
-    const T av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
-    T axm[3], axmxm[3];
+    const double *a = J.cot_k + (size_t)j * 3 * N;
+    const double m[3] = {(double)y[cell], (double)y[N + cell], (double)y[2 * N + cell]};
+    const double av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
+    double axm[3], axmxm[3];
     cross3(av, m, axm);
     cross3(axm, m, axmxm);
+    const double gb = (double)A.gamma_bar, al = (double)A.alpha;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) b[c] = -A.gamma_bar * ((A.precession ? axm[c] : T(0)) + A.alpha * axmxm[c]);
+    for (int c = 0; c < 3; ++c) b[c] = -gb * ((A.precession ? axm[c] : 0.0) + al * axmxm[c]);
 }
 
 // Cotangent of y_j at `cell` (hd = (-K beta)_i, zero without demag),
 // parameter gradients, then the normalisation of stage j and the
-// accumulation into cot m_n and cot k_q.
+// accumulation into cot m_n and cot k_q.  FP64 arithmetic on the stored
+// state, field and demag values.
 template <class T>
 __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T> &J, const Method &M, int j,
-                                               int cell, int x, int y, int z, const T hd[3], T wf, double amp[3]) {
+                                               int cell, int x, int y, int z, const T hdT[3], T wf, double amp[3]) {
     const int N = (int)A.ncell;
     if (A.mask && !A.mask[cell]) return;
     const T *yv = J.rec_y + (size_t)(j - 1) * 3 * N;
     const T *Hv = J.rec_h + (size_t)(j - 1) * 3 * N;
-    const T *a = J.cot_k + (size_t)j * 3 * N;
-    const T m[3] = {yv[cell], yv[N + cell], yv[2 * N + cell]};
-    const T H[3] = {Hv[cell], Hv[N + cell], Hv[2 * N + cell]};
-    const T av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
-    const T bi[3] = {J.beta[cell], J.beta[N + cell], J.beta[2 * N + cell]};
-    const T ms_i = A.ms_cell ? A.ms_cell[cell] : A.ms;
-    T Hxa[3], mxH[3], mxHxa[3], axm[3], Hxaxm[3];
+    const double *a = J.cot_k + (size_t)j * 3 * N;
+    const double m[3] = {(double)yv[cell], (double)yv[N + cell], (double)yv[2 * N + cell]};
+    const double H[3] = {(double)Hv[cell], (double)Hv[N + cell], (double)Hv[2 * N + cell]};
+    const double av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
+    const double bi[3] = {J.beta[cell], J.beta[N + cell], J.beta[2 * N + cell]};
+    const double hd[3] = {(double)hdT[0], (double)hdT[1], (double)hdT[2]};
+    const double ms_i = A.ms_cell ? (double)A.ms_cell[cell] : (double)A.ms;
+    const double gb = (double)A.gamma_bar, al = (double)A.alpha;
+    double Hxa[3], mxH[3], mxHxa[3], axm[3], Hxaxm[3];
     cross3(H, av, Hxa);
     cross3(m, H, mxH);
     cross3(mxH, av, mxHxa);
     cross3(av, m, axm);
     cross3(H, axm, Hxaxm);
-    T r[3];
+    double r[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) r[c] = -A.gamma_bar * ((A.precession ? Hxa[c] : T(0)) + A.alpha * (mxHxa[c] + Hxaxm[c]));
+    for (int c = 0; c < 3; ++c) r[c] = -gb * ((A.precession ? Hxa[c] : 0.0) + al * (mxHxa[c] + Hxaxm[c]));
     // exchange: adjoint stencil on beta/Ms, and H_ex at y_j for the Ms gradient
     const int sy = A.nx, sz = A.nx * A.ny;
     const int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
                        y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
                        z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
-    const T wq[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
-    T hex[3] = {T(0), T(0), T(0)};
-    const T inv_ms_i = T(1) / ms_i;
+    const double wq[6] = {(double)A.idx2, (double)A.idx2, (double)A.idy2, (double)A.idy2, (double)A.idz2,
+                          (double)A.idz2};
+    const double c_ex = (double)A.two_a_mu0;
+    double hex[3] = {0.0, 0.0, 0.0};
+    const double inv_ms_i = 1.0 / ms_i;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int o = nb[q];
         const bool use = o != cell && (!A.mask || A.mask[o]);
         if (!use) continue;
-        const T inv_ms_o = A.ms_cell ? T(1) / A.ms_cell[o] : inv_ms_i;
+        const double inv_ms_o = A.ms_cell ? 1.0 / (double)A.ms_cell[o] : inv_ms_i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            r[c] += A.two_a_mu0 * wq[q] * (J.beta[c * N + o] * inv_ms_o - bi[c] * inv_ms_i);
-            hex[c] += wq[q] * (yv[c * N + o] - m[c]);
+            r[c] += c_ex * wq[q] * (J.beta[c * N + o] * inv_ms_o - bi[c] * inv_ms_i);
+            hex[c] += wq[q] * ((double)yv[c * N + o] - m[c]);
         }
     }
-    const T pre = A.ms_cell ? A.two_a_mu0 * inv_ms_i : A.ex_pre;
+    const double pre = A.ms_cell ? c_ex * inv_ms_i : (double)A.ex_pre;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         hex[c] *= pre;
         r[c] += ms_i * hd[c];
     }
-    T hani[3] = {T(0), T(0), T(0)};
+    double hani[3] = {0.0, 0.0, 0.0};
     if (A.has_uni) {
-        T u[3];
+        double u[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[c] = A.axis_cell ? A.axis_cell[c * N + cell] : A.axis[c];
-        T c1 = A.uni_c1, c2 = A.uni_c2;
+        for (int c = 0; c < 3; ++c) u[c] = A.axis_cell ? (double)A.axis_cell[c * N + cell] : (double)A.axis[c];
+        double c1 = (double)A.uni_c1, c2 = (double)A.uni_c2;
         if (A.ku1_cell || A.ku2_cell || A.ms_cell) {
-            const T k1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
-            const T k2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
-            c1 = A.two_mu0inv * k1 * inv_ms_i;
-            c2 = T(2) * A.two_mu0inv * k2 * inv_ms_i;
+            const double k1 = A.ku1_cell ? (double)A.ku1_cell[cell] : (double)A.ku1_u;
+            const double k2 = A.ku2_cell ? (double)A.ku2_cell[cell] : (double)A.ku2_u;
+            c1 = (double)A.two_mu0inv * k1 * inv_ms_i;
+            c2 = 2.0 * (double)A.two_mu0inv * k2 * inv_ms_i;
         }
-        const T mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
-        const T bu = bi[0] * u[0] + bi[1] * u[1] + bi[2] * u[2];
-        const T dcoef = (c1 + T(3) * c2 * mu * mu) * bu;
-        const T coef = c1 * mu + c2 * mu * mu * mu;
+        const double mu = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+        const double bu = bi[0] * u[0] + bi[1] * u[1] + bi[2] * u[2];
+        const double dcoef = (c1 + 3.0 * c2 * mu * mu) * bu;
+        const double coef = c1 * mu + c2 * mu * mu * mu;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             r[c] += dcoef * u[c];
             hani[c] = coef * u[c];
         }
     }
-    // parameter gradients (FP64 accumulation)
+    // parameter gradients
     {
         double g = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            g += -(double)bi[c] * ((double)hex[c] + (double)hani[c]) / (double)ms_i + (double)m[c] * (double)hd[c];
+        for (int c = 0; c < 3; ++c) g += -bi[c] * (hex[c] + hani[c]) * inv_ms_i + m[c] * hd[c];
         J.grad_ms[cell] += g;
         if (A.wave_kind && (!A.wave_mask || A.wave_mask[cell])) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) amp[c] += (double)bi[c] * (double)wf;
+            for (int c = 0; c < 3; ++c) amp[c] += bi[c] * (double)wf;
         }
     }
     if (j >= 2) {
-        const T vn = J.rec_vn[(size_t)(j - 2) * N + cell];
-        const T yr = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
-        T cv[3];
+        const double vn = (double)J.rec_vn[(size_t)(j - 2) * N + cell];
+        const double yr = m[0] * r[0] + m[1] * r[1] + m[2] * r[2];
+        double cv[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) cv[c] = vn > T(0) ? (r[c] - m[c] * yr) / vn : T(0);
+        for (int c = 0; c < 3; ++c) cv[c] = vn > 0.0 ? (r[c] - m[c] * yr) / vn : 0.0;
         const StageRow &R = M.row[j];
-        const T f = T(J.dt * R.scale);
+        const double f = J.dt * R.scale;
 #pragma unroll
         for (int c = 0; c < 3; ++c) J.cot_m[c * N + cell] += cv[c];
         for (int q = 0; q < R.nk; ++q) {
-            T *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
-            const T w = f * T(R.a[q]);
+            double *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
+            const double w = f * R.a[q];
 #pragma unroll
             for (int c = 0; c < 3; ++c) ck[c * N + cell] += w * cv[c];
         }
@@ -1636,24 +1643,24 @@ __global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J, 
     if (cell >= N) return;
     const Method &M = c_methods[A.method];
     const T *y = J.rec_y + (size_t)(nstage - 1) * 3 * N;
-    const T m[3] = {y[cell], y[N + cell], y[2 * N + cell]};
-    T lam[3] = {J.cot_m[cell], J.cot_m[N + cell], J.cot_m[2 * N + cell]};
-    if (with_obj) lam[J.obj_comp] += T(2.0 * J.obj_scale) * J.obj_w[cell] * m[J.obj_comp];
-    const T vn = J.rec_vn[(size_t)(nstage - 2) * N + cell];
-    const T yl = m[0] * lam[0] + m[1] * lam[1] + m[2] * lam[2];
-    T cv[3];
+    const double m[3] = {(double)y[cell], (double)y[N + cell], (double)y[2 * N + cell]};
+    double lam[3] = {J.cot_m[cell], J.cot_m[N + cell], J.cot_m[2 * N + cell]};
+    if (with_obj) lam[J.obj_comp] += 2.0 * J.obj_scale * (double)J.obj_w[cell] * m[J.obj_comp];
+    const double vn = (double)J.rec_vn[(size_t)(nstage - 2) * N + cell];
+    const double yl = m[0] * lam[0] + m[1] * lam[1] + m[2] * lam[2];
+    double cv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) cv[c] = vn > T(0) ? (lam[c] - m[c] * yl) / vn : T(0);
+    for (int c = 0; c < 3; ++c) cv[c] = vn > 0.0 ? (lam[c] - m[c] * yl) / vn : 0.0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) J.cot_m[c * N + cell] = cv[c];
     for (int j = 1; j < nstage; ++j)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) J.cot_k[((size_t)j * 3 + c) * N + cell] = T(0);
+        for (int c = 0; c < 3; ++c) J.cot_k[((size_t)j * 3 + c) * N + cell] = 0.0;
     const StageRow &R = M.row[nstage];
-    const T f = T(J.dt * R.scale);
+    const double f = J.dt * R.scale;
     for (int q = 0; q < R.nk; ++q) {
-        T *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
-        const T w = f * T(R.a[q]);
+        double *ck = J.cot_k + (size_t)R.kidx[q] * 3 * N;
+        const double w = f * R.a[q];
 #pragma unroll
         for (int c = 0; c < 3; ++c) ck[c * N + cell] += w * cv[c];
     }
@@ -1677,7 +1684,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J, 
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= N) return;
-    T b[3];
+    double b[3];
     adj_beta_cell(A, J, j, cell, b);
 #pragma unroll
     for (int c = 0; c < 3; ++c) J.beta[c * N + cell] = b[c];
@@ -1717,7 +1724,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J, in
     for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
         const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
         const int c0 = xr.cell0[rr];
-        T b[3] = {T(0), T(0), T(0)};
+        double b[3] = {0.0, 0.0, 0.0};
         if (c0 >= 0) {
             adj_beta_cell(A, J, j, c0 + x, b);
 #pragma unroll
@@ -1727,7 +1734,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J, in
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
             T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
-            e[l & 1] = b[c];
+            e[l & 1] = (T)b[c];
         }
     }
     if (nl & 1)

@@ -117,9 +117,9 @@ int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out);
  * (grad_ms, n values, 0 on inactive cells) and dJ/d field_amplitude_am of
  * the plan's waveform (grad_field_amplitude, 3 values).  Either output may
  * be NULL.  After either call the handle holds the final state m_steps at
- * t0 + steps*dt.  Memory: the checkpoints plus one segment's record,
- * about (3 + 7 (nstage-1)) x 3N values of the state precision per step of
- * a segment.
+ * t0 + steps*dt.  Device memory: 3N state values per checkpoint, 7 (nstage-1) N
+ * per step of one segment for the record (stage states, fields and
+ * pre-normalisation norms), and 27N FP64 cotangents.
  */
 typedef struct fullmag_fdm_adjoint_desc {
     uint64_t struct_size;             /* sizeof(fullmag_fdm_adjoint_desc) */
