@@ -46,7 +46,8 @@ long long adj_blocks(const Args<T> &A);
 template <class T>
 hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipStream_t st);
 template <class T>
-hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, int with_obj, hipStream_t st);
+hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, hipStream_t st);
+hipError_t enqueue_adj_step(long long *step, long long value, int add, hipStream_t st);
 template <class T>
 hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st);
 template <class T>
@@ -97,7 +98,8 @@ struct fullmag_fdm_backend {
         void *cp = nullptr;                       // ncp x 3N checkpoints
         void *rec_y = nullptr, *rec_h = nullptr, *rec_vn = nullptr;
         void *cot_m = nullptr, *cot_k = nullptr, *beta = nullptr, *obj_w = nullptr, *m_end = nullptr;
-        double *grad_ms = nullptr, *grad_amp = nullptr, *obj_part = nullptr;
+        double *grad_ms = nullptr, *grad_amp = nullptr, *obj_part = nullptr, *tn = nullptr;
+        long long *step = nullptr;
         uint64_t steps = 0, seg = 0, ncp = 0, first = 0;
         uint32_t comp = 0;
         double dt = 0.0, t0 = 0.0, obj_scale = 0.0;
@@ -1242,6 +1244,21 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
 // ------------------------------------------------------------ adjoint --
 namespace {
 
+// Captures the launches `f` enqueues on the handle's stream into an
+// executable graph (nullptr on failure).
+template <class F>
+hipGraphExec_t capture_graph(Backend *h, F &&f) {
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const bool ok = f();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->stream, &g);
+    hipGraphExec_t ex = nullptr;
+    if (ok && e == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) ex = nullptr;
+    if (g) hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return ex;
+}
+
 void adj_free(Backend *h) {
     for (void *p : h->adj.bufs) hipFree(p);
     h->adj = Backend::Adj{};
@@ -1258,6 +1275,16 @@ template <class T>
 fm::AdjArgs<T> adj_args(Backend *h) {
     auto &D = h->adj;
     fm::AdjArgs<T> J{};
+    const int ns = h->nstage;
+    J.rec_y0 = static_cast<const T *>(D.rec_y);
+    J.rec_h0 = static_cast<const T *>(D.rec_h);
+    J.rec_vn0 = static_cast<const T *>(D.rec_vn);
+    J.step = D.step;
+    J.seg = (long long)D.seg;
+    J.sv = (long long)(ns - 1) * 3 * (long long)h->n;
+    J.ss = (long long)(ns - 1) * (long long)h->n;
+    J.tn = D.tn;
+    J.first = (long long)D.first;
     J.cot_m = static_cast<double *>(D.cot_m);
     J.cot_k = static_cast<double *>(D.cot_k);
     J.beta = static_cast<double *>(D.beta);
@@ -1299,8 +1326,10 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     D.grad_ms = static_cast<double *>(adj_alloc(h, n * sizeof(double)));
     D.grad_amp = static_cast<double *>(adj_alloc(h, (size_t)D.blocks * 3 * sizeof(double)));
     D.obj_part = static_cast<double *>(adj_alloc(h, (size_t)D.blocks * 3 * sizeof(double)));
+    D.tn = static_cast<double *>(adj_alloc(h, (steps + 1) * sizeof(double)));
+    D.step = static_cast<long long *>(adj_alloc(h, sizeof(long long)));
     if (!D.cp || !D.rec_y || !D.rec_h || !D.rec_vn || !D.cot_m || !D.cot_k || !D.beta || !D.obj_w || !D.m_end ||
-        !D.grad_ms || !D.grad_amp || !D.obj_part) {
+        !D.grad_ms || !D.grad_amp || !D.obj_part || !D.tn || !D.step) {
         adj_free(h);
         return fail(h, FULLMAG_FDM_ERR_CUDA, "adjoint: device allocation failed");
     }
@@ -1321,20 +1350,32 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     c.dt_carry = D.dt;
     c.dt = D.dt;
     c.done = 0;
-    if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
-    ensure_graph(h);
+    // host replica of the device time sequence t_n = t_{n-1} + dt (the
+    // controller adds dt in FP64 after every accepted step)
+    std::vector<double> tn(steps + 1);
+    tn[0] = D.t0;
+    for (uint64_t s = 1; s <= steps; ++s) tn[s] = tn[s - 1] + D.dt;
+    hipMemcpyAsync(D.tn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    fm::AdjArgs<T> J = adj_args<T>(h);
+    J.obj_from = (long long)(c.step + D.first);
+    if (!push_ctl(h) || !hip_ok(h, hipStreamSynchronize(h->stream), "adjoint setup")) return FULLMAG_FDM_ERR_CUDA;
     h->obs_fresh = false;
-    const fm::AdjArgs<T> J = adj_args<T>(h);
+    auto one_step = [&]() {
+        return hip_ok(h, fm::enqueue_trial(A, ns, h->stream), "adjoint forward step") &&
+               hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective");
+    };
+    hipGraphExec_t g = capture_graph(h, one_step);
     int par = D.parity0;
-    for (uint64_t s = 1; s <= steps; ++s) {
-        if (!launch_attempt(h)) return FULLMAG_FDM_ERR_CUDA;
+    bool ok = true;
+    for (uint64_t s = 1; s <= steps && ok; ++s) {
+        ok = g ? hip_ok(h, hipGraphLaunch(g, h->stream), "adjoint forward graph") : one_step();
         par ^= 1;
-        if (s >= D.first && !hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective"))
-            return FULLMAG_FDM_ERR_CUDA;
         if (s % D.seg == 0 && s < steps)
             hipMemcpyAsync(static_cast<char *>(D.cp) + (s / D.seg) * v3, A.m_all + (size_t)par * n3, v3,
                            hipMemcpyDeviceToDevice, h->stream);
     }
+    if (g) hipGraphExecDestroy(g);
+    if (!ok) return FULLMAG_FDM_ERR_CUDA;
     std::vector<double> part((size_t)D.blocks * 3);
     if (!hip_ok(h, hipMemcpyAsync(part.data(), D.obj_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
                                   h->stream),
@@ -1364,23 +1405,43 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
     hipMemsetAsync(D.grad_ms, 0, n * sizeof(double), h->stream);
     hipMemsetAsync(D.grad_amp, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
     hipMemsetAsync(D.cot_m, 0, 3 * n * sizeof(double), h->stream);
-    // host replica of the device time sequence t_n = t_{n-1} + dt
-    std::vector<double> tn(D.steps + 1);
-    tn[0] = D.t0;
-    for (uint64_t s = 1; s <= D.steps; ++s) tn[s] = tn[s - 1] + D.dt;
-    fm::AdjArgs<T> J = adj_args<T>(h);
+    const fm::AdjArgs<T> J = adj_args<T>(h);
+    // recording form of the arguments: slot = device step index mod seg
+    fm::Args<T> B = A;
+    B.rec_y = static_cast<T *>(D.rec_y);
+    B.rec_h = static_cast<T *>(D.rec_h);
+    B.rec_vn = static_cast<T *>(D.rec_vn);
+    B.rec_step = D.step;
+    B.rec_seg = J.seg;
+    B.rec_sv = J.sv;
+    B.rec_ss = J.ss;
+    auto rec_step = [&]() { return hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint recompute"); };
+    auto rev_step = [&]() {
+        if (!hip_ok(h, fm::enqueue_adj_init(A, J, ns, h->stream), "adjoint init")) return false;
+        for (int j = ns - 1; j >= 1; --j)
+            if (!hip_ok(h, fm::enqueue_adj_stage(A, J, j, h->stream), "adjoint stage")) return false;
+        return hip_ok(h, fm::enqueue_adj_step(D.step, -1, 1, h->stream), "adjoint step");
+    };
+    hipGraphExec_t g_rec = capture_graph(h, rec_step);
+    hipGraphExec_t g_rev = capture_graph(h, rev_step);
+    auto run = [&](hipGraphExec_t g, auto &f) {
+        return g ? hip_ok(h, hipGraphLaunch(g, h->stream), "adjoint graph") : f();
+    };
     fm::Ctl &c = *h->ctl_h;
     h->obs_fresh = false;
-    T *rec_y = static_cast<T *>(D.rec_y), *rec_h = static_cast<T *>(D.rec_h), *rec_vn = static_cast<T *>(D.rec_vn);
-    const size_t step_v = (size_t)(ns - 1) * n3, step_s = (size_t)(ns - 1) * n;
-    for (uint64_t k = D.ncp; k-- > 0;) {
+    bool ok = true;
+    for (uint64_t k = D.ncp; k-- > 0 && ok;) {
         const uint64_t n0 = k * D.seg, n1 = std::min<uint64_t>(n0 + D.seg, D.steps);
         // the pinned control block is read by the queued upload only when the
         // stream reaches it: drain the stream before rewriting it
-        if (!hip_ok(h, hipStreamSynchronize(h->stream), "adjoint sync")) return FULLMAG_FDM_ERR_CUDA;
+        if (!hip_ok(h, hipStreamSynchronize(h->stream), "adjoint sync")) {
+            ok = false;
+            break;
+        }
         // recompute the segment from its checkpoint, recording every stage
         c = D.ctl_end;
-        c.t = tn[n0];
+        c.t = D.t0;
+        for (uint64_t s = 0; s < n0; ++s) c.t += D.dt;
         c.parity = 0;
         c.loop = 1;
         c.t_end = std::numeric_limits<double>::infinity();
@@ -1390,35 +1451,21 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
         c.done = 0;
         const char *cpk = static_cast<const char *>(D.cp) + k * v3;
         hipMemcpyAsync(A.m_all, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
-        hipMemcpyAsync(rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
-        if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
-        fm::Args<T> B = A;
-        B.rec_h = rec_h;
-        if (!hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint recompute (k1)")) return FULLMAG_FDM_ERR_CUDA;
-        for (uint64_t i = 0; i < n1 - n0; ++i) {
-            B.rec_y = rec_y + i * step_v;
-            B.rec_h = rec_h + i * step_v;
-            B.rec_vn = rec_vn + i * step_s;
-            if (!hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint recompute")) return FULLMAG_FDM_ERR_CUDA;
-        }
+        hipMemcpyAsync(D.rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
+        ok = push_ctl(h) && hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n0, 0, h->stream), "adjoint step") &&
+             hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint recompute (k1)");
+        for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rec, rec_step);
+        if (!ok) break;
         // the recompute ends with the device loop marked done; the y/z passes
         // that the reverse sweep shares with the forward step return early
         // while it is set
         hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream);
-        // reverse sweep through the segment
-        for (uint64_t s = n1; s-- > n0;) {
-            const uint64_t i = s - n0;
-            J.rec_y = rec_y + i * step_v;
-            J.rec_h = rec_h + i * step_v;
-            J.rec_vn = rec_vn + i * step_s;
-            J.t_n = tn[s];
-            if (!hip_ok(h, fm::enqueue_adj_init(A, J, ns, s + 1 >= D.first ? 1 : 0, h->stream), "adjoint init"))
-                return FULLMAG_FDM_ERR_CUDA;
-            for (int j = ns - 1; j >= 1; --j)
-                if (!hip_ok(h, fm::enqueue_adj_stage(A, J, j, h->stream), "adjoint stage"))
-                    return FULLMAG_FDM_ERR_CUDA;
-        }
+        ok = hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n1 - 1, 0, h->stream), "adjoint step");
+        for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rev, rev_step);
     }
+    if (g_rec) hipGraphExecDestroy(g_rec);
+    if (g_rev) hipGraphExecDestroy(g_rev);
+    if (!ok) return FULLMAG_FDM_ERR_CUDA;
     std::vector<double> gms(n), gamp((size_t)D.blocks * 3);
     if (!hip_ok(h, hipMemcpyAsync(gms.data(), D.grad_ms, n * sizeof(double), hipMemcpyDeviceToHost, h->stream),
                 "gradient readback") ||

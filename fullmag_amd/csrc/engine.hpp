@@ -126,10 +126,14 @@ struct Args {
     unsigned *counter;      // finished work-groups of the current partial-producing launch
     // observables (written only when non-NULL)
     T *obs_hex, *obs_hd, *obs_heff;
-    // trajectory record of one step for the adjoint sweep (written only when
-    // non-NULL): stage state y_s at rec_y[(s-1)*3N], effective field at y_s
-    // at rec_h[(s-1)*3N], pre-normalisation norm |v_s| at rec_vn[(s-2)*N]
+    // trajectory record for the adjoint sweep (written only when non-NULL):
+    // for the step in flight, slot i = (*rec_step mod rec_seg), stage state
+    // y_s at rec_y[i*rec_sv + (s-1)*3N], effective field at y_s at
+    // rec_h[i*rec_sv + (s-1)*3N], pre-normalisation norm |v_s| at
+    // rec_vn[i*rec_ss + (s-2)*N]; the controller advances *rec_step
     T *rec_y, *rec_h, *rec_vn;
+    long long *rec_step;
+    long long rec_seg, rec_sv, rec_ss;
     // physics
     T ms;
     T ex_pre;               // 2A/(mu0 Ms)
@@ -167,7 +171,13 @@ struct Args {
 // gradients (see the adjoint kernels for the formulas).
 template <class T>
 struct AdjArgs {
-    const T *rec_y, *rec_h, *rec_vn;  // record of the step being reversed (layout as in Args)
+    const T *rec_y, *rec_h, *rec_vn;  // record of the step being reversed (bound per launch by adj_bind)
+    const T *rec_y0, *rec_h0, *rec_vn0;  // record bases (layout as in Args)
+    long long *step;                  // global index n of the step being reversed (device)
+    long long seg, sv, ss;            // record slots = n mod seg, slot strides
+    const double *tn;                 // t_n for n = 0..steps (device)
+    long long first;                  // objective_first_step
+    long long obj_from;               // forward: control-block step count from which J accumulates
     double *cot_m;                    // 3N: cotangent of the step's start state (FP64: it accumulates over steps)
     double *cot_k;                    // kMaxStages x 3N: cotangent of k_j at [j*3N]
     double *beta;                     // 3N: field cotangent of the stage being reversed

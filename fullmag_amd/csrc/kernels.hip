@@ -301,6 +301,12 @@ __device__ __forceinline__ int kslot(const Method &M, int j, int p) {
 
 // Intermediate stage states alternate between two buffers, so that a fused
 // kernel can write y_{s+1} while other work-groups still read y_s.
+// Offset of the record slot of the step in flight (see Args::rec_step).
+template <class T>
+__device__ __forceinline__ size_t rec_slot(const Args<T> &A, long long stride) {
+    return A.rec_step ? (size_t)((*A.rec_step) % A.rec_seg) * (size_t)stride : 0;
+}
+
 template <class T>
 __device__ __forceinline__ T *stage_buf(const Args<T> &A, int s) {
     return A.ybuf + (size_t)(s & 1) * 3 * (size_t)A.ncell;
@@ -369,11 +375,11 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     out[N + cell] = (T)y;
     out[2 * N + cell] = (T)z;
     if (A.rec_y) {
-        T *ry = A.rec_y + (size_t)(s - 1) * 3 * N;
+        T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
         ry[cell] = (T)x;
         ry[N + cell] = (T)y;
         ry[2 * N + cell] = (T)z;
-        A.rec_vn[(size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+        A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
     }
     v[0] = (T)x;
     v[1] = (T)y;
@@ -838,9 +844,10 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
                 v[2] = mv[j][2];
                 if (A.rec_y && c0 >= 0) {
                     const int cell = c0 + x;
-                    A.rec_y[cell] = v[0];
-                    A.rec_y[N + cell] = v[1];
-                    A.rec_y[2 * N + cell] = v[2];
+                    T *ry = A.rec_y + rec_slot(A, A.rec_sv);
+                    ry[cell] = v[0];
+                    ry[N + cell] = v[1];
+                    ry[2 * N + cell] = v[2];
                 }
             } else {
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -863,11 +870,11 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
                     yout[N + cell] = v[1];
                     yout[2 * N + cell] = v[2];
                     if (A.rec_y) {
-                        T *ry = A.rec_y + (size_t)(s - 1) * 3 * N;
+                        T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
                         ry[cell] = v[0];
                         ry[N + cell] = v[1];
                         ry[2 * N + cell] = v[2];
-                        A.rec_vn[(size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+                        A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
                     }
                 }
             }
@@ -979,7 +986,7 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
     const float rnx = 1.0f / (float)nx;
     const T *yin = stage_input(A, M, s, p);
     T *kout = stage_rhs(A, M, s, p);
-    T *hrec = A.rec_h ? A.rec_h + (size_t)(s - 1) * 3 * A.ncell : nullptr;
+    T *hrec = A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
     constexpr int CH = Chunk<T>::value;
     for (int base = 0; base < ncl; base += CH * bd) {
@@ -1486,6 +1493,20 @@ __device__ __forceinline__ void cross3(const T a[3], const T b[3], T o[3]) {
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
+// Binds the record of the step being reversed (n = *J.step, slot n mod seg)
+// and its start time.
+template <class T>
+__device__ __forceinline__ AdjArgs<T> adj_bind(const AdjArgs<T> &J0) {
+    AdjArgs<T> J = J0;
+    const long long n = *J0.step;
+    const size_t i = (size_t)(n % J0.seg);
+    J.rec_y = J0.rec_y0 + i * (size_t)J0.sv;
+    J.rec_h = J0.rec_h0 + i * (size_t)J0.sv;
+    J.rec_vn = J0.rec_vn0 + i * (size_t)J0.ss;
+    J.t_n = J0.tn[n];
+    return J;
+}
+
 // beta of stage j at `cell` from the recorded state and the cotangent of k_j
 // (FP64 arithmetic for both storage precisions: the reverse sweep sums
 // small differences of cotangents over many steps).
@@ -1637,10 +1658,12 @@ __device__ void block_add3(double v[3], double *out) {
 // derivative at m_{n+1}), normalisation of the final stage, k cotangents
 // of the final row; cot m_n restarts from there.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J, int nstage, int with_obj) {
+__global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J0, int nstage) {
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= N) return;
+    const AdjArgs<T> J = adj_bind(J0);
+    const bool with_obj = *J0.step + 1 >= J0.first;
     const Method &M = c_methods[A.method];
     const T *y = J.rec_y + (size_t)(nstage - 1) * 3 * N;
     const double m[3] = {(double)y[cell], (double)y[N + cell], (double)y[2 * N + cell]};
@@ -1669,6 +1692,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J, 
 // Objective partial sums of the current state: obj_part[block] += sum w_i m_c,i^2.
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_adj_objective(Args<T> A, AdjArgs<T> J) {
+    if ((long long)A.ctl->step < J.obj_from) return;
     const int N = (int)A.ncell;
     const T *m = A.m_all + (size_t)A.ctl->parity * 3 * N + (size_t)J.obj_comp * N;
     double s = 0.0;
@@ -1678,9 +1702,15 @@ __global__ void __launch_bounds__(kThreads) k_adj_objective(Args<T> A, AdjArgs<T
     block_add3(v, J.obj_part + (size_t)blockIdx.x * 3);
 }
 
+// Sets (add = 0) or steps (add != 0) the device step index of the sweep.
+static __global__ void k_adj_step(long long *step, long long value, int add) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step = add ? *step + value : value;
+}
+
 // Without demag: beta of stage j, one thread per cell.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J0, int j) {
+    const AdjArgs<T> J = adj_bind(J0);
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= N) return;
@@ -1692,7 +1722,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J, 
 
 // Without demag: cotangent of y_j, one thread per cell.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J0, int j) {
+    const AdjArgs<T> J = adj_bind(J0);
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     const Method &M = c_methods[A.method];
@@ -1709,8 +1740,9 @@ __global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J,
 // With demag: beta of stage j for the work-group's x-rows, stored and
 // packed (unit Ms) two real rows per complex sequence, forward x FFT.
 template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J0, int j) {
     constexpr bool FAST = R1 > 0;
+    const AdjArgs<T> J = adj_bind(J0);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
     C2<T> *tw = b1 + A.nseq_x * A.px;
@@ -1745,8 +1777,9 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J, in
 // With demag: inverse x FFT of the convolved beta spectrum (1/P), then the
 // cotangent of y_j for the work-group's cells.
 template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_bx(Args<T> A, AdjArgs<T> J, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_bx(Args<T> A, AdjArgs<T> J0, int j) {
     constexpr bool FAST = R1 > 0;
+    const AdjArgs<T> J = adj_bind(J0);
     const Method &M = c_methods[A.method];
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
@@ -1809,7 +1842,7 @@ __global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats,
         CellIn<T> g;
         gather_cell<T, LUT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
         compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc,
-                             A.rec_h ? A.rec_h + (size_t)(s - 1) * 3 * A.ncell : nullptr);
+                             A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr);
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
     if (control) finish_last_block(A, control);
@@ -1862,6 +1895,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
         return;
     }
     for (int k = 0; k < kStatSlots; ++k) c->stats[k] = tot[k];
+    if (A.rec_step) *A.rec_step += 1;
     c->t += dt;
     c->step += 1;
     c->parity ^= 1;
@@ -2457,11 +2491,17 @@ hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipSt
     return hipGetLastError();
 }
 template <class T>
-hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, int with_obj, hipStream_t st) {
+hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, hipStream_t st) {
     const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
-    k_adj_init<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, nstage, with_obj);
+    k_adj_init<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, nstage);
     return hipGetLastError();
 }
+#ifdef FM_PRIMARY
+hipError_t enqueue_adj_step(long long *step, long long value, int add, hipStream_t st) {
+    k_adj_step<<<dim3(1), dim3(64), 0, st>>>(step, value, add);
+    return hipGetLastError();
+}
+#endif
 template <class T>
 hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st) {
     const unsigned gc = (unsigned)std::min<long long>((A.ncell + kThreads - 1) / kThreads, 1024);
@@ -2492,7 +2532,7 @@ template hipError_t enqueue_trial<FM_REAL>(const Args<FM_REAL> &, int, hipStream
 template hipError_t enqueue_control<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
 template long long adj_blocks<FM_REAL>(const Args<FM_REAL> &);
 template hipError_t enqueue_adj_stage<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, hipStream_t);
-template hipError_t enqueue_adj_init<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, int, hipStream_t);
+template hipError_t enqueue_adj_init<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, hipStream_t);
 template hipError_t enqueue_adj_objective<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, hipStream_t);
 template hipError_t adj_configure<FM_REAL>(const Args<FM_REAL> &);
 
