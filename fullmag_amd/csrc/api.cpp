@@ -92,6 +92,9 @@ struct fullmag_fdm_backend {
     bool graph_tried = false;
     bool obs_fresh = false;
     void *obs_base[3] = {nullptr, nullptr, nullptr};  // hex, hd, heff
+    // device copies of the kernel arguments: 0 stepping, 1 with observable
+    // outputs, 2 with trajectory recording (adjoint)
+    void *args_d[3] = {nullptr, nullptr, nullptr};
     // adjoint sweep (fullmag_fdm_adjoint_*)
     struct Adj {
         std::vector<void *> bufs;
@@ -148,6 +151,17 @@ void *dalloc(Backend *h, size_t bytes) {
     if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
     h->allocs.push_back(p);
     return p;
+}
+
+// Publishes B as the kernel arguments in device slot `slot` (the kernels
+// read their arguments from device memory).  The stream is drained first so
+// that no queued launch still reads the slot.
+template <class T>
+bool put_args(Backend *h, fm::Args<T> &B, int slot) {
+    B.self_d = static_cast<const fm::Args<T> *>(h->args_d[slot]);
+    B.nsys = 1;
+    return hip_ok(h, hipStreamSynchronize(h->stream), "stream sync") &&
+           hip_ok(h, hipMemcpy(h->args_d[slot], &B, sizeof(B), hipMemcpyHostToDevice), "kernel arguments");
 }
 
 bool plan_fft(int n, fm::FftDesc &fd, std::string *err) {
@@ -271,6 +285,7 @@ bool eval_current(Backend *h, bool observables) {
             B.obs_hex = reinterpret_cast<T *>(h->obs_base[0]);
             B.obs_hd = reinterpret_cast<T *>(h->obs_base[1]);
             B.obs_heff = reinterpret_cast<T *>(h->obs_base[2]);
+            if (!put_args(h, B, 1)) return false;
         }
         return hip_ok(h, fm::enqueue_eval(B, 1, 2, h->stream), "launch (evaluation)");
     });
@@ -595,6 +610,9 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.counter = static_cast<unsigned *>(dalloc(h, 64));
     if (!A.partials || !A.counter) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
     if (!hip_ok(h, hipMemset(A.counter, 0, 64), "clear counter")) return false;
+    for (int i = 0; i < 3; ++i)
+        if (!(h->args_d[i] = dalloc(h, sizeof(fm::Args<T>)))) return hip_ok(h, hipErrorOutOfMemory, "alloc arguments");
+    if (!put_args(h, A, 0)) return false;
     return upload_state<T>(h, p->initial_magnetization_xyz, nullptr, 0);
 }
 
@@ -1186,6 +1204,8 @@ int fullmag_fdm_backend_set_adaptive_ext(fullmag_fdm_backend *h, double rtol, do
     h->af.rtol = h->ad.rtol = rtol;
     h->af.growth = h->ad.growth = g;
     h->af.shrink = h->ad.shrink = shrink;
+    const bool put = h->fp64 ? put_args(h, h->ad, 0) : put_args(h, h->af, 0);
+    if (!put) return FULLMAG_FDM_ERR_CUDA;
     // the captured graph holds the old kernel arguments
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->graph) hipGraphDestroy(h->graph);
@@ -1415,6 +1435,7 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
     B.rec_seg = J.seg;
     B.rec_sv = J.sv;
     B.rec_ss = J.ss;
+    if (!put_args(h, B, 2)) return FULLMAG_FDM_ERR_CUDA;
     auto rec_step = [&]() { return hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint recompute"); };
     auto rev_step = [&]() {
         if (!hip_ok(h, fm::enqueue_adj_init(A, J, ns, h->stream), "adjoint init")) return false;

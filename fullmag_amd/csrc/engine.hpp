@@ -94,6 +94,10 @@ struct Ctl {
 
 template <class T>
 struct Args {
+    // device copy of these arguments (kernels read AP[blockIdx.y]) and the
+    // number of systems sharing a launch (batched sweeps; 1 otherwise)
+    const Args *self_d;
+    int nsys;
     // grid
     int nx, ny, nz;
     long long ncell;

@@ -1020,7 +1020,8 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
 
 // Forward x pass of stage s: combination, Ms*y packing, R2C FFT along x.
 template <class T, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
+__global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ AP, int s) {
+    const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -1040,7 +1041,8 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(Args<T> A, int s) {
 // Inverse x pass and local pass of stage s; the last work-group reduces the
 // partials and runs the controller (control 1) or stores statistics (2).
 template <class T, bool LUT, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats, int control) {
+__global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats, int control) {
+    const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -1069,7 +1071,8 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(Args<T> A, int s, int stats,
 // and k_s of stage s feed the forward x pass of stage s+1 directly.  NK =
 // k-vectors of stage s+1.  Results are those of the separate launches.
 template <class T, bool LUT, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads) k_xx(Args<T> A, int s) {
+__global__ void __launch_bounds__(kThreads) k_xx(const Args<T> *__restrict__ AP, int s) {
+    const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -1096,7 +1099,8 @@ __global__ void __launch_bounds__(kThreads) k_xx(Args<T> A, int s) {
 // MODE 0: thin-film fused forward FFT + tensor multiply + inverse FFT.
 // MODE 1: 3-D forward y FFT.   MODE 2: 3-D inverse y FFT.
 template <class T, int MODE>
-__global__ void __launch_bounds__(kThreads) k_y(Args<T> A) {
+__global__ void __launch_bounds__(kThreads) k_y(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     const int py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
@@ -1153,7 +1157,8 @@ __global__ void __launch_bounds__(kThreads) k_y(Args<T> A) {
 // tensor multiply H = -N M; the inverse runs the same steps backwards and
 // keeps the first ny outputs.
 template <class T, int R1, int R2>
-__global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(Args<T> A) {
+__global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     constexpr int TS = R1 > R2 ? R1 : R2;
     constexpr int N = R1 * R2;
@@ -1246,7 +1251,8 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(Args<T> A) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
+__global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     const int pz = A.pz, nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, TY = A.ty;
     const int ntile = (py + TY - 1) / TY;
@@ -1301,7 +1307,8 @@ __global__ void __launch_bounds__(kThreads) k_z(Args<T> A) {
 // zero padding), the full padded spectrum is written.  Inverse: py points
 // are read, the first ny written.
 template <class T, int R1, int R2, bool INV>
-__global__ void __launch_bounds__(kThreads) k_yl(Args<T> A) {
+__global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     constexpr int TS = R1 > R2 ? R1 : R2, LPB = kThreads / TS;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
@@ -1373,7 +1380,8 @@ __global__ void __launch_bounds__(kThreads) k_yl(Args<T> A) {
 // as [component][element][ky] so that consecutive lanes (consecutive ky) hit
 // consecutive addresses.
 template <class T, int R1, int R2, int UB>
-__global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(Args<T> A) {
+__global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
@@ -1658,7 +1666,8 @@ __device__ void block_add3(double v[3], double *out) {
 // derivative at m_{n+1}), normalisation of the final stage, k cotangents
 // of the final row; cot m_n restarts from there.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J0, int nstage) {
+__global__ void __launch_bounds__(kThreads) k_adj_init(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int nstage) {
+    const Args<T> &A = AP[blockIdx.y];
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= N) return;
@@ -1691,7 +1700,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_init(Args<T> A, AdjArgs<T> J0,
 
 // Objective partial sums of the current state: obj_part[block] += sum w_i m_c,i^2.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_objective(Args<T> A, AdjArgs<T> J) {
+__global__ void __launch_bounds__(kThreads) k_adj_objective(const Args<T> *__restrict__ AP, AdjArgs<T> J) {
+    const Args<T> &A = AP[blockIdx.y];
     if ((long long)A.ctl->step < J.obj_from) return;
     const int N = (int)A.ncell;
     const T *m = A.m_all + (size_t)A.ctl->parity * 3 * N + (size_t)J.obj_comp * N;
@@ -1709,7 +1719,8 @@ static __global__ void k_adj_step(long long *step, long long value, int add) {
 
 // Without demag: beta of stage j, one thread per cell.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J0, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_beta(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+    const Args<T> &A = AP[blockIdx.y];
     const AdjArgs<T> J = adj_bind(J0);
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1722,7 +1733,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_beta(Args<T> A, AdjArgs<T> J0,
 
 // Without demag: cotangent of y_j, one thread per cell.
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J0, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_local(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+    const Args<T> &A = AP[blockIdx.y];
     const AdjArgs<T> J = adj_bind(J0);
     const int N = (int)A.ncell;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1740,7 +1752,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_local(Args<T> A, AdjArgs<T> J0
 // With demag: beta of stage j for the work-group's x-rows, stored and
 // packed (unit Ms) two real rows per complex sequence, forward x FFT.
 template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J0, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+    const Args<T> &A = AP[blockIdx.y];
     constexpr bool FAST = R1 > 0;
     const AdjArgs<T> J = adj_bind(J0);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
@@ -1777,7 +1790,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(Args<T> A, AdjArgs<T> J0, i
 // With demag: inverse x FFT of the convolved beta spectrum (1/P), then the
 // cotangent of y_j for the work-group's cells.
 template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_bx(Args<T> A, AdjArgs<T> J0, int j) {
+__global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+    const Args<T> &A = AP[blockIdx.y];
     constexpr bool FAST = R1 > 0;
     const AdjArgs<T> J = adj_bind(J0);
     const Method &M = c_methods[A.method];
@@ -1812,7 +1826,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_bx(Args<T> A, AdjArgs<T> J0, i
 }
 
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
+__global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ AP, int s) {
+    const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1823,7 +1838,8 @@ __global__ void __launch_bounds__(kThreads) k_stage(Args<T> A, int s) {
 }
 
 template <class T, bool LUT>
-__global__ void __launch_bounds__(kThreads) k_local(Args<T> A, int s, int stats, int control) {
+__global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats, int control) {
+    const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
     const int p = ctl->parity;
@@ -1916,7 +1932,8 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
 // Stand-alone controller launch (used for timing and by callers that keep
 // the partial-producing kernel and the controller apart).
 template <class T>
-__global__ void __launch_bounds__(kThreads) k_control(Args<T> A, int nparts, int observe) {
+__global__ void __launch_bounds__(kThreads) k_control(const Args<T> *__restrict__ AP, int nparts, int observe) {
+    const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     control_block(A, nparts, observe);
 }
@@ -2039,6 +2056,14 @@ static bool yfast_shape(int py, int &r1, int &r2) {
         case 2048: r1 = 64; r2 = 32; return true;
         default: return false;
     }
+}
+
+// Launch grid of a (possibly batched) launch: x = work-groups of one
+// system, y = systems sharing the launch (each reads its arguments at
+// AP[blockIdx.y]).
+template <class T>
+static dim3 bgrid(const Args<T> &A, dim3 g) {
+    return dim3(g.x, (unsigned)A.nsys, 1);
 }
 
 // Four-step factorisation pz = R1*R2 of the 3-D fast z pass.
@@ -2238,7 +2263,7 @@ const char *configure(Args<T> &A, size_t lds_limit) {
 template <class T, int R1, int R2>
 static void launch_yf_shape(const Args<T> &A, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
-    k_yf<T, R1, R2><<<dim3(A.nkx), dim3(3 * TS), lds_yf<T, R1, R2>(), st>>>(A);
+    k_yf<T, R1, R2><<<bgrid(A, dim3(A.nkx)), dim3(3 * TS), lds_yf<T, R1, R2>(), st>>>(A.self_d);
 }
 
 template <class T>
@@ -2265,30 +2290,30 @@ static void launch_fwd_x_shape(const Args<T> &A, int s, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
     const size_t l = lds_x(A);
     switch (A.stage_nk[s]) {
-        case 0: k_fwd_x<T, R1, R2, 0><<<g, b, l, st>>>(A, s); break;
-        case 1: k_fwd_x<T, R1, R2, 1><<<g, b, l, st>>>(A, s); break;
-        case 2: k_fwd_x<T, R1, R2, 2><<<g, b, l, st>>>(A, s); break;
-        case 3: k_fwd_x<T, R1, R2, 3><<<g, b, l, st>>>(A, s); break;
-        case 4: k_fwd_x<T, R1, R2, 4><<<g, b, l, st>>>(A, s); break;
-        default: k_fwd_x<T, R1, R2, 5><<<g, b, l, st>>>(A, s); break;
+        case 0: k_fwd_x<T, R1, R2, 0><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 1: k_fwd_x<T, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 2: k_fwd_x<T, R1, R2, 2><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 3: k_fwd_x<T, R1, R2, 3><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 4: k_fwd_x<T, R1, R2, 4><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        default: k_fwd_x<T, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
     }
 }
 template <class T, int R1, int R2>
 static void launch_inv_x_shape(const Args<T> &A, int s, int stats, int control, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
-    if (A.region) k_inv_x<T, true, R1, R2><<<g, b, lds_x(A), st>>>(A, s, stats, control);
-    else k_inv_x<T, false, R1, R2><<<g, b, lds_x(A), st>>>(A, s, stats, control);
+    if (A.region) k_inv_x<T, true, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats, control);
+    else k_inv_x<T, false, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats, control);
 }
 template <class T, bool LUT, int R1, int R2>
 static void launch_xx_lut(const Args<T> &A, int s, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
     const size_t l = lds_x(A);
     switch (A.stage_nk[s + 1]) {
-        case 1: k_xx<T, LUT, R1, R2, 1><<<g, b, l, st>>>(A, s); break;
-        case 2: k_xx<T, LUT, R1, R2, 2><<<g, b, l, st>>>(A, s); break;
-        case 3: k_xx<T, LUT, R1, R2, 3><<<g, b, l, st>>>(A, s); break;
-        case 4: k_xx<T, LUT, R1, R2, 4><<<g, b, l, st>>>(A, s); break;
-        default: k_xx<T, LUT, R1, R2, 5><<<g, b, l, st>>>(A, s); break;
+        case 1: k_xx<T, LUT, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 2: k_xx<T, LUT, R1, R2, 2><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 3: k_xx<T, LUT, R1, R2, 3><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 4: k_xx<T, LUT, R1, R2, 4><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        default: k_xx<T, LUT, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
     }
 }
 template <class T, int R1, int R2>
@@ -2337,8 +2362,8 @@ template <class T, int R1, int R2>
 static void launch_yl_shape(const Args<T> &A, bool inv, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
     const unsigned g = (unsigned)((3ll * A.nzs * A.nkx + kThreads / TS - 1) / (kThreads / TS));
-    if (inv) k_yl<T, R1, R2, true><<<dim3(g), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A);
-    else k_yl<T, R1, R2, false><<<dim3(g), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A);
+    if (inv) k_yl<T, R1, R2, true><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
+    else k_yl<T, R1, R2, false><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
 }
 // 3-D y pass (forward: inv = false; inverse: inv = true).
 template <class T>
@@ -2352,14 +2377,14 @@ static void launch_y3(const Args<T> &A, bool inv, hipStream_t st) {
         case 2048: launch_yl_shape<T, 64, 32>(A, inv, st); return;
         default: break;
     }
-    if (inv) k_y<T, 2><<<dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st>>>(A);
-    else k_y<T, 1><<<dim3(A.nkx * A.nz), dim3(kThreads), lds_y(A), st>>>(A);
+    if (inv) k_y<T, 2><<<bgrid(A, dim3(A.nkx * A.nz)), dim3(kThreads), lds_y(A), st>>>(A.self_d);
+    else k_y<T, 1><<<bgrid(A, dim3(A.nkx * A.nz)), dim3(kThreads), lds_y(A), st>>>(A.self_d);
 }
 template <class T, int R1, int R2>
 static void launch_zf_shape(const Args<T> &A, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
     const unsigned g = (unsigned)(A.nkx * ((A.py + kZfUnits - 1) / kZfUnits));
-    k_zf<T, R1, R2, kZfUnits><<<dim3(g), dim3(kZfUnits * TS), lds_zf<T, R1, R2>(), st>>>(A);
+    k_zf<T, R1, R2, kZfUnits><<<bgrid(A, dim3(g)), dim3(kZfUnits * TS), lds_zf<T, R1, R2>(), st>>>(A.self_d);
 }
 // 3-D z pass (forward FFT, tensor multiply, inverse FFT).
 template <class T>
@@ -2373,7 +2398,7 @@ static void launch_z3(const Args<T> &A, hipStream_t st) {
         default: break;
     }
     const unsigned gz = (unsigned)(A.nkx * ((A.py + A.ty - 1) / A.ty));
-    k_z<T><<<dim3(gz), dim3(kThreads), lds_z(A), st>>>(A);
+    k_z<T><<<bgrid(A, dim3(gz)), dim3(kThreads), lds_z(A), st>>>(A.self_d);
 }
 
 // The y (and for 3-D z) passes between the forward and inverse x passes.
@@ -2381,7 +2406,7 @@ template <class T>
 static void launch_yz(const Args<T> &A, hipStream_t st) {
     if (A.nz == 1) {
         if (A.yfast) launch_yf(A, st);
-        else k_y<T, 0><<<dim3(A.nkx), dim3(kThreads), lds_y(A), st>>>(A);
+        else k_y<T, 0><<<bgrid(A, dim3(A.nkx)), dim3(kThreads), lds_y(A), st>>>(A.self_d);
     } else {
         launch_y3(A, false, st);
         launch_z3(A, st);
@@ -2401,9 +2426,9 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
         launch_yz(A, st);
         launch_inv_x(A, s, stats, control, st);
     } else {
-        if (s >= 2) hipLaunchKernelGGL(k_stage<T>, dim3(gpart), dim3(kThreads), 0, st, A, s);
-        if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
-        else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, stats, control);
+        if (s >= 2) k_stage<T><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s);
+        if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats, control);
+        else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats, control);
     }
     return hipGetLastError();
 }
@@ -2433,8 +2458,8 @@ hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
 template <class T, int R1, int R2>
 static void launch_adj_x_shape(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
-    if (inverse) k_adj_bx<T, R1, R2><<<g, b, lds_x(A), st>>>(A, J, j);
-    else k_adj_fx<T, R1, R2><<<g, b, lds_x(A), st>>>(A, J, j);
+    if (inverse) k_adj_bx<T, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, J, j);
+    else k_adj_fx<T, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, J, j);
 }
 template <class T>
 static void launch_adj_x(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
@@ -2485,15 +2510,15 @@ hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipSt
         launch_yz(A, st);
         launch_adj_x(A, J, j, true, st);
     } else {
-        k_adj_beta<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, j);
-        k_adj_local<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, j);
+        k_adj_beta<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J, j);
+        k_adj_local<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J, j);
     }
     return hipGetLastError();
 }
 template <class T>
 hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, hipStream_t st) {
     const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
-    k_adj_init<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J, nstage);
+    k_adj_init<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J, nstage);
     return hipGetLastError();
 }
 #ifdef FM_PRIMARY
@@ -2505,7 +2530,7 @@ hipError_t enqueue_adj_step(long long *step, long long value, int add, hipStream
 template <class T>
 hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st) {
     const unsigned gc = (unsigned)std::min<long long>((A.ncell + kThreads - 1) / kThreads, 1024);
-    k_adj_objective<T><<<dim3(gc), dim3(kThreads), 0, st>>>(A, J);
+    k_adj_objective<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J);
     return hipGetLastError();
 }
 template <class T>
@@ -2515,7 +2540,7 @@ hipError_t adj_configure(const Args<T> &A) {
 
 template <class T>
 hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st) {
-    hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(kThreads), 0, st, A, (int)num_partials(A), observe);
+    k_control<T><<<bgrid(A, dim3(1)), dim3(kThreads), 0, st>>>(A.self_d, (int)num_partials(A), observe);
     return hipGetLastError();
 }
 
@@ -2550,12 +2575,12 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
     switch (kind) {
         case 0:
             if (A.demag) launch_fwd_x(A, s, st);
-            else hipLaunchKernelGGL(k_stage<T>, dim3(gpart), dim3(kThreads), 0, st, A, s);
+            else k_stage<T><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s);
             break;
         case 1:
             if (!A.demag) return hipErrorInvalidValue;
             if (film && A.yfast) launch_yf(A, st);
-            else if (film) hipLaunchKernelGGL((k_y<T, 0>), dim3(A.nkx), dim3(kThreads), lds_y(A), st, A);
+            else if (film) k_y<T, 0><<<bgrid(A, dim3(A.nkx)), dim3(kThreads), lds_y(A), st>>>(A.self_d);
             else launch_y3(A, false, st);
             break;
         case 2:
@@ -2570,12 +2595,12 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             if (A.demag) {
                 launch_inv_x(A, s, 1, 0, st);
             } else {
-                if (A.region) hipLaunchKernelGGL((k_local<T, true>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
-                else hipLaunchKernelGGL((k_local<T, false>), dim3(gpart), dim3(kThreads), 0, st, A, s, 1, 0);
+                if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1, 0);
+                else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1, 0);
             }
             break;
         case 5:
-            hipLaunchKernelGGL(k_control<T>, dim3(1), dim3(kThreads), 0, st, A, (int)gpart, 0);
+            k_control<T><<<bgrid(A, dim3(1)), dim3(kThreads), 0, st>>>(A.self_d, (int)gpart, 0);
             break;
         case 6:
             if (!A.demag) return hipErrorInvalidValue;
