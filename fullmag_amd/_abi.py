@@ -237,6 +237,7 @@ class AdjointDesc(ctypes.Structure):
 # name -> (restype, argtypes); every symbol of include/fullmag_fdm.h first,
 # then include/fullmag_fdm_ext.h.
 _H = ctypes.c_void_p
+_B = ctypes.c_void_p  # fullmag_fdm_batch *
 REFERENCE_SYMBOLS = {
     "fullmag_fdm_is_available": (ctypes.c_int, []),
     "fullmag_fdm_backend_create": (_H, [ctypes.POINTER(PlanDesc)]),
@@ -281,6 +282,14 @@ EXT_SYMBOLS = {
     "fullmag_fdm_backend_create_ext": (_H, [ctypes.POINTER(PlanDesc), ctypes.POINTER(PlanExt)]),
     "fullmag_fdm_adjoint_forward": (ctypes.c_int, [_H, ctypes.POINTER(AdjointDesc), c_double_p]),
     "fullmag_fdm_adjoint_backward": (ctypes.c_int, [_H, c_double_p, c_double_p]),
+    "fullmag_fdm_batch_create": (_B, [ctypes.POINTER(ctypes.POINTER(PlanDesc)),
+                                      ctypes.POINTER(ctypes.POINTER(PlanExt)), ctypes.c_uint64]),
+    "fullmag_fdm_batch_groups": (ctypes.c_int, [_B, ctypes.POINTER(ctypes.c_uint64)]),
+    "fullmag_fdm_batch_advance": (ctypes.c_int, [_B, ctypes.c_double, ctypes.c_uint64, ctypes.c_double, c_double_p,
+                                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "fullmag_fdm_batch_system": (_H, [_B, ctypes.c_uint64]),
+    "fullmag_fdm_batch_last_error": (ctypes.c_char_p, [_B]),
+    "fullmag_fdm_batch_destroy": (None, [_B]),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -96,6 +96,92 @@ def is_available() -> bool:
     return bool(abi.load().fullmag_fdm_is_available())
 
 
+def marshal_plan(plan: FdmPlan):
+    """PlanDesc (and PlanExt when the plan uses extensions) for `plan`, plus
+    the numpy arrays they point into (keep them alive while the library
+    reads the plan)."""
+    keep = []
+    desc = abi.PlanDesc()
+    desc.grid = abi.GridDesc(plan.nx, plan.ny, plan.nz, plan.dx, plan.dy, plan.dz)
+    desc.material = abi.MaterialDesc(plan.ms, plan.a_ex, plan.alpha, plan.gamma)
+    desc.precision = abi.PRECISION_DOUBLE if plan.precision == "double" else abi.PRECISION_SINGLE
+    desc.integrator = _INTEGRATORS[plan.integrator]
+    desc.disable_precession = 0 if plan.precession else 1
+    desc.enable_exchange = int(plan.exchange)
+    desc.enable_demag = int(plan.demag)
+    if plan.h_ext is not None:
+        desc.has_external_field = 1
+        desc.external_field_am[:] = list(plan.h_ext)
+    if plan.axis is not None:
+        desc.has_uniaxial_anisotropy = 1
+        desc.uniaxial_anisotropy_constant = plan.ku1
+        desc.uniaxial_anisotropy_k2 = plan.ku2
+        desc.anisotropy_axis[:] = list(plan.axis)
+    m0 = np.ascontiguousarray(np.asarray(plan.m0, dtype=np.float64).reshape(-1))
+    keep.append(m0)
+    desc.initial_magnetization_xyz = _dp(m0)
+    desc.initial_magnetization_len = m0.size
+    if plan.mask is not None:
+        mk = np.ascontiguousarray(np.asarray(plan.mask, dtype=np.uint8).reshape(-1))
+        keep.append(mk)
+        desc.active_mask = mk.ctypes.data_as(abi.c_u8_p)
+        desc.active_mask_len = mk.size
+    if plan.regions is not None:
+        rg = np.ascontiguousarray(np.asarray(plan.regions, dtype=np.uint32).reshape(-1))
+        keep.append(rg)
+        desc.region_mask = rg.ctypes.data_as(abi.c_u32_p)
+        desc.region_mask_len = rg.size
+    if plan.exchange_lut is not None:
+        lt = np.ascontiguousarray(np.asarray(plan.exchange_lut, dtype=np.float64).reshape(-1))
+        keep.append(lt)
+        desc.exchange_lut = _dp(lt)
+        desc.exchange_lut_len = lt.size
+    if plan.spectra is not None:
+        ln = None
+        for nm in ("xx", "yy", "zz", "xy", "xz", "yz"):
+            if nm in plan.spectra:
+                a = np.ascontiguousarray(np.asarray(plan.spectra[nm], dtype=np.float64))
+                keep.append(a)
+                setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
+                ln = a.size
+        desc.demag_kernel_spectrum_len = ln
+    for nm in ("ku1_field", "ku2_field"):
+        a = getattr(plan, nm)
+        if a is not None:
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+            keep.append(a)
+            setattr(desc, nm, _dp(a))
+    desc.adaptive_max_error = plan.atol
+    desc.adaptive_dt_min = plan.dt_min
+    desc.adaptive_dt_max = plan.dt_max
+    desc.adaptive_headroom = plan.headroom
+    for k, v in plan.extra.items():
+        setattr(desc, k, v)
+    ext = None
+    if plan.uses_ext():
+        ext = abi.PlanExt()
+        ext.struct_size = ctypes.sizeof(abi.PlanExt)
+        if plan.ms_field is not None:
+            a = np.ascontiguousarray(np.asarray(plan.ms_field, dtype=np.float64).reshape(-1))
+            keep.append(a)
+            ext.ms_field = _dp(a)
+        if plan.axis_field is not None:
+            a = np.ascontiguousarray(np.asarray(plan.axis_field, dtype=np.float64).reshape(-1))
+            keep.append(a)
+            ext.anisotropy_axis_field = _dp(a)
+        ext.field_waveform = plan.wave
+        ext.field_amplitude_am[:] = list(plan.wave_amp)
+        ext.field_t0, ext.field_t1 = plan.wave_t0, plan.wave_t1
+        ext.field_frequency, ext.field_phase = plan.wave_freq, plan.wave_phase
+        if plan.wave_mask is not None:
+            wm = np.ascontiguousarray(np.asarray(plan.wave_mask, dtype=np.uint8).reshape(-1))
+            keep.append(wm)
+            ext.field_mask = wm.ctypes.data_as(abi.c_u8_p)
+        ext.adaptive_rtol, ext.adaptive_growth_limit = plan.rtol, plan.growth
+        ext.adaptive_shrink_limit = plan.shrink
+    return desc, ext, keep
+
+
 class FdmBackend:
     """One engine handle (one host thread drives it)."""
 
@@ -103,86 +189,9 @@ class FdmBackend:
         self.lib = abi.load(lib_path)
         self.plan = plan
         self.n = plan.nx * plan.ny * plan.nz
-        self._keep = []
-        desc = abi.PlanDesc()
-        desc.grid = abi.GridDesc(plan.nx, plan.ny, plan.nz, plan.dx, plan.dy, plan.dz)
-        desc.material = abi.MaterialDesc(plan.ms, plan.a_ex, plan.alpha, plan.gamma)
-        desc.precision = abi.PRECISION_DOUBLE if plan.precision == "double" else abi.PRECISION_SINGLE
-        desc.integrator = _INTEGRATORS[plan.integrator]
-        desc.disable_precession = 0 if plan.precession else 1
-        desc.enable_exchange = int(plan.exchange)
-        desc.enable_demag = int(plan.demag)
-        if plan.h_ext is not None:
-            desc.has_external_field = 1
-            desc.external_field_am[:] = list(plan.h_ext)
-        if plan.axis is not None:
-            desc.has_uniaxial_anisotropy = 1
-            desc.uniaxial_anisotropy_constant = plan.ku1
-            desc.uniaxial_anisotropy_k2 = plan.ku2
-            desc.anisotropy_axis[:] = list(plan.axis)
-        m0 = np.ascontiguousarray(np.asarray(plan.m0, dtype=np.float64).reshape(-1))
-        self._keep.append(m0)
-        desc.initial_magnetization_xyz = _dp(m0)
-        desc.initial_magnetization_len = m0.size
-        if plan.mask is not None:
-            mk = np.ascontiguousarray(np.asarray(plan.mask, dtype=np.uint8).reshape(-1))
-            self._keep.append(mk)
-            desc.active_mask = mk.ctypes.data_as(abi.c_u8_p)
-            desc.active_mask_len = mk.size
-        if plan.regions is not None:
-            rg = np.ascontiguousarray(np.asarray(plan.regions, dtype=np.uint32).reshape(-1))
-            self._keep.append(rg)
-            desc.region_mask = rg.ctypes.data_as(abi.c_u32_p)
-            desc.region_mask_len = rg.size
-        if plan.exchange_lut is not None:
-            lt = np.ascontiguousarray(np.asarray(plan.exchange_lut, dtype=np.float64).reshape(-1))
-            self._keep.append(lt)
-            desc.exchange_lut = _dp(lt)
-            desc.exchange_lut_len = lt.size
-        if plan.spectra is not None:
-            ln = None
-            for nm in ("xx", "yy", "zz", "xy", "xz", "yz"):
-                if nm in plan.spectra:
-                    a = np.ascontiguousarray(np.asarray(plan.spectra[nm], dtype=np.float64))
-                    self._keep.append(a)
-                    setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
-                    ln = a.size
-            desc.demag_kernel_spectrum_len = ln
-        for nm in ("ku1_field", "ku2_field"):
-            a = getattr(plan, nm)
-            if a is not None:
-                a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
-                self._keep.append(a)
-                setattr(desc, nm, _dp(a))
-        desc.adaptive_max_error = plan.atol
-        desc.adaptive_dt_min = plan.dt_min
-        desc.adaptive_dt_max = plan.dt_max
-        desc.adaptive_headroom = plan.headroom
-        for k, v in plan.extra.items():
-            setattr(desc, k, v)
-        self._desc = desc
-        if plan.uses_ext():
-            ext = abi.PlanExt()
-            ext.struct_size = ctypes.sizeof(abi.PlanExt)
-            if plan.ms_field is not None:
-                a = np.ascontiguousarray(np.asarray(plan.ms_field, dtype=np.float64).reshape(-1))
-                self._keep.append(a)
-                ext.ms_field = _dp(a)
-            if plan.axis_field is not None:
-                a = np.ascontiguousarray(np.asarray(plan.axis_field, dtype=np.float64).reshape(-1))
-                self._keep.append(a)
-                ext.anisotropy_axis_field = _dp(a)
-            ext.field_waveform = plan.wave
-            ext.field_amplitude_am[:] = list(plan.wave_amp)
-            ext.field_t0, ext.field_t1 = plan.wave_t0, plan.wave_t1
-            ext.field_frequency, ext.field_phase = plan.wave_freq, plan.wave_phase
-            if plan.wave_mask is not None:
-                wm = np.ascontiguousarray(np.asarray(plan.wave_mask, dtype=np.uint8).reshape(-1))
-                self._keep.append(wm)
-                ext.field_mask = wm.ctypes.data_as(abi.c_u8_p)
-            ext.adaptive_rtol, ext.adaptive_growth_limit = plan.rtol, plan.growth
-            ext.adaptive_shrink_limit = plan.shrink
-            self._ext = ext
+        desc, ext, self._keep = marshal_plan(plan)
+        self._desc, self._ext = desc, ext
+        if ext is not None:
             self.h = self.lib.fullmag_fdm_backend_create_ext(ctypes.byref(desc), ctypes.byref(ext))
         else:
             self.h = self.lib.fullmag_fdm_backend_create(ctypes.byref(desc))
@@ -208,8 +217,18 @@ class FdmBackend:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.fullmag_fdm_backend_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.fullmag_fdm_backend_destroy(self.h)
             self.h = None
+
+    @classmethod
+    def _view(cls, lib, handle, plan: FdmPlan):
+        """Non-owning wrapper of a handle that a batch owns."""
+        obj = cls.__new__(cls)
+        obj.lib, obj.h, obj.plan = lib, handle, plan
+        obj.n = plan.nx * plan.ny * plan.nz
+        obj._keep, obj._poll, obj._owned = [], None, False
+        return obj
 
     def __del__(self):
         try:
@@ -365,3 +384,68 @@ def hole_mask(nx: int, ny: int, nz: int, dx: float, dy: float, radius: float) ->
     r2 = x[None, :] ** 2 + y[:, None] ** 2
     plane = (r2 > radius * radius).astype(np.uint8)
     return np.ascontiguousarray(np.broadcast_to(plane, (nz, ny, nx)).reshape(-1))
+
+
+class FdmBatch:
+    """Batched sweep of independent systems (fullmag_fdm_batch_*): systems
+    with the same launch geometry share every kernel launch."""
+
+    def __init__(self, plans: list, lib_path: str | None = None):
+        self.lib = abi.load(lib_path)
+        self.plans = list(plans)
+        marshalled = [marshal_plan(p) for p in self.plans]
+        self._keep = marshalled
+        n = len(self.plans)
+        descs = (ctypes.POINTER(abi.PlanDesc) * n)(*[ctypes.pointer(d) for d, _, _ in marshalled])
+        exts = (ctypes.POINTER(abi.PlanExt) * n)(*[ctypes.pointer(e) if e is not None else None
+                                                    for _, e, _ in marshalled])
+        self._arrays = (descs, exts)
+        self.b = self.lib.fullmag_fdm_batch_create(descs, exts, n)
+        if not self.b:
+            raise FdmError(abi.ERR_INTERNAL, "fullmag_fdm_batch_create returned NULL")
+        err = self.lib.fullmag_fdm_batch_last_error(self.b)
+        if err:
+            msg = err.decode()
+            self.lib.fullmag_fdm_batch_destroy(self.b)
+            self.b = None
+            raise FdmError(abi.ERR_INVALID, msg)
+        self.systems = [FdmBackend._view(self.lib, self.lib.fullmag_fdm_batch_system(self.b, i), p)
+                        for i, p in enumerate(self.plans)]
+
+    def groups(self) -> int:
+        g = ctypes.c_uint64(0)
+        if self.lib.fullmag_fdm_batch_groups(self.b, ctypes.byref(g)) != 0:
+            raise FdmError(abi.ERR_INVALID, "batch_groups failed")
+        return int(g.value)
+
+    def advance(self, t_end: float, max_steps: int, dt0: float):
+        """Every system to t_end with its own dt control; returns (wall
+        seconds after which each system was seen finished, accepted steps)."""
+        n = len(self.plans)
+        wall = np.zeros(n)
+        steps = np.zeros(n, dtype=np.uint64)
+        rc = self.lib.fullmag_fdm_batch_advance(self.b, t_end, max_steps, dt0, _dp(wall),
+                                                steps.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if rc != 0:
+            err = self.lib.fullmag_fdm_batch_last_error(self.b)
+            raise FdmError(rc, err.decode() if err else "batch advance failed")
+        return wall, steps.astype(np.int64)
+
+    def close(self):
+        if getattr(self, "b", None):
+            for s_ in self.systems:
+                s_.h = None
+            self.lib.fullmag_fdm_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

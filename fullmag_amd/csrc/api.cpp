@@ -1546,3 +1546,253 @@ int fullmag_fdm_adjoint_backward(fullmag_fdm_backend *h, double *grad_ms, double
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ batches --
+struct fullmag_fdm_batch {
+    std::string err;
+    std::vector<fullmag_fdm_backend *> sys;
+    struct Group {
+        std::vector<size_t> members;
+        bool fp64 = false;
+        int nstage = 0;
+        void *args_d = nullptr;  // members' kernel arguments, one per system
+        hipStream_t stream = nullptr;
+        hipGraphExec_t graph = nullptr;
+        fm::Args<float> gf{};
+        fm::Args<double> gd{};
+        bool finished = false;
+        hipEvent_t ev = nullptr;
+    };
+    std::vector<Group> groups;
+    fm::Ctl *flags = nullptr;  // pinned: control blocks read back per chunk
+};
+
+namespace {
+
+using Batch = fullmag_fdm_batch;
+
+// Launch geometry that must agree for systems to share launches.
+template <class T>
+std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
+    return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
+            A.ylfast, A.zfast, A.ty, A.nseq_x};
+}
+
+template <class T>
+fm::Args<T> &group_args(Batch::Group &g);
+template <>
+fm::Args<float> &group_args<float>(Batch::Group &g) { return g.gf; }
+template <>
+fm::Args<double> &group_args<double>(Batch::Group &g) { return g.gd; }
+
+// Uploads the members' current kernel arguments into the group array and
+// captures one trial step of the whole group (once).
+template <class T>
+bool group_prepare(Batch *b, Batch::Group &g) {
+    const size_t ns = g.members.size();
+    if (!g.args_d && hipMalloc(&g.args_d, ns * sizeof(fm::Args<T>)) != hipSuccess)
+        return (b->err = "batch: argument allocation failed", false);
+    std::vector<fm::Args<T>> host(ns);
+    for (size_t i = 0; i < ns; ++i) {
+        Backend *h = b->sys[g.members[i]];
+        host[i] = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
+    }
+    if (hipStreamSynchronize(g.stream) != hipSuccess ||
+        hipMemcpy(g.args_d, host.data(), ns * sizeof(fm::Args<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return (b->err = "batch: argument upload failed", false);
+    fm::Args<T> &G = group_args<T>(g);
+    G = host[0];
+    G.self_d = static_cast<const fm::Args<T> *>(g.args_d);
+    G.nsys = (int)ns;
+    if (!g.graph) {
+        if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            const hipError_t e = fm::enqueue_trial(G, g.nstage, g.stream);
+            hipGraph_t gr = nullptr;
+            const hipError_t ec = hipStreamEndCapture(g.stream, &gr);
+            if (e == hipSuccess && ec == hipSuccess && gr &&
+                hipGraphInstantiate(&g.graph, gr, nullptr, nullptr, 0) != hipSuccess)
+                g.graph = nullptr;
+            if (gr) hipGraphDestroy(gr);
+            (void)hipGetLastError();
+        }
+    }
+    return true;
+}
+
+template <class T>
+bool group_launch(Batch *b, Batch::Group &g) {
+    if (g.graph) return hipGraphLaunch(g.graph, g.stream) == hipSuccess || (b->err = "batch: graph launch", false);
+    return fm::enqueue_trial(group_args<T>(g), g.nstage, g.stream) == hipSuccess ||
+           (b->err = "batch: launch failed", false);
+}
+
+}  // namespace
+
+extern "C" {
+
+fullmag_fdm_batch *fullmag_fdm_batch_create(const fullmag_fdm_plan_desc *const *plans,
+                                            const fullmag_fdm_plan_ext *const *exts, uint64_t count) {
+    if (!plans || count == 0) return nullptr;
+    Batch *b = new (std::nothrow) Batch();
+    if (!b) return nullptr;
+    for (uint64_t i = 0; i < count; ++i) {
+        Backend *h = fullmag_fdm_backend_create_ext(plans[i], exts ? exts[i] : nullptr);
+        if (!h) {
+            b->err = "batch: system " + std::to_string(i) + ": create returned NULL";
+            return b;
+        }
+        b->sys.push_back(h);
+        if (!h->ok) {
+            b->err = "batch: system " + std::to_string(i) + ": " + h->err;
+            return b;
+        }
+    }
+    std::vector<std::vector<long long>> keys;
+    for (size_t i = 0; i < b->sys.size(); ++i) {
+        Backend *h = b->sys[i];
+        const auto key = h->fp64 ? group_key(h->ad, true) : group_key(h->af, false);
+        size_t gi = 0;
+        while (gi < keys.size() && keys[gi] != key) ++gi;
+        if (gi == keys.size()) {
+            keys.push_back(key);
+            b->groups.emplace_back();
+            b->groups.back().fp64 = h->fp64;
+            b->groups.back().nstage = h->nstage;
+        }
+        b->groups[gi].members.push_back(i);
+    }
+    hipSetDevice(b->sys[0]->device);
+    for (auto &g : b->groups) {
+        if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&g.ev, hipEventDisableTiming) != hipSuccess) {
+            b->err = "batch: stream creation failed";
+            return b;
+        }
+    }
+    if (hipHostMalloc((void **)&b->flags, b->sys.size() * sizeof(fm::Ctl), hipHostMallocDefault) != hipSuccess)
+        b->err = "batch: pinned allocation failed";
+    return b;
+}
+
+int fullmag_fdm_batch_groups(const fullmag_fdm_batch *b, uint64_t *out) {
+    if (!b || !out) return FULLMAG_FDM_ERR_INVALID;
+    *out = b->groups.size();
+    return FULLMAG_FDM_OK;
+}
+
+fullmag_fdm_backend *fullmag_fdm_batch_system(fullmag_fdm_batch *b, uint64_t i) {
+    return (b && i < b->sys.size()) ? b->sys[i] : nullptr;
+}
+
+const char *fullmag_fdm_batch_last_error(const fullmag_fdm_batch *b) {
+    return (b && !b->err.empty()) ? b->err.c_str() : nullptr;
+}
+
+int fullmag_fdm_batch_advance(fullmag_fdm_batch *b, double t_end, uint64_t max_steps, double dt0,
+                              double *out_wall, uint64_t *out_steps) {
+    if (!b) return FULLMAG_FDM_ERR_INVALID;
+    if (!b->err.empty() || b->sys.empty() || !b->flags) return FULLMAG_FDM_ERR_INVALID;
+    if (!(dt0 > 0.0) || !std::isfinite(dt0) || std::isnan(t_end)) {
+        b->err = "batch: dt must be positive and finite";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    hipSetDevice(b->sys[0]->device);
+    const size_t nsys = b->sys.size();
+    std::vector<uint64_t> step0(nsys);
+    std::vector<double> wall(nsys, -1.0);
+    // arm every system's device loop (as fullmag_fdm_backend_advance does)
+    for (size_t i = 0; i < nsys; ++i) {
+        Backend *h = b->sys[i];
+        fm::Ctl &c = *h->ctl_h;
+        step0[i] = c.step;
+        const double dmin = h->fp64 ? h->ad.dt_min : h->af.dt_min, dmax = h->fp64 ? h->ad.dt_max : h->af.dt_max;
+        c.loop = 1;
+        c.t_end = t_end;
+        c.step_end = c.step + max_steps;
+        c.dt_carry = dt0;
+        double d = std::min(dt0, t_end - c.t);
+        if (h->adaptive) d = std::min(std::max(d, dmin), dmax);
+        c.dt = d;
+        c.done = (c.t >= t_end || max_steps == 0) ? 1 : 0;
+        h->obs_fresh = false;
+        if (!push_ctl(h) || hipStreamSynchronize(h->stream) != hipSuccess) {
+            b->err = "batch: control upload failed: " + h->err;
+            return FULLMAG_FDM_ERR_CUDA;
+        }
+    }
+    for (auto &g : b->groups) {
+        g.finished = false;
+        const bool ok = g.fp64 ? group_prepare<double>(b, g) : group_prepare<float>(b, g);
+        if (!ok) return FULLMAG_FDM_ERR_CUDA;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    int rc = FULLMAG_FDM_OK;
+    for (;;) {
+        bool active = false;
+        for (auto &g : b->groups) {
+            if (g.finished) continue;
+            active = true;
+            for (int a = 0; a < kAttemptChunk && rc == FULLMAG_FDM_OK; ++a)
+                if (!(g.fp64 ? group_launch<double>(b, g) : group_launch<float>(b, g))) rc = FULLMAG_FDM_ERR_CUDA;
+            for (size_t m : g.members)
+                hipMemcpyAsync(&b->flags[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream);
+            hipEventRecord(g.ev, g.stream);
+        }
+        if (!active || rc != FULLMAG_FDM_OK) break;
+        for (auto &g : b->groups) {
+            if (g.finished) continue;
+            if (hipEventSynchronize(g.ev) != hipSuccess) {
+                b->err = "batch: event sync failed";
+                rc = FULLMAG_FDM_ERR_CUDA;
+                break;
+            }
+            bool all = true;
+            for (size_t m : g.members) {
+                if (b->flags[m].done) {
+                    if (wall[m] < 0) wall[m] = now();
+                } else {
+                    all = false;
+                }
+            }
+            g.finished = all;
+        }
+        if (rc != FULLMAG_FDM_OK) break;
+    }
+    for (auto &g : b->groups) hipStreamSynchronize(g.stream);
+    const hipError_t le = hipGetLastError();
+    if (rc == FULLMAG_FDM_OK && le != hipSuccess) {
+        b->err = std::string("batch: device error: ") + hipGetErrorString(le);
+        rc = FULLMAG_FDM_ERR_CUDA;
+    }
+    for (size_t i = 0; i < nsys; ++i) {
+        Backend *h = b->sys[i];
+        if (!pull_ctl(h)) {
+            rc = FULLMAG_FDM_ERR_CUDA;
+            continue;
+        }
+        fm::Ctl &c = *h->ctl_h;
+        if (out_steps) out_steps[i] = c.step - step0[i];
+        if (out_wall) out_wall[i] = wall[i] < 0 ? now() : wall[i];
+        c.loop = 0;
+        c.done = 0;
+        if (!push_ctl(h) || !pull_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
+    }
+    return rc;
+}
+
+void fullmag_fdm_batch_destroy(fullmag_fdm_batch *b) {
+    if (!b) return;
+    for (auto &g : b->groups) {
+        if (g.stream) hipStreamSynchronize(g.stream);
+        if (g.graph) hipGraphExecDestroy(g.graph);
+        if (g.args_d) hipFree(g.args_d);
+        if (g.ev) hipEventDestroy(g.ev);
+        if (g.stream) hipStreamDestroy(g.stream);
+    }
+    for (auto *h : b->sys) fullmag_fdm_backend_destroy(h);
+    if (b->flags) hipHostFree(b->flags);
+    delete b;
+}
+
+}  // extern "C"

@@ -57,3 +57,24 @@ def granular_block(nx: int = 256, ny: int = 256, nz: int = 64, grains: int = 512
                 integrator="dp45", ku1=5e5, axis=(0.0, 0.0, 1.0), axis_field=axes, wave=abi.WAVE_RAMP,
                 wave_amp=(0.0, 0.0, 1.0 / MU0), wave_t0=0.0, wave_t1=1e-9, atol=1e-5, dt_min=1e-18,
                 dt_max=1e-10, headroom=0.8)
+
+
+BATCH_SHAPES = ((64, 64, 1), (128, 64, 2), (256, 128, 1), (256, 256, 4))
+
+
+def batch_sweep(count: int = 96, seed: int = 4, shapes=BATCH_SHAPES, precision: str = "single") -> list:
+    """Independent systems of a parameter sweep: sizes drawn by seed from
+    `shapes` (cells of 4x4x4 nm), Ms ~ U[6e5, 1.2e6] A/m, A ~ U[1e-11,
+    2e-11] J/m, alpha ~ U[0.01, 0.5], seeded random m0 per system, exchange
+    + demag, DP45 atol 1e-5 with its own adaptive dt.  Returns FdmPlan
+    keyword dicts."""
+    from .backend import random_unit_vectors
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nx, ny, nz = shapes[int(rng.integers(len(shapes)))]
+        out.append(dict(nx=nx, ny=ny, nz=nz, dx=4e-9, dy=4e-9, dz=4e-9, ms=float(rng.uniform(6e5, 1.2e6)),
+                        a_ex=float(rng.uniform(1e-11, 2e-11)), alpha=float(rng.uniform(0.01, 0.5)),
+                        m0=random_unit_vectors(seed * 1000 + i + 1, nx * ny * nz), precision=precision,
+                        integrator="dp45", atol=1e-5, dt_min=1e-18, dt_max=1e-10, headroom=0.8))
+    return out

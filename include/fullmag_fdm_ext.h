@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 3
+#define FULLMAG_FDM_EXT_VERSION 4
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -136,6 +136,38 @@ typedef struct fullmag_fdm_adjoint_desc {
 int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *handle, const fullmag_fdm_adjoint_desc *desc,
                                 double *out_objective);
 int fullmag_fdm_adjoint_backward(fullmag_fdm_backend *handle, double *grad_ms, double *grad_field_amplitude);
+
+/*
+ * Batched sweep of independent systems (no reference counterpart; for
+ * parameter sweeps of many small systems that cannot fill the device one at
+ * a time).  Each plan becomes a system with its own state, material, step
+ * control and adaptive dt.  Systems whose launch geometry agrees (grid
+ * shape, precision, integrator, demag and exchange form, tuned tile) form a
+ * group: every kernel of a trial step covers the whole group (one row of
+ * work-groups per system), one captured graph per trial step, and the
+ * groups run concurrently on their own streams.
+ *
+ * create: plans[i] with optional exts[i] (exts or its entries may be NULL);
+ * returns non-NULL with last_error set on failure, like backend_create.
+ * advance: integrates every system to t_end (at most max_steps accepted
+ * steps each) with its own dt control starting from dt0, as
+ * fullmag_fdm_backend_advance does for one system; out_wall_seconds[i]
+ * (optional) receives the wall time after which system i was seen
+ * finished (resolution: one polling chunk), out_steps[i] (optional) its
+ * accepted steps.  system(i): the handle of system i, owned by the batch
+ * and valid until batch_destroy, for field copies, snapshots and stats
+ * (not for stepping while a batch advance runs).
+ */
+typedef struct fullmag_fdm_batch fullmag_fdm_batch;
+
+fullmag_fdm_batch *fullmag_fdm_batch_create(const fullmag_fdm_plan_desc *const *plans,
+                                            const fullmag_fdm_plan_ext *const *exts, uint64_t count);
+int fullmag_fdm_batch_groups(const fullmag_fdm_batch *batch, uint64_t *out_groups);
+int fullmag_fdm_batch_advance(fullmag_fdm_batch *batch, double t_end, uint64_t max_steps, double dt0,
+                              double *out_wall_seconds, uint64_t *out_steps);
+fullmag_fdm_backend *fullmag_fdm_batch_system(fullmag_fdm_batch *batch, uint64_t index);
+const char *fullmag_fdm_batch_last_error(const fullmag_fdm_batch *batch);
+void fullmag_fdm_batch_destroy(fullmag_fdm_batch *batch);
 
 #ifdef __cplusplus
 }
