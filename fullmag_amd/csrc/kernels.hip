@@ -2069,6 +2069,8 @@ static dim3 bgrid(const Args<T> &A, dim3 g) {
 // Four-step factorisation pz = R1*R2 of the 3-D fast z pass.
 static bool zfast_shape(int pz, int &r1, int &r2) {
     switch (pz) {
+        case 4: r1 = 2; r2 = 2; return true;
+        case 8: r1 = 2; r2 = 4; return true;
         case 16: r1 = 4; r2 = 4; return true;
         case 32: r1 = 4; r2 = 8; return true;
         case 64: r1 = 8; r2 = 8; return true;
@@ -2077,7 +2079,11 @@ static bool zfast_shape(int pz, int &r1, int &r2) {
         default: return false;
     }
 }
-constexpr int kZfUnits = 16;  // ky lines per work-group of the fast z pass
+// ky lines per work-group of the fast z pass: 256 threads in all
+template <int R1, int R2>
+constexpr int zf_units() {
+    return kThreads / (R1 > R2 ? R1 : R2);
+}
 
 template <class T, int R1, int R2>
 static size_t lds_yl() {
@@ -2090,7 +2096,7 @@ template <class T, int R1, int R2>
 static size_t lds_zf() {
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
-    return 3ull * TB * kZfUnits * sizeof(C2<T>);
+    return 3ull * TB * zf_units<R1, R2>() * sizeof(C2<T>);
 }
 
 template <class T>
@@ -2186,11 +2192,13 @@ template <class T, int R1, int R2>
 static hipError_t set_lds_zf_shape(size_t limit) {
     const size_t b = lds_zf<T, R1, R2>();
     if (b > limit) return hipErrorInvalidValue;
-    return set_lds<T>((const void *)k_zf<T, R1, R2, kZfUnits>, b);
+    return set_lds<T>((const void *)k_zf<T, R1, R2, zf_units<R1, R2>()>, b);
 }
 template <class T>
 static hipError_t set_lds_zf(int pz, size_t limit) {
     switch (pz) {
+        case 4: return set_lds_zf_shape<T, 2, 2>(limit);
+        case 8: return set_lds_zf_shape<T, 2, 4>(limit);
         case 16: return set_lds_zf_shape<T, 4, 4>(limit);
         case 32: return set_lds_zf_shape<T, 4, 8>(limit);
         case 64: return set_lds_zf_shape<T, 8, 8>(limit);
@@ -2383,13 +2391,16 @@ static void launch_y3(const Args<T> &A, bool inv, hipStream_t st) {
 template <class T, int R1, int R2>
 static void launch_zf_shape(const Args<T> &A, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
-    const unsigned g = (unsigned)(A.nkx * ((A.py + kZfUnits - 1) / kZfUnits));
-    k_zf<T, R1, R2, kZfUnits><<<bgrid(A, dim3(g)), dim3(kZfUnits * TS), lds_zf<T, R1, R2>(), st>>>(A.self_d);
+    constexpr int UB = zf_units<R1, R2>();
+    const unsigned g = (unsigned)(A.nkx * ((A.py + UB - 1) / UB));
+    k_zf<T, R1, R2, UB><<<bgrid(A, dim3(g)), dim3(UB * TS), lds_zf<T, R1, R2>(), st>>>(A.self_d);
 }
 // 3-D z pass (forward FFT, tensor multiply, inverse FFT).
 template <class T>
 static void launch_z3(const Args<T> &A, hipStream_t st) {
     switch (A.zfast) {
+        case 4: launch_zf_shape<T, 2, 2>(A, st); return;
+        case 8: launch_zf_shape<T, 2, 4>(A, st); return;
         case 16: launch_zf_shape<T, 4, 4>(A, st); return;
         case 32: launch_zf_shape<T, 4, 8>(A, st); return;
         case 64: launch_zf_shape<T, 8, 8>(A, st); return;

@@ -105,7 +105,8 @@ def test_fields_register_four_step_x_shapes(precision, nx, ny):
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-@pytest.mark.parametrize("nx,ny,nz", [(6, 32, 8), (5, 64, 16), (4, 128, 32), (3, 256, 64), (4, 32, 128)])
+@pytest.mark.parametrize("nx,ny,nz", [(6, 32, 2), (5, 64, 4), (6, 32, 8), (5, 64, 16), (4, 128, 32), (3, 256, 64),
+                                      (4, 32, 128)])
 def test_fields_register_four_step_3d_shapes(precision, nx, ny, nz):
     # 3-D grids whose padded y (64..512) and z (16..256) lengths take the
     # register four-step y-line and z passes
