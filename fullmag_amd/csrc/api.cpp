@@ -575,9 +575,12 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.twx = upload_twiddles<T>(h, A.px);
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
-        auto four_step_twiddles = [&](int n, const void **dst, bool zshape = false) -> bool {
+        // swap: the table of the transposed factorisation n = r2*r1 (the
+        // forward 3-D y-line pass runs with the factors exchanged)
+        auto four_step_twiddles = [&](int n, const void **dst, bool zshape = false, bool swap = false) -> bool {
             int r1 = 0, r2 = 0;
             if (!(zshape ? fm::zfast_factors(n, &r1, &r2) : fm::yfast_factors(n, &r1, &r2))) return true;
+            if (swap) std::swap(r1, r2);
             // W_n^{t1 k2} stored at [k2*r1 + t1]
             std::vector<T> t4(2 * (size_t)n);
             for (int k2 = 0; k2 < r2; ++k2)
@@ -593,6 +596,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             return true;
         };
         if (!four_step_twiddles(A.py, &A.tw4y)) return false;
+        if (nz > 1 && !four_step_twiddles(A.py, &A.tw4y_f, false, true)) return false;
         if (!four_step_twiddles(A.px, &A.tw4x)) return false;
         if (nz > 1 && !four_step_twiddles(A.pz, &A.tw4z, true)) return false;
         if (!A.spec || !A.twx || !A.twy || !A.twz) return hip_ok(h, hipErrorOutOfMemory, "alloc spectrum");

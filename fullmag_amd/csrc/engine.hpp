@@ -116,6 +116,7 @@ struct Args {
     const T *ntens;
     const void *twx, *twy, *twz;   // complex<T> forward twiddles
     const void *tw4y;       // four-step twiddles W_py^{t1 k2} at [k2*R1 + t1] (thin-film fast y pass)
+    const void *tw4y_f;     // 3-D forward y-lines: the same for the exchanged factors (R2, R1)
     int yfast;              // 0 = Stockham y pass, else py of the register four-step y pass
     const void *tw4x;       // four-step twiddles of the x passes (W_px^{t1 k2} at [k2*R1 + t1])
     int xfast;              // 0 = Stockham x passes, else px of the register four-step x passes

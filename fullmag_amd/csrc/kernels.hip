@@ -1305,7 +1305,9 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
 // group of TS = max(R1,R2) threads owns one (component, z, kx) y-line of the
 // spectrum.  Forward: the first ny = py/2 points are read (the rest is the
 // zero padding), the full padded spectrum is written.  Inverse: py points
-// are read, the first ny written.
+// are read, the first ny written.  The forward pass is launched with the
+// factors of the inverse exchanged, so that in both directions the step
+// with every thread active is the one that reads HBM.
 template <class T, int R1, int R2, bool INV>
 __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
@@ -1318,7 +1320,7 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
     const int line = blockIdx.x * LPB + g;
     const bool on = line < nlines;
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + (size_t)(on ? line : 0) * A.ly;
-    const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4y);
+    const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(INV ? A.tw4y : A.tw4y_f);
     C2<T> *tb = reinterpret_cast<C2<T> *>(fm_smem) + g * TB;
     if constexpr (!INV) {
         if (t < R1) {
@@ -2070,11 +2072,11 @@ static dim3 bgrid(const Args<T> &A, dim3 g) {
 static bool zfast_shape(int pz, int &r1, int &r2) {
     switch (pz) {
         case 4: r1 = 2; r2 = 2; return true;
-        case 8: r1 = 2; r2 = 4; return true;
+        case 8: r1 = 4; r2 = 2; return true;
         case 16: r1 = 4; r2 = 4; return true;
-        case 32: r1 = 4; r2 = 8; return true;
+        case 32: r1 = 8; r2 = 4; return true;
         case 64: r1 = 8; r2 = 8; return true;
-        case 128: r1 = 8; r2 = 16; return true;
+        case 128: r1 = 16; r2 = 8; return true;
         case 256: r1 = 16; r2 = 16; return true;
         default: return false;
     }
@@ -2172,7 +2174,7 @@ template <class T, int R1, int R2>
 static hipError_t set_lds_yl_shape(size_t limit) {
     const size_t b = lds_yl<T, R1, R2>();
     if (b > limit) return hipErrorInvalidValue;
-    hipError_t e = set_lds<T>((const void *)k_yl<T, R1, R2, false>, b);
+    hipError_t e = set_lds<T>((const void *)k_yl<T, R2, R1, false>, b);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_yl<T, R1, R2, true>, b);
     return e;
 }
@@ -2198,11 +2200,11 @@ template <class T>
 static hipError_t set_lds_zf(int pz, size_t limit) {
     switch (pz) {
         case 4: return set_lds_zf_shape<T, 2, 2>(limit);
-        case 8: return set_lds_zf_shape<T, 2, 4>(limit);
+        case 8: return set_lds_zf_shape<T, 4, 2>(limit);
         case 16: return set_lds_zf_shape<T, 4, 4>(limit);
-        case 32: return set_lds_zf_shape<T, 4, 8>(limit);
+        case 32: return set_lds_zf_shape<T, 8, 4>(limit);
         case 64: return set_lds_zf_shape<T, 8, 8>(limit);
-        case 128: return set_lds_zf_shape<T, 8, 16>(limit);
+        case 128: return set_lds_zf_shape<T, 16, 8>(limit);
         case 256: return set_lds_zf_shape<T, 16, 16>(limit);
         default: return hipErrorInvalidValue;
     }
@@ -2235,7 +2237,8 @@ const char *configure(Args<T> &A, size_t lds_limit) {
     if (A.nz > 1) {
         // register four-step passes where the shape has one and its LDS fits
         int r1 = 0, r2 = 0;
-        if (A.tw4y && yfast_shape(A.py, r1, r2) && set_lds_yl<T>(A.py, lds_limit) == hipSuccess) A.ylfast = A.py;
+        if (A.tw4y && A.tw4y_f && yfast_shape(A.py, r1, r2) && set_lds_yl<T>(A.py, lds_limit) == hipSuccess)
+            A.ylfast = A.py;
         if (A.tw4z && zfast_shape(A.pz, r1, r2) && set_lds_zf<T>(A.pz, lds_limit) == hipSuccess) A.zfast = A.pz;
     }
     if (!A.yfast && !A.ylfast && lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
@@ -2371,7 +2374,7 @@ static void launch_yl_shape(const Args<T> &A, bool inv, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
     const unsigned g = (unsigned)((3ll * A.nzs * A.nkx + kThreads / TS - 1) / (kThreads / TS));
     if (inv) k_yl<T, R1, R2, true><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
-    else k_yl<T, R1, R2, false><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
+    else k_yl<T, R2, R1, false><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
 }
 // 3-D y pass (forward: inv = false; inverse: inv = true).
 template <class T>
@@ -2400,11 +2403,11 @@ template <class T>
 static void launch_z3(const Args<T> &A, hipStream_t st) {
     switch (A.zfast) {
         case 4: launch_zf_shape<T, 2, 2>(A, st); return;
-        case 8: launch_zf_shape<T, 2, 4>(A, st); return;
+        case 8: launch_zf_shape<T, 4, 2>(A, st); return;
         case 16: launch_zf_shape<T, 4, 4>(A, st); return;
-        case 32: launch_zf_shape<T, 4, 8>(A, st); return;
+        case 32: launch_zf_shape<T, 8, 4>(A, st); return;
         case 64: launch_zf_shape<T, 8, 8>(A, st); return;
-        case 128: launch_zf_shape<T, 8, 16>(A, st); return;
+        case 128: launch_zf_shape<T, 16, 8>(A, st); return;
         case 256: launch_zf_shape<T, 16, 16>(A, st); return;
         default: break;
     }
