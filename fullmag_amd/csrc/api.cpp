@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -151,6 +153,24 @@ void *dalloc(Backend *h, size_t bytes) {
     if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
     h->allocs.push_back(p);
     return p;
+}
+
+// Reduced Newell spectrum of a grid, computed once per process for each of
+// the last few geometries (batched sweeps and repeated creates share it).
+std::vector<double> cached_newell(int nx, int ny, int nz, double dx, double dy, double dz) {
+    static std::mutex mu;
+    static std::vector<std::pair<std::vector<double>, std::shared_ptr<const std::vector<double>>>> cache;
+    const std::vector<double> key = {(double)nx, (double)ny, (double)nz, dx, dy, dz};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto &e : cache)
+            if (e.first == key) return *e.second;
+    }
+    auto v = std::make_shared<const std::vector<double>>(fm::newell_reduced_spectrum(nx, ny, nz, dx, dy, dz));
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 8) cache.erase(cache.begin());
+    cache.emplace_back(key, v);
+    return *v;
 }
 
 // Publishes B as the kernel arguments in device slot `slot` (the kernels
@@ -562,9 +582,21 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
                 return false;
             }
         } else {
-            red = fm::newell_reduced_spectrum(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz);
+            red = cached_newell(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz);
         }
-        std::vector<T> rt(red.begin(), red.end());
+        std::vector<T> rt(red.size());
+        if (nz == 1) {
+            for (size_t i = 0; i < red.size(); ++i) rt[i] = (T)red[i];
+        } else {
+            // 3-D: [kx][kz][ky][component], the six components of a
+            // frequency contiguous for the z pass
+            const size_t X = nx + 1, Y = ny + 1, Z = nz + 1;
+            for (size_t c = 0; c < 6; ++c)
+                for (size_t kx = 0; kx < X; ++kx)
+                    for (size_t ky = 0; ky < Y; ++ky)
+                        for (size_t kz = 0; kz < Z; ++kz)
+                            rt[((kx * Z + kz) * Y + ky) * 6 + c] = (T)red[((c * X + kx) * Y + ky) * Z + kz];
+        }
         T *nt = static_cast<T *>(dalloc(h, rt.size() * sizeof(T)));
         if (!nt) return hip_ok(h, hipErrorOutOfMemory, "alloc tensor");
         if (!hip_ok(h, hipMemcpy(nt, rt.data(), rt.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))

@@ -6,12 +6,12 @@
 //   m_all  : 2 slots x 3 x ncell  (ping-pong state; slot `parity` is current)
 //   k_all  : 7 slots x 3 x ncell  (slots 0/1: k1 of the current state and
 //            the last-stage RHS of the trial state; slots 2..6: k2..k6)
-//   ybuf   : 3 x ncell            (intermediate stage state)
+//   ybuf   : 2 x 3 x ncell        (intermediate stage states, by stage parity)
 //   spec   : 3 x nzs x nkx x ly complex (x-half-spectrum of Ms*m, y-lines
 //            contiguous; nkx = nx+1, ly = ny for nz == 1 else 2ny, nzs = nz)
 //   ntens  : real symmetric Newell spectrum, symmetry-reduced:
 //            thin film: 4 comps (xx,yy,zz,xy) x nkx x (ny+1)
-//            3-D:       6 comps (xx,yy,zz,xy,xz,yz) x nkx x (ny+1) x (nz+1)
+//            3-D:       nkx x (nz+1) x (ny+1) x 6 comps (xx,yy,zz,xy,xz,yz)
 // The control block `Ctl` lives in device memory; every kernel reads dt,
 // parity and the done flag from it, so a whole adaptive step (including
 // rejections) runs without host round trips.

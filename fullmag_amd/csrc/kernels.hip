@@ -1271,7 +1271,6 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
     }
     __syncthreads();
     C2<T> *res = lds_fft<T, false>(b0, b1, A.fz, tw, 3 * TY, nz, pz);
-    const long long cs = (long long)nkx * (ny + 1) * (nz + 1);
     for (int i = threadIdx.x; i < TY * pz; i += blockDim.x) {
         const int t = i / pz, kz = i - t * pz;
         const int ky = ky0 + t;
@@ -1279,9 +1278,9 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
         const bool ylo = ky <= ny, zlo = kz <= nz;
         const int kyr = ylo ? ky : py - ky, kzr = zlo ? kz : pz - kz;
         const T sy = ylo ? T(1) : T(-1), sz = zlo ? T(1) : T(-1);
-        const T *N = A.ntens + ((long long)kx * (ny + 1) + kyr) * (nz + 1) + kzr;
-        const T nxx = N[0], nyy = N[cs], nzz = N[2 * cs];
-        const T nxy = sy * N[3 * cs], nxz = sz * N[4 * cs], nyz = sy * sz * N[5 * cs];
+        const T *N = A.ntens + (((size_t)kx * (nz + 1) + kzr) * (ny + 1) + kyr) * 6;
+        const T nxx = N[0], nyy = N[1], nzz = N[2];
+        const T nxy = sy * N[3], nxz = sz * N[4], nyz = sy * sz * N[5];
         C2<T> &ex = res[(0 * TY + t) * pz + kz];
         C2<T> &ey = res[(1 * TY + t) * pz + kz];
         C2<T> &ez = res[(2 * TY + t) * pz + kz];
@@ -1426,17 +1425,17 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
         const bool ylo = ky <= ny;
         const int kyr = ylo ? ky : (on ? py - ky : 0);
         const T sy = ylo ? T(1) : T(-1);
-        const size_t cs = (size_t)nkx * (ny + 1) * (nz + 1);
-        const T *NT = A.ntens + ((size_t)kx * (ny + 1) + kyr) * (nz + 1);
+        // tensor [kx][kz][ky][6]: consecutive ky (lanes) read consecutive records
+        const T *NT = A.ntens + (size_t)kx * (nz + 1) * (ny + 1) * 6 + (size_t)kyr * 6;
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
             const int kz = t + R2 * k1;
             const bool zlo = kz <= nz;
             const int kzr = zlo ? kz : 2 * nz - kz;
             const T sz = zlo ? T(1) : T(-1);
-            const T *N = NT + kzr;
-            const T nxx = N[0], nyy = N[cs], nzz = N[2 * cs];
-            const T nxy = sy * N[3 * cs], nxz = sz * N[4 * cs], nyz = sy * sz * N[5 * cs];
+            const T *N = NT + (size_t)kzr * (ny + 1) * 6;
+            const T nxx = N[0], nyy = N[1], nzz = N[2];
+            const T nxy = sy * N[3], nxz = sz * N[4], nyz = sy * sz * N[5];
             const C2<T> mx = x[0][k1], my = x[1][k1], mz = x[2][k1];
             x[0][k1] = {-(nxx * mx.x + nxy * my.x + nxz * mz.x), -(nxx * mx.y + nxy * my.y + nxz * mz.y)};
             x[1][k1] = {-(nxy * mx.x + nyy * my.x + nyz * mz.x), -(nxy * mx.y + nyy * my.y + nyz * mz.y)};
