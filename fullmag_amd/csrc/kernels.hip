@@ -1172,14 +1172,30 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);  // transpose block per component
     C2<T> *tb = lds + c * TB;
 
-    // ---- forward step 1: thread t1 < R1
+    // ---- tensor values of this thread's frequencies, loaded before any
+    // use so their latency overlaps the forward transform: component c
+    // needs N_cc and (for x, y) N_xy with the sign of ky's half
+    const long long cs = (long long)nkx * (ny + 1);
+    const T *NT = A.ntens + (long long)kx * (ny + 1);
+    const int other = c == 2 ? 2 : (c ^ 1);
+    T na[R1], nb[R1];
+    if (t < R2) {
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) {
+            const int k = t + R2 * k1;
+            const bool lo = k <= ny;
+            const int kr = lo ? k : N - k;
+            na[k1] = NT[c * cs + kr];
+            const T x = NT[3 * cs + kr];
+            nb[k1] = c == 2 ? T(0) : (lo ? x : -x);
+        }
+    }
+    // ---- forward step 1: thread t1 < R1 (py = 2 ny: the upper half of
+    // every thread's inputs is the zero padding)
     C2<T> v[R2];
     if (t < R1) {
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) {
-            const int n = t + R1 * n2;
-            v[n2] = n < ny ? S[n] : C2<T>{T(0), T(0)};
-        }
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? S[t + R1 * n2] : C2<T>{T(0), T(0)};
         DftReg<T, R2, false>::run(v);
 #pragma unroll
         for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
@@ -1203,24 +1219,11 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     }
     __syncthreads();
     if (t < R2) {
-        const long long cs = (long long)nkx * (ny + 1);
-        const T *NT = A.ntens + (long long)kx * (ny + 1);
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
             const int k = t + R2 * k1;
-            const bool lo = k <= ny;
-            const int kr = lo ? k : N - k;
-            const C2<T> mx = ex[k], my = ex[N + k], mz = ex[2 * N + k];
-            if (c == 0) {
-                const T nxx = NT[kr], nxy = lo ? NT[3 * cs + kr] : -NT[3 * cs + kr];
-                u[k1] = {-(nxx * mx.x + nxy * my.x), -(nxx * mx.y + nxy * my.y)};
-            } else if (c == 1) {
-                const T nyy = NT[cs + kr], nxy = lo ? NT[3 * cs + kr] : -NT[3 * cs + kr];
-                u[k1] = {-(nxy * mx.x + nyy * my.x), -(nxy * mx.y + nyy * my.y)};
-            } else {
-                const T nzz = NT[2 * cs + kr];
-                u[k1] = {-(nzz * mz.x), -(nzz * mz.y)};
-            }
+            const C2<T> ms = ex[c * N + k], mo = ex[other * N + k];
+            u[k1] = {-(na[k1] * ms.x + nb[k1] * mo.x), -(na[k1] * ms.y + nb[k1] * mo.y)};
         }
     }
     __syncthreads();
@@ -1243,10 +1246,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         for (int k2 = 0; k2 < R2; ++k2) v[k2] = tb[t * P2 + k2];
         DftReg<T, R2, true>::run(v);
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) {
-            const int n = t + R1 * n2;
-            if (n < ny) S[n] = v[n2];
-        }
+        for (int n2 = 0; n2 < R2 / 2; ++n2) S[t + R1 * n2] = v[n2];
     }
 }
 
