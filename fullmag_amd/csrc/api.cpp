@@ -91,6 +91,7 @@ struct fullmag_fdm_backend {
     void *poll_user = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_chunk = nullptr;  // kAttemptChunk trial steps in one graph
     bool graph_tried = false;
     bool obs_fresh = false;
     void *obs_base[3] = {nullptr, nullptr, nullptr};  // hex, hd, heff
@@ -342,6 +343,18 @@ void ensure_graph(Backend *h) {
     }
     h->graph = g;
     h->graph_exec = ex;
+    // the device loop's chunks as one graph: the kernels of attempts after
+    // the loop finished return at once
+    if (hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return;
+    bool ok2 = true;
+    for (int a = 0; a < kAttemptChunk && ok2; ++a) ok2 = enqueue_attempt(h);
+    hipGraph_t g2 = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(h->stream, &g2);
+    if (ok2 && e2 == hipSuccess && g2 && hipGraphInstantiate(&h->graph_chunk, g2, nullptr, nullptr, 0) != hipSuccess)
+        h->graph_chunk = nullptr;
+    if (g2) hipGraphDestroy(g2);
+    (void)hipGetLastError();
+    h->err.clear();
 }
 
 bool launch_attempt(Backend *h) {
@@ -1096,6 +1109,7 @@ void fullmag_fdm_backend_destroy(fullmag_fdm_backend *h) {
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph_chunk) hipGraphExecDestroy(h->graph_chunk);
     if (h->graph) hipGraphDestroy(h->graph);
     for (void *p : h->allocs) hipFree(p);
     for (void *p : h->adj.bufs) hipFree(p);
@@ -1184,11 +1198,15 @@ int fullmag_fdm_backend_advance(fullmag_fdm_backend *h, double t_end, uint64_t m
             nl = std::min<uint64_t>(kAttemptChunk, missing);
             planned = launched + nl;
         }
-        for (uint64_t a = 0; a < nl; ++a)
-            if (!launch_attempt(h)) {
-                rc = FULLMAG_FDM_ERR_CUDA;
-                break;
-            }
+        if (nl == (uint64_t)kAttemptChunk && h->graph_chunk) {
+            if (!hip_ok(h, hipGraphLaunch(h->graph_chunk, h->stream), "graph launch")) rc = FULLMAG_FDM_ERR_CUDA;
+        } else {
+            for (uint64_t a = 0; a < nl; ++a)
+                if (!launch_attempt(h)) {
+                    rc = FULLMAG_FDM_ERR_CUDA;
+                    break;
+                }
+        }
         if (rc != FULLMAG_FDM_OK) break;
         launched += nl;
         hipMemcpyAsync(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream);
@@ -1242,12 +1260,7 @@ int fullmag_fdm_backend_set_adaptive_ext(fullmag_fdm_backend *h, double rtol, do
     h->af.shrink = h->ad.shrink = shrink;
     const bool put = h->fp64 ? put_args(h, h->ad, 0) : put_args(h, h->af, 0);
     if (!put) return FULLMAG_FDM_ERR_CUDA;
-    // the captured graph holds the old kernel arguments
-    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
-    if (h->graph) hipGraphDestroy(h->graph);
-    h->graph_exec = nullptr;
-    h->graph = nullptr;
-    h->graph_tried = false;
+    // the graphs read the kernel arguments from device memory (updated above)
     return FULLMAG_FDM_OK;
 }
 
@@ -1622,7 +1635,7 @@ template <>
 fm::Args<double> &group_args<double>(Batch::Group &g) { return g.gd; }
 
 // Uploads the members' current kernel arguments into the group array and
-// captures one trial step of the whole group (once).
+// captures one polling chunk of trial steps of the whole group (once).
 template <class T>
 bool group_prepare(Batch *b, Batch::Group &g) {
     const size_t ns = g.members.size();
@@ -1642,7 +1655,8 @@ bool group_prepare(Batch *b, Batch::Group &g) {
     G.nsys = (int)ns;
     if (!g.graph) {
         if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-            const hipError_t e = fm::enqueue_trial(G, g.nstage, g.stream);
+            hipError_t e = hipSuccess;
+            for (int a = 0; a < kAttemptChunk && e == hipSuccess; ++a) e = fm::enqueue_trial(G, g.nstage, g.stream);
             hipGraph_t gr = nullptr;
             const hipError_t ec = hipStreamEndCapture(g.stream, &gr);
             if (e == hipSuccess && ec == hipSuccess && gr &&
@@ -1655,11 +1669,14 @@ bool group_prepare(Batch *b, Batch::Group &g) {
     return true;
 }
 
+// One polling chunk (kAttemptChunk trial steps) of a group.
 template <class T>
 bool group_launch(Batch *b, Batch::Group &g) {
     if (g.graph) return hipGraphLaunch(g.graph, g.stream) == hipSuccess || (b->err = "batch: graph launch", false);
-    return fm::enqueue_trial(group_args<T>(g), g.nstage, g.stream) == hipSuccess ||
-           (b->err = "batch: launch failed", false);
+    for (int a = 0; a < kAttemptChunk; ++a)
+        if (fm::enqueue_trial(group_args<T>(g), g.nstage, g.stream) != hipSuccess)
+            return (b->err = "batch: launch failed", false);
+    return true;
 }
 
 }  // namespace
@@ -1769,8 +1786,7 @@ int fullmag_fdm_batch_advance(fullmag_fdm_batch *b, double t_end, uint64_t max_s
         for (auto &g : b->groups) {
             if (g.finished) continue;
             active = true;
-            for (int a = 0; a < kAttemptChunk && rc == FULLMAG_FDM_OK; ++a)
-                if (!(g.fp64 ? group_launch<double>(b, g) : group_launch<float>(b, g))) rc = FULLMAG_FDM_ERR_CUDA;
+            if (!(g.fp64 ? group_launch<double>(b, g) : group_launch<float>(b, g))) rc = FULLMAG_FDM_ERR_CUDA;
             for (size_t m : g.members)
                 hipMemcpyAsync(&b->flags[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream);
             hipEventRecord(g.ev, g.stream);
