@@ -282,6 +282,8 @@ EXT_SYMBOLS = {
     "fullmag_fdm_backend_create_ext": (_H, [ctypes.POINTER(PlanDesc), ctypes.POINTER(PlanExt)]),
     "fullmag_fdm_adjoint_forward": (ctypes.c_int, [_H, ctypes.POINTER(AdjointDesc), c_double_p]),
     "fullmag_fdm_adjoint_backward": (ctypes.c_int, [_H, c_double_p, c_double_p]),
+    "fullmag_fdm_backend_autotune": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                    ctypes.POINTER(ctypes.c_int)]),
     "fullmag_fdm_batch_create": (_B, [ctypes.POINTER(ctypes.POINTER(PlanDesc)),
                                       ctypes.POINTER(ctypes.POINTER(PlanExt)), ctypes.c_uint64]),
     "fullmag_fdm_batch_groups": (ctypes.c_int, [_B, ctypes.POINTER(ctypes.c_uint64)]),

@@ -282,6 +282,12 @@ class FdmBackend:
         self._check(self.lib.fullmag_fdm_adjoint_backward(self.h, _dp(g), _dp(ga)))
         return g, ga
 
+    def autotune(self, reps: int = 10) -> dict:
+        """Times the x tiles and keeps the fastest (fullmag_fdm_backend_autotune)."""
+        rows, fs = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.fullmag_fdm_backend_autotune(self.h, reps, ctypes.byref(rows), ctypes.byref(fs)))
+        return {"rows_per_group": rows.value, "four_step_x": bool(fs.value)}
+
     def set_adaptive_ext(self, rtol: float, growth: float = 0.0, shrink: float = 0.0):
         self._check(self.lib.fullmag_fdm_backend_set_adaptive_ext(self.h, rtol, growth, shrink))
 

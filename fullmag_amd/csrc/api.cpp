@@ -34,7 +34,7 @@ hipError_t upload_methods();
 int method_nstage(int method);
 int method_stage_nk(int method, int s);
 template <class T>
-const char *configure(Args<T> &A, size_t lds_limit);
+const char *configure(Args<T> &A, size_t lds_limit, int rows_req = 0, int xfast_req = -1);
 template <class T>
 long long num_partials(const Args<T> &A);
 template <class T>
@@ -93,6 +93,7 @@ struct fullmag_fdm_backend {
     hipGraphExec_t graph_exec = nullptr;
     hipGraphExec_t graph_chunk = nullptr;  // kAttemptChunk trial steps in one graph
     bool graph_tried = false;
+    size_t lds_limit = 0;                  // dynamic LDS available to one work-group
     bool obs_fresh = false;
     void *obs_base[3] = {nullptr, nullptr, nullptr};  // hex, hd, heff
     // device copies of the kernel arguments: 0 stepping, 1 with observable
@@ -172,6 +173,37 @@ std::vector<double> cached_newell(int nx, int ny, int nz, double dx, double dy, 
     if (cache.size() >= 8) cache.erase(cache.begin());
     cache.emplace_back(key, v);
     return *v;
+}
+
+// Process-wide x-tile choices of the tuner (fullmag_fdm_backend_autotune),
+// keyed by precision, grid and method; applied by later creates.
+struct TuneEntry {
+    std::vector<int> key;
+    int rows, xfast;
+};
+std::mutex g_tune_mu;
+std::vector<TuneEntry> g_tune;
+
+void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int *rows, int *xfast) {
+    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &e : g_tune)
+        if (e.key == key) {
+            *rows = e.rows;
+            *xfast = e.xfast;
+        }
+}
+
+void tuned_store(bool fp64, int nx, int ny, int nz, int method, int rows, int xfast) {
+    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method};
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto &e : g_tune)
+        if (e.key == key) {
+            e.rows = rows;
+            e.xfast = xfast;
+            return;
+        }
+    g_tune.push_back({key, rows, xfast});
 }
 
 // Publishes B as the kernel arguments in device slot `slot` (the kernels
@@ -648,13 +680,19 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)
             lds = 65536;
-        const char *ce = fm::configure(A, (size_t)lds - 1024);
+        h->lds_limit = (size_t)lds - 1024;
+        int rows_req = 0, xfast_req = -1;
+        tuned_lookup(h->fp64, nx, ny, nz, A.method, &rows_req, &xfast_req);
+        const char *ce = fm::configure(A, h->lds_limit, rows_req, xfast_req);
+        if (rows_req && ce && *ce) ce = fm::configure(A, h->lds_limit);
         if (ce && *ce) {
             h->err = ce;
             return false;
         }
     }
-    const long long nparts = fm::num_partials(A);
+    // partials for the smallest x tile (one row per work-group), so that the
+    // tuner may change the tile later
+    const long long nparts = std::max<long long>(A.demag ? A.nrows : 0, fm::num_partials(A));
     A.partials = static_cast<double *>(dalloc(h, (size_t)nparts * fm::kStatSlots * sizeof(double)));
     A.counter = static_cast<unsigned *>(dalloc(h, 64));
     if (!A.partials || !A.counter) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
@@ -1310,6 +1348,91 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
 
 }  // extern "C"
 
+
+namespace {
+
+// Times the x-pass launches of one trial step (stage-2 forward, joined,
+// final inverse) for the current arguments; the state is not changed.
+template <class T>
+double time_x_passes(Backend *h, const fm::Args<T> &A, int reps, hipStream_t st) {
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double total = 0.0;
+    const int kinds[3] = {0, 6, 4};
+    const double weight[3] = {1.0, (double)std::max(h->nstage - 2, 0), 1.0};
+    for (int i = 0; i < 3; ++i) {
+        if (fm::enqueue_kind(A, kinds[i], 2, st) != hipSuccess) continue;
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) fm::enqueue_kind(A, kinds[i], 2, st);
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        total += weight[i] * ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return total;
+}
+
+template <class T>
+int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out_fs) {
+    if (!A.demag) {
+        if (out_rows) *out_rows = 0;
+        if (out_fs) *out_fs = 0;
+        return FULLMAG_FDM_OK;
+    }
+    if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    const fm::Ctl saved = *h->ctl_h;
+    const fm::Args<T> orig = A;
+    double best = std::numeric_limits<double>::infinity();
+    fm::Args<T> best_args = A;
+    for (int fs = 1; fs >= 0; --fs) {
+        for (int rows : {1, 2, 4}) {
+            fm::Args<T> B = orig;
+            const char *ce = fm::configure(B, h->lds_limit, rows, fs);
+            if (ce && *ce) continue;
+            if (fs == 1 && !B.xfast) continue;  // no four-step shape: the fs = 0 pass covers it
+            if (!put_args(h, B, 0)) return FULLMAG_FDM_ERR_CUDA;
+            time_x_passes(h, B, 2, h->stream);  // warm-up
+            const double t = time_x_passes(h, B, reps, h->stream);
+            if (t < best) {
+                best = t;
+                best_args = B;
+            }
+        }
+    }
+    A = best_args;
+    const char *ce = fm::configure(A, h->lds_limit, A.rows_per_wg, A.xfast ? 1 : 0);  // kernel LDS attributes
+    if (ce && *ce) return fail(h, FULLMAG_FDM_ERR_INTERNAL, ce);
+    if (!put_args(h, A, 0)) return FULLMAG_FDM_ERR_CUDA;
+    // graphs were captured with the old launch geometry
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->graph_chunk) hipGraphExecDestroy(h->graph_chunk);
+    if (h->graph) hipGraphDestroy(h->graph);
+    h->graph_exec = h->graph_chunk = nullptr;
+    h->graph = nullptr;
+    h->graph_tried = false;
+    *h->ctl_h = saved;
+    if (!push_ctl(h) || !eval_current(h, false)) return FULLMAG_FDM_ERR_CUDA;
+    tuned_store(h->fp64, (int)h->nx, (int)h->ny, (int)h->nz, A.method, A.rows_per_wg, A.xfast ? 1 : 0);
+    if (out_rows) *out_rows = A.rows_per_wg;
+    if (out_fs) *out_fs = A.xfast ? 1 : 0;
+    return FULLMAG_FDM_OK;
+}
+
+}  // namespace
+
+extern "C" int fullmag_fdm_backend_autotune(fullmag_fdm_backend *h, int reps, int *out_rows_per_group,
+                                            int *out_four_step_x) {
+    if (!h || reps < 1) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    h->err.clear();
+    hipSetDevice(h->device);
+    return with_args(h, [&](auto &A) { return autotune_typed(h, A, reps, out_rows_per_group, out_four_step_x); });
+}
+
 // ------------------------------------------------------------ adjoint --
 namespace {
 
@@ -1637,7 +1760,7 @@ fm::Args<double> &group_args<double>(Batch::Group &g) { return g.gd; }
 // Uploads the members' current kernel arguments into the group array and
 // captures one polling chunk of trial steps of the whole group (once).
 template <class T>
-bool group_prepare(Batch *b, Batch::Group &g) {
+bool group_prepare(Batch *b, Batch::Group &g, bool capture = true) {
     const size_t ns = g.members.size();
     if (!g.args_d && hipMalloc(&g.args_d, ns * sizeof(fm::Args<T>)) != hipSuccess)
         return (b->err = "batch: argument allocation failed", false);
@@ -1653,7 +1776,7 @@ bool group_prepare(Batch *b, Batch::Group &g) {
     G = host[0];
     G.self_d = static_cast<const fm::Args<T> *>(g.args_d);
     G.nsys = (int)ns;
-    if (!g.graph) {
+    if (capture && !g.graph) {
         if (hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
             hipError_t e = hipSuccess;
             for (int a = 0; a < kAttemptChunk && e == hipSuccess; ++a) e = fm::enqueue_trial(G, g.nstage, g.stream);
@@ -1670,6 +1793,42 @@ bool group_prepare(Batch *b, Batch::Group &g) {
 }
 
 // One polling chunk (kAttemptChunk trial steps) of a group.
+// Chooses the x tile (rows per work-group, FFT form) of a group by timing
+// the group's batched x-pass launches for each candidate; every member gets
+// the winner.  Members' states are not changed.
+template <class T>
+bool group_tune(Batch *b, Batch::Group &g) {
+    Backend *h0 = b->sys[g.members[0]];
+    fm::Args<T> &A0 = *reinterpret_cast<fm::Args<T> *>(h0->fp64 ? (void *)&h0->ad : (void *)&h0->af);
+    if (!A0.demag) return true;
+    auto apply = [&](int rows, int fs) {
+        for (size_t m : g.members) {
+            Backend *h = b->sys[m];
+            fm::Args<T> &A = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
+            const char *ce = fm::configure(A, h->lds_limit, rows, fs);
+            if ((ce && *ce) || !put_args(h, A, 0)) return false;
+        }
+        return true;
+    };
+    double best = std::numeric_limits<double>::infinity();
+    int best_rows = A0.rows_per_wg, best_fs = A0.xfast ? 1 : 0;
+    for (int fs = 1; fs >= 0; --fs)
+        for (int rows : {1, 2, 4}) {
+            if (!apply(rows, fs)) continue;
+            if (fs == 1 && !A0.xfast) continue;
+            if (!group_prepare<T>(b, g, false)) return false;
+            time_x_passes(h0, group_args<T>(g), 2, g.stream);
+            const double t = time_x_passes(h0, group_args<T>(g), 5, g.stream);
+            if (t < best) {
+                best = t;
+                best_rows = rows;
+                best_fs = fs;
+            }
+        }
+    if (!apply(best_rows, best_fs)) return (b->err = "batch: tile configuration failed", false);
+    return true;
+}
+
 template <class T>
 bool group_launch(Batch *b, Batch::Group &g) {
     if (g.graph) return hipGraphLaunch(g.graph, g.stream) == hipSuccess || (b->err = "batch: graph launch", false);
@@ -1721,6 +1880,9 @@ fullmag_fdm_batch *fullmag_fdm_batch_create(const fullmag_fdm_plan_desc *const *
             b->err = "batch: stream creation failed";
             return b;
         }
+        // x tile of the group, timed on the group's own batched launches
+        const bool ok = g.fp64 ? group_tune<double>(b, g) : group_tune<float>(b, g);
+        if (!ok) return b;
     }
     if (hipHostMalloc((void **)&b->flags, b->sys.size() * sizeof(fm::Ctl), hipHostMallocDefault) != hipSuccess)
         b->err = "batch: pinned allocation failed";

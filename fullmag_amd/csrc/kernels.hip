@@ -509,12 +509,14 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 template <class T, bool LUT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
                                              const CellIn<T> &g, T *__restrict__ kout, const T hd[3], T wf,
-                                             bool stats, bool err, Acc &acc, T *__restrict__ hrec) {
+                                             bool stats, bool err, Acc &acc, T *__restrict__ hrec,
+                                             T kret[3] = nullptr) {
     const int N = (int)A.ncell;
     if (!g.act) {
         kout[cell] = T(0);
         kout[N + cell] = T(0);
         kout[2 * N + cell] = T(0);
+        if (kret) kret[0] = kret[1] = kret[2] = T(0);
         if (hrec) hrec[cell] = hrec[N + cell] = hrec[2 * N + cell] = T(0);
         if (A.obs_hex) {
             A.obs_hex[cell] = A.obs_hex[N + cell] = A.obs_hex[2 * N + cell] = T(0);
@@ -588,6 +590,11 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     kout[cell] = k[0];
     kout[N + cell] = k[1];
     kout[2 * N + cell] = k[2];
+    if (kret) {
+        kret[0] = k[0];
+        kret[1] = k[1];
+        kret[2] = k[2];
+    }
     if (hrec) {
         hrec[cell] = h[0];
         hrec[N + cell] = h[1];
@@ -791,104 +798,138 @@ __device__ __forceinline__ void x_rows(const Args<T> &A, int r0, XRows &xr) {
 // work-group's rows (accumulated in FP64 for both storage precisions: the
 // Dormand-Prince rows have coefficients up to ~12 that cancel to O(1)),
 // y_s stored for s >= 2, Ms*y packed two real x-lines per complex sequence
-// into b0.  NK = k-vectors of the combination (0: stage 1, y = m0).  The
-// loads of a chunk of cells are all issued before any is consumed.
+// into b0.  NK = k-vectors of the combination (0: stage 1, y = m0).  Cells
+// are taken in chunks of CH per thread (cell i = base + j*blockDim + tid);
+// the loads of a chunk are all issued before any is consumed.
+template <class T, int NK>
+struct PackIn {
+    static constexpr int CH = Chunk<T>::value;
+    T mv[CH][3], kv[CH][NK > 0 ? NK : 1][3], msv[CH];
+};
+
+// Issues the loads of one chunk; the k-vector at row position `skip` (-1:
+// none) is not loaded (the caller has it in registers).
 template <class T, int R1, int R2, int NK>
-__device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, int s, int p, double dt,
-                                             const XRows &xr, C2<T> *b0) {
+__device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, int s, int p, const XRows &xr,
+                                            int base, int skip, PackIn<T, NK> &in) {
     constexpr bool FAST = R1 > 0;
-    constexpr int KM = NK > 0 ? NK : 1;
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int rows = A.rows_per_wg, ncl = rows * nx, nl = 3 * rows;
-    const float rnx = 1.0f / (float)nx;
     constexpr int CH = Chunk<T>::value;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int ncl = A.rows_per_wg * nx;
+    const float rnx = 1.0f / (float)nx;
     const int N = (int)A.ncell, bd = blockDim.x;
     const T *__restrict__ m0 = A.m_all + (size_t)p * 3 * N;
     const StageRow &R = M.row[NK > 0 ? s : 2];
-    const T *kp[KM];
-    double wa[KM];
 #pragma unroll
-    for (int q = 0; q < NK; ++q) {
-        kp[q] = A.k_all + (size_t)kslot(M, R.kidx[q], p) * 3 * N;
-        wa[q] = R.a[q];
+    for (int j = 0; j < CH; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        const int ic = i < ncl ? i : ncl - 1;
+        const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
+        const int c0 = xr.cell0[rr];
+        const int cell = (c0 < 0 ? 0 : c0) + x;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) in.mv[j][c] = m0[c * N + cell];
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            if (q == skip) continue;
+            const T *kp = A.k_all + (size_t)kslot(M, R.kidx[q], p) * 3 * N;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) in.kv[j][q][c] = kp[c * N + cell];
+        }
+        in.msv[j] = A.ms_cell ? A.ms_cell[cell] : A.ms;
     }
+}
+
+// Combination, normalisation, store and packing of one loaded chunk.
+template <class T, int R1, int R2, int NK>
+__device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, int s, int p, double dt,
+                                           const XRows &xr, int base, const PackIn<T, NK> &in, C2<T> *b0) {
+    constexpr bool FAST = R1 > 0;
+    constexpr int CH = Chunk<T>::value;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int ncl = A.rows_per_wg * nx;
+    const float rnx = 1.0f / (float)nx;
+    const int N = (int)A.ncell, bd = blockDim.x;
+    const StageRow &R = M.row[NK > 0 ? s : 2];
     const double f = dt * R.scale;
     T *__restrict__ yout = NK > 0 ? stage_output(A, M, s, p) : nullptr;
-    for (int base = 0; base < ncl; base += CH * bd) {
-        T mv[CH][3], kv[CH][KM][3], msv[CH];
 #pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            const int ic = i < ncl ? i : ncl - 1;
-            const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
-            const int c0 = xr.cell0[rr];
-            const int cell = (c0 < 0 ? 0 : c0) + x;
+    for (int j = 0; j < CH; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        if (i >= ncl) continue;
+        const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+        const int c0 = xr.cell0[rr];
+        T v[3];
+        if constexpr (NK == 0) {
+            v[0] = in.mv[j][0];
+            v[1] = in.mv[j][1];
+            v[2] = in.mv[j][2];
+            if (A.rec_y && c0 >= 0) {
+                const int cell = c0 + x;
+                T *ry = A.rec_y + rec_slot(A, A.rec_sv);
+                ry[cell] = v[0];
+                ry[N + cell] = v[1];
+                ry[2 * N + cell] = v[2];
+            }
+        } else {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) mv[j][c] = m0[c * N + cell];
-#pragma unroll
-            for (int q = 0; q < NK; ++q)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) kv[j][q][c] = kp[q][c * N + cell];
-            msv[j] = A.ms_cell ? A.ms_cell[cell] : A.ms;
-        }
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            if (i >= ncl) continue;
-            const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
-            const int c0 = xr.cell0[rr];
-            T v[3];
-            if constexpr (NK == 0) {
-                v[0] = mv[j][0];
-                v[1] = mv[j][1];
-                v[2] = mv[j][2];
-                if (A.rec_y && c0 >= 0) {
-                    const int cell = c0 + x;
-                    T *ry = A.rec_y + rec_slot(A, A.rec_sv);
+            for (int q = 0; q < NK; ++q) {
+                const double w = R.a[q];
+                a0 += w * (double)in.kv[j][q][0];
+                a1 += w * (double)in.kv[j][q][1];
+                a2 += w * (double)in.kv[j][q][2];
+            }
+            const double xx = (double)in.mv[j][0] + a0 * f, yy = (double)in.mv[j][1] + a1 * f,
+                         zz = (double)in.mv[j][2] + a2 * f;
+            const double n2 = xx * xx + yy * yy + zz * zz;
+            const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+            v[0] = (T)(xx * inv);
+            v[1] = (T)(yy * inv);
+            v[2] = (T)(zz * inv);
+            if (c0 >= 0) {
+                const int cell = c0 + x;
+                yout[cell] = v[0];
+                yout[N + cell] = v[1];
+                yout[2 * N + cell] = v[2];
+                if (A.rec_y) {
+                    T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
                     ry[cell] = v[0];
                     ry[N + cell] = v[1];
                     ry[2 * N + cell] = v[2];
+                    A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
                 }
-            } else {
-                double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-#pragma unroll
-                for (int q = 0; q < NK; ++q) {
-                    a0 += wa[q] * (double)kv[j][q][0];
-                    a1 += wa[q] * (double)kv[j][q][1];
-                    a2 += wa[q] * (double)kv[j][q][2];
-                }
-                const double xx = (double)mv[j][0] + a0 * f, yy = (double)mv[j][1] + a1 * f,
-                             zz = (double)mv[j][2] + a2 * f;
-                const double n2 = xx * xx + yy * yy + zz * zz;
-                const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
-                v[0] = (T)(xx * inv);
-                v[1] = (T)(yy * inv);
-                v[2] = (T)(zz * inv);
-                if (c0 >= 0) {
-                    const int cell = c0 + x;
-                    yout[cell] = v[0];
-                    yout[N + cell] = v[1];
-                    yout[2 * N + cell] = v[2];
-                    if (A.rec_y) {
-                        T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
-                        ry[cell] = v[0];
-                        ry[N + cell] = v[1];
-                        ry[2 * N + cell] = v[2];
-                        A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
-                    }
-                }
-            }
-            const T w = c0 >= 0 ? msv[j] : T(0);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int l = 3 * rr + c;
-                T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
-                e[l & 1] = w * v[c];
             }
         }
+        const T w = c0 >= 0 ? in.msv[j] : T(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int l = 3 * rr + c;
+            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
+            e[l & 1] = w * v[c];
+        }
     }
+}
+
+// Zero imaginary half of the last sequence when the line count is odd.
+template <class T, int R1, int R2>
+__device__ __forceinline__ void x_pack_tail(const Args<T> &A, C2<T> *b0) {
+    const int px = R1 > 0 ? R1 * R2 : A.px, nx = px >> 1, nl = 3 * A.rows_per_wg;
     if (nl & 1)
-        for (int x = threadIdx.x; x < nx; x += bd) b0[(nl >> 1) * px + x].y = T(0);
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * px + x].y = T(0);
+}
+
+template <class T, int R1, int R2, int NK>
+__device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, int s, int p, double dt,
+                                             const XRows &xr, C2<T> *b0) {
+    const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
+    constexpr int CH = Chunk<T>::value;
+    for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
+        PackIn<T, NK> in;
+        x_pack_load<T, R1, R2, NK>(A, M, s, p, xr, base, -1, in);
+        x_pack_put<T, R1, R2, NK>(A, M, s, p, dt, xr, base, in, b0);
+    }
+    x_pack_tail<T, R1, R2>(A, b0);
 }
 
 // Zero-padded forward FFT of the packed sequences in b0 and the unpacking of
@@ -924,46 +965,70 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
 
 // Hermitian rebuild of the packed sequences from the half spectra (two real
 // output lines per complex sequence) and the C2R inverse FFT along x, only
-// the first nx outputs kept.  Returns the LDS buffer holding the result.
-template <class T, int R1, int R2>
-__device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
-                                                       const C2<T> *tw) {
+// the first nx outputs kept.  x_inv_issue loads the first chunk (CZ
+// elements per thread), x_inv_finish consumes it, rebuilds the remaining
+// chunks and transforms; the caller may issue other loads in between.
+template <class T, int CZ>
+struct InvIn {
+    C2<T> av[CZ], bv[CZ];
+    bool ma[CZ], mb[CZ];
+};
+
+template <class T, int R1, int R2, int CZ>
+__device__ __forceinline__ void x_inv_issue(const Args<T> &A, const XRows &xr, int base, InvIn<T, CZ> &in) {
     constexpr bool FAST = R1 > 0;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int nl = 3 * A.rows_per_wg, ly = A.ly, bd = blockDim.x;
     const float rpx = 1.0f / (float)px;
     const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
-    constexpr int CZ = 8;
     const int nz_tot = A.nseq_x * px;
-    for (int base = 0; base < nz_tot; base += CZ * bd) {
-        C2<T> av[CZ], bv[CZ];
-        bool ma[CZ], mb[CZ];
 #pragma unroll
-        for (int j = 0; j < CZ; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            const int ic = i < nz_tot ? i : nz_tot - 1;
-            const int sq = small_div<FAST>(ic, px, rpx), k = ic - sq * px;
-            const int kk = k > nx ? px - k : k;
-            const int la = 2 * sq, lb = la + 1;
-            const int oa = xr.spec[la], ob = lb < nl ? xr.spec[lb] : -1;
-            ma[j] = oa >= 0;
-            mb[j] = ob >= 0;
-            av[j] = S[(oa < 0 ? 0 : oa) + kk * ly];
-            bv[j] = S[(ob < 0 ? 0 : ob) + kk * ly];
-        }
+    for (int j = 0; j < CZ; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        const int ic = i < nz_tot ? i : nz_tot - 1;
+        const int sq = small_div<FAST>(ic, px, rpx), k = ic - sq * px;
+        const int kk = k > nx ? px - k : k;
+        const int la = 2 * sq, lb = la + 1;
+        const int oa = xr.spec[la], ob = lb < nl ? xr.spec[lb] : -1;
+        in.ma[j] = oa >= 0;
+        in.mb[j] = ob >= 0;
+        in.av[j] = S[(oa < 0 ? 0 : oa) + kk * ly];
+        in.bv[j] = S[(ob < 0 ? 0 : ob) + kk * ly];
+    }
+}
+
+template <class T, int R1, int R2, int CZ>
+__device__ __forceinline__ void x_inv_put(const Args<T> &A, int base, const InvIn<T, CZ> &in, C2<T> *b0) {
+    constexpr bool FAST = R1 > 0;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1, bd = blockDim.x;
+    const float rpx = 1.0f / (float)px;
+    const int nz_tot = A.nseq_x * px;
 #pragma unroll
-        for (int j = 0; j < CZ; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            if (i >= nz_tot) continue;
-            const int sq = small_div<FAST>(i, px, rpx), k = i - sq * px;
-            C2<T> a = ma[j] ? av[j] : C2<T>{T(0), T(0)};
-            C2<T> b = mb[j] ? bv[j] : C2<T>{T(0), T(0)};
-            if (k > nx) {
-                a.y = -a.y;
-                b.y = -b.y;
-            }
-            b0[i] = {a.x - b.y, a.y + b.x};
+    for (int j = 0; j < CZ; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        if (i >= nz_tot) continue;
+        const int sq = small_div<FAST>(i, px, rpx), k = i - sq * px;
+        C2<T> a = in.ma[j] ? in.av[j] : C2<T>{T(0), T(0)};
+        C2<T> b = in.mb[j] ? in.bv[j] : C2<T>{T(0), T(0)};
+        if (k > nx) {
+            a.y = -a.y;
+            b.y = -b.y;
         }
+        b0[i] = {a.x - b.y, a.y + b.x};
+    }
+}
+
+template <class T, int R1, int R2, int CZ>
+__device__ __forceinline__ const C2<T> *x_inv_finish(const Args<T> &A, const XRows &xr, const InvIn<T, CZ> &first,
+                                                     C2<T> *b0, C2<T> *b1, const C2<T> *tw) {
+    constexpr bool FAST = R1 > 0;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int nz_tot = A.nseq_x * px, step = CZ * (int)blockDim.x;
+    x_inv_put<T, R1, R2, CZ>(A, 0, first, b0);
+    for (int base = step; base < nz_tot; base += step) {
+        InvIn<T, CZ> in;
+        x_inv_issue<T, R1, R2, CZ>(A, xr, base, in);
+        x_inv_put<T, R1, R2, CZ>(A, base, in, b0);
     }
     __syncthreads();
     if constexpr (FAST) {
@@ -974,47 +1039,76 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
     }
 }
 
+template <class T, int R1, int R2>
+__device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
+                                                       const C2<T> *tw) {
+    InvIn<T, 8> in;
+    x_inv_issue<T, R1, R2, 8>(A, xr, 0, in);
+    return x_inv_finish<T, R1, R2, 8>(A, xr, in, b0, b1, tw);
+}
+
 // Local pass of stage s over the work-group's rows: demag field from the
 // inverse x FFT (1/P scale), exchange, Zeeman, anisotropy, LLG torque,
-// optional statistics and error norm.
+// optional statistics and error norm.  x_local_load issues the loads of one
+// chunk of cells, x_local_put consumes it (and returns the chunk's k).
 template <class T, bool LUT, int R1, int R2>
-__device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
-                                        const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
+__device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, int p, const T *yin,
+                                             const XRows &xr, int base, bool err, CellIn<T> *g) {
     constexpr bool FAST = R1 > 0;
+    constexpr int CH = Chunk<T>::value;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
     const float rnx = 1.0f / (float)nx;
-    const T *yin = stage_input(A, M, s, p);
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        const int ic = i < ncl ? i : ncl - 1;
+        const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
+        const int c0 = xr.cell0[rr];
+        gather_cell<T, LUT>(A, M, p, (c0 < 0 ? 0 : c0) + x, x, xr.y[rr], xr.z[rr], yin, err, g[j]);
+    }
+}
+
+template <class T, bool LUT, int R1, int R2>
+__device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
+                                            const XRows &xr, int base, const CellIn<T> *g, const C2<T> *res,
+                                            bool stats, bool err, Acc &acc, T (*kk)[3]) {
+    constexpr bool FAST = R1 > 0;
+    constexpr int CH = Chunk<T>::value;
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
+    const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
+    const float rnx = 1.0f / (float)nx;
     T *kout = stage_rhs(A, M, s, p);
     T *hrec = A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        const int i = base + j * bd + (int)threadIdx.x;
+        if (i >= ncl) continue;
+        const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
+        const int c0 = xr.cell0[rr];
+        if (c0 < 0) continue;
+        T hd[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int l = 3 * rr + c;
+            const C2<T> zv = res[(l >> 1) * px + x];
+            hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
+        }
+        compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec, kk ? kk[j] : nullptr);
+    }
+}
+
+template <class T, bool LUT, int R1, int R2>
+__device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
+                                        const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
     constexpr int CH = Chunk<T>::value;
-    for (int base = 0; base < ncl; base += CH * bd) {
+    const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
+    const T *yin = stage_input(A, M, s, p);
+    for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
         CellIn<T> g[CH];
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            const int ic = i < ncl ? i : ncl - 1;
-            const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
-            const int c0 = xr.cell0[rr];
-            gather_cell<T, LUT>(A, M, p, (c0 < 0 ? 0 : c0) + x, x, xr.y[rr], xr.z[rr], yin, err, g[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const int i = base + j * bd + (int)threadIdx.x;
-            if (i >= ncl) continue;
-            const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
-            const int c0 = xr.cell0[rr];
-            if (c0 < 0) continue;
-            T hd[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const int l = 3 * rr + c;
-                const C2<T> zv = res[(l >> 1) * px + x];
-                hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
-            }
-            compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec);
-        }
+        x_local_load<T, LUT, R1, R2>(A, M, p, yin, xr, base, err, g);
+        x_local_put<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, base, g, res, stats, err, acc, nullptr);
     }
 }
 
@@ -1070,6 +1164,9 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
 // work-group owns all cells of its rows, so the inverse x pass, local pass
 // and k_s of stage s feed the forward x pass of stage s+1 directly.  NK =
 // k-vectors of stage s+1.  Results are those of the separate launches.
+// (Issuing all three phases' loads up front was measured slower: the
+// register footprint doubles and the 3-D grid loses two thirds of its
+// occupancy.)
 template <class T, bool LUT, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads) k_xx(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
@@ -2210,12 +2307,12 @@ static hipError_t set_lds_zf(int pz, size_t limit) {
 }
 
 template <class T>
-const char *configure(Args<T> &A, size_t lds_limit) {
+const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req) {
     if (!A.demag) return "";
     A.xfast = 0;
     {
         int r1 = 0, r2 = 0;
-        if (A.tw4x && yfast_shape(A.px, r1, r2)) A.xfast = A.px;
+        if (A.tw4x && yfast_shape(A.px, r1, r2) && xfast_req != 0) A.xfast = A.px;
     }
     auto fits_x = [&](int rows) {
         A.rows_per_wg = rows;
@@ -2223,8 +2320,14 @@ const char *configure(Args<T> &A, size_t lds_limit) {
         return lds_x(A) <= lds_limit;
     };
     int rows = 2;
-    if (A.nrows >= 4096 && fits_x(4)) rows = 4;
-    else if (!fits_x(2)) rows = 1;
+    if (rows_req > 0) {
+        if (rows_req > kMaxRowsWg || !fits_x(rows_req)) return "requested x tile does not fit";
+        rows = rows_req;
+    } else if (A.nrows >= 4096 && fits_x(4)) {
+        rows = 4;
+    } else if (!fits_x(2)) {
+        rows = 1;
+    }
     if (!fits_x(rows)) return "padded x length does not fit the LDS of one work-group";
     A.yfast = 0;
     {
@@ -2563,7 +2666,7 @@ bool zfast_factors(int pz, int *r1, int *r2) { return zfast_shape(pz, *r1, *r2);
 #endif
 
 template hipError_t upload_methods<FM_REAL>();
-template const char *configure<FM_REAL>(Args<FM_REAL> &, size_t);
+template const char *configure<FM_REAL>(Args<FM_REAL> &, size_t, int, int);
 template long long num_partials<FM_REAL>(const Args<FM_REAL> &);
 template hipError_t enqueue_eval<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
 template hipError_t enqueue_trial<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);

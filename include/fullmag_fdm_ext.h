@@ -93,6 +93,18 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *handle, int reps, doub
                                      int n_out);
 
 /*
+ * Tile tuner (no reference counterpart): times the x-pass launches of a
+ * trial step for each x tile (rows per work-group 1, 2, 4; register
+ * four-step or LDS Stockham FFT) at `reps` launches each, keeps the fastest
+ * for this handle and records it for later creates of the same precision,
+ * grid and integrator in this process.  The state is not changed.  Outputs
+ * (optional): the chosen rows per work-group and 1 if the four-step FFT is
+ * used (both 0 without demag).
+ */
+int fullmag_fdm_backend_autotune(fullmag_fdm_backend *handle, int reps, int *out_rows_per_group,
+                                 int *out_four_step_x);
+
+/*
  * Seeded random unit vectors of the planner's RandomSeeded initial state
  * (crates/fullmag-plan/src/util.rs:118-147, xorshift64 13/7/17, components
  * uniform in [-1,1], rejection of near-zero norms, then normalised).  Host

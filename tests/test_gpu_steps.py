@@ -282,3 +282,28 @@ def test_rejected_plans_report_errors():
     with pytest.raises(FdmError) as ei:
         FdmBackend(bad)
     assert "active_mask_len" in str(ei.value)
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("shape", [(64, 48, 1), (50, 30, 1), (16, 32, 8)])
+def test_autotune_keeps_state_and_results(precision, shape):
+    # the tuner only changes the x tile (rows per work-group, FFT form): the
+    # state is untouched and later steps agree with an untuned handle to
+    # roundoff (the two-for-one packing pairs different rows)
+    from fullmag_amd.backend import FdmBackend
+    nx, ny, nz = shape
+    case = random_case(nx, ny, nz, seed=nx + nz, hole_radius_cells=min(nx, ny) // 5 if nz == 1 else None)
+    with FdmBackend(case.plan(precision=precision)) as a, FdmBackend(case.plan(precision=precision)) as b:
+        m0 = a.field("m")
+        choice = a.autotune(reps=3)
+        assert choice["rows_per_group"] in (1, 2, 4)
+        assert np.array_equal(a.field("m"), m0)
+        dt_a = dt_b = 1e-13
+        for _ in range(4):
+            sa, sb = a.step(dt_a), b.step(dt_b)
+            dt_a, dt_b = sa["suggested_next_dt"], sb["suggested_next_dt"]
+        d = float(np.abs(a.field("m") - b.field("m")).max())
+        print(shape, precision, choice, f"max |dm| {d:.2e}", flush=True)
+        # FP32 from a random state: roundoff differences grow ~10x per step
+        # (chaotic relaxation), so 4 steps end near 1e-5
+        assert d <= (1e-12 if precision == "double" else 1e-4)
