@@ -1795,7 +1795,8 @@ bool group_prepare(Batch *b, Batch::Group &g, bool capture = true) {
 // One polling chunk (kAttemptChunk trial steps) of a group.
 // Chooses the x tile (rows per work-group, FFT form) of a group by timing
 // the group's batched x-pass launches for each candidate; every member gets
-// the winner.  Members' states are not changed.
+// the winner, which is also recorded for later creates of that shape.
+// Members' states are not changed.
 template <class T>
 bool group_tune(Batch *b, Batch::Group &g) {
     Backend *h0 = b->sys[g.members[0]];
@@ -1826,6 +1827,10 @@ bool group_tune(Batch *b, Batch::Group &g) {
             }
         }
     if (!apply(best_rows, best_fs)) return (b->err = "batch: tile configuration failed", false);
+    // k1 of the initial states with the chosen tile (as a create with it gives)
+    for (size_t m : g.members)
+        if (!eval_current(b->sys[m], false)) return (b->err = "batch: evaluation failed", false);
+    tuned_store(h0->fp64, (int)h0->nx, (int)h0->ny, (int)h0->nz, A0.method, best_rows, best_fs);
     return true;
 }
 

@@ -161,6 +161,9 @@ int fullmag_fdm_adjoint_backward(fullmag_fdm_backend *handle, double *grad_ms, d
  *
  * create: plans[i] with optional exts[i] (exts or its entries may be NULL);
  * returns non-NULL with last_error set on failure, like backend_create.
+ * Each group's x tile is chosen by timing the group's own launches (as
+ * fullmag_fdm_backend_autotune does for one handle) and recorded for later
+ * creates of the same shape.
  * advance: integrates every system to t_end (at most max_steps accepted
  * steps each) with its own dt control starting from dt0, as
  * fullmag_fdm_backend_advance does for one system; out_wall_seconds[i]
