@@ -33,6 +33,7 @@ template <class T>
 hipError_t upload_methods();
 int method_nstage(int method);
 int method_stage_nk(int method, int s);
+int method_efuse(int method);
 template <class T>
 const char *configure(Args<T> &A, size_t lds_limit, int rows_req = 0, int xfast_req = -1);
 template <class T>
@@ -603,6 +604,10 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));
     A.k_all = static_cast<T *>(dalloc(h, 7 * 3 * n * sizeof(T)));
     A.ybuf = static_cast<T *>(dalloc(h, 6 * n * sizeof(T)));  // two stage-state buffers
+    if (fm::method_efuse(A.method)) {
+        A.errv = static_cast<double *>(dalloc(h, 3 * n * sizeof(double)));
+        if (!A.errv) return hip_ok(h, hipErrorOutOfMemory, "alloc error terms");
+    }
     for (int i = 0; i < 3; ++i) h->obs_base[i] = dalloc(h, 3 * n * sizeof(T));
     h->ctl_d = static_cast<fm::Ctl *>(dalloc(h, sizeof(fm::Ctl)));
     if (!A.m_all || !A.k_all || !A.ybuf || !h->obs_base[0] || !h->obs_base[1] || !h->obs_base[2] || !h->ctl_d)

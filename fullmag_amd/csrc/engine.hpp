@@ -65,6 +65,10 @@ struct Method {
     int nerr;
     int eidx[kMaxStages];
     double e[kMaxStages];
+    // efuse = 1: the error terms other than the final stage's are exactly the
+    // final row's k-vectors in row order (DP45), so the final combination
+    // also forms their sum (Args::errv) and the last pass adds only k_final
+    int efuse;
     double node[kMaxStages + 1];   // stage s is evaluated at t + node[s]*dt
     StageRow row[kMaxStages + 1];  // row[s] for s = 2..nstage
 };
@@ -131,6 +135,9 @@ struct Args {
     unsigned *counter;      // finished work-groups of the current partial-producing launch
     // observables (written only when non-NULL)
     T *obs_hex, *obs_hd, *obs_heff;
+    // adaptive methods with Method::efuse: sum over the final row of
+    // e_q k_q (FP64, unscaled), written by the final stage combination
+    double *errv;
     // trajectory record for the adjoint sweep (written only when non-NULL):
     // for the step in flight, slot i = (*rec_step mod rec_seg), stage state
     // y_s at rec_y[i*rec_sv + (s-1)*3N], effective field at y_s at

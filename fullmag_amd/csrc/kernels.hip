@@ -349,13 +349,25 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     // FP32 several bits; the kernel is memory-bound, so the FP64 ALU work is
     // free.
     const StageRow &R = M.row[s];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    const bool efin = A.errv && s == M.nstage;
     for (int q = 0; q < R.nk; ++q) {
         const T *k = A.k_all + kslot(M, R.kidx[q], p) * 3 * N;
         const double w = R.a[q];
         a0 += w * (double)k[cell];
         a1 += w * (double)k[N + cell];
         a2 += w * (double)k[2 * N + cell];
+        if (efin) {
+            const double we = M.e[q];
+            e0 += we * (double)k[cell];
+            e1 += we * (double)k[N + cell];
+            e2 += we * (double)k[2 * N + cell];
+        }
+    }
+    if (efin) {
+        A.errv[cell] = e0;
+        A.errv[N + cell] = e1;
+        A.errv[2 * N + cell] = e2;
     }
     const double f = dt * R.scale;
     double x = (double)m0[cell] + a0 * f;
@@ -455,6 +467,7 @@ struct CellIn {
     T ms, ku1, ku2, ax[3];
     T ke[kMaxErr][3]; // k-vectors of the error estimate (final adaptive stage)
     T m0[3];          // state at the start of the step (mixed tolerance)
+    double ev[3];     // sum of the error terms before the final stage (Args::errv)
     unsigned char act, nact[6], wm;
     uint32_t rc, rn[6];
 };
@@ -489,7 +502,10 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int q = 0; q < 6; ++q) g.rn[q] = A.region[nb[q]];
     }
-    if (err) {
+    if (err && A.errv) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.ev[c] = A.errv[c * N + cell];
+    } else if (err) {
 #pragma unroll
         for (int q = 0; q < kMaxErr; ++q) {
             if (q >= M.nerr || M.eidx[q] == M.nstage) continue;
@@ -497,6 +513,8 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 #pragma unroll
             for (int c = 0; c < 3; ++c) g.ke[q][c] = kk[c * N + cell];
         }
+    }
+    if (err) {
         if (A.rtol > 0.0) {
             const T *m0 = A.m_all + (size_t)p * 3 * N;
 #pragma unroll
@@ -625,14 +643,23 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     }
     if (err) {
         double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+        if (A.errv) {
+            // same order as below: the earlier terms were summed by the
+            // final stage combination
+            const double w = M.e[M.nerr - 1];
+            e0 = g.ev[0] + w * (double)k[0];
+            e1 = g.ev[1] + w * (double)k[1];
+            e2 = g.ev[2] + w * (double)k[2];
+        } else {
 #pragma unroll
-        for (int q = 0; q < kMaxErr; ++q) {
-            if (q >= M.nerr) continue;
-            const double w = M.e[q];
-            const T *kq = M.eidx[q] == M.nstage ? k : g.ke[q];
-            e0 += w * (double)kq[0];
-            e1 += w * (double)kq[1];
-            e2 += w * (double)kq[2];
+            for (int q = 0; q < kMaxErr; ++q) {
+                if (q >= M.nerr) continue;
+                const double w = M.e[q];
+                const T *kq = M.eidx[q] == M.nstage ? k : g.ke[q];
+                e0 += w * (double)kq[0];
+                e1 += w * (double)kq[1];
+                e2 += w * (double)kq[2];
+            }
         }
         e0 *= dt;
         e1 *= dt;
@@ -879,6 +906,22 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
                 a0 += w * (double)in.kv[j][q][0];
                 a1 += w * (double)in.kv[j][q][1];
                 a2 += w * (double)in.kv[j][q][2];
+            }
+            if (A.errv && s == M.nstage && c0 >= 0) {
+                // error terms of the final row (Method::efuse: the same k's,
+                // in the same order, as the error estimate's)
+                double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < NK; ++q) {
+                    const double w = M.e[q];
+                    e0 += w * (double)in.kv[j][q][0];
+                    e1 += w * (double)in.kv[j][q][1];
+                    e2 += w * (double)in.kv[j][q][2];
+                }
+                const int cell = c0 + x;
+                A.errv[cell] = e0;
+                A.errv[N + cell] = e1;
+                A.errv[2 * N + cell] = e2;
             }
             const double xx = (double)in.mv[j][0] + a0 * f, yy = (double)in.mv[j][1] + a1 * f,
                          zz = (double)in.mv[j][2] + a2 * f;
@@ -2114,6 +2157,13 @@ static void build_methods(Method *out) {
     row(r, 3, {2}, {1.0}, 0.5);
     row(r, 4, {3}, {1.0}, 1.0);
     row(r, 5, {1, 2, 3, 4}, {1.0, 2.0, 2.0, 1.0}, 1.0 / 6.0);
+    for (int i = 0; i < M_COUNT; ++i) {
+        Method &m = out[i];
+        const StageRow &fr = m.row[m.nstage];
+        bool f = m.adaptive && m.nerr >= 1 && m.eidx[m.nerr - 1] == m.nstage && fr.nk == m.nerr - 1;
+        for (int q = 0; f && q < fr.nk; ++q) f = fr.kidx[q] == m.eidx[q];
+        m.efuse = f ? 1 : 0;
+    }
 }
 
 template <class T>
@@ -2134,6 +2184,12 @@ int method_nstage(int method) {
     Method tab[M_COUNT];
     build_methods(tab);
     return tab[method].nstage;
+}
+
+int method_efuse(int method) {
+    Method tab[M_COUNT];
+    build_methods(tab);
+    return tab[method].efuse;
 }
 
 int method_stage_nk(int method, int s) {
