@@ -494,12 +494,13 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             return false;
         }
     }
-    A.method = p->integrator == FULLMAG_FDM_INTEGRATOR_DP45 ? fm::M_DP45
-               : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4 ? fm::M_RK4
+    A.method = p->integrator == FULLMAG_FDM_INTEGRATOR_DP45   ? fm::M_DP45
+               : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4  ? fm::M_RK4
+               : p->integrator == FULLMAG_FDM_INTEGRATOR_RK23 ? fm::M_RK23
                                                                : fm::M_HEUN;
     h->nstage = fm::method_nstage(A.method);
     for (int s = 0; s <= fm::kMaxStages; ++s) A.stage_nk[s] = fm::method_stage_nk(A.method, s);
-    h->adaptive = A.method == fm::M_DP45;
+    h->adaptive = A.method == fm::M_DP45 || A.method == fm::M_RK23;
 
     // geometry
     if (p->active_mask) {
@@ -717,8 +718,8 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
     if (p->precision != FULLMAG_FDM_PRECISION_SINGLE && p->precision != FULLMAG_FDM_PRECISION_DOUBLE)
         return (h->err = "invalid precision", false);
     if (p->integrator != FULLMAG_FDM_INTEGRATOR_DP45 && p->integrator != FULLMAG_FDM_INTEGRATOR_RK4 &&
-        p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN)
-        return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4)", false);
+        p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN && p->integrator != FULLMAG_FDM_INTEGRATOR_RK23)
+        return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4, rk23)", false);
     const std::string bad = unsupported_terms(p);
     if (!bad.empty()) return (h->err = "terms not supported by fdm_gpu_native: " + bad, false);
     h->nx = g.nx;

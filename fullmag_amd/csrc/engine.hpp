@@ -62,6 +62,7 @@ struct StageRow {
 struct Method {
     int nstage;         // index of the final stage (DP45: 7, RK4: 5, Heun: 3)
     int adaptive;       // 1 = embedded error estimate + step control
+    double ctrl_exp;    // controller exponent 1/(order+1): DP45 1/5, RK23 1/3
     int nerr;
     int eidx[kMaxStages];
     double e[kMaxStages];
@@ -204,6 +205,6 @@ struct AdjArgs {
 };
 
 // Method indices in the constant table.
-enum MethodId { M_HEUN = 0, M_DP45 = 1, M_RK4 = 2, M_COUNT = 3 };
+enum MethodId { M_HEUN = 0, M_DP45 = 1, M_RK4 = 2, M_RK23 = 3, M_COUNT = 4 };
 
 }  // namespace fm

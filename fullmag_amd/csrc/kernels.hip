@@ -2039,7 +2039,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
         const double thr = A.rtol > 0.0 ? 1.0 : A.atol;
         accept = err <= thr || dt <= A.dt_min;
         const double ratio =
-            fmax(fmin(A.headroom * pow(thr / (accept ? fmax(err, 1e-30) : err), 0.2), A.growth), A.shrink);
+            fmax(fmin(A.headroom * pow(thr / (accept ? fmax(err, 1e-30) : err), M.ctrl_exp), A.growth), A.shrink);
         const double dtn = fmin(fmax(dt * ratio, A.dt_min), A.dt_max);
         if (accept) c->dt_next = dtn;
         else c->dt = dtn;
@@ -2128,6 +2128,7 @@ static void build_methods(Method *out) {
     Method &d = out[M_DP45];
     d.nstage = 7;
     d.adaptive = 1;
+    d.ctrl_exp = 0.2;
     const double dnode[8] = {0.0, 0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
     for (int i = 0; i < 8; ++i) d.node[i] = dnode[i];
     row(d, 2, {1}, {1.0}, 1.0 / 5.0);
@@ -2157,6 +2158,26 @@ static void build_methods(Method *out) {
     row(r, 3, {2}, {1.0}, 0.5);
     row(r, 4, {3}, {1.0}, 1.0);
     row(r, 5, {1, 2, 3, 4}, {1.0, 2.0, 2.0, 1.0}, 1.0 / 6.0);
+    // Bogacki-Shampine 3(2) with normalised stages (fdm_integrators.rs:290-410);
+    // the k of the new state is the next step's k1 (the reference evaluates
+    // it again at the start of the next step, with the same result)
+    Method &b = out[M_RK23];
+    b.nstage = 4;
+    b.adaptive = 1;
+    b.ctrl_exp = 1.0 / 3.0;
+    b.node[2] = 0.5;
+    b.node[3] = 0.75;
+    b.node[4] = 1.0;
+    row(b, 2, {1}, {1.0}, 0.5);
+    row(b, 3, {2}, {1.0}, 0.75);
+    row(b, 4, {1, 2, 3}, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}, 1.0);
+    const int bj[4] = {1, 2, 3, 4};
+    const double bw[4] = {-5.0 / 72.0, 1.0 / 12.0, 1.0 / 9.0, -1.0 / 8.0};
+    b.nerr = 4;
+    for (int i = 0; i < 4; ++i) {
+        b.eidx[i] = bj[i];
+        b.e[i] = bw[i];
+    }
     for (int i = 0; i < M_COUNT; ++i) {
         Method &m = out[i];
         const StageRow &fr = m.row[m.nstage];

@@ -16,6 +16,7 @@
 //   Heun ................. crates/fullmag-engine/src/fdm_integrators.rs:84-150
 //   RK4 .................. crates/fullmag-engine/src/fdm_integrators.rs:155-264
 //   RK45 (Dormand-Prince)  crates/fullmag-engine/src/fdm_integrators.rs:429-680
+//   RK23 (Bogacki-Shampine) crates/fullmag-engine/src/fdm_integrators.rs:290-410
 //   error norm ........... crates/fullmag-engine/src/fdm_integrators.rs:828-870
 //   neighbour clamp ...... crates/fullmag-engine/src/fdm_types.rs:126-132
 //   runner time loop ..... crates/fullmag-runner/src/cpu_reference.rs:448-455,622-639
@@ -926,8 +927,80 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
     }
 }
 
+// Bogacki-Shampine 3(2), fdm_integrators.rs:290-410: k1 is evaluated on
+// every attempt (no FSAL), stages normalised, 3rd-order solution, error
+// dt*(-5/72 k1 + 1/12 k2 + 1/9 k3 - 1/8 k4) with k4 at the new state, and
+// the controller exponent 1/3.  The waveform is evaluated at the stage
+// times t0, t0 + dt/2, t0 + 3dt/4, t0 + dt.
+int orc_rk23_step(void *h, double *m, double *t, double dt_in, orc_report *rep) {
+    Problem *p = static_cast<Problem *>(h);
+    const orc_desc &c = p->d;
+    const size_t n = p->n;
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    double dt = std::max(std::min(dt_in, c.dt_max), c.dt_min);
+    std::copy(mm, mm + n, p->m0.begin());
+    auto &K = p->k;
+    const V3 *m0 = p->m0.data();
+    V3 *st = p->ms.data();
+    uint64_t rejected = 0;
+    const double t0 = *t;
+    for (;;) {
+        p->t_eval = t0;
+        rhs_eval(*p, m0, K[0].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s) st[s] = v_normalized(v_add(m0[s], v_scale(K[0][s], 0.5 * dt)));
+        p->t_eval = t0 + 0.5 * dt;
+        rhs_eval(*p, st, K[1].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s) st[s] = v_normalized(v_add(m0[s], v_scale(K[1][s], 0.75 * dt)));
+        p->t_eval = t0 + 0.75 * dt;
+        rhs_eval(*p, st, K[2].data());
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)n; ++s)
+            st[s] = v_normalized(v_add(
+                m0[s], v_scale(v_add(v_add(v_scale(K[0][s], 2.0 / 9.0), v_scale(K[1][s], 1.0 / 3.0)),
+                                     v_scale(K[2][s], 4.0 / 9.0)),
+                               dt)));
+        p->t_eval = t0 + dt;
+        rhs_eval(*p, st, K[3].data());
+        const bool use_rtol = c.rtol > 0.0;
+        double err = 0.0;
+        const double w[4] = {-5.0 / 72.0, 1.0 / 12.0, 1.0 / 9.0, -1.0 / 8.0};
+#pragma omp parallel for reduction(max : err) schedule(static)
+        for (long s = 0; s < (long)n; ++s) {
+            double e[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < 4; ++q)
+                for (int a = 0; a < 3; ++a) e[a] += w[q] * K[q][s].c[a];
+            for (int a = 0; a < 3; ++a) e[a] *= dt;
+            double v = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (use_rtol) v /= (c.atol + c.rtol * std::max(v_norm(m0[s]), 1e-30));
+            err = std::max(err, v);
+        }
+        const double thr = use_rtol ? 1.0 : c.atol;
+        if (err <= thr || dt <= c.dt_min) {
+            std::copy(st, st + n, mm);
+            *t += dt;
+            const double ratio =
+                std::max(std::min(c.headroom * std::pow(thr / std::max(err, 1e-30), 1.0 / 3.0), c.growth), c.shrink);
+            const double dt_next = std::min(std::max(dt * ratio, c.dt_min), c.dt_max);
+            p->t_eval = *t;
+            observables(*p, mm, rep);
+            rep->time = *t;
+            rep->dt_used = dt;
+            rep->dt_next = dt_next;
+            rep->rejected = rejected;
+            p->fsal_valid = false;
+            return 0;
+        }
+        ++rejected;
+        const double ratio = std::max(std::min(c.headroom * std::pow(thr / err, 1.0 / 3.0), c.growth), c.shrink);
+        dt = std::min(std::max(dt * ratio, c.dt_min), c.dt_max);
+    }
+}
+
 // Runner loop, cpu_reference.rs:622-639: dt_step = min(dt, until - t); the
-// next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 4 RK4.
+// next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 4 RK4,
+// 5 RK23.
 // Records each accepted step's (time, dt) in `trace` (2 doubles per step) if
 // non-NULL.  Returns the number of steps taken.
 uint64_t orc_run(void *h, int integrator, double *m, double *t, double until, double dt0, uint64_t max_steps,
@@ -940,6 +1013,9 @@ uint64_t orc_run(void *h, int integrator, double *m, double *t, double until, do
         const double dt_step = std::min(dt, until - *t);
         if (integrator == 2) {
             orc_rk45_step(p, m, t, dt_step, &rep);
+            dt = rep.dt_next;
+        } else if (integrator == 5) {
+            orc_rk23_step(p, m, t, dt_step, &rep);
             dt = rep.dt_next;
         } else if (integrator == 4) {
             orc_rk4_step(p, m, t, dt_step, &rep);

@@ -92,7 +92,7 @@ def lib():
         L.orc_prepare_state.argtypes = [ctypes.c_void_p, _dp]
         L.orc_invalidate_fsal.argtypes = [ctypes.c_void_p]
         L.orc_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step"):
+        for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step", "orc_rk23_step"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.POINTER(Report)]
         L.orc_run.restype = ctypes.c_uint64
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
@@ -290,12 +290,13 @@ class Oracle:
         """One step in place on `m` (n,3 float64 C-contiguous). Returns (t, report)."""
         tt = ctypes.c_double(t)
         r = Report()
-        fn = {"heun": lib().orc_heun_step, "rk4": lib().orc_rk4_step, "rk45": lib().orc_rk45_step}[integrator]
+        fn = {"heun": lib().orc_heun_step, "rk4": lib().orc_rk4_step, "rk45": lib().orc_rk45_step,
+              "rk23": lib().orc_rk23_step}[integrator]
         fn(self.h, _ptr(m), ctypes.byref(tt), dt, ctypes.byref(r))
         return tt.value, r.as_dict()
 
     def run(self, integrator, m, t, until, dt0, max_steps, trace=False):
-        code = {"heun": 1, "rk45": 2, "rk4": 4}[integrator]
+        code = {"heun": 1, "rk45": 2, "rk4": 4, "rk23": 5}[integrator]
         tt = ctypes.c_double(t)
         r = Report()
         tr = np.zeros(2 * max_steps) if trace else None

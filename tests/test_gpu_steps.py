@@ -1,5 +1,6 @@
 """GPU parity of whole integrator steps against the FP64 oracle, through the
-C ABI: DP45 (adaptive, FSAL, device-side step control), RK4, Heun, the
+C ABI: DP45 (adaptive, FSAL, device-side step control), RK23 (adaptive,
+Bogacki-Shampine), RK4, Heun, the
 device-resident runner loop, interrupts, uploads and snapshots.
 
 Tolerances: FP64 engine vs oracle, m within 1e-11 after a few steps (same
@@ -15,6 +16,12 @@ from tests.cases import masked_3x3, random_case, rel_err, thin_film_8x6
 
 pytestmark = pytest.mark.gpu
 
+ADAPTIVE = ("dp45", "rk23")
+
+
+def _orc_name(integrator):
+    return "rk45" if integrator == "dp45" else integrator
+
 
 def _run_pair(case, integrator, precision, nsteps, dt):
     from fullmag_amd.backend import FdmBackend
@@ -28,14 +35,14 @@ def _run_pair(case, integrator, precision, nsteps, dt):
         dt_e = dt_o = dt
         for _ in range(nsteps):
             st = be.step(dt_e)
-            t, rep = orc.step("rk45" if integrator == "dp45" else integrator, m, t, dt_o)
-            if integrator == "dp45":
+            t, rep = orc.step(_orc_name(integrator), m, t, dt_o)
+            if integrator in ADAPTIVE:
                 dt_e, dt_o = st["suggested_next_dt"], rep["dt_next"]
             out.append((st, rep, be.field("m"), m.copy()))
     return out
 
 
-@pytest.mark.parametrize("integrator", ["dp45", "rk4", "heun"])
+@pytest.mark.parametrize("integrator", ["dp45", "rk4", "heun", "rk23"])
 def test_steps_fp64_match_oracle_film(integrator):
     case = random_case(48, 40, 1, seed=21, hole_radius_cells=8)
     res = _run_pair(case, integrator, "double", 6, 1e-13)
@@ -45,7 +52,7 @@ def test_steps_fp64_match_oracle_film(integrator):
         assert np.abs(me - mo).max() < 1e-11
         assert abs(st["total_energy_joules"] - rep["e_total"]) <= 1e-9 * abs(rep["e_total"])
         assert abs(st["max_rhs_amplitude"] - rep["max_rhs"]) <= 1e-9 * rep["max_rhs"]
-        if integrator == "dp45":
+        if integrator in ADAPTIVE:
             assert abs(st["suggested_next_dt"] - rep["dt_next"]) <= 1e-9 * rep["dt_next"]
 
 
@@ -58,11 +65,12 @@ def test_dp45_fp64_3d_and_masked():
             assert abs(st["dt_seconds"] - rep["dt_used"]) <= 1e-9 * rep["dt_used"]
 
 
-def test_dp45_rejection_path_matches_oracle():
+@pytest.mark.parametrize("integrator", ADAPTIVE)
+def test_rejection_path_matches_oracle(integrator):
     # a too-large first trial step forces rejections; both sides must shrink
     # dt identically and accept the same step
     case = random_case(32, 32, 1, seed=8, alpha=0.3)
-    res = _run_pair(case, "dp45", "double", 3, 1e-11)
+    res = _run_pair(case, integrator, "double", 3, 1e-11)
     st, rep, me, mo = res[0]
     assert rep["rejected"] >= 1
     assert abs(st["dt_seconds"] - rep["dt_used"]) <= 1e-9 * rep["dt_used"]
@@ -102,9 +110,8 @@ def _fp32_vs_oracle(case, integrator, nsteps, dt0):
         for _ in range(nsteps):
             st = be.step(dt)
             # the oracle takes exactly the step the engine accepted
-            t, rep = orc.step("rk4" if integrator == "rk4" else "heun" if integrator == "heun" else "rk45",
-                              m, t, st["dt_seconds"])
-            if integrator == "dp45":
+            t, rep = orc.step(_orc_name(integrator), m, t, st["dt_seconds"])
+            if integrator in ADAPTIVE:
                 assert rep["rejected"] == 0 and abs(rep["dt_used"] - st["dt_seconds"]) < 1e-30
                 dt = st["suggested_next_dt"]
             errs.append(float(np.abs(be.field("m") - m).max()))
@@ -112,7 +119,7 @@ def _fp32_vs_oracle(case, integrator, nsteps, dt0):
     return errs
 
 
-@pytest.mark.parametrize("integrator", ["dp45", "rk4"])
+@pytest.mark.parametrize("integrator", ["dp45", "rk4", "rk23"])
 def test_fp32_random_state_vs_oracle(integrator):
     # random m0 with a hole: exchange fields ~1e7 A/m; a few steps stay
     # inside the Tier-B band before the dynamics amplify FP32 roundoff
@@ -157,15 +164,16 @@ def test_advance_equals_step_loop():
         assert r["suggested_next_dt"] == st["suggested_next_dt"]
 
 
-def test_advance_matches_oracle_runner_loop_until():
+@pytest.mark.parametrize("integrator", ADAPTIVE)
+def test_advance_matches_oracle_runner_loop_until(integrator):
     from fullmag_amd.backend import FdmBackend
     from oracle.oracle import Oracle
     case = random_case(24, 20, 1, seed=13)
     orc = Oracle(case.problem())
     m = orc.prepare(case.m0)
     until = 3e-12
-    ro = orc.run("rk45", m, 0.0, until, 1e-13, 10000)
-    with FdmBackend(case.plan()) as be:
+    ro = orc.run(_orc_name(integrator), m, 0.0, until, 1e-13, 10000)
+    with FdmBackend(case.plan(integrator=integrator)) as be:
         r = be.advance(until, 10000, 1e-13)
         assert r["steps_taken"] == ro["steps"]
         assert abs(r["time_seconds"] - ro["t"]) <= 1e-20
@@ -187,7 +195,7 @@ def test_fixed_step_advance_rk4_matches_oracle():
 
 
 @pytest.mark.parametrize("precision", ["double", "single"])
-@pytest.mark.parametrize("integrator", ["dp45", "rk4", "heun"])
+@pytest.mark.parametrize("integrator", ["dp45", "rk4", "heun", "rk23"])
 @pytest.mark.parametrize("shape", [(64, 48, 1), (50, 30, 1), (12, 10, 4), (8, 32, 16)])
 def test_joined_stage_launches_equal_separate(precision, integrator, shape):
     # A trial step joins the inverse x pass of stage s with the forward x pass
@@ -205,7 +213,7 @@ def test_joined_stage_launches_equal_separate(precision, integrator, shape):
         for _ in range(6):
             sa, sb = a.step(dt_a), b.step(dt_b)
             assert sa["time_seconds"] == sb["time_seconds"]
-            if integrator == "dp45":
+            if integrator in ADAPTIVE:
                 dt_a, dt_b = sa["suggested_next_dt"], sb["suggested_next_dt"]
         ma, mb = a.field("m"), b.field("m")
         d = float(np.abs(ma - mb).max())

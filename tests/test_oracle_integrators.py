@@ -3,9 +3,11 @@ invariants rather than by restating the tableaux:
 
 - single-cell macrospin in a static field has the analytic LLG solution
   phi(t) = gb H t, tan(theta/2) = tan(theta0/2) exp(-alpha gb H t);
-- observed convergence orders (RK4: 4, Heun: 2, DP45 5th-order solution: 5)
-  fail for any wrong tableau coefficient;
-- the DP45 error estimate scales like dt^5 (embedded 4th-order pair);
+- observed convergence orders (RK4: 4, Heun: 2, DP45 5th-order solution: 5,
+  RK23 3rd-order solution: 3) fail for any wrong tableau coefficient;
+- the DP45 error estimate scales like dt^5 (embedded 4th-order pair), the
+  RK23 one like dt^3 (embedded 2nd-order pair), and each controller uses
+  its own exponent 1/(order+1);
 - norm preservation, monotone energy decay with damping, and the reference's
   guardrail that RK45 and Heun relax to energies within 5%
   (crates/fullmag-engine/tests/physics_guardrails.rs:122-152);
@@ -43,7 +45,7 @@ def _integrate(integ, dt, t_end):
     return m[0], _exact(th0, t)
 
 
-@pytest.mark.parametrize("integ,order", [("rk4", 4), ("heun", 2), ("rk45", 5)])
+@pytest.mark.parametrize("integ,order", [("rk4", 4), ("heun", 2), ("rk45", 5), ("rk23", 3)])
 def test_macrospin_convergence_order(integ, order):
     t_end = 2e-10
     errs = []
@@ -64,6 +66,37 @@ def test_dp45_error_estimate_scales_as_dt5():
         # dt_next = headroom * dt * (thr/err)^(1/5) -> err = thr*(0.8 dt/dt_next)^5
         ests.append(1e9 * (0.8 * dt / rep["dt_next"]) ** 5)
     assert abs(np.log2(ests[0] / ests[1]) - 5.0) < 0.3
+
+
+def test_rk23_error_estimate_scales_as_dt3():
+    o = _macrospin(atol=1e9, dt_max=1.0)
+    ests = []
+    for dt in (4e-12, 2e-12):
+        m = np.array([[np.sin(1.0), 0.0, np.cos(1.0)]])
+        _, rep = o.step("rk23", m, 0.0, dt)
+        ests.append(1e9 * (0.8 * dt / rep["dt_next"]) ** 3)
+    assert abs(np.log2(ests[0] / ests[1]) - 3.0) < 0.2
+
+
+@pytest.mark.parametrize("integ,expo", [("rk45", 0.2), ("rk23", 1.0 / 3.0)])
+def test_controller_exponent(integ, expo):
+    # same attempt at two tolerances -> same err; dt_next ratio = (atol ratio)^expo
+    res = []
+    for atol in (1e-3, 3.2e-2):
+        o = _macrospin(atol=atol, dt_max=1.0)
+        m = np.array([[np.sin(1.0), 0.0, np.cos(1.0)]])
+        _, rep = o.step(integ, m, 0.0, 1e-13)
+        assert rep["rejected"] == 0
+        res.append(rep["dt_next"])
+    assert abs(res[1] / res[0] - (3.2e-2 / 1e-3) ** expo) < 1e-12
+
+
+def test_rk23_rejection_shrinks_and_retries():
+    o = _macrospin(atol=1e-7, dt_max=1.0)
+    m = np.array([[np.sin(1.0), 0.0, np.cos(1.0)]])
+    _, rep = o.step("rk23", m, 0.0, 5e-11)
+    assert rep["rejected"] >= 1 and rep["dt_used"] < 5e-11
+    assert np.abs(m[0] - _exact(1.0, rep["dt_used"])).max() < 1e-6
 
 
 def test_adaptive_controller_formula():
