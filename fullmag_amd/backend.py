@@ -44,7 +44,7 @@ class FdmPlan:
     m0: np.ndarray                      # (n,3) float64
     gamma: float = 2.211e5
     precision: str = "double"           # "single" | "double"
-    integrator: str = "dp45"            # "heun" | "dp45" | "rk4"
+    integrator: str = "dp45"            # "heun" | "dp45" | "rk4" | "rk23" | "abm3"
     exchange: bool = True
     demag: bool = True
     precession: bool = True

@@ -79,6 +79,7 @@ struct fullmag_fdm_backend {
     uint64_t n = 0;
     int nstage = 0;
     bool adaptive = false;
+    bool multistep = false;  // ABM3: RHS history in k_all slots kHistSlot.. (Ctl::abm_*)
     fm::Args<float> af{};
     fm::Args<double> ad{};
     std::vector<void *> allocs;
@@ -497,10 +498,12 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.method = p->integrator == FULLMAG_FDM_INTEGRATOR_DP45   ? fm::M_DP45
                : p->integrator == FULLMAG_FDM_INTEGRATOR_RK4  ? fm::M_RK4
                : p->integrator == FULLMAG_FDM_INTEGRATOR_RK23 ? fm::M_RK23
+               : p->integrator == FULLMAG_FDM_INTEGRATOR_ABM3 ? fm::M_ABM3
                                                                : fm::M_HEUN;
     h->nstage = fm::method_nstage(A.method);
     for (int s = 0; s <= fm::kMaxStages; ++s) A.stage_nk[s] = fm::method_stage_nk(A.method, s);
     h->adaptive = A.method == fm::M_DP45 || A.method == fm::M_RK23;
+    h->multistep = A.method == fm::M_ABM3;
 
     // geometry
     if (p->active_mask) {
@@ -718,8 +721,9 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
     if (p->precision != FULLMAG_FDM_PRECISION_SINGLE && p->precision != FULLMAG_FDM_PRECISION_DOUBLE)
         return (h->err = "invalid precision", false);
     if (p->integrator != FULLMAG_FDM_INTEGRATOR_DP45 && p->integrator != FULLMAG_FDM_INTEGRATOR_RK4 &&
-        p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN && p->integrator != FULLMAG_FDM_INTEGRATOR_RK23)
-        return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4, rk23)", false);
+        p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN && p->integrator != FULLMAG_FDM_INTEGRATOR_RK23 &&
+        p->integrator != FULLMAG_FDM_INTEGRATOR_ABM3)
+        return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4, rk23, abm3)", false);
     const std::string bad = unsupported_terms(p);
     if (!bad.empty()) return (h->err = "terms not supported by fdm_gpu_native: " + bad, false);
     h->nx = g.nx;
@@ -1096,6 +1100,13 @@ static int upload_common(fullmag_fdm_backend *h, const double *d, const float *f
     const int slot = h->ctl_h->parity;
     const bool ok = h->fp64 ? upload_state<double>(h, d, f, slot) : upload_state<float>(h, d, f, slot);
     if (!ok) return FULLMAG_FDM_ERR_CUDA;
+    if (h->multistep) {
+        // external state change: the RHS history no longer belongs to the
+        // trajectory (ExchangeLlgState::reset_abm_history, fdm_state.rs:172)
+        h->ctl_h->abm_count = 0;
+        h->ctl_h->abm_dt = 0.0;
+        if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    }
     // the first stage of the next step needs k1 of the new state
     if (!eval_current(h, true)) return FULLMAG_FDM_ERR_CUDA;
     return FULLMAG_FDM_OK;
@@ -1317,6 +1328,38 @@ int fullmag_fdm_backend_get_counters(fullmag_fdm_backend *h, uint64_t *acc, uint
     return FULLMAG_FDM_OK;
 }
 
+namespace {
+
+// The timing launches of a trial stage write that stage's RHS; with ABM3 in
+// its predictor-corrector phase that is a slot of the RHS history, so tuning
+// calls keep a copy of the ring and put it back when they end.
+struct HistoryGuard {
+    Backend *h;
+    void *buf = nullptr, *ring = nullptr;
+    size_t bytes = 0;
+    explicit HistoryGuard(Backend *b) : h(b) {
+        if (!h->multistep) return;
+        const size_t es = h->fp64 ? sizeof(double) : sizeof(float);
+        ring = h->fp64 ? (void *)(h->ad.k_all + (size_t)fm::kHistSlot * 3 * h->n)
+                       : (void *)(h->af.k_all + (size_t)fm::kHistSlot * 3 * h->n);
+        bytes = 3 * 3 * h->n * es;
+        if (hipMalloc(&buf, bytes) != hipSuccess) {
+            buf = nullptr;
+            return;
+        }
+        hipMemcpyAsync(buf, ring, bytes, hipMemcpyDeviceToDevice, h->stream);
+    }
+    bool ok() const { return !h->multistep || buf; }
+    ~HistoryGuard() {
+        if (!buf) return;
+        hipMemcpyAsync(ring, buf, bytes, hipMemcpyDeviceToDevice, h->stream);
+        hipStreamSynchronize(h->stream);
+        hipFree(buf);
+    }
+};
+
+}  // namespace
+
 int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *out_us, int n_out) {
     if (!h || !out_us || n_out < 7 || reps < 1) return FULLMAG_FDM_ERR_INVALID;
     if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
@@ -1325,8 +1368,10 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
     if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
     // The kernels of stage 2 only write trial buffers (stage state, stage-2
     // RHS, spectrum, partials); the controller may change dt or parity, so the
-    // control block is restored afterwards.  m and k1 are never written.
+    // control block is restored after every kind.  m and k1 are never written.
     const fm::Ctl saved = *h->ctl_h;
+    HistoryGuard keep(h);
+    if (!keep.ok()) return fail(h, FULLMAG_FDM_ERR_CUDA, "history copy allocation failed");
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -1343,6 +1388,11 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
         out_us[kind] = 1000.0 * ms / reps;
+        // the controller (kind 5) accepts and flips the state parity; the
+        // next kind must see the original slots, or a joined launch into
+        // the final stage would write the current state
+        *h->ctl_h = saved;
+        if (!push_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -1391,6 +1441,8 @@ int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out
     }
     if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
     const fm::Ctl saved = *h->ctl_h;
+    HistoryGuard keep(h);
+    if (!keep.ok()) return fail(h, FULLMAG_FDM_ERR_CUDA, "history copy allocation failed");
     const fm::Args<T> orig = A;
     double best = std::numeric_limits<double>::infinity();
     fm::Args<T> best_args = A;
@@ -1698,7 +1750,8 @@ int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *h, const fullmag_fdm_adjoin
     if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
     if (d->struct_size < sizeof(fullmag_fdm_adjoint_desc))
         return fail(h, FULLMAG_FDM_ERR_INVALID, "fullmag_fdm_adjoint_desc.struct_size too small");
-    if (h->adaptive) return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint needs a fixed-step integrator (rk4 or heun)");
+    if (h->adaptive || h->multistep)
+        return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint needs a one-step fixed-step integrator (rk4 or heun)");
     if (d->steps == 0 || !(d->dt > 0.0) || !std::isfinite(d->dt))
         return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: steps must be >= 1 and dt positive and finite");
     if (d->objective_first_step < 1 || d->objective_first_step > d->steps)

@@ -5,7 +5,8 @@
 // exchange_fp64.cu:46-50):
 //   m_all  : 2 slots x 3 x ncell  (ping-pong state; slot `parity` is current)
 //   k_all  : 7 slots x 3 x ncell  (slots 0/1: k1 of the current state and
-//            the last-stage RHS of the trial state; slots 2..6: k2..k6)
+//            the last-stage RHS of the trial state; slots 2..6: k2..k6;
+//            ABM3: slots 4..6 are the ring of f_n, f_{n-1}, f_{n-2})
 //   ybuf   : 2 x 3 x ncell        (intermediate stage states, by stage parity)
 //   spec   : 3 x nzs x nkx x ly complex (x-half-spectrum of Ms*m, y-lines
 //            contiguous; nkx = nx+1, ly = ny for nz == 1 else 2ny, nzs = nz)
@@ -28,6 +29,8 @@ constexpr int kStatSlots = 8;       // per-block partial reductions
 constexpr int kMaxStages = 8;
 constexpr int kMaxK = 5;            // k-vectors in one stage combination (DP45 rows 6, 7)
 constexpr int kMaxErr = 6;          // terms of the embedded error estimate (DP45)
+constexpr int kHist = 8;            // StageRow::kidx kHist + r names the ABM3 history value f_{n-r}
+constexpr int kHistSlot = 4;        // first k_all slot of the ABM3 history ring
 
 // Per-block partial layout (doubles).
 enum StatSlot {
@@ -70,6 +73,10 @@ struct Method {
     // final row's k-vectors in row order (DP45), so the final combination
     // also forms their sum (Args::errv) and the last pass adds only k_final
     int efuse;
+    // abm = 1: ABM3 predictor-corrector (stage 2's RHS is f*, stored in the
+    // history slot it replaces); 2: its Heun start-up (the final RHS is also
+    // pushed into the history).  0 for one-step methods.
+    int abm;
     double node[kMaxStages + 1];   // stage s is evaluated at t + node[s]*dt
     StageRow row[kMaxStages + 1];  // row[s] for s = 2..nstage
 };
@@ -86,6 +93,7 @@ struct Ctl {
     double dt_used;    // last accepted dt
     double dt_next;    // controller proposal after the last accepted step
     double err;        // error norm of the last attempt
+    double abm_dt;     // ABM3: dt of the last history push (0 = none)
     double stats[kStatSlots];  // reduced statistics of the last evaluation
     unsigned long long step;       // accepted steps since create
     unsigned long long step_end;   // loop mode: stop after this many steps
@@ -95,6 +103,8 @@ struct Ctl {
     int accepted;      // last attempt accepted
     int done;          // loop mode finished: kernels return at once
     int loop;          // 1 = loop mode, 0 = single step driven by the host
+    int abm_count;     // ABM3: history values since the last restart (0..3)
+    int abm_head;      // ABM3: f_n is in k_all slot kHistSlot + abm_head
 };
 
 template <class T>
@@ -205,6 +215,8 @@ struct AdjArgs {
 };
 
 // Method indices in the constant table.
-enum MethodId { M_HEUN = 0, M_DP45 = 1, M_RK4 = 2, M_RK23 = 3, M_COUNT = 4 };
+// M_ABM3S is the start-up table of M_ABM3 (selected on the device while the
+// history holds fewer than three values).
+enum MethodId { M_HEUN = 0, M_DP45 = 1, M_RK4 = 2, M_RK23 = 3, M_ABM3 = 4, M_ABM3S = 5, M_COUNT = 6 };
 
 }  // namespace fm

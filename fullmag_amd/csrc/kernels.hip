@@ -63,6 +63,14 @@ __device__ __forceinline__ C2<T> mul_mi(C2<T> a) {
 // double) with its own copy of the method table.
 static __constant__ Method c_methods[M_COUNT];
 
+// Tableau of the step in flight: ABM3 runs its Heun start-up table until
+// the history holds three values (Ctl::abm_count changes only in the
+// controller, after every kernel of the step has read it).
+template <class T>
+__device__ __forceinline__ const Method &method_for(const Args<T> &A) {
+    return c_methods[A.method == M_ABM3 && A.ctl->abm_count < 3 ? M_ABM3S : A.method];
+}
+
 extern __shared__ __align__(16) unsigned char fm_smem[];
 
 // ------------------------------------------------------------ butterflies --
@@ -295,8 +303,23 @@ __device__ C2<T> *lds_fft(C2<T> *a, C2<T> *b, const FftDesc &fd, const C2<T> *__
 }
 
 // ------------------------------------------------------------ RK helpers --
-__device__ __forceinline__ int kslot(const Method &M, int j, int p) {
+template <class T>
+__device__ __forceinline__ int kslot(const Args<T> &A, const Method &M, int j, int p) {
+    if (M.abm) {
+        // ABM3 history ring: f_{n-r} for kidx kHist + r; the predictor's
+        // RHS f* goes to the slot of f_{n-2}, which becomes f_n at the push
+        if (j >= kHist) return kHistSlot + (A.ctl->abm_head + 3 - (j - kHist)) % 3;
+        if (M.abm == 1 && j == 2) return kHistSlot + (A.ctl->abm_head + 1) % 3;
+    }
     return j == 1 ? p : (j == M.nstage ? 1 - p : j);
+}
+
+// ABM3 start-up: the RHS of the new state is also pushed into the history
+// (the slot that becomes f_n); NULL otherwise.
+template <class T>
+__device__ __forceinline__ T *stage_push(const Args<T> &A, const Method &M, int s) {
+    if (M.abm != 2 || s != M.nstage) return nullptr;
+    return A.k_all + (size_t)(kHistSlot + (A.ctl->abm_head + 1) % 3) * 3 * (size_t)A.ncell;
 }
 
 // Intermediate stage states alternate between two buffers, so that a fused
@@ -328,7 +351,7 @@ __device__ __forceinline__ T *stage_output(const Args<T> &A, const Method &M, in
 
 template <class T>
 __device__ __forceinline__ T *stage_rhs(const Args<T> &A, const Method &M, int s, int p) {
-    const int slot = s <= 1 ? p : kslot(M, s, p);
+    const int slot = s <= 1 ? p : kslot(A, M, s, p);
     return A.k_all + (size_t)slot * 3 * (size_t)A.ncell;
 }
 
@@ -352,7 +375,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
     const bool efin = A.errv && s == M.nstage;
     for (int q = 0; q < R.nk; ++q) {
-        const T *k = A.k_all + kslot(M, R.kidx[q], p) * 3 * N;
+        const T *k = A.k_all + kslot(A, M, R.kidx[q], p) * 3 * N;
         const double w = R.a[q];
         a0 += w * (double)k[cell];
         a1 += w * (double)k[N + cell];
@@ -509,7 +532,7 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int q = 0; q < kMaxErr; ++q) {
             if (q >= M.nerr || M.eidx[q] == M.nstage) continue;
-            const T *kk = A.k_all + (size_t)kslot(M, M.eidx[q], p) * 3 * N;
+            const T *kk = A.k_all + (size_t)kslot(A, M, M.eidx[q], p) * 3 * N;
 #pragma unroll
             for (int c = 0; c < 3; ++c) g.ke[q][c] = kk[c * N + cell];
         }
@@ -528,12 +551,13 @@ template <class T, bool LUT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
                                              const CellIn<T> &g, T *__restrict__ kout, const T hd[3], T wf,
                                              bool stats, bool err, Acc &acc, T *__restrict__ hrec,
-                                             T kret[3] = nullptr) {
+                                             T kret[3] = nullptr, T *__restrict__ kpush = nullptr) {
     const int N = (int)A.ncell;
     if (!g.act) {
         kout[cell] = T(0);
         kout[N + cell] = T(0);
         kout[2 * N + cell] = T(0);
+        if (kpush) kpush[cell] = kpush[N + cell] = kpush[2 * N + cell] = T(0);
         if (kret) kret[0] = kret[1] = kret[2] = T(0);
         if (hrec) hrec[cell] = hrec[N + cell] = hrec[2 * N + cell] = T(0);
         if (A.obs_hex) {
@@ -608,6 +632,11 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     kout[cell] = k[0];
     kout[N + cell] = k[1];
     kout[2 * N + cell] = k[2];
+    if (kpush) {
+        kpush[cell] = k[0];
+        kpush[N + cell] = k[1];
+        kpush[2 * N + cell] = k[2];
+    }
     if (kret) {
         kret[0] = k[0];
         kret[1] = k[1];
@@ -859,7 +888,7 @@ __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
             if (q == skip) continue;
-            const T *kp = A.k_all + (size_t)kslot(M, R.kidx[q], p) * 3 * N;
+            const T *kp = A.k_all + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N;
 #pragma unroll
             for (int c = 0; c < 3; ++c) in.kv[j][q][c] = kp[c * N + cell];
         }
@@ -1122,6 +1151,7 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
     const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
     const float rnx = 1.0f / (float)nx;
     T *kout = stage_rhs(A, M, s, p);
+    T *kpush = stage_push(A, M, s);
     T *hrec = A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
 #pragma unroll
@@ -1138,7 +1168,8 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
             const C2<T> zv = res[(l >> 1) * px + x];
             hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
         }
-        compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec, kk ? kk[j] : nullptr);
+        compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec, kk ? kk[j] : nullptr,
+                             kpush);
     }
 }
 
@@ -1163,7 +1194,7 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
     if (ctl->done) return;
     const int p = ctl->parity;
     const double dt = ctl->dt;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
     C2<T> *tw = b1 + A.nseq_x * A.px;
@@ -1184,7 +1215,7 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     if (ctl->done) return;
     const int p = ctl->parity;
     const double dt = ctl->dt;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
     C2<T> *tw = b1 + A.nseq_x * A.px;
@@ -1217,7 +1248,7 @@ __global__ void __launch_bounds__(kThreads) k_xx(const Args<T> *__restrict__ AP,
     if (ctl->done) return;
     const int p = ctl->parity;
     const double dt = ctl->dt;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.px;
     C2<T> *tw = b1 + A.nseq_x * A.px;
@@ -1973,7 +2004,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ 
     if (ctl->done) return;
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.ncell) return;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     T v[3];
     stage_state(A, M, s, ctl->parity, ctl->dt, cell, v);
 }
@@ -1985,7 +2016,7 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
     if (ctl->done) return;
     const int p = ctl->parity;
     const double dt = ctl->dt;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool err = M.adaptive && s == M.nstage;
     Acc acc;
@@ -1999,7 +2030,8 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
         CellIn<T> g;
         gather_cell<T, LUT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
         compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc,
-                             A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr);
+                             A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
+                             nullptr, stage_push(A, M, s));
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
     if (control) finish_last_block(A, control);
@@ -2025,7 +2057,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     block_reduce_store(acc, tot);
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const Method &M = c_methods[A.method];
+    const Method &M = method_for(A);
     if (observe) {
         for (int k = 0; k < kStatSlots; ++k) c->stats[k] = tot[k];
         return;
@@ -2058,6 +2090,15 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     c->parity ^= 1;
     c->dt_used = dt;
     c->accepted = 1;
+    if (M.abm) {
+        // AbmHistory::push (fdm_state.rs:247-257): the kernels stored the
+        // pushed value in the slot after the head; a dt change above 10%
+        // restarts the history first
+        if (c->abm_dt > 0.0 && fabs(dt - c->abm_dt) / c->abm_dt > 0.1) c->abm_count = 0;
+        c->abm_head = (c->abm_head + 1) % 3;
+        c->abm_count = c->abm_count < 3 ? c->abm_count + 1 : 3;
+        c->abm_dt = dt;
+    }
     if (c->loop) {
         if (M.adaptive) c->dt_carry = c->dt_next;
         if (c->t >= c->t_end || c->step >= c->step_end) {
@@ -2178,6 +2219,27 @@ static void build_methods(Method *out) {
         b.eidx[i] = bj[i];
         b.e[i] = bw[i];
     }
+    // ABM3 (fdm_integrators.rs:751-820): predictor y_2 = m0 + dt (23 f_n -
+    // 16 f_{n-1} + 5 f_{n-2})/12, f* = RHS(y_2), corrector m0 + dt (5 f* +
+    // 8 f_n - f_{n-1})/12; both evaluated at t + dt.  The RHS of the new
+    // state is evaluated for the step's observables and a later start-up.
+    Method &ab = out[M_ABM3];
+    ab.nstage = 3;
+    ab.abm = 1;
+    ab.node[2] = 1.0;
+    ab.node[3] = 1.0;
+    row(ab, 2, {kHist, kHist + 1, kHist + 2}, {23.0 / 12.0, -16.0 / 12.0, 5.0 / 12.0}, 1.0);
+    row(ab, 3, {2, kHist, kHist + 1}, {5.0 / 12.0, 8.0 / 12.0, -1.0 / 12.0}, 1.0);
+    // its start-up (fdm_integrators.rs:693-749): the Heun rows, padded with
+    // zero-weight terms to the same k-vector counts so that one captured
+    // launch sequence serves both tables
+    Method &as = out[M_ABM3S];
+    as.nstage = 3;
+    as.abm = 2;
+    as.node[2] = 1.0;
+    as.node[3] = 1.0;
+    row(as, 2, {1, 1, 1}, {1.0, 0.0, 0.0}, 1.0);
+    row(as, 3, {1, 2, 1}, {1.0, 1.0, 0.0}, 0.5);
     for (int i = 0; i < M_COUNT; ++i) {
         Method &m = out[i];
         const StageRow &fr = m.row[m.nstage];

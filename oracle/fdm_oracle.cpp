@@ -17,6 +17,9 @@
 //   RK4 .................. crates/fullmag-engine/src/fdm_integrators.rs:155-264
 //   RK45 (Dormand-Prince)  crates/fullmag-engine/src/fdm_integrators.rs:429-680
 //   RK23 (Bogacki-Shampine) crates/fullmag-engine/src/fdm_integrators.rs:290-410
+//   ABM3 (Adams-Bashforth-Moulton, Heun start-up)
+//                           crates/fullmag-engine/src/fdm_integrators.rs:683-820,
+//                           history: crates/fullmag-engine/src/fdm_state.rs:204-262
 //   error norm ........... crates/fullmag-engine/src/fdm_integrators.rs:828-870
 //   neighbour clamp ...... crates/fullmag-engine/src/fdm_types.rs:126-132
 //   runner time loop ..... crates/fullmag-runner/src/cpu_reference.rs:448-455,622-639
@@ -374,6 +377,11 @@ struct Problem {
     std::vector<V3> k[7];
     bool fsal_valid = false;
     std::vector<V3> kfsal;
+    // ABM3 history (fdm_state.rs:204-262): f_n, f_{n-1}, f_{n-2}, pushes
+    // since the last restart (saturating at 3) and the dt of the last push
+    std::vector<V3> abm_f[3];
+    int abm_count = 0;
+    double abm_dt = 0.0;
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
     double ms_at(size_t i) const { return ms_c.empty() ? d.ms : ms_c[i]; }
@@ -757,10 +765,21 @@ int orc_prepare_state(void *h, double *m) {
     V3 *mm = reinterpret_cast<V3 *>(m);
     for (size_t s = 0; s < p->n; ++s) mm[s] = p->active(s) ? v_normalized(mm[s]) : V3{{0.0, 0.0, 0.0}};
     p->fsal_valid = false;
+    p->abm_count = 0;
+    p->abm_dt = 0.0;
     return 0;
 }
 
 void orc_invalidate_fsal(void *h) { static_cast<Problem *>(h)->fsal_valid = false; }
+
+// ExchangeLlgState::reset_abm_history (fdm_state.rs:172-174).
+void orc_reset_abm(void *h) {
+    Problem *p = static_cast<Problem *>(h);
+    p->abm_count = 0;
+    p->abm_dt = 0.0;
+}
+
+int orc_abm_count(void *h) { return static_cast<Problem *>(h)->abm_count; }
 
 // Time at which orc_fields / orc_rhs / orc_observables evaluate the waveform.
 void orc_set_time(void *h, double t) { static_cast<Problem *>(h)->t_eval = t; }
@@ -998,9 +1017,66 @@ int orc_rk23_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
     }
 }
 
+// AbmHistory::push (fdm_state.rs:247-257): a dt that differs from the last
+// push by more than 10% restarts the history first; then f becomes f_n.
+static void abm_push(Problem &p, std::vector<V3> &f, double dt) {
+    if (p.abm_dt > 0.0 && std::fabs(dt - p.abm_dt) / p.abm_dt > 0.1) {
+        p.abm_count = 0;
+        p.abm_dt = 0.0;
+    }
+    std::swap(p.abm_f[2], p.abm_f[1]);
+    std::swap(p.abm_f[1], p.abm_f[0]);
+    std::swap(p.abm_f[0], f);
+    p.abm_count = std::min(p.abm_count + 1, 3);
+    p.abm_dt = dt;
+}
+
+// fdm_integrators.rs:683-820.  Until three RHS values are in the history
+// the step is a Heun step and pushes f(m_new); afterwards the
+// Adams-Bashforth predictor, f* = f(pred) and the Adams-Moulton corrector,
+// pushing f*.
+int orc_abm3_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    Problem *p = static_cast<Problem *>(h);
+    const size_t n = p->n;
+    V3 *mm = reinterpret_cast<V3 *>(m);
+    std::vector<V3> f(n);
+    if (p->abm_count < 3) {
+        orc_heun_step(p, m, t, dt, rep);
+        p->t_eval = *t;
+        rhs_eval(*p, mm, f.data());
+        abm_push(*p, f, dt);
+        return 0;
+    }
+    std::copy(mm, mm + n, p->m0.begin());
+    const V3 *fn = p->abm_f[0].data(), *fn1 = p->abm_f[1].data(), *fn2 = p->abm_f[2].data();
+    const double t0 = *t;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)n; ++s) {
+        const V3 pr = v_add(v_add(v_scale(fn[s], 23.0 / 12.0), v_scale(fn1[s], -16.0 / 12.0)), v_scale(fn2[s], 5.0 / 12.0));
+        p->ms[s] = v_normalized(v_add(p->m0[s], v_scale(pr, dt)));
+    }
+    p->t_eval = t0 + dt;
+    rhs_eval(*p, p->ms.data(), f.data());
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)n; ++s) {
+        const V3 co = v_add(v_add(v_scale(f[s], 5.0 / 12.0), v_scale(fn[s], 8.0 / 12.0)), v_scale(fn1[s], -1.0 / 12.0));
+        mm[s] = v_normalized(v_add(p->m0[s], v_scale(co, dt)));
+    }
+    *t += dt;
+    abm_push(*p, f, dt);
+    p->t_eval = *t;
+    observables(*p, mm, rep);
+    rep->time = *t;
+    rep->dt_used = dt;
+    rep->dt_next = 0.0;
+    rep->rejected = 0;
+    p->fsal_valid = false;
+    return 0;
+}
+
 // Runner loop, cpu_reference.rs:622-639: dt_step = min(dt, until - t); the
-// next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 4 RK4,
-// 5 RK23.
+// next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 3 ABM3,
+// 4 RK4, 5 RK23.
 // Records each accepted step's (time, dt) in `trace` (2 doubles per step) if
 // non-NULL.  Returns the number of steps taken.
 uint64_t orc_run(void *h, int integrator, double *m, double *t, double until, double dt0, uint64_t max_steps,
@@ -1019,6 +1095,8 @@ uint64_t orc_run(void *h, int integrator, double *m, double *t, double until, do
             dt = rep.dt_next;
         } else if (integrator == 4) {
             orc_rk4_step(p, m, t, dt_step, &rep);
+        } else if (integrator == 3) {
+            orc_abm3_step(p, m, t, dt_step, &rep);
         } else {
             orc_heun_step(p, m, t, dt_step, &rep);
         }

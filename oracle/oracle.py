@@ -91,8 +91,11 @@ def lib():
         L.orc_observables.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(Report)]
         L.orc_prepare_state.argtypes = [ctypes.c_void_p, _dp]
         L.orc_invalidate_fsal.argtypes = [ctypes.c_void_p]
+        L.orc_reset_abm.argtypes = [ctypes.c_void_p]
+        L.orc_abm_count.argtypes = [ctypes.c_void_p]
+        L.orc_abm_count.restype = ctypes.c_int
         L.orc_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step", "orc_rk23_step"):
+        for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step", "orc_rk23_step", "orc_abm3_step"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.POINTER(Report)]
         L.orc_run.restype = ctypes.c_uint64
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double,
@@ -282,6 +285,14 @@ class Oracle:
     def invalidate_fsal(self):
         lib().orc_invalidate_fsal(self.h)
 
+    def reset_abm(self):
+        """Drop the ABM3 history (the next three steps are Heun start-up steps)."""
+        lib().orc_reset_abm(self.h)
+
+    def abm_count(self):
+        """RHS values in the ABM3 history since the last restart (0..3)."""
+        return lib().orc_abm_count(self.h)
+
     def set_time(self, t):
         """Time at which fields()/rhs()/observables() evaluate the waveform."""
         lib().orc_set_time(self.h, t)
@@ -291,12 +302,12 @@ class Oracle:
         tt = ctypes.c_double(t)
         r = Report()
         fn = {"heun": lib().orc_heun_step, "rk4": lib().orc_rk4_step, "rk45": lib().orc_rk45_step,
-              "rk23": lib().orc_rk23_step}[integrator]
+              "rk23": lib().orc_rk23_step, "abm3": lib().orc_abm3_step}[integrator]
         fn(self.h, _ptr(m), ctypes.byref(tt), dt, ctypes.byref(r))
         return tt.value, r.as_dict()
 
     def run(self, integrator, m, t, until, dt0, max_steps, trace=False):
-        code = {"heun": 1, "rk45": 2, "rk4": 4, "rk23": 5}[integrator]
+        code = {"heun": 1, "rk45": 2, "abm3": 3, "rk4": 4, "rk23": 5}[integrator]
         tt = ctypes.c_double(t)
         r = Report()
         tr = np.zeros(2 * max_steps) if trace else None

@@ -1,24 +1,28 @@
 """Batched sweeps (fullmag_fdm_batch_*): systems that share launches give
 the same results as the same systems stepped alone, each with its own
-adaptive dt sequence (bit-identical: the batched launch only adds a grid
-row per system)."""
+adaptive dt sequence or ABM3 history (bit-identical: the batched launch only
+adds a grid row per system)."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _plans(precision):
+def _plans(precision, integrator="dp45"):
     from fullmag_amd.backend import FdmPlan
     from fullmag_amd.workloads import batch_sweep
     shapes = ((32, 16, 1), (16, 16, 2), (24, 20, 1))
-    return [FdmPlan(**kw) for kw in batch_sweep(count=7, seed=9, shapes=shapes, precision=precision)]
+    out = [FdmPlan(**kw) for kw in batch_sweep(count=7, seed=9, shapes=shapes, precision=precision)]
+    for p in out:
+        p.integrator = integrator
+    return out
 
 
-@pytest.mark.parametrize("precision", ["single", "double"])
-def test_batch_equals_individual_runs(precision):
+@pytest.mark.parametrize("precision,integrator", [("single", "dp45"), ("double", "dp45"), ("double", "abm3"),
+                                                  ("single", "rk23")])
+def test_batch_equals_individual_runs(precision, integrator):
     from fullmag_amd.backend import FdmBackend, FdmBatch
-    plans = _plans(precision)
+    plans = _plans(precision, integrator)
     t_end = 2e-12
     with FdmBatch(plans) as b:
         assert b.groups() == len({(p.nx, p.ny, p.nz) for p in plans})
@@ -54,7 +58,7 @@ def test_batch_second_advance_continues_and_max_steps_caps():
 def test_batch_reports_bad_plan():
     from fullmag_amd.backend import FdmBatch, FdmError
     plans = _plans("single")
-    plans[3].integrator = "abm3"
+    plans[3].extra = {"has_cubic_anisotropy": 1}
     with pytest.raises(FdmError) as e:
         FdmBatch(plans)
     assert "system 3" in str(e.value)

@@ -4,7 +4,12 @@ invariants rather than by restating the tableaux:
 - single-cell macrospin in a static field has the analytic LLG solution
   phi(t) = gb H t, tan(theta/2) = tan(theta0/2) exp(-alpha gb H t);
 - observed convergence orders (RK4: 4, Heun: 2, DP45 5th-order solution: 5,
-  RK23 3rd-order solution: 3) fail for any wrong tableau coefficient;
+  RK23 3rd-order solution: 3, ABM3: 3) fail for any wrong tableau coefficient;
+- ABM3 (fdm_integrators.rs:683-820) against a scheme built here from the
+  definition of Adams methods: weights are integrals of the Lagrange
+  polynomials through the history nodes (numpy.polynomial), three Heun
+  start-up steps, predictor-evaluate-correct pushing f(pred); plus the
+  history's restart rule (fdm_state.rs:247-257: a dt change above 10%);
 - the DP45 error estimate scales like dt^5 (embedded 4th-order pair), the
   RK23 one like dt^3 (embedded 2nd-order pair), and each controller uses
   its own exponent 1/(order+1);
@@ -45,11 +50,13 @@ def _integrate(integ, dt, t_end):
     return m[0], _exact(th0, t)
 
 
-@pytest.mark.parametrize("integ,order", [("rk4", 4), ("heun", 2), ("rk45", 5), ("rk23", 3)])
-def test_macrospin_convergence_order(integ, order):
+@pytest.mark.parametrize("integ,order,n0", [("rk4", 4, 80), ("heun", 2, 80), ("rk45", 5, 80), ("rk23", 3, 80),
+                                            ("abm3", 3, 160)])
+def test_macrospin_convergence_order(integ, order, n0):
+    # (the multistep method leaves its pre-asymptotic range at ~0.05 rad/step)
     t_end = 2e-10
     errs = []
-    for nsteps in (80, 160, 320):
+    for nsteps in (n0, 2 * n0, 4 * n0):
         m, ex = _integrate(integ, t_end / nsteps, t_end)
         errs.append(np.abs(m - ex).max())
     rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
@@ -166,3 +173,67 @@ def test_rk45_and_heun_relax_to_same_energy():
     r2 = o2.run("heun", m2, 0.0, 2e-10, 2e-14, 200000)
     e1, e2 = r1["last"]["e_total"], r2["last"]["e_total"]
     assert abs(e1 - e2) <= 0.05 * abs(e1)
+
+
+def _adams_weights(nodes):
+    """Weights w_j of int_0^1 p(s) ds where p interpolates f at the given
+    nodes (in units of dt, 0 = start of the step)."""
+    from numpy.polynomial import polynomial as P
+    w = []
+    for j, xj in enumerate(nodes):
+        poly = np.array([1.0])
+        for i, xi in enumerate(nodes):
+            if i != j:
+                poly = P.polymul(poly, np.array([-xi, 1.0]) / (xj - xi))
+        integ = P.polyint(poly)
+        w.append(P.polyval(1.0, integ) - P.polyval(0.0, integ))
+    return np.array(w)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def test_abm3_matches_adams_scheme_built_from_definition():
+    ab = _adams_weights([0.0, -1.0, -2.0])     # f_n, f_{n-1}, f_{n-2}
+    am = _adams_weights([1.0, 0.0, -1.0])      # f*, f_n, f_{n-1}
+    o = _macrospin()
+    o2 = _macrospin()
+    m = np.array([[np.sin(1.0), 0.0, np.cos(1.0)]])
+    x = m.copy()
+    hist = []
+    dt, t = 2e-12, 0.0
+    for step in range(10):
+        t, _ = o.step("abm3", m, t, dt)
+        if len(hist) < 3:
+            k1 = o2.rhs(x)
+            pred = _unit(x + dt * k1)
+            x = _unit(x + 0.5 * dt * (k1 + o2.rhs(pred)))
+            hist.insert(0, o2.rhs(x))
+        else:
+            pred = _unit(x + dt * sum(w * f for w, f in zip(ab, hist)))
+            fs = o2.rhs(pred)
+            x = _unit(x + dt * (am[0] * fs + am[1] * hist[0] + am[2] * hist[1]))
+            hist.insert(0, fs)
+        hist = hist[:3]
+        assert o.abm_count() == min(step + 1, 3)
+        assert np.abs(m - x).max() < 1e-13, step
+
+
+def test_abm3_restarts_on_dt_change_above_ten_percent():
+    o = _macrospin()
+    m = np.array([[np.sin(1.0), 0.0, np.cos(1.0)]])
+    t = 0.0
+    for _ in range(5):
+        t, _ = o.step("abm3", m, t, 2e-12)
+    t, _ = o.step("abm3", m, t, 2.1e-12)        # +5%: no restart
+    assert o.abm_count() == 3
+    t, _ = o.step("abm3", m, t, 2.5e-12)        # +19%: full step, then restart at the push
+    assert o.abm_count() == 1
+    # the next step is a Heun start-up step
+    ref = _macrospin()
+    x = m.copy()
+    ref.step("heun", x, t, 2.5e-12)
+    o.step("abm3", m, t, 2.5e-12)
+    assert o.abm_count() == 2
+    assert np.array_equal(m, x)
