@@ -608,7 +608,9 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));
     A.k_all = static_cast<T *>(dalloc(h, 7 * 3 * n * sizeof(T)));
     A.ybuf = static_cast<T *>(dalloc(h, 6 * n * sizeof(T)));  // two stage-state buffers
-    if (fm::method_efuse(A.method)) {
+    if (h->adaptive) {
+        // the error estimate reads the final combination's error-term sum
+        if (!fm::method_efuse(A.method)) return (h->err = "internal: adaptive tableau without fused error terms", false);
         A.errv = static_cast<double *>(dalloc(h, 3 * n * sizeof(double)));
         if (!A.errv) return hip_ok(h, hipErrorOutOfMemory, "alloc error terms");
     }

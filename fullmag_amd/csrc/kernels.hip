@@ -488,7 +488,6 @@ struct CellIn {
     T m[3];           // state at the cell
     T v[6][3];        // box-clamped neighbour states (-x,+x,-y,+y,-z,+z)
     T ms, ku1, ku2, ax[3];
-    T ke[kMaxErr][3]; // k-vectors of the error estimate (final adaptive stage)
     T m0[3];          // state at the start of the step (mixed tolerance)
     double ev[3];     // sum of the error terms before the final stage (Args::errv)
     unsigned char act, nact[6], wm;
@@ -525,19 +524,11 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int q = 0; q < 6; ++q) g.rn[q] = A.region[nb[q]];
     }
-    if (err && A.errv) {
+    if (err) {
+        // every adaptive tableau has Method::efuse: the final stage
+        // combination left the other error terms' sum in errv
 #pragma unroll
         for (int c = 0; c < 3; ++c) g.ev[c] = A.errv[c * N + cell];
-    } else if (err) {
-#pragma unroll
-        for (int q = 0; q < kMaxErr; ++q) {
-            if (q >= M.nerr || M.eidx[q] == M.nstage) continue;
-            const T *kk = A.k_all + (size_t)kslot(A, M, M.eidx[q], p) * 3 * N;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) g.ke[q][c] = kk[c * N + cell];
-        }
-    }
-    if (err) {
         if (A.rtol > 0.0) {
             const T *m0 = A.m_all + (size_t)p * 3 * N;
 #pragma unroll
@@ -671,25 +662,12 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         acc.v[ST_RHS2] = fmax(acc.v[ST_RHS2], sq(k));
     }
     if (err) {
-        double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-        if (A.errv) {
-            // same order as below: the earlier terms were summed by the
-            // final stage combination
-            const double w = M.e[M.nerr - 1];
-            e0 = g.ev[0] + w * (double)k[0];
-            e1 = g.ev[1] + w * (double)k[1];
-            e2 = g.ev[2] + w * (double)k[2];
-        } else {
-#pragma unroll
-            for (int q = 0; q < kMaxErr; ++q) {
-                if (q >= M.nerr) continue;
-                const double w = M.e[q];
-                const T *kq = M.eidx[q] == M.nstage ? k : g.ke[q];
-                e0 += w * (double)kq[0];
-                e1 += w * (double)kq[1];
-                e2 += w * (double)kq[2];
-            }
-        }
+        // terms in tableau order: the earlier ones were summed by the final
+        // stage combination, the final stage's k is the last term
+        const double w = M.e[M.nerr - 1];
+        double e0 = g.ev[0] + w * (double)k[0];
+        double e1 = g.ev[1] + w * (double)k[1];
+        double e2 = g.ev[2] + w * (double)k[2];
         e0 *= dt;
         e1 *= dt;
         e2 *= dt;
