@@ -122,6 +122,7 @@ struct Args {
     int rows_per_wg;        // x-lines per work-group in the x passes
     int ty;                 // y-lines per work-group in the z pass
     int nseq_x;             // complex sequences per x work-group
+    int xsb;                // LDS stride between those sequences (complex values; px for Stockham)
     double inv_p;           // 1 / (px*py*pz)
     // buffers
     T *m_all;

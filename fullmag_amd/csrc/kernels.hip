@@ -712,13 +712,13 @@ struct FourStep {
 };
 
 template <class T, int R1, int R2>
-__device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in) {
+__device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in, int sb) {
     using F = FourStep<R1, R2>;
     const int spb = blockDim.x / F::TS;  // sequences per round
     for (int base = 0; base < nseq; base += spb) {
         const int sq = base + threadIdx.x / F::TS, t = threadIdx.x % F::TS;
         const bool on = sq < nseq && (int)threadIdx.x < spb * F::TS;
-        C2<T> *b = buf + sq * F::N;
+        C2<T> *b = buf + sq * sb;
         C2<T> *tt = tb + (sq - base) * F::TB;
         if (on && t < R1) {
             C2<T> v[R2];
@@ -748,13 +748,13 @@ __device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
 
 // Inverse of fs_forward (unnormalised); only outputs n < n_out are written.
 template <class T, int R1, int R2>
-__device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out) {
+__device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out, int sb) {
     using F = FourStep<R1, R2>;
     const int spb = blockDim.x / F::TS;
     for (int base = 0; base < nseq; base += spb) {
         const int sq = base + threadIdx.x / F::TS, t = threadIdx.x % F::TS;
         const bool on = sq < nseq && (int)threadIdx.x < spb * F::TS;
-        C2<T> *b = buf + sq * F::N;
+        C2<T> *b = buf + sq * sb;
         C2<T> *tt = tb + (sq - base) * F::TB;
         if (on && t < R2) {
             C2<T> u[R1];
@@ -886,6 +886,7 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
     const int N = (int)A.ncell, bd = blockDim.x;
     const StageRow &R = M.row[NK > 0 ? s : 2];
     const double f = dt * R.scale;
+    const int sb = FAST ? A.xsb : px;
     T *__restrict__ yout = NK > 0 ? stage_output(A, M, s, p) : nullptr;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
@@ -955,7 +956,7 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
+            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * sb + x]);
             e[l & 1] = w * v[c];
         }
     }
@@ -965,8 +966,9 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
 template <class T, int R1, int R2>
 __device__ __forceinline__ void x_pack_tail(const Args<T> &A, C2<T> *b0) {
     const int px = R1 > 0 ? R1 * R2 : A.px, nx = px >> 1, nl = 3 * A.rows_per_wg;
+    const int sb = R1 > 0 ? A.xsb : px;
     if (nl & 1)
-        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * px + x].y = T(0);
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * sb + x].y = T(0);
 }
 
 template <class T, int R1, int R2, int NK>
@@ -992,20 +994,24 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1, nkx = nx + 1;
     __syncthreads();
     const C2<T> *res;
+    const int sb = FAST ? A.xsb : px;
     if constexpr (FAST) {
-        fs_forward<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx);
+        fs_forward<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
         res = b0;
     } else {
         res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
     }
+    // lanes take the lines of one kx together (line fastest): the rows of a
+    // component are consecutive y-points, so one store instruction writes
+    // runs of rows_per_wg values instead of single values a y-line apart
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
-    const int ly = A.ly, tot = 3 * A.rows_per_wg * nkx;
-    const float rnk = 1.0f / (float)nkx;
+    const int ly = A.ly, nl = 3 * A.rows_per_wg, tot = nl * nkx;
+    const float rnl = 1.0f / (float)nl;
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int l = small_div<FAST>(i, nkx, rnk), kx = i - l * nkx;
+        const int kx = small_div<false>(i, nl, rnl), l = i - kx * nl;
         const int off = xr.spec[l];
         if (off < 0) continue;
-        const C2<T> *zq = res + (l >> 1) * px;
+        const C2<T> *zq = res + (l >> 1) * sb;
         const C2<T> zk = zq[kx], zm = zq[kx == 0 ? 0 : px - kx];
         const C2<T> X = (l & 1) ? C2<T>{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)}
                                 : C2<T>{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
@@ -1015,86 +1021,52 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
 
 // Hermitian rebuild of the packed sequences from the half spectra (two real
 // output lines per complex sequence) and the C2R inverse FFT along x, only
-// the first nx outputs kept.  x_inv_issue loads the first chunk (CZ
-// elements per thread), x_inv_finish consumes it, rebuilds the remaining
-// chunks and transforms; the caller may issue other loads in between.
-template <class T, int CZ>
-struct InvIn {
-    C2<T> av[CZ], bv[CZ];
-    bool ma[CZ], mb[CZ];
-};
-
-template <class T, int R1, int R2, int CZ>
-__device__ __forceinline__ void x_inv_issue(const Args<T> &A, const XRows &xr, int base, InvIn<T, CZ> &in) {
+// the first nx outputs kept.  Lanes take the lines of one kx together (line
+// fastest, the two lines of a packed sequence in an even/odd lane pair, so
+// that the rows of a component are consecutive y-points of one load); each
+// half-spectrum value is loaded once, and the even lane writes Z = A + iB at
+// kx and, from the conjugates, at px - kx.
+template <class T, int R1, int R2>
+__device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
+                                                       const C2<T> *tw) {
     constexpr bool FAST = R1 > 0;
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int nl = 3 * A.rows_per_wg, ly = A.ly, bd = blockDim.x;
-    const float rpx = 1.0f / (float)px;
+    constexpr int CZ = 8;  // loads in flight per thread
+    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1, nkx = nx + 1;
+    const int sb = FAST ? A.xsb : px;
+    const int nlp = 2 * A.nseq_x, nl = 3 * A.rows_per_wg, ly = A.ly, bd = blockDim.x;
+    const int tot = nlp * nkx;  // even: lane pairs never straddle its end
+    const float rnl = 1.0f / (float)nlp;
     const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
-    const int nz_tot = A.nseq_x * px;
+    for (int base = 0; base < tot; base += CZ * bd) {
+        C2<T> v[CZ];
 #pragma unroll
-    for (int j = 0; j < CZ; ++j) {
-        const int i = base + j * bd + (int)threadIdx.x;
-        const int ic = i < nz_tot ? i : nz_tot - 1;
-        const int sq = small_div<FAST>(ic, px, rpx), k = ic - sq * px;
-        const int kk = k > nx ? px - k : k;
-        const int la = 2 * sq, lb = la + 1;
-        const int oa = xr.spec[la], ob = lb < nl ? xr.spec[lb] : -1;
-        in.ma[j] = oa >= 0;
-        in.mb[j] = ob >= 0;
-        in.av[j] = S[(oa < 0 ? 0 : oa) + kk * ly];
-        in.bv[j] = S[(ob < 0 ? 0 : ob) + kk * ly];
-    }
-}
-
-template <class T, int R1, int R2, int CZ>
-__device__ __forceinline__ void x_inv_put(const Args<T> &A, int base, const InvIn<T, CZ> &in, C2<T> *b0) {
-    constexpr bool FAST = R1 > 0;
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1, bd = blockDim.x;
-    const float rpx = 1.0f / (float)px;
-    const int nz_tot = A.nseq_x * px;
-#pragma unroll
-    for (int j = 0; j < CZ; ++j) {
-        const int i = base + j * bd + (int)threadIdx.x;
-        if (i >= nz_tot) continue;
-        const int sq = small_div<FAST>(i, px, rpx), k = i - sq * px;
-        C2<T> a = in.ma[j] ? in.av[j] : C2<T>{T(0), T(0)};
-        C2<T> b = in.mb[j] ? in.bv[j] : C2<T>{T(0), T(0)};
-        if (k > nx) {
-            a.y = -a.y;
-            b.y = -b.y;
+        for (int j = 0; j < CZ; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            const int ic = i < tot ? i : tot - 1;
+            const int kk = small_div<false>(ic, nlp, rnl), l = ic - kk * nlp;
+            const int off = l < nl ? xr.spec[l] : -1;
+            v[j] = off >= 0 ? S[off + kk * ly] : C2<T>{T(0), T(0)};
         }
-        b0[i] = {a.x - b.y, a.y + b.x};
-    }
-}
-
-template <class T, int R1, int R2, int CZ>
-__device__ __forceinline__ const C2<T> *x_inv_finish(const Args<T> &A, const XRows &xr, const InvIn<T, CZ> &first,
-                                                     C2<T> *b0, C2<T> *b1, const C2<T> *tw) {
-    constexpr bool FAST = R1 > 0;
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int nz_tot = A.nseq_x * px, step = CZ * (int)blockDim.x;
-    x_inv_put<T, R1, R2, CZ>(A, 0, first, b0);
-    for (int base = step; base < nz_tot; base += step) {
-        InvIn<T, CZ> in;
-        x_inv_issue<T, R1, R2, CZ>(A, xr, base, in);
-        x_inv_put<T, R1, R2, CZ>(A, base, in, b0);
+#pragma unroll
+        for (int j = 0; j < CZ; ++j) {
+            const int i = base + j * bd + (int)threadIdx.x;
+            // every lane takes part in the exchange (wave-uniform control flow)
+            const C2<T> b = {__shfl_xor(v[j].x, 1, 64), __shfl_xor(v[j].y, 1, 64)};
+            if (i >= tot || (threadIdx.x & 1)) continue;
+            const int kk = small_div<false>(i, nlp, rnl), l = i - kk * nlp;
+            const C2<T> a = v[j];
+            C2<T> *q = b0 + (l >> 1) * sb;
+            q[kk] = {a.x - b.y, a.y + b.x};
+            if (kk > 0 && kk < nx) q[px - kk] = {a.x + b.y, b.x - a.y};
+        }
     }
     __syncthreads();
     if constexpr (FAST) {
-        fs_inverse<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx);
+        fs_inverse<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
         return b0;
     } else {
         return lds_fft<T, true>(b0, b1, A.fx, tw, A.nseq_x, px, nx);
     }
-}
-
-template <class T, int R1, int R2>
-__device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
-                                                       const C2<T> *tw) {
-    InvIn<T, 8> in;
-    x_inv_issue<T, R1, R2, 8>(A, xr, 0, in);
-    return x_inv_finish<T, R1, R2, 8>(A, xr, in, b0, b1, tw);
 }
 
 // Local pass of stage s over the work-group's rows: demag field from the
@@ -1132,6 +1104,7 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
     T *kpush = stage_push(A, M, s);
     T *hrec = A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
+    const int sb = FAST ? A.xsb : px;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         const int i = base + j * bd + (int)threadIdx.x;
@@ -1143,7 +1116,7 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            const C2<T> zv = res[(l >> 1) * px + x];
+            const C2<T> zv = res[(l >> 1) * sb + x];
             hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
         }
         compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec, kk ? kk[j] : nullptr,
@@ -1174,8 +1147,8 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
     const double dt = ctl->dt;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    C2<T> *tw = b1 + A.nseq_x * A.xsb;
     __shared__ XRows xr;
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
@@ -1195,8 +1168,8 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     const double dt = ctl->dt;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    C2<T> *tw = b1 + A.nseq_x * A.xsb;
     __shared__ XRows xr;
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
@@ -1228,8 +1201,8 @@ __global__ void __launch_bounds__(kThreads) k_xx(const Args<T> *__restrict__ AP,
     const double dt = ctl->dt;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    C2<T> *tw = b1 + A.nseq_x * A.xsb;
     __shared__ XRows xr;
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
@@ -1526,15 +1499,16 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
 // of each ky hold all three components, so the symmetric 6-component real
 // tensor multiply H = -N M needs no exchange.  The first nz of the pz
 // z-points are read (the rest is the zero padding) and, after the inverse,
-// the first nz are written back in place.  LDS holds the padded transposes
-// as [component][element][ky] so that consecutive lanes (consecutive ky) hit
-// consecutive addresses.
+// the first nz are written back in place.  The three components pass one
+// at a time through one padded LDS transpose [element][ky] (consecutive
+// lanes, consecutive ky, hit consecutive addresses), so that a work-group
+// needs a third of the LDS and three fit on a CU; the z-point loads of all
+// three are issued before the first transform.
 template <class T, int R1, int R2, int UB>
 __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
-    constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
     const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
     const int ntile = (py + UB - 1) / UB;
     const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x - kx * ntile) * UB;
@@ -1545,32 +1519,40 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + (size_t)kx * ly + (on ? ky : py - 1);
     const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4z);
     C2<T> *tb = reinterpret_cast<C2<T> *>(fm_smem);
-    auto L = [&](int c, int i) -> C2<T> & { return tb[(c * TB + i) * UB + u]; };
-    // forward step 1: thread t < R1 holds z = t + R1 n2 (n2 < R2/2 non-zero)
+    auto L = [&](int i) -> C2<T> & { return tb[i * UB + u]; };
+    // every z-point load of the three components is issued first; thread
+    // t < R1 holds z = t + R1 n2 (n2 < R2/2 non-zero)
+    C2<T> raw[3][R2 / 2];
     if (t < R1) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int n2 = 0; n2 < R2 / 2; ++n2) raw[c][n2] = S[(size_t)(c * nzs + t + R1 * n2) * zs];
+    }
+    // forward, one component at a time through one transpose buffer: thread
+    // t < R2 ends with kz = t + R2 k1 of every component
+    C2<T> x[3][R1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (t < R1) {
             C2<T> v[R2];
 #pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2)
-                v[n2] = n2 < R2 / 2 ? S[(size_t)(c * nzs + t + R1 * n2) * zs] : C2<T>{T(0), T(0)};
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? raw[c][n2] : C2<T>{T(0), T(0)};
             DftReg<T, R2, false>::run(v);
 #pragma unroll
             for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
 #pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) L(c, k2 * P1 + t) = v[k2];
+            for (int k2 = 0; k2 < R2; ++k2) L(k2 * P1 + t) = v[k2];
         }
-    }
-    __syncthreads();
-    // forward step 2, multiply, inverse step A: thread t < R2 holds kz = t + R2 k1
-    C2<T> x[3][R1];
-    if (t < R2) {
+        __syncthreads();
+        if (t < R2) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-#pragma unroll
-            for (int t1 = 0; t1 < R1; ++t1) x[c][t1] = L(c, t * P1 + t1);
+            for (int t1 = 0; t1 < R1; ++t1) x[c][t1] = L(t * P1 + t1);
             DftReg<T, R1, false>::run(x[c]);
         }
+        __syncthreads();
+    }
+    if (t < R2) {
         const bool ylo = ky <= ny;
         const int kyr = ylo ? ky : (on ? py - ky : 0);
         const T sy = ylo ? T(1) : T(-1);
@@ -1601,27 +1583,26 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             }
         }
     }
-    __syncthreads();  // every read of the forward transpose is done
-    if (t < R2) {
+    // inverse step B per component: thread t < R1 holds z = t + R1 n2, the
+    // first nz kept
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
+        if (t < R2) {
 #pragma unroll
-            for (int t1 = 0; t1 < R1; ++t1) L(c, t1 * P2 + t) = x[c][t1];
-    }
-    __syncthreads();
-    // inverse step B: thread t < R1 holds z = t + R1 n2, the first nz kept
-    if (t < R1) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
+            for (int t1 = 0; t1 < R1; ++t1) L(t1 * P2 + t) = x[c][t1];
+        }
+        __syncthreads();
+        if (t < R1) {
             C2<T> v[R2];
 #pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) v[k2] = L(c, t * P2 + k2);
+            for (int k2 = 0; k2 < R2; ++k2) v[k2] = L(t * P2 + k2);
             DftReg<T, R2, true>::run(v);
             if (on) {
 #pragma unroll
                 for (int n2 = 0; n2 < R2 / 2; ++n2) S[(size_t)(c * nzs + t + R1 * n2) * zs] = v[n2];
             }
         }
+        if (c < 2) __syncthreads();
     }
 }
 
@@ -1907,14 +1888,15 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__
     constexpr bool FAST = R1 > 0;
     const AdjArgs<T> J = adj_bind(J0);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    C2<T> *tw = b1 + A.nseq_x * A.xsb;
     __shared__ XRows xr;
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int rows = A.rows_per_wg, ncl = rows * nx, nl = 3 * rows, N = (int)A.ncell;
+    const int sb = FAST ? A.xsb : px;
     const float rnx = 1.0f / (float)nx;
     for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
         const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
@@ -1928,12 +1910,12 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * px + x]);
+            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * sb + x]);
             e[l & 1] = (T)b[c];
         }
     }
     if (nl & 1)
-        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * px + x].y = T(0);
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * sb + x].y = T(0);
     x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
 }
 
@@ -1946,15 +1928,15 @@ __global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__
     const AdjArgs<T> J = adj_bind(J0);
     const Method &M = c_methods[A.method];
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.px;
-    C2<T> *tw = b1 + A.nseq_x * A.px;
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    C2<T> *tw = b1 + A.nseq_x * A.xsb;
     __shared__ XRows xr;
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
     const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int ncl = A.rows_per_wg * nx;
+    const int ncl = A.rows_per_wg * nx, sb = FAST ? A.xsb : px;
     const float rnx = 1.0f / (float)nx;
     const T scale = T(A.inv_p);
     const T wf = A.wave_kind ? wave_factor(A, J.t_n + M.node[j] * J.dt) : T(0);
@@ -1967,7 +1949,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            const C2<T> zv = res[(l >> 1) * px + x];
+            const C2<T> zv = res[(l >> 1) * sb + x];
             hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
         }
         adj_local_cell(A, J, M, j, c0 + x, x, xr.y[rr], xr.z[rr], hd, wf, amp);
@@ -2311,7 +2293,7 @@ template <class T, int R1, int R2>
 static size_t lds_zf() {
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
-    return 3ull * TB * zf_units<R1, R2>() * sizeof(C2<T>);
+    return 1ull * TB * zf_units<R1, R2>() * sizeof(C2<T>);
 }
 
 template <class T>
@@ -2322,7 +2304,7 @@ static size_t lds_x(const Args<T> &A) {
         const int ts = r1 > r2 ? r1 : r2;
         const long long tb = (long long)(r2 * (r1 + 1) > r1 * (r2 + 1) ? r2 * (r1 + 1) : r1 * (r2 + 1));
         const long long rounds = A.nseq_x < kThreads / ts ? A.nseq_x : kThreads / ts;
-        return ((long long)A.nseq_x * A.px + rounds * tb) * sizeof(C2<T>);
+        return ((long long)A.nseq_x * A.xsb + rounds * tb) * sizeof(C2<T>);
     }
     return (2ull * A.nseq_x + 1) * A.px * sizeof(C2<T>);
 }
@@ -2434,6 +2416,11 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
     auto fits_x = [&](int rows) {
         A.rows_per_wg = rows;
         A.nseq_x = (3 * rows + 1) / 2;
+        // register four-step passes: the packed sequences sit ceil(32/nseq)
+        // complex values further apart than px, so that the lanes of one kx
+        // in the spectrum load/store phases (one per sequence) fall in
+        // different LDS banks
+        A.xsb = A.xfast ? A.px + (32 + A.nseq_x - 1) / A.nseq_x : A.px;
         return lds_x(A) <= lds_limit;
     };
     int rows = 2;
