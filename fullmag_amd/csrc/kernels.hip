@@ -697,7 +697,7 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned g) {
 }
 
 // Register four-step FFT of `nseq` sequences of length N = R1*R2 held in
-// natural order in LDS `buf` (stride N), in place.  Thread t1 < R1 of a
+// natural order in LDS `buf` (sequence stride sb >= N), in place.  Thread t1 < R1 of a
 // sequence holds x[t1 + R1 n2] (inputs n >= n_in are zero), does the
 // R2-point DFT in VGPRs, applies W_N^{t1 k2} (tw4[k2*R1 + t1]) and writes a
 // padded transpose to `tb`; thread k2 < R2 then does the R1-point DFT and
@@ -1184,6 +1184,15 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     if (control) finish_last_block(A, control);
 }
 
+// Waves per SIMD the joined x pass is compiled for: FP32 with the uniform
+// exchange and px <= 1024 fits 168 VGPRs (three work-groups per CU,
+// measured 11% faster on a 256x256x64 block than two); FP64, the region
+// table form and px = 2048 would spill.
+template <class T, bool LUT, int R1, int R2>
+constexpr int xx_waves() {
+    return sizeof(T) == 4 && !LUT && R1 * R2 <= 1024 ? 3 : 1;
+}
+
 // Stage s (not the last of the step) ends and stage s+1 begins in one
 // launch: the combination of stage s+1 is pointwise in k_1..k_s, and the
 // work-group owns all cells of its rows, so the inverse x pass, local pass
@@ -1193,7 +1202,7 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
 // register footprint doubles and the 3-D grid loses two thirds of its
 // occupancy.)
 template <class T, bool LUT, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads) k_xx(const Args<T> *__restrict__ AP, int s) {
+__global__ void __launch_bounds__(kThreads, (xx_waves<T, LUT, R1, R2>())) k_xx(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
@@ -1511,6 +1520,8 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
     const int ntile = (py + UB - 1) / UB;
+    // (pairing tiles j and ntile-1-j, which read the same tensor records,
+    // was measured slower than consecutive tiles)
     const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x - kx * ntile) * UB;
     const int u = threadIdx.x % UB, t = threadIdx.x / UB;
     const int ky = ky0 + u;
