@@ -705,9 +705,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     // tuner may change the tile later
     const long long nparts = std::max<long long>(A.demag ? A.nrows : 0, fm::num_partials(A));
     A.partials = static_cast<double *>(dalloc(h, (size_t)nparts * fm::kStatSlots * sizeof(double)));
-    A.counter = static_cast<unsigned *>(dalloc(h, 64));
-    if (!A.partials || !A.counter) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
-    if (!hip_ok(h, hipMemset(A.counter, 0, 64), "clear counter")) return false;
+    if (!A.partials) return hip_ok(h, hipErrorOutOfMemory, "alloc partials");
     for (int i = 0; i < 3; ++i)
         if (!(h->args_d[i] = dalloc(h, sizeof(fm::Args<T>)))) return hip_ok(h, hipErrorOutOfMemory, "alloc arguments");
     if (!put_args(h, A, 0)) return false;

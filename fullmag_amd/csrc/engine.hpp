@@ -144,7 +144,6 @@ struct Args {
     const T *lut;           // 256 x 256 A_ij when region != NULL
     Ctl *ctl;
     double *partials;       // [nblocks][kStatSlots]
-    unsigned *counter;      // finished work-groups of the current partial-producing launch
     // observables (written only when non-NULL)
     T *obs_hex, *obs_hd, *obs_heff;
     // adaptive methods with Method::efuse: sum over the final row of

@@ -1157,10 +1157,10 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
     x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
 }
 
-// Inverse x pass and local pass of stage s; the last work-group reduces the
-// partials and runs the controller (control 1) or stores statistics (2).
+// Inverse x pass and local pass of stage s, with the work-group partials of
+// the statistics / error norm when asked (k_control reduces them).
 template <class T, bool LUT, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats, int control) {
+__global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
@@ -1181,7 +1181,6 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     acc.init();
     x_local<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, res, stats != 0, err, acc);
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
-    if (control) finish_last_block(A, control);
 }
 
 // Waves per SIMD the joined x pass is compiled for: FP32 with the uniform
@@ -1981,7 +1980,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ 
 }
 
 template <class T, bool LUT>
-__global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats, int control) {
+__global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
@@ -2005,7 +2004,6 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
                              nullptr, stage_push(A, M, s));
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
-    if (control) finish_last_block(A, control);
 }
 
 // Reduction of all per-block partials and the step controller, executed by
@@ -2016,11 +2014,11 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     Acc acc;
     acc.init();
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        // written by the previous launch: the kernel boundary made them visible
         const double *q = A.partials + (long long)i * kStatSlots;
 #pragma unroll
         for (int k = 0; k < kStatSlots; ++k) {
-            // agent-scope loads: the partials were written by other CUs
-            const double v = __hip_atomic_load(q + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double v = q[k];
             acc.v[k] = slot_is_max(k) ? fmax(acc.v[k], v) : acc.v[k] + v;
         }
     }
@@ -2082,33 +2080,17 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     }
 }
 
-// Stand-alone controller launch (used for timing and by callers that keep
-// the partial-producing kernel and the controller apart).
+// Reduction of the partials and the step controller (observe != 0: only the
+// statistics), one work-group per system, launched after the
+// partial-producing kernel: the kernel boundary makes every work-group's
+// partials visible.  (Fusing it into that kernel's last work-group needs a
+// device-scope release in every work-group, which writes back the XCD's L2
+// each time; measured 11% of a 256x256x64 step.)
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_control(const Args<T> *__restrict__ AP, int nparts, int observe) {
     const Args<T> &A = AP[blockIdx.y];
     if (A.ctl->done) return;
     control_block(A, nparts, observe);
-}
-
-// End of a partial-producing kernel: the last work-group to finish reduces
-// the partials of the whole grid and runs the controller (control 1) or
-// stores the statistics (control 2), then re-arms the counter for the next
-// launch.  Every work-group of the grid must reach this point.
-template <class T>
-__device__ void finish_last_block(const Args<T> &A, int control) {
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(A.counter, 1u);
-        is_last = prev == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    control_block(A, (int)gridDim.x, control == 2);
-    if (threadIdx.x == 0) atomicExch(A.counter, 0u);
 }
 
 // ------------------------------------------------------------ host side --
@@ -2527,10 +2509,10 @@ static void launch_fwd_x_shape(const Args<T> &A, int s, hipStream_t st) {
     }
 }
 template <class T, int R1, int R2>
-static void launch_inv_x_shape(const Args<T> &A, int s, int stats, int control, hipStream_t st) {
+static void launch_inv_x_shape(const Args<T> &A, int s, int stats, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
-    if (A.region) k_inv_x<T, true, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats, control);
-    else k_inv_x<T, false, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats, control);
+    if (A.region) k_inv_x<T, true, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats);
+    else k_inv_x<T, false, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats);
 }
 template <class T, bool LUT, int R1, int R2>
 static void launch_xx_lut(const Args<T> &A, int s, hipStream_t st) {
@@ -2575,15 +2557,15 @@ static void launch_fwd_x(const Args<T> &A, int s, hipStream_t st) {
     }
 }
 template <class T>
-static void launch_inv_x(const Args<T> &A, int s, int stats, int control, hipStream_t st) {
+static void launch_inv_x(const Args<T> &A, int s, int stats, hipStream_t st) {
     switch (A.xfast) {
-        case 64: launch_inv_x_shape<T, 8, 8>(A, s, stats, control, st); break;
-        case 128: launch_inv_x_shape<T, 8, 16>(A, s, stats, control, st); break;
-        case 256: launch_inv_x_shape<T, 16, 16>(A, s, stats, control, st); break;
-        case 512: launch_inv_x_shape<T, 16, 32>(A, s, stats, control, st); break;
-        case 1024: launch_inv_x_shape<T, 32, 32>(A, s, stats, control, st); break;
-        case 2048: launch_inv_x_shape<T, 64, 32>(A, s, stats, control, st); break;
-        default: launch_inv_x_shape<T, 0, 0>(A, s, stats, control, st); break;
+        case 64: launch_inv_x_shape<T, 8, 8>(A, s, stats, st); break;
+        case 128: launch_inv_x_shape<T, 8, 16>(A, s, stats, st); break;
+        case 256: launch_inv_x_shape<T, 16, 16>(A, s, stats, st); break;
+        case 512: launch_inv_x_shape<T, 16, 32>(A, s, stats, st); break;
+        case 1024: launch_inv_x_shape<T, 32, 32>(A, s, stats, st); break;
+        case 2048: launch_inv_x_shape<T, 64, 32>(A, s, stats, st); break;
+        default: launch_inv_x_shape<T, 0, 0>(A, s, stats, st); break;
     }
 }
 template <class T, int R1, int R2>
@@ -2646,8 +2628,8 @@ static void launch_yz(const Args<T> &A, hipStream_t st) {
 }
 
 // One RHS evaluation of stage s (1 = the current state m0).  control:
-// 0 none, 1 step controller, 2 statistics only (fused into the last
-// work-group of the final kernel; implies per-block statistics).
+// 0 none, 1 step controller, 2 statistics only (k_control after the final
+// kernel; implies per-block statistics).
 template <class T>
 hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
     const int stats = control != 0;
@@ -2655,18 +2637,19 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
     if (A.demag) {
         launch_fwd_x(A, s, st);
         launch_yz(A, st);
-        launch_inv_x(A, s, stats, control, st);
+        launch_inv_x(A, s, stats, st);
     } else {
         if (s >= 2) k_stage<T><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s);
-        if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats, control);
-        else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats, control);
+        if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats);
+        else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats);
     }
+    if (control) k_control<T><<<bgrid(A, dim3(1)), dim3(kThreads), 0, st>>>(A.self_d, (int)gpart, control == 2);
     return hipGetLastError();
 }
 
-// Stages 2..nstage of one trial step, the controller fused into the last
-// launch.  With demag, the inverse x pass of stage s and the forward x pass
-// of stage s+1 share one launch.
+// Stages 2..nstage of one trial step and the controller.  With demag, the
+// inverse x pass of stage s and the forward x pass of stage s+1 share one
+// launch.
 template <class T>
 hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
     if (!A.demag) {
@@ -2680,8 +2663,9 @@ hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
     for (int s = 2; s <= nstage; ++s) {
         launch_yz(A, st);
         if (s < nstage) launch_xx(A, s, st);
-        else launch_inv_x(A, s, 1, 1, st);
+        else launch_inv_x(A, s, 1, st);
     }
+    k_control<T><<<bgrid(A, dim3(1)), dim3(kThreads), 0, st>>>(A.self_d, (int)num_partials(A), 0);
     return hipGetLastError();
 }
 
@@ -2824,10 +2808,10 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             break;
         case 4:
             if (A.demag) {
-                launch_inv_x(A, s, 1, 0, st);
+                launch_inv_x(A, s, 1, st);
             } else {
-                if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1, 0);
-                else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1, 0);
+                if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1);
+                else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1);
             }
             break;
         case 5:
