@@ -37,10 +37,14 @@ METRIC_3D = "Mcell-steps/s (RK45, exch+demag+anisotropy+ramp) at 256²×64, fp32
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# on the film workload (FP32, stage 2; FETCH_SIZE scaled by 2.0, WRITE_SIZE by
-# 1.0, the factors a plain 256 MiB device copy gives); the passes and their
-# commands are recorded in profiles/film512_dp45_hbm_traffic.md
-MEASURED_TRAFFIC_FP32 = {"film512": {0: 16.01e6, 1: 17.53e6, 4: 13.71e6, 6: 29.87e6}}
+# (FP32, the stage-2 launches that the kernel timings use; FETCH_SIZE scaled
+# by 2.0, WRITE_SIZE by 1.0, the factors a plain 256 MiB device copy gives);
+# the passes and their commands are recorded in
+# profiles/film512_dp45_hbm_traffic.md and profiles/granular3d_dp45_profile.md
+MEASURED_TRAFFIC_FP32 = {
+    "film512": {0: 16.12e6, 1: 17.42e6, 4: 13.64e6, 6: 31.83e6},
+    "granular3d": {0: 252.5e6, 1: 303.3e6, 2: 658.9e6, 3: 303.3e6, 4: 340.3e6, 6: 669.7e6},
+}
 
 NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xx"}
 NAMES_3D = {0: "k_fwd_x", 1: "k_y (forward)", 2: "k_z", 3: "k_y (inverse)", 4: "k_inv_x", 6: "k_xx"}
