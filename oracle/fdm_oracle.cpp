@@ -22,6 +22,16 @@
 //                           history: crates/fullmag-engine/src/fdm_state.rs:204-262
 //   error norm ........... crates/fullmag-engine/src/fdm_integrators.rs:828-870
 //   neighbour clamp ...... crates/fullmag-engine/src/fdm_types.rs:126-132
+//   cubic anisotropy ..... crates/fullmag-engine/src/fdm_fields.rs:648-705 (Kc1, Kc2);
+//                          Kc3 term native/backends/fdm/src/demag_fp64.cu:301-330
+//   interfacial/bulk DMI . crates/fullmag-engine/src/fdm_fields.rs:707-816
+//   magnetoelastic ....... crates/fullmag-engine/src/magnetoelastic.rs:52-75,155-176
+//   Oersted cylinder ..... crates/fullmag-engine/src/fdm_fields.rs:1049-1150 (geometry);
+//                          time envelopes native/backends/fdm/src/demag_fp64.cu:628-650
+//   Zhang-Li / Slonczewski / SOT torques
+//                          crates/fullmag-engine/src/fdm_fields.rs:1446-1680, added to
+//                          every RHS as native/backends/fdm/src/llg_fp64.cu:58-205 does
+//                          (the CPU integrators leave them out of the stage RHS)
 //   runner time loop ..... crates/fullmag-runner/src/cpu_reference.rs:448-455,622-639
 //   random unit vectors .. crates/fullmag-plan/src/util.rs:118-147
 //   hole mask ............ crates/fullmag-plan/src/geometry.rs:159-231
@@ -345,6 +355,31 @@ typedef struct {
     double wave_amp[3];
     double wave_t0, wave_t1, wave_freq, wave_phase;
     const uint8_t *wave_mask;
+    // cubic anisotropy: energy Kc1 S + Kc2 P + Kc3 S^2 in the frame c1, c2,
+    // c3 = c1 x c2 (S = sum_{i<j} m_i^2 m_j^2, P = m1^2 m2^2 m3^2); per-cell
+    // constants override the uniform ones
+    int has_cub;
+    double kc1, kc2, kc3, cub_axis1[3], cub_axis2[3];
+    const double *kc1_cell, *kc2_cell, *kc3_cell;
+    // DMI constants (J/m^2), 0 = off
+    double dmi_i, dmi_b;
+    // magnetoelastic coupling with a uniform prescribed strain, Voigt order
+    // [e11, e22, e33, 2e23, 2e13, 2e12]
+    int has_mel;
+    double mel_b1, mel_b2, mel_strain[6];
+    // current density (A/m^2) of the Zhang-Li and Slonczewski torques,
+    // spin polarisation degree, non-adiabaticity; Slonczewski polarisation
+    // axis, lambda and epsilon' (layer thickness dz)
+    double j[3], stt_degree, stt_beta, slon_p[3], slon_lambda, slon_eps;
+    // spin-orbit torque
+    int has_sot;
+    double sot_je, sot_xi_dl, sot_xi_fl, sot_sigma[3], sot_thickness;
+    // Oersted field of an infinite cylinder; envelope 0 constant,
+    // 1 sin(2 pi f t + phase) + offset, 2 pulse [t_on, t_off)
+    int has_oe;
+    double oe_current, oe_radius, oe_center[3], oe_axis[3];
+    int oe_kind;
+    double oe_freq, oe_phase, oe_offset, oe_t_on, oe_t_off;
 } orc_desc;
 
 typedef struct {
@@ -352,6 +387,7 @@ typedef struct {
     double e_ex, e_demag, e_ext, e_ani, e_total;
     double max_heff, max_hdemag, max_rhs;
     uint64_t rejected;
+    double e_cub, e_dmi, e_mel;  // cubic anisotropy, DMI, magnetoelastic (in e_total)
 } orc_report;
 
 }  // extern "C"
@@ -363,8 +399,9 @@ struct Problem {
     size_t n;
     std::vector<uint8_t> mask;
     bool has_mask;
-    std::vector<double> ms_c, ku1_c, ku2_c, axis_c;
+    std::vector<double> ms_c, ku1_c, ku2_c, axis_c, kc1_c, kc2_c, kc3_c;
     std::vector<uint8_t> wmask;
+    std::vector<V3> oe_prof;  // Oersted field at the full current, per cell
     double t_eval = 0.0;  // time at which the Zeeman waveform is evaluated
     // demag workspace (fdm_fft.rs:64-131)
     int px = 0, py = 0, pz = 0;
@@ -498,13 +535,183 @@ void uni_add(const Problem &p, const V3 *m, V3 *h) {
     }
 }
 
-// fdm_fields.rs:1153-1183 (MEL/cubic/DMI/Oersted/thermal are off on this path)
+V3 unit_or_zero(const double *a) {
+    V3 u = {{a[0], a[1], a[2]}};
+    return v_scale(u, 1.0 / std::max(v_norm(u), 1e-30));
+}
+
+// Cubic anisotropy, fdm_fields.rs:648-705 (axes normalised, c3 = c1 x c2),
+// plus the Kc3 S^2 term of demag_fp64.cu:301-330.
+void cubic_add(const Problem &p, const V3 *m, V3 *h) {
+    if (!p.d.has_cub) return;
+    const V3 c1 = unit_or_zero(p.d.cub_axis1), c2 = unit_or_zero(p.d.cub_axis2), c3 = v_cross(c1, c2);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const double ms = std::max(p.ms_at(s), 1e-30);
+        const double k1 = p.kc1_c.empty() ? p.d.kc1 : p.kc1_c[s];
+        const double k2 = p.kc2_c.empty() ? p.d.kc2 : p.kc2_c[s];
+        const double k3 = p.kc3_c.empty() ? p.d.kc3 : p.kc3_c[s];
+        const double m1 = v_dot(m[s], c1), m2 = v_dot(m[s], c2), m3 = v_dot(m[s], c3);
+        const double a1 = m1 * m1, a2 = m2 * m2, a3 = m3 * m3;
+        const double sig = a1 * a2 + a2 * a3 + a1 * a3;
+        const double pf = 2.0 / (MU0 * ms), pf3 = 4.0 * k3 / (MU0 * ms);
+        const double g1 = -pf * (k1 * m1 * (a2 + a3) + k2 * m1 * a2 * a3) - pf3 * sig * m1 * (a2 + a3);
+        const double g2 = -pf * (k1 * m2 * (a1 + a3) + k2 * m2 * a1 * a3) - pf3 * sig * m2 * (a1 + a3);
+        const double g3 = -pf * (k1 * m3 * (a1 + a2) + k2 * m3 * a1 * a2) - pf3 * sig * m3 * (a1 + a2);
+        for (int k = 0; k < 3; ++k) h[s].c[k] += g1 * c1.c[k] + g2 * c2.c[k] + g3 * c3.c[k];
+    }
+}
+
+// Interfacial and bulk DMI, fdm_fields.rs:707-816: central differences over
+// box-clamped neighbours (fdm_types.rs:126-132), neighbour values as stored
+// (zero on empty cells); H_i = 2D/(mu0 Ms) (dz mz... ), H_b = -2D/(mu0 Ms) curl m.
+void dmi_add(const Problem &p, const V3 *m, V3 *h) {
+    if (p.d.dmi_i == 0.0 && p.d.dmi_b == 0.0) return;
+    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    const double i2x = 1.0 / (2.0 * p.d.dx), i2y = 1.0 / (2.0 * p.d.dy), i2z = 1.0 / (2.0 * p.d.dz);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const double ms = std::max(p.ms_at(s), 1e-30);
+        const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
+        auto at = [&](long xi, long yi, long zi) { return m[((size_t)zi * ny + yi) * nx + xi]; };
+        const V3 xm = at(std::max(x - 1, 0L), y, z), xp = at(std::min(x + 1, nx - 1), y, z);
+        const V3 ym = at(x, std::max(y - 1, 0L), z), yp = at(x, std::min(y + 1, ny - 1), z);
+        const V3 zm = at(x, y, std::max(z - 1, 0L)), zp = at(x, y, std::min(z + 1, nz - 1));
+        if (p.d.dmi_i != 0.0) {
+            const double pf = 2.0 * p.d.dmi_i / (MU0 * ms);
+            h[s].c[0] += pf * (xp.c[2] - xm.c[2]) * i2x;
+            h[s].c[1] += pf * (yp.c[2] - ym.c[2]) * i2y;
+            h[s].c[2] += -pf * ((xp.c[0] - xm.c[0]) * i2x + (yp.c[1] - ym.c[1]) * i2y);
+        }
+        if (p.d.dmi_b != 0.0) {
+            const double pf = -2.0 * p.d.dmi_b / (MU0 * ms);
+            h[s].c[0] += pf * ((yp.c[2] - ym.c[2]) * i2y - (zp.c[1] - zm.c[1]) * i2z);
+            h[s].c[1] += pf * ((zp.c[0] - zm.c[0]) * i2z - (xp.c[2] - xm.c[2]) * i2x);
+            h[s].c[2] += pf * ((xp.c[1] - xm.c[1]) * i2x - (yp.c[0] - ym.c[0]) * i2y);
+        }
+    }
+}
+
+// Magnetoelastic field and energy density, magnetoelastic.rs:52-75,155-176.
+V3 mel_field(const Problem &p, V3 m, double ms) {
+    const double *e = p.d.mel_strain;
+    const double e12 = 0.5 * e[5], e13 = 0.5 * e[4], e23 = 0.5 * e[3];
+    const double b1 = p.d.mel_b1, b2 = p.d.mel_b2, inv = -1.0 / (MU0 * ms);
+    return {{inv * (2.0 * b1 * m.c[0] * e[0] + 2.0 * b2 * (m.c[1] * e12 + m.c[2] * e13)),
+             inv * (2.0 * b1 * m.c[1] * e[1] + 2.0 * b2 * (m.c[0] * e12 + m.c[2] * e23)),
+             inv * (2.0 * b1 * m.c[2] * e[2] + 2.0 * b2 * (m.c[0] * e13 + m.c[1] * e23))}};
+}
+double mel_density(const Problem &p, V3 m) {
+    const double *e = p.d.mel_strain;
+    const double e12 = 0.5 * e[5], e13 = 0.5 * e[4], e23 = 0.5 * e[3];
+    return p.d.mel_b1 * (m.c[0] * m.c[0] * e[0] + m.c[1] * m.c[1] * e[1] + m.c[2] * m.c[2] * e[2]) +
+           2.0 * p.d.mel_b2 * (m.c[0] * m.c[1] * e12 + m.c[0] * m.c[2] * e13 + m.c[1] * m.c[2] * e23);
+}
+void mel_add(const Problem &p, const V3 *m, V3 *h) {
+    if (!p.d.has_mel) return;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s)
+        if (p.active(s)) h[s] = v_add(h[s], mel_field(p, m[s], std::max(p.ms_at(s), 1e-30)));
+}
+
+// Oersted profile at the full current, fdm_fields.rs:1049-1150 (cell
+// centres, radial distance from the axis line, azimuthal axis x r_hat).
+void oe_profile(Problem &p) {
+    const long nx = p.d.nx, ny = p.d.ny;
+    const V3 ax = unit_or_zero(p.d.oe_axis);
+    const double I = p.d.oe_current, R = p.d.oe_radius;
+    p.oe_prof.assign(p.n, V3{{0.0, 0.0, 0.0}});
+    for (size_t s = 0; s < p.n; ++s) {
+        if (!p.active(s)) continue;
+        const long x = (long)s % nx, y = ((long)s / nx) % ny, z = (long)s / (nx * ny);
+        const V3 r0 = {{(x + 0.5) * p.d.dx - p.d.oe_center[0], (y + 0.5) * p.d.dy - p.d.oe_center[1],
+                        (z + 0.5) * p.d.dz - p.d.oe_center[2]}};
+        const double proj = v_dot(r0, ax);
+        const V3 r = v_add(r0, v_scale(ax, -proj));
+        const double rp = v_norm(r);
+        if (rp < 1e-30) continue;
+        const V3 phi = v_cross(ax, v_scale(r, 1.0 / rp));
+        const double hm = rp <= R ? I * rp / (2.0 * PI * R * R) : I / (2.0 * PI * rp);
+        p.oe_prof[s] = v_scale(phi, hm);
+    }
+}
+double oe_envelope(const Problem &p, double t) {
+    switch (p.d.oe_kind) {
+        case 1: return std::sin(2.0 * PI * p.d.oe_freq * t + p.d.oe_phase) + p.d.oe_offset;
+        case 2: return (t >= p.d.oe_t_on && t < p.d.oe_t_off) ? 1.0 : 0.0;
+        default: return 1.0;
+    }
+}
+void oe_add(const Problem &p, V3 *h) {
+    if (p.oe_prof.empty()) return;
+    const double f = oe_envelope(p, p.t_eval);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) h[s] = v_add(h[s], v_scale(p.oe_prof[s], f));
+}
+
+// Zhang-Li (upwind differences), Slonczewski and spin-orbit torques,
+// fdm_fields.rs:1446-1680, activation as native/backends/fdm/src/api.cpp:250-303.
+void torque_add(const Problem &p, const V3 *m, V3 *k) {
+    const double jm = std::sqrt(p.d.j[0] * p.d.j[0] + p.d.j[1] * p.d.j[1] + p.d.j[2] * p.d.j[2]);
+    const bool zl = jm > 0.0 && p.d.stt_degree > 0.0;
+    const double pp = v_dot(V3{{p.d.slon_p[0], p.d.slon_p[1], p.d.slon_p[2]}}, V3{{p.d.slon_p[0], p.d.slon_p[1], p.d.slon_p[2]}});
+    const bool sl = pp > 0.0 && p.d.slon_lambda > 0.0 && jm > 0.0;
+    const bool so = p.d.has_sot && p.d.sot_je != 0.0 && p.d.sot_thickness > 0.0;
+    if (!zl && !sl && !so) return;
+    constexpr double MU_B = 9.274009994e-24, E_CH = 1.60217662e-19, HBAR = 1.054571817e-34;
+    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    const V3 sg = unit_or_zero(p.d.sot_sigma);
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const double ms = std::max(p.ms_at(s), 1e-30);
+        const V3 mm = m[s];
+        if (zl) {
+            const double beta = p.d.stt_beta;
+            const double b = p.d.stt_degree * MU_B / (E_CH * ms * (1.0 + beta * beta));
+            const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
+            const long st[3] = {1, nx, nx * ny}, pos[3] = {x, y, z}, len[3] = {nx, ny, nz};
+            const double dd[3] = {p.d.dx, p.d.dy, p.d.dz};
+            V3 du = {{0.0, 0.0, 0.0}};
+            for (int a = 0; a < 3; ++a) {
+                const double u = b * p.d.j[a];
+                if (u > 0.0 && pos[a] > 0) du = v_add(du, v_scale(v_add(mm, v_scale(m[s - st[a]], -1.0)), u / dd[a]));
+                else if (u < 0.0 && pos[a] + 1 < len[a]) du = v_add(du, v_scale(v_add(m[s + st[a]], v_scale(mm, -1.0)), u / dd[a]));
+            }
+            const V3 c = v_cross(mm, du), dc = v_cross(mm, c);
+            k[s] = v_add(k[s], v_add(v_scale(dc, -1.0), v_scale(c, -beta)));
+        }
+        if (sl) {
+            const V3 pv = {{p.d.slon_p[0], p.d.slon_p[1], p.d.slon_p[2]}};
+            const double pre = jm * HBAR / (2.0 * E_CH * MU0 * ms * std::max(p.d.dz, 1e-30));
+            const double l2 = p.d.slon_lambda * p.d.slon_lambda;
+            const double deg = p.d.stt_degree > 0.0 ? p.d.stt_degree : 1.0;
+            const double g = deg * l2 / ((l2 + 1.0) + (l2 - 1.0) * v_dot(mm, pv));
+            const V3 mp = v_cross(mm, pv), mmp = v_cross(mm, mp);
+            k[s] = v_add(k[s], v_scale(v_add(mmp, v_scale(mp, p.d.slon_eps)), pre * g));
+        }
+        if (so) {
+            const double amp = std::fabs(p.d.sot_je) * HBAR / (2.0 * E_CH * MU0 * ms * p.d.sot_thickness);
+            const V3 ms_ = v_cross(mm, sg), mms = v_cross(mm, ms_);
+            k[s] = v_add(k[s], v_scale(v_add(v_scale(mms, -p.d.sot_xi_dl), v_scale(ms_, p.d.sot_xi_fl)), amp));
+        }
+    }
+}
+
+// fdm_fields.rs:1153-1183; the Oersted envelope and the torques of
+// torque_add as the reference's GPU engine applies them.
 void effective_field(Problem &p, const V3 *m, V3 *h) {
     std::fill(h, h + p.n, V3{{0.0, 0.0, 0.0}});
     if (p.d.exchange) exchange_add(p, m, h);
     if (p.d.demag) demag_add(p, m, h);
+    mel_add(p, m, h);
     ext_add(p, h);
     uni_add(p, m, h);
+    cubic_add(p, m, h);
+    dmi_add(p, m, h);
+    oe_add(p, h);
 }
 
 inline V3 llg(const Problem &p, V3 m, V3 h) {
@@ -520,6 +727,7 @@ void rhs_eval(Problem &p, const V3 *m, V3 *k) {
     effective_field(p, m, p.heff.data());
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) k[s] = llg(p, m[s], p.heff[s]);
+    torque_add(p, m, k);
 }
 
 double max_norm(const V3 *v, size_t n) {
@@ -561,14 +769,33 @@ void observables(Problem &p, const V3 *m, orc_report *r) {
         r->e_ext = energy_field_form(p, m, p.hs.data(), -1.0);
         for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
     }
-    uni_add(p, m, p.heff.data());
+    // uniaxial, cubic and DMI energies in field form (-1/2 mu0 Ms m.H V),
+    // magnetoelastic from its density; all of them in the total
     std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
     uni_add(p, m, p.hs.data());
     r->e_ani = energy_field_form(p, m, p.hs.data(), -0.5);
+    for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+    cubic_add(p, m, p.hs.data());
+    r->e_cub = energy_field_form(p, m, p.hs.data(), -0.5);
+    for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    std::fill(p.hs.begin(), p.hs.end(), V3{{0.0, 0.0, 0.0}});
+    dmi_add(p, m, p.hs.data());
+    r->e_dmi = energy_field_form(p, m, p.hs.data(), -0.5);
+    for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
+    mel_add(p, m, p.heff.data());
+    oe_add(p, p.heff.data());
+    r->e_mel = 0.0;
+    if (p.d.has_mel) {
+        const double vol = p.d.dx * p.d.dy * p.d.dz;
+        for (size_t s = 0; s < n; ++s)
+            if (p.active(s)) r->e_mel += mel_density(p, m[s]) * vol;
+    }
     r->max_heff = max_norm(p.heff.data(), n);
     for (size_t s = 0; s < n; ++s) p.rhs[s] = llg(p, m[s], p.heff[s]);
+    torque_add(p, m, p.rhs.data());
     r->max_rhs = max_norm(p.rhs.data(), n);
-    r->e_total = r->e_ex + r->e_demag + r->e_ext + r->e_ani;
+    r->e_total = r->e_ex + r->e_demag + r->e_ext + r->e_ani + r->e_cub + r->e_dmi + r->e_mel;
 }
 
 }  // namespace
@@ -647,8 +874,13 @@ void *orc_create(const orc_desc *d) {
     if (d->ku2_cell) p->ku2_c.assign(d->ku2_cell, d->ku2_cell + p->n);
     if (d->axis_cell) p->axis_c.assign(d->axis_cell, d->axis_cell + 3 * p->n);
     if (d->wave_mask) p->wmask.assign(d->wave_mask, d->wave_mask + p->n);
+    if (d->kc1_cell) p->kc1_c.assign(d->kc1_cell, d->kc1_cell + p->n);
+    if (d->kc2_cell) p->kc2_c.assign(d->kc2_cell, d->kc2_cell + p->n);
+    if (d->kc3_cell) p->kc3_c.assign(d->kc3_cell, d->kc3_cell + p->n);
     p->d.ms_cell = p->d.ku1_cell = p->d.ku2_cell = p->d.axis_cell = nullptr;
+    p->d.kc1_cell = p->d.kc2_cell = p->d.kc3_cell = nullptr;
     p->d.wave_mask = nullptr;
+    if (d->has_oe && d->oe_radius > 0.0) oe_profile(*p);
     const size_t n = p->n;
     p->m0.resize(n);
     p->ms.resize(n);

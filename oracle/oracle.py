@@ -48,6 +48,21 @@ class _Desc(ctypes.Structure):
         ("wave_kind", ctypes.c_int), ("wave_amp", ctypes.c_double * 3),
         ("wave_t0", ctypes.c_double), ("wave_t1", ctypes.c_double), ("wave_freq", ctypes.c_double),
         ("wave_phase", ctypes.c_double), ("wave_mask", ctypes.POINTER(ctypes.c_uint8)),
+        ("has_cub", ctypes.c_int), ("kc1", ctypes.c_double), ("kc2", ctypes.c_double), ("kc3", ctypes.c_double),
+        ("cub_axis1", ctypes.c_double * 3), ("cub_axis2", ctypes.c_double * 3),
+        ("kc1_cell", ctypes.POINTER(ctypes.c_double)), ("kc2_cell", ctypes.POINTER(ctypes.c_double)),
+        ("kc3_cell", ctypes.POINTER(ctypes.c_double)),
+        ("dmi_i", ctypes.c_double), ("dmi_b", ctypes.c_double),
+        ("has_mel", ctypes.c_int), ("mel_b1", ctypes.c_double), ("mel_b2", ctypes.c_double),
+        ("mel_strain", ctypes.c_double * 6),
+        ("j", ctypes.c_double * 3), ("stt_degree", ctypes.c_double), ("stt_beta", ctypes.c_double),
+        ("slon_p", ctypes.c_double * 3), ("slon_lambda", ctypes.c_double), ("slon_eps", ctypes.c_double),
+        ("has_sot", ctypes.c_int), ("sot_je", ctypes.c_double), ("sot_xi_dl", ctypes.c_double),
+        ("sot_xi_fl", ctypes.c_double), ("sot_sigma", ctypes.c_double * 3), ("sot_thickness", ctypes.c_double),
+        ("has_oe", ctypes.c_int), ("oe_current", ctypes.c_double), ("oe_radius", ctypes.c_double),
+        ("oe_center", ctypes.c_double * 3), ("oe_axis", ctypes.c_double * 3), ("oe_kind", ctypes.c_int),
+        ("oe_freq", ctypes.c_double), ("oe_phase", ctypes.c_double), ("oe_offset", ctypes.c_double),
+        ("oe_t_on", ctypes.c_double), ("oe_t_off", ctypes.c_double),
     ]
 
 
@@ -58,6 +73,7 @@ class Report(ctypes.Structure):
         ("e_ani", ctypes.c_double), ("e_total", ctypes.c_double),
         ("max_heff", ctypes.c_double), ("max_hdemag", ctypes.c_double), ("max_rhs", ctypes.c_double),
         ("rejected", ctypes.c_uint64),
+        ("e_cub", ctypes.c_double), ("e_dmi", ctypes.c_double), ("e_mel", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -194,6 +210,22 @@ class Problem:
     wave_freq: float = 0.0
     wave_phase: float = 0.0
     wave_mask: np.ndarray | None = None
+    # further terms of the plan descriptor (fullmag_fdm.h:117-178)
+    cubic: tuple | None = None          # (Kc1, Kc2, Kc3, axis1, axis2)
+    kc1_cell: np.ndarray | None = None
+    kc2_cell: np.ndarray | None = None
+    kc3_cell: np.ndarray | None = None
+    dmi_i: float = 0.0
+    dmi_b: float = 0.0
+    mel: tuple | None = None            # (B1, B2, Voigt strain[6])
+    j: tuple = (0.0, 0.0, 0.0)          # current density (A/m^2)
+    stt_degree: float = 0.0
+    stt_beta: float = 0.0
+    slon_p: tuple = (0.0, 0.0, 0.0)
+    slon_lambda: float = 0.0
+    slon_eps: float = 0.0
+    sot: tuple | None = None            # (je, xi_dl, xi_fl, sigma[3], thickness)
+    oersted: tuple | None = None        # (current, radius, center[3], axis[3], kind, freq, phase, offset, t_on, t_off)
 
 
 class Oracle:
@@ -231,6 +263,40 @@ class Oracle:
             wm = np.ascontiguousarray(pb.wave_mask, dtype=np.uint8).ravel()
             self._keep.append(wm)
             d.wave_mask = wm.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        if pb.cubic is not None:
+            d.has_cub = 1
+            d.kc1, d.kc2, d.kc3 = pb.cubic[0], pb.cubic[1], pb.cubic[2]
+            d.cub_axis1[:] = list(pb.cubic[3])
+            d.cub_axis2[:] = list(pb.cubic[4])
+            for nm in ("kc1_cell", "kc2_cell", "kc3_cell"):
+                a = getattr(pb, nm)
+                if a is not None:
+                    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+                    self._keep.append(a)
+                    setattr(d, nm, _ptr(a))
+        d.dmi_i, d.dmi_b = pb.dmi_i, pb.dmi_b
+        if pb.mel is not None:
+            d.has_mel = 1
+            d.mel_b1, d.mel_b2 = pb.mel[0], pb.mel[1]
+            d.mel_strain[:] = list(pb.mel[2])
+        d.j[:] = list(pb.j)
+        d.stt_degree, d.stt_beta = pb.stt_degree, pb.stt_beta
+        d.slon_p[:] = list(pb.slon_p)
+        d.slon_lambda, d.slon_eps = pb.slon_lambda, pb.slon_eps
+        if pb.sot is not None:
+            d.has_sot = 1
+            d.sot_je, d.sot_xi_dl, d.sot_xi_fl = pb.sot[0], pb.sot[1], pb.sot[2]
+            d.sot_sigma[:] = list(pb.sot[3])
+            d.sot_thickness = pb.sot[4]
+        if pb.oersted is not None:
+            o = pb.oersted
+            d.has_oe = 1
+            d.oe_current, d.oe_radius = o[0], o[1]
+            d.oe_center[:] = list(o[2])
+            d.oe_axis[:] = list(o[3])
+            d.oe_kind = int(o[4]) if len(o) > 4 else 0
+            if len(o) > 5:
+                d.oe_freq, d.oe_phase, d.oe_offset, d.oe_t_on, d.oe_t_off = o[5:10]
         self.pb = pb
         self.n = pb.nx * pb.ny * pb.nz
         self.h = L.orc_create(ctypes.byref(d))
