@@ -62,6 +62,9 @@ class FdmPlan:
     headroom: float = 0.0
     ku1_field: np.ndarray | None = None      # per-cell Ku1 (reference ABI ku1_field)
     ku2_field: np.ndarray | None = None
+    kc1_field: np.ndarray | None = None      # per-cell cubic Kc1..Kc3 (reference ABI kc*_field)
+    kc2_field: np.ndarray | None = None
+    kc3_field: np.ndarray | None = None
     # plan extension (include/fullmag_fdm_ext.h)
     ms_field: np.ndarray | None = None       # per-cell Ms
     axis_field: np.ndarray | None = None     # (n,3) per-cell uniaxial axis
@@ -145,7 +148,7 @@ def marshal_plan(plan: FdmPlan):
                 setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
                 ln = a.size
         desc.demag_kernel_spectrum_len = ln
-    for nm in ("ku1_field", "ku2_field"):
+    for nm in ("ku1_field", "ku2_field", "kc1_field", "kc2_field", "kc3_field"):
         a = getattr(plan, nm)
         if a is not None:
             a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))

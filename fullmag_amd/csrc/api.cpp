@@ -288,21 +288,140 @@ int select_device(Backend *h) {
 
 std::string unsupported_terms(const fullmag_fdm_plan_desc *p) {
     std::vector<std::string> bad;
-    if (p->has_cubic_anisotropy) bad.push_back("cubic anisotropy");
-    if (p->kc1_field || p->kc2_field || p->kc3_field) bad.push_back("per-cell cubic constants");
-    if (p->has_interfacial_dmi && p->dmi_D_interfacial != 0.0) bad.push_back("interfacial DMI");
-    if (p->has_bulk_dmi && p->dmi_D_bulk != 0.0) bad.push_back("bulk DMI");
-    if (p->has_magnetoelastic) bad.push_back("magnetoelastic coupling");
     if (p->temperature > 0.0) bad.push_back("thermal field");
-    if (p->current_density_x != 0.0 || p->current_density_y != 0.0 || p->current_density_z != 0.0)
-        bad.push_back("spin-transfer torque");
-    if (p->has_sot) bad.push_back("spin-orbit torque");
-    if (p->has_oersted_cylinder) bad.push_back("Oersted field");
     if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE) bad.push_back("boundary correction tiers");
     if (p->has_demag_boundary_corr) bad.push_back("sparse demag boundary correction");
     std::string s;
     for (size_t i = 0; i < bad.size(); ++i) s += (i ? ", " : "") + bad[i];
     return s;
+}
+
+// Cubic anisotropy, DMI, magnetoelastic coupling, Zhang-Li / Slonczewski /
+// spin-orbit torques and the Oersted cylinder: activation and prefactors as
+// native/backends/fdm/src/api.cpp:220-318 (physical constants as there);
+// prefactors exclude 1/Ms, applied per cell by the kernels.
+template <class T>
+bool parse_further_terms(Backend *h, fm::Args<T> &A, const fullmag_fdm_plan_desc *p) {
+    constexpr double kMuB = 9.274009994e-24, kE = 1.60217662e-19, kHbar = 1.054571817e-34;
+    const auto unit = [](const double *a, double *o) {
+        const double n = std::max(std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]), 1e-30);
+        for (int c = 0; c < 3; ++c) o[c] = a[c] / n;
+    };
+    const uint64_t n = h->n;
+    if (p->has_cubic_anisotropy) {
+        double c1[3], c2[3];
+        unit(p->cubic_axis1, c1);
+        unit(p->cubic_axis2, c2);
+        const double c3[3] = {c1[1] * c2[2] - c1[2] * c2[1], c1[2] * c2[0] - c1[0] * c2[2], c1[0] * c2[1] - c1[1] * c2[0]};
+        for (int c = 0; c < 3; ++c) {
+            A.cub_ax[c] = (T)c1[c];
+            A.cub_ax[3 + c] = (T)c2[c];
+            A.cub_ax[6 + c] = (T)c3[c];
+        }
+        A.kc1_u = (T)p->cubic_Kc1;
+        A.kc2_u = (T)p->cubic_Kc2;
+        A.kc3_u = (T)p->cubic_Kc3;
+        A.has_cub = 1;
+        const double *src[3] = {p->kc1_field, p->kc2_field, p->kc3_field};
+        const T **dst[3] = {&A.kc1_cell, &A.kc2_cell, &A.kc3_cell};
+        for (int q = 0; q < 3; ++q) {
+            if (!src[q]) continue;
+            std::vector<T> v(src[q], src[q] + n);
+            T *d = static_cast<T *>(dalloc(h, n * sizeof(T)));
+            if (!d || hipMemcpy(d, v.data(), n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+                return (h->err = "upload cubic constant field", false);
+            *dst[q] = d;
+        }
+    }
+    if (p->has_interfacial_dmi) A.dmi_i = (T)p->dmi_D_interfacial;
+    if (p->has_bulk_dmi) A.dmi_b = (T)p->dmi_D_bulk;
+    A.i2x = (T)(0.5 / p->grid.dx);
+    A.i2y = (T)(0.5 / p->grid.dy);
+    A.i2z = (T)(0.5 / p->grid.dz);
+    A.i1x = (T)(1.0 / p->grid.dx);
+    A.i1y = (T)(1.0 / p->grid.dy);
+    A.i1z = (T)(1.0 / p->grid.dz);
+    if (p->has_magnetoelastic) {
+        // Voigt [e11 e22 e33 2e23 2e13 2e12] -> tensor components
+        const double *e = p->mel_strain;
+        const double t[6] = {e[0], e[1], e[2], 0.5 * e[3], 0.5 * e[4], 0.5 * e[5]};
+        for (int i = 0; i < 6; ++i) A.mel_e[i] = (T)t[i];
+        A.mel_b1 = (T)p->mel_b1;
+        A.mel_b2 = (T)p->mel_b2;
+        A.has_mel = 1;
+    }
+    const double j[3] = {p->current_density_x, p->current_density_y, p->current_density_z};
+    const double jm = std::sqrt(j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+    if (jm > 0.0 && p->stt_degree > 0.0) {
+        const double beta = p->stt_beta;
+        const double b = p->stt_degree * kMuB / (kE * (1.0 + beta * beta));
+        for (int c = 0; c < 3; ++c) A.zl_u[c] = (T)(b * j[c]);
+        A.zl_beta = (T)beta;
+        A.has_zl = 1;
+    }
+    const double pp = p->stt_p_x * p->stt_p_x + p->stt_p_y * p->stt_p_y + p->stt_p_z * p->stt_p_z;
+    if (pp > 0.0 && p->stt_lambda > 0.0 && jm > 0.0) {
+        A.sl_p[0] = (T)p->stt_p_x;
+        A.sl_p[1] = (T)p->stt_p_y;
+        A.sl_p[2] = (T)p->stt_p_z;
+        A.sl_l2 = (T)(p->stt_lambda * p->stt_lambda);
+        A.sl_deg = (T)(p->stt_degree > 0.0 ? p->stt_degree : 1.0);
+        A.sl_eps = (T)p->stt_epsilon_prime;
+        A.sl_pf = (T)(jm * kHbar / (2.0 * kE * kMu0 * p->grid.dz));
+        A.has_slon = 1;
+    }
+    if (p->has_sot) {
+        if (!(p->sot_thickness > 0.0)) return (h->err = "sot_thickness must be positive", false);
+        if (p->sot_je != 0.0) {
+            double sg[3];
+            unit(p->sot_sigma, sg);
+            for (int c = 0; c < 3; ++c) A.sot_s[c] = (T)sg[c];
+            A.sot_pf = (T)(std::fabs(p->sot_je) * kHbar / (2.0 * kE * kMu0 * p->sot_thickness));
+            A.sot_dl = (T)p->sot_xi_dl;
+            A.sot_fl = (T)p->sot_xi_fl;
+            A.has_sot = 1;
+        }
+    }
+    if (p->has_oersted_cylinder) {
+        if (!(p->oersted_radius > 0.0)) return (h->err = "oersted_radius must be positive", false);
+        if (p->oersted_time_dep_kind > 2) return (h->err = "oersted_time_dep_kind must be 0, 1 or 2", false);
+        // profile at the full current (fdm_fields.rs:1049-1150 geometry:
+        // cell centres, distance from the axis line, azimuthal axis x r_hat)
+        double ax[3];
+        unit(p->oersted_axis, ax);
+        const double I = p->oersted_current, R = p->oersted_radius;
+        std::vector<T> prof(3 * n, (T)0);
+        const uint64_t nx = h->nx, ny = h->ny;
+        for (uint64_t i = 0; i < n; ++i) {  // (empty cells never read it)
+            const uint64_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+            const double r0[3] = {(x + 0.5) * p->grid.dx - p->oersted_center[0],
+                                  (y + 0.5) * p->grid.dy - p->oersted_center[1],
+                                  (z + 0.5) * p->grid.dz - p->oersted_center[2]};
+            const double pr = r0[0] * ax[0] + r0[1] * ax[1] + r0[2] * ax[2];
+            const double r[3] = {r0[0] - pr * ax[0], r0[1] - pr * ax[1], r0[2] - pr * ax[2]};
+            const double rp = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            if (rp < 1e-30) continue;
+            const double hm = rp <= R ? I * rp / (2.0 * M_PI * R * R) : I / (2.0 * M_PI * rp);
+            const double phi[3] = {(ax[1] * r[2] - ax[2] * r[1]) / rp, (ax[2] * r[0] - ax[0] * r[2]) / rp,
+                                   (ax[0] * r[1] - ax[1] * r[0]) / rp};
+            for (int c = 0; c < 3; ++c) prof[c * n + i] = (T)(phi[c] * hm);
+        }
+        T *d = static_cast<T *>(dalloc(h, 3 * n * sizeof(T)));
+        if (!d || hipMemcpy(d, prof.data(), prof.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return (h->err = "upload Oersted field", false);
+        A.oe_field = d;
+        A.oe_kind = (int)p->oersted_time_dep_kind;
+        A.oe_freq = p->oersted_time_dep_freq;
+        A.oe_phase = p->oersted_time_dep_phase;
+        A.oe_offset = p->oersted_time_dep_offset;
+        A.oe_t_on = p->oersted_time_dep_t_on;
+        A.oe_t_off = p->oersted_time_dep_t_off;
+    }
+    A.xt = (A.has_cub || A.dmi_i != (T)0 || A.dmi_b != (T)0 || A.has_mel || A.has_zl || A.has_slon || A.has_sot ||
+            A.oe_field)
+               ? 1
+               : 0;
+    return true;
 }
 
 template <class T>
@@ -407,8 +526,12 @@ void fill_stats(Backend *h, fullmag_fdm_step_stats *s, double dt, uint64_t wall_
     s->demag_energy_joules = c.stats[fm::ST_EDEMAG];
     s->external_energy_joules = c.stats[fm::ST_EEXT];
     s->anisotropy_energy_joules = c.stats[fm::ST_EANI];
-    s->total_energy_joules =
-        c.stats[fm::ST_EEX] + c.stats[fm::ST_EDEMAG] + c.stats[fm::ST_EEXT] + c.stats[fm::ST_EANI];
+    s->cubic_energy_joules = c.stats[fm::ST_ECUB];
+    s->dmi_energy_joules = c.stats[fm::ST_EDMI];
+    // every evaluated term, magnetoelastic included (it has no field of its own here)
+    s->total_energy_joules = c.stats[fm::ST_EEX] + c.stats[fm::ST_EDEMAG] + c.stats[fm::ST_EEXT] +
+                             c.stats[fm::ST_EANI] + c.stats[fm::ST_ECUB] + c.stats[fm::ST_EDMI] +
+                             c.stats[fm::ST_EMEL];
     s->max_effective_field_amplitude = std::sqrt(c.stats[fm::ST_HEFF2]);
     s->max_demag_field_amplitude = std::sqrt(c.stats[fm::ST_HD2]);
     s->max_rhs_amplitude = std::sqrt(c.stats[fm::ST_RHS2]);
@@ -472,6 +595,8 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.shrink = 0.0;
     A.mu0_vol = kMu0 * p->grid.dx * p->grid.dy * p->grid.dz;
     A.two_mu0inv = (T)(2.0 / kMu0);
+    A.cell_vol = p->grid.dx * p->grid.dy * p->grid.dz;
+    if (!parse_further_terms(h, A, p)) return false;
     if (x) {
         if (x->adaptive_rtol < 0 || x->adaptive_shrink_limit < 0) {
             h->err = "invalid adaptive tolerances in plan extension";
@@ -1752,6 +1877,9 @@ int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *h, const fullmag_fdm_adjoin
         return fail(h, FULLMAG_FDM_ERR_INVALID, "fullmag_fdm_adjoint_desc.struct_size too small");
     if (h->adaptive || h->multistep)
         return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint needs a one-step fixed-step integrator (rk4 or heun)");
+    if (h->fp64 ? h->ad.xt : h->af.xt)
+        return fail(h, FULLMAG_FDM_ERR_INVALID,
+                    "adjoint covers exchange, demag, uniaxial anisotropy and Zeeman terms only");
     if (d->steps == 0 || !(d->dt > 0.0) || !std::isfinite(d->dt))
         return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint: steps must be >= 1 and dt positive and finite");
     if (d->objective_first_step < 1 || d->objective_first_step > d->steps)
@@ -1806,7 +1934,7 @@ using Batch = fullmag_fdm_batch;
 template <class T>
 std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
     return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
-            A.ylfast, A.zfast, A.ty, A.nseq_x};
+            A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt};
 }
 
 template <class T>

@@ -25,7 +25,7 @@ constexpr int kThreads = 256;       // work-group size of every kernel
 constexpr int kMaxRowsWg = 4;       // x-rows owned by one x-pass work-group
 constexpr int kMaxPasses = 24;      // FFT passes per length
 constexpr int kMaxRadix = 64;       // largest prime factor of a padded length
-constexpr int kStatSlots = 8;       // per-block partial reductions
+constexpr int kStatSlots = 11;      // per-block partial reductions
 constexpr int kMaxStages = 8;
 constexpr int kMaxK = 5;            // k-vectors in one stage combination (DP45 rows 6, 7)
 constexpr int kMaxErr = 6;          // terms of the embedded error estimate (DP45)
@@ -42,6 +42,9 @@ enum StatSlot {
     ST_HEFF2 = 5,     // max |H_eff|^2 (max)
     ST_HD2 = 6,       // max |H_d|^2 (max)
     ST_RHS2 = 7,      // max |dm/dt|^2 (max)
+    ST_ECUB = 8,      // cubic anisotropy energy (sum)
+    ST_EDMI = 9,      // DMI energy (sum)
+    ST_EMEL = 10,     // magnetoelastic energy (sum)
 };
 
 struct FftDesc {
@@ -183,6 +186,28 @@ struct Args {
     int wave_kind;
     double wave_amp[3], wave_t0, wave_t1, wave_freq, wave_phase;
     const uint8_t *wave_mask;
+    // further terms (the kernels' XT instantiations evaluate them; xt = 0
+    // when all are off).  Prefactors exclude 1/Ms, applied per cell.
+    int xt;
+    int has_cub;            // cubic anisotropy in the frame cub_ax = (c1, c2, c3)
+    T cub_ax[9];
+    T kc1_u, kc2_u, kc3_u;
+    const T *kc1_cell, *kc2_cell, *kc3_cell;
+    T dmi_i, dmi_b;         // interfacial / bulk D (J/m^2), 0 = off
+    T i2x, i2y, i2z;        // 1/(2dx) ... (DMI central differences)
+    T i1x, i1y, i1z;        // 1/dx ... (Zhang-Li upwind differences)
+    int has_mel;            // magnetoelastic, uniform tensor strain e11 e22 e33 e23 e13 e12
+    T mel_b1, mel_b2, mel_e[6];
+    int has_zl;             // Zhang-Li: u = zl_u / Ms, non-adiabaticity zl_beta
+    T zl_u[3], zl_beta;
+    int has_slon;           // Slonczewski: pf/Ms * g(m.p) (m x (m x p) + eps m x p)
+    T sl_p[3], sl_l2, sl_deg, sl_eps, sl_pf;
+    int has_sot;            // spin-orbit: pf/Ms (-xi_dl m x (m x s) + xi_fl m x s)
+    T sot_s[3], sot_pf, sot_dl, sot_fl;
+    const T *oe_field;      // Oersted field at the full current, 3 x ncell SoA (NULL = off)
+    int oe_kind;            // envelope: 0 constant, 1 sin(2 pi f t + phase) + offset, 2 pulse [t_on, t_off)
+    double oe_freq, oe_phase, oe_offset, oe_t_on, oe_t_off;
+    double cell_vol;        // dx dy dz
     // step control
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
     int method;             // index into the constant method table

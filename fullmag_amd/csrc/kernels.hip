@@ -473,6 +473,23 @@ __device__ __forceinline__ T wave_factor(const Args<T> &A, double t) {
     }
 }
 
+// Time-dependent factors of one stage: Zeeman waveform and Oersted current
+// envelope (native/backends/fdm/src/demag_fp64.cu:628-650 forms).
+template <class T>
+struct TimeF {
+    T wave, oe;
+};
+template <class T>
+__device__ __forceinline__ TimeF<T> time_factors(const Args<T> &A, double t) {
+    TimeF<T> f;
+    f.wave = A.wave_kind ? wave_factor(A, t) : T(0);
+    double e = 1.0;
+    if (A.oe_kind == 1) e = sin(2.0 * 3.14159265358979323846 * A.oe_freq * t + A.oe_phase) + A.oe_offset;
+    else if (A.oe_kind == 2) e = (t >= A.oe_t_on && t < A.oe_t_off) ? 1.0 : 0.0;
+    f.oe = T(e);
+    return f;
+}
+
 // Evaluation time of stage s of the trial step from the control block.
 template <class T>
 __device__ __forceinline__ double stage_time(const Args<T> &A, const Method &M, int s, const Ctl *ctl, double dt) {
@@ -488,15 +505,18 @@ struct CellIn {
     T m[3];           // state at the cell
     T v[6][3];        // box-clamped neighbour states (-x,+x,-y,+y,-z,+z)
     T ms, ku1, ku2, ax[3];
+    T kc[3];          // cubic constants (further terms)
+    T oe[3];          // Oersted field at the full current (further terms)
     T m0[3];          // state at the start of the step (mixed tolerance)
     double ev[3];     // sum of the error terms before the final stage (Args::errv)
     unsigned char act, nact[6], wm;
     uint32_t rc, rn[6];
 };
 
-template <class T, bool LUT>
+template <class T, int FT>
 __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, int p, int cell, int x, int y, int z,
                                             const T *__restrict__ yin, bool err, CellIn<T> &g) {
+    constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
     const int N = (int)A.ncell;
     const int sy = A.nx, sz = A.nx * A.ny;
     const int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
@@ -519,6 +539,17 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
         for (int c = 0; c < 3; ++c) g.ax[c] = A.axis_cell ? A.axis_cell[c * N + cell] : A.axis[c];
     }
     g.wm = A.wave_mask ? A.wave_mask[cell] : 1;
+    if constexpr (XT) {
+        if (A.has_cub) {
+            g.kc[0] = A.kc1_cell ? A.kc1_cell[cell] : A.kc1_u;
+            g.kc[1] = A.kc2_cell ? A.kc2_cell[cell] : A.kc2_u;
+            g.kc[2] = A.kc3_cell ? A.kc3_cell[cell] : A.kc3_u;
+        }
+        if (A.oe_field) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g.oe[c] = A.oe_field[c * N + cell];
+        }
+    }
     if (LUT) {
         g.rc = A.region[cell];
 #pragma unroll
@@ -538,11 +569,13 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 }
 
 // Local field terms, LLG torque, and (optionally) statistics for one cell.
-template <class T, bool LUT>
+template <class T, int FT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
-                                             const CellIn<T> &g, T *__restrict__ kout, const T hd[3], T wf,
+                                             const CellIn<T> &g, T *__restrict__ kout, const T hd[3], TimeF<T> tf,
                                              bool stats, bool err, Acc &acc, T *__restrict__ hrec,
                                              T kret[3] = nullptr, T *__restrict__ kpush = nullptr) {
+    constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
+    const T wf = tf.wave;
     const int N = (int)A.ncell;
     if (!g.act) {
         kout[cell] = T(0);
@@ -614,12 +647,108 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     T h[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) h[c] = hex[c] + hd[c] + hext[c] + hani[c];
+    // further terms: cubic anisotropy (fdm_fields.rs:648-705, Kc3 as
+    // demag_fp64.cu:301-330), magnetoelastic (magnetoelastic.rs:52-75), DMI
+    // over the box-clamped neighbour values as stored (fdm_fields.rs:707-816),
+    // Oersted field times the current envelope
+    T hcub[3] = {T(0), T(0), T(0)}, hdmi[3] = {T(0), T(0), T(0)};
+    double e_mel = 0.0;
+    if constexpr (XT) {
+        const T inv_ms = T(1) / ms_i;
+        if (A.has_cub) {
+            const T *c1 = A.cub_ax, *c2 = A.cub_ax + 3, *c3 = A.cub_ax + 6;
+            const T m1 = m[0] * c1[0] + m[1] * c1[1] + m[2] * c1[2];
+            const T m2 = m[0] * c2[0] + m[1] * c2[1] + m[2] * c2[2];
+            const T m3 = m[0] * c3[0] + m[1] * c3[1] + m[2] * c3[2];
+            const T a1 = m1 * m1, a2 = m2 * m2, a3 = m3 * m3;
+            const T sig = a1 * a2 + a2 * a3 + a1 * a3;
+            const T pf = A.two_mu0inv * inv_ms, pf3 = T(2) * pf * g.kc[2] * sig;
+            const T g1 = -(pf * (g.kc[0] * m1 * (a2 + a3) + g.kc[1] * m1 * a2 * a3) + pf3 * m1 * (a2 + a3));
+            const T g2 = -(pf * (g.kc[0] * m2 * (a1 + a3) + g.kc[1] * m2 * a1 * a3) + pf3 * m2 * (a1 + a3));
+            const T g3 = -(pf * (g.kc[0] * m3 * (a1 + a2) + g.kc[1] * m3 * a1 * a2) + pf3 * m3 * (a1 + a2));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hcub[c] = g1 * c1[c] + g2 * c2[c] + g3 * c3[c];
+        }
+        if (A.has_mel) {
+            const T *e = A.mel_e;  // e11 e22 e33 e23 e13 e12
+            const T pf = -T(0.5) * A.two_mu0inv * inv_ms, b1 = T(2) * A.mel_b1, b2 = T(2) * A.mel_b2;
+            h[0] += pf * (b1 * m[0] * e[0] + b2 * (m[1] * e[5] + m[2] * e[4]));
+            h[1] += pf * (b1 * m[1] * e[1] + b2 * (m[0] * e[5] + m[2] * e[3]));
+            h[2] += pf * (b1 * m[2] * e[2] + b2 * (m[0] * e[4] + m[1] * e[3]));
+            if (stats) {
+                const double mx = m[0], my = m[1], mz = m[2];
+                e_mel = A.cell_vol * ((double)A.mel_b1 * (mx * mx * e[0] + my * my * e[1] + mz * mz * e[2]) +
+                                      2.0 * (double)A.mel_b2 * (mx * my * e[5] + mx * mz * e[4] + my * mz * e[3]));
+            }
+        }
+        if (A.dmi_i != T(0)) {
+            const T pf = A.two_mu0inv * A.dmi_i * inv_ms;
+            hdmi[0] += pf * (g.v[1][2] - g.v[0][2]) * A.i2x;
+            hdmi[1] += pf * (g.v[3][2] - g.v[2][2]) * A.i2y;
+            hdmi[2] -= pf * ((g.v[1][0] - g.v[0][0]) * A.i2x + (g.v[3][1] - g.v[2][1]) * A.i2y);
+        }
+        if (A.dmi_b != T(0)) {
+            const T pf = -A.two_mu0inv * A.dmi_b * inv_ms;
+            hdmi[0] += pf * ((g.v[3][2] - g.v[2][2]) * A.i2y - (g.v[5][1] - g.v[4][1]) * A.i2z);
+            hdmi[1] += pf * ((g.v[5][0] - g.v[4][0]) * A.i2z - (g.v[1][2] - g.v[0][2]) * A.i2x);
+            hdmi[2] += pf * ((g.v[1][1] - g.v[0][1]) * A.i2x - (g.v[3][0] - g.v[2][0]) * A.i2y);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) h[c] += hcub[c] + hdmi[c] + (A.oe_field ? tf.oe * g.oe[c] : T(0));
+    }
     // LLG: k = -gb (m x H + alpha m x (m x H))
     const T pr[3] = {m[1] * h[2] - m[2] * h[1], m[2] * h[0] - m[0] * h[2], m[0] * h[1] - m[1] * h[0]};
     const T dm[3] = {m[1] * pr[2] - m[2] * pr[1], m[2] * pr[0] - m[0] * pr[2], m[0] * pr[1] - m[1] * pr[0]};
     T k[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) k[c] = -A.gamma_bar * ((A.precession ? pr[c] : T(0)) + A.alpha * dm[c]);
+    // torques added to every RHS (native/backends/fdm/src/llg_fp64.cu:58-205;
+    // forms of fdm_fields.rs:1446-1680)
+    if constexpr (XT) {
+        const T inv_ms = T(1) / ms_i;
+        auto cross = [](const T *a, const T *b, T *o) {
+            o[0] = a[1] * b[2] - a[2] * b[1];
+            o[1] = a[2] * b[0] - a[0] * b[2];
+            o[2] = a[0] * b[1] - a[1] * b[0];
+        };
+        if (A.has_zl) {
+            // upwind (u.grad) m; a box face gives a zero difference, as the
+            // reference's skipped term
+            T du[3] = {T(0), T(0), T(0)};
+            const T inv_d[3] = {A.i1x, A.i1y, A.i1z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T u = A.zl_u[a] * inv_ms;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    du[c] += u > T(0) ? u * (m[c] - g.v[2 * a][c]) * inv_d[a]
+                                      : (u < T(0) ? u * (g.v[2 * a + 1][c] - m[c]) * inv_d[a] : T(0));
+            }
+            T c1[3], c2[3];
+            cross(m, du, c1);
+            cross(m, c1, c2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) k[c] += -c2[c] - A.zl_beta * c1[c];
+        }
+        if (A.has_slon) {
+            const T mp_dot = m[0] * A.sl_p[0] + m[1] * A.sl_p[1] + m[2] * A.sl_p[2];
+            const T gg = A.sl_deg * A.sl_l2 / ((A.sl_l2 + T(1)) + (A.sl_l2 - T(1)) * mp_dot);
+            const T b = A.sl_pf * inv_ms * gg;
+            T c1[3], c2[3];
+            cross(m, A.sl_p, c1);
+            cross(m, c1, c2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) k[c] += b * (c2[c] + A.sl_eps * c1[c]);
+        }
+        if (A.has_sot) {
+            const T amp = A.sot_pf * inv_ms;
+            T c1[3], c2[3];
+            cross(m, A.sot_s, c1);
+            cross(m, c1, c2);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) k[c] += amp * (-A.sot_dl * c2[c] + A.sot_fl * c1[c]);
+        }
+    }
     kout[cell] = k[0];
     kout[N + cell] = k[1];
     kout[2 * N + cell] = k[2];
@@ -657,6 +786,11 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         acc.v[ST_EDEMAG] += -0.5 * ep * dotd(hd);
         acc.v[ST_EEXT] += -1.0 * ep * dotd(hext);
         acc.v[ST_EANI] += -0.5 * ep * dotd(hani);
+        if constexpr (XT) {
+            acc.v[ST_ECUB] += -0.5 * ep * dotd(hcub);
+            acc.v[ST_EDMI] += -0.5 * ep * dotd(hdmi);
+            acc.v[ST_EMEL] += e_mel;
+        }
         acc.v[ST_HEFF2] = fmax(acc.v[ST_HEFF2], sq(h));
         acc.v[ST_HD2] = fmax(acc.v[ST_HD2], sq(hd));
         acc.v[ST_RHS2] = fmax(acc.v[ST_RHS2], sq(k));
@@ -1073,7 +1207,7 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
 // inverse x FFT (1/P scale), exchange, Zeeman, anisotropy, LLG torque,
 // optional statistics and error norm.  x_local_load issues the loads of one
 // chunk of cells, x_local_put consumes it (and returns the chunk's k).
-template <class T, bool LUT, int R1, int R2>
+template <class T, int FT, int R1, int R2>
 __device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, int p, const T *yin,
                                              const XRows &xr, int base, bool err, CellIn<T> *g) {
     constexpr bool FAST = R1 > 0;
@@ -1087,12 +1221,12 @@ __device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, 
         const int ic = i < ncl ? i : ncl - 1;
         const int rr = small_div<FAST>(ic, nx, rnx), x = ic - rr * nx;
         const int c0 = xr.cell0[rr];
-        gather_cell<T, LUT>(A, M, p, (c0 < 0 ? 0 : c0) + x, x, xr.y[rr], xr.z[rr], yin, err, g[j]);
+        gather_cell<T, FT>(A, M, p, (c0 < 0 ? 0 : c0) + x, x, xr.y[rr], xr.z[rr], yin, err, g[j]);
     }
 }
 
-template <class T, bool LUT, int R1, int R2>
-__device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
+template <class T, int FT, int R1, int R2>
+__device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, int s, int p, double dt, TimeF<T> tf,
                                             const XRows &xr, int base, const CellIn<T> *g, const C2<T> *res,
                                             bool stats, bool err, Acc &acc, T (*kk)[3]) {
     constexpr bool FAST = R1 > 0;
@@ -1119,21 +1253,21 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
             const C2<T> zv = res[(l >> 1) * sb + x];
             hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
         }
-        compute_cell<T, LUT>(A, M, c0 + x, dt, g[j], kout, hd, wf, stats, err, acc, hrec, kk ? kk[j] : nullptr,
+        compute_cell<T, FT>(A, M, c0 + x, dt, g[j], kout, hd, tf, stats, err, acc, hrec, kk ? kk[j] : nullptr,
                              kpush);
     }
 }
 
-template <class T, bool LUT, int R1, int R2>
-__device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, T wf,
+template <class T, int FT, int R1, int R2>
+__device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, TimeF<T> tf,
                                         const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
     constexpr int CH = Chunk<T>::value;
     const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
     const T *yin = stage_input(A, M, s, p);
     for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
         CellIn<T> g[CH];
-        x_local_load<T, LUT, R1, R2>(A, M, p, yin, xr, base, err, g);
-        x_local_put<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, base, g, res, stats, err, acc, nullptr);
+        x_local_load<T, FT, R1, R2>(A, M, p, yin, xr, base, err, g);
+        x_local_put<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, base, g, res, stats, err, acc, nullptr);
     }
 }
 
@@ -1159,7 +1293,7 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
 
 // Inverse x pass and local pass of stage s, with the work-group partials of
 // the statistics / error norm when asked (k_control reduces them).
-template <class T, bool LUT, int R1, int R2>
+template <class T, int FT, int R1, int R2>
 __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
@@ -1176,20 +1310,20 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     __syncthreads();
     const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
     const bool err = M.adaptive && s == M.nstage;
-    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+    const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     Acc acc;
     acc.init();
-    x_local<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, res, stats != 0, err, acc);
+    x_local<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, res, stats != 0, err, acc);
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
 }
 
 // Waves per SIMD the joined x pass is compiled for: FP32 with the uniform
-// exchange and px <= 1024 fits 168 VGPRs (three work-groups per CU,
-// measured 11% faster on a 256x256x64 block than two); FP64, the region
-// table form and px = 2048 would spill.
-template <class T, bool LUT, int R1, int R2>
+// exchange, no further terms and px <= 1024 fits 168 VGPRs (three
+// work-groups per CU, measured 11% faster on a 256x256x64 block than two);
+// FP64, the region table, the further terms and px = 2048 would spill.
+template <class T, int FT, int R1, int R2>
 constexpr int xx_waves() {
-    return sizeof(T) == 4 && !LUT && R1 * R2 <= 1024 ? 3 : 1;
+    return sizeof(T) == 4 && FT == 0 && R1 * R2 <= 1024 ? 3 : 1;
 }
 
 // Stage s (not the last of the step) ends and stage s+1 begins in one
@@ -1200,8 +1334,8 @@ constexpr int xx_waves() {
 // (Issuing all three phases' loads up front was measured slower: the
 // register footprint doubles and the 3-D grid loses two thirds of its
 // occupancy.)
-template <class T, bool LUT, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads, (xx_waves<T, LUT, R1, R2>())) k_xx(const Args<T> *__restrict__ AP, int s) {
+template <class T, int FT, int R1, int R2, int NK>
+__global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
     if (ctl->done) return;
@@ -1216,9 +1350,9 @@ __global__ void __launch_bounds__(kThreads, (xx_waves<T, LUT, R1, R2>())) k_xx(c
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
     const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
-    const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+    const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     Acc acc;
-    x_local<T, LUT, R1, R2>(A, M, s, p, dt, wf, xr, res, false, false, acc);
+    x_local<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, res, false, false, acc);
     // k_s of these rows is written by this work-group; the barrier publishes
     // it (work-group scope) and frees the LDS result buffer
     __syncthreads();
@@ -1979,7 +2113,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ 
     stage_state(A, M, s, ctl->parity, ctl->dt, cell, v);
 }
 
-template <class T, bool LUT>
+template <class T, int FT>
 __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
@@ -1996,10 +2130,10 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
         const int y = (int)((cell / A.nx) % A.ny);
         const int z = (int)(cell / ((long long)A.nx * A.ny));
         const T hd[3] = {T(0), T(0), T(0)};
-        const T wf = A.wave_kind ? wave_factor(A, stage_time(A, M, s, ctl, dt)) : T(0);
+        const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
         CellIn<T> g;
-        gather_cell<T, LUT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
-        compute_cell<T, LUT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, wf, stats != 0, err, acc,
+        gather_cell<T, FT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
+        compute_cell<T, FT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, tf, stats != 0, err, acc,
                              A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
                              nullptr, stage_push(A, M, s));
     }
@@ -2321,13 +2455,13 @@ static size_t lds_yf() {
     return 3ull * TB * sizeof(C2<T>);
 }
 
-template <class T, bool LUT, int R1, int R2>
+template <class T, int FT, int R1, int R2>
 static hipError_t set_lds_xx(size_t bytes) {
-    hipError_t e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 1>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 2>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 3>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 4>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, LUT, R1, R2, 5>, bytes);
+    hipError_t e = set_lds<T>((const void *)k_xx<T, FT, R1, R2, 1>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, FT, R1, R2, 2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, FT, R1, R2, 3>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, FT, R1, R2, 4>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xx<T, FT, R1, R2, 5>, bytes);
     return e;
 }
 
@@ -2339,10 +2473,14 @@ static hipError_t set_lds_x_shape(size_t bytes) {
     if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 3>, bytes);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 4>, bytes);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 5>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, false, R1, R2>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, true, R1, R2>, bytes);
-    if (e == hipSuccess) e = set_lds_xx<T, false, R1, R2>(bytes);
-    if (e == hipSuccess) e = set_lds_xx<T, true, R1, R2>(bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, 0, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, 1, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, 2, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, 3, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, 0, R1, R2>(bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, 1, R1, R2>(bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, 2, R1, R2>(bytes);
+    if (e == hipSuccess) e = set_lds_xx<T, 3, R1, R2>(bytes);
     return e;
 }
 template <class T>
@@ -2508,28 +2646,42 @@ static void launch_fwd_x_shape(const Args<T> &A, int s, hipStream_t st) {
         default: k_fwd_x<T, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
     }
 }
+// Kernel feature set of a plan: bit 0 region exchange table, bit 1 further
+// terms (Args::xt).
+template <class T>
+static int feature_set(const Args<T> &A) {
+    return (A.region ? 1 : 0) | (A.xt ? 2 : 0);
+}
 template <class T, int R1, int R2>
 static void launch_inv_x_shape(const Args<T> &A, int s, int stats, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
-    if (A.region) k_inv_x<T, true, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats);
-    else k_inv_x<T, false, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats);
+    switch (feature_set(A)) {
+        case 0: k_inv_x<T, 0, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
+        case 1: k_inv_x<T, 1, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
+        case 2: k_inv_x<T, 2, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
+        default: k_inv_x<T, 3, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
+    }
 }
-template <class T, bool LUT, int R1, int R2>
-static void launch_xx_lut(const Args<T> &A, int s, hipStream_t st) {
+template <class T, int FT, int R1, int R2>
+static void launch_xx_ft(const Args<T> &A, int s, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
     const size_t l = lds_x(A);
     switch (A.stage_nk[s + 1]) {
-        case 1: k_xx<T, LUT, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
-        case 2: k_xx<T, LUT, R1, R2, 2><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
-        case 3: k_xx<T, LUT, R1, R2, 3><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
-        case 4: k_xx<T, LUT, R1, R2, 4><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
-        default: k_xx<T, LUT, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 1: k_xx<T, FT, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 2: k_xx<T, FT, R1, R2, 2><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 3: k_xx<T, FT, R1, R2, 3><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 4: k_xx<T, FT, R1, R2, 4><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        default: k_xx<T, FT, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
     }
 }
 template <class T, int R1, int R2>
 static void launch_xx_shape(const Args<T> &A, int s, hipStream_t st) {
-    if (A.region) launch_xx_lut<T, true, R1, R2>(A, s, st);
-    else launch_xx_lut<T, false, R1, R2>(A, s, st);
+    switch (feature_set(A)) {
+        case 0: launch_xx_ft<T, 0, R1, R2>(A, s, st); break;
+        case 1: launch_xx_ft<T, 1, R1, R2>(A, s, st); break;
+        case 2: launch_xx_ft<T, 2, R1, R2>(A, s, st); break;
+        default: launch_xx_ft<T, 3, R1, R2>(A, s, st); break;
+    }
 }
 // Fused end of stage s and start of stage s+1 (2 <= s < nstage).
 template <class T>
@@ -2627,6 +2779,18 @@ static void launch_yz(const Args<T> &A, hipStream_t st) {
     }
 }
 
+// Local pass without demag (one cell per thread).
+template <class T>
+static void launch_local(const Args<T> &A, int s, int stats, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    switch (feature_set(A)) {
+        case 0: k_local<T, 0><<<bgrid(A, g), b, 0, st>>>(A.self_d, s, stats); break;
+        case 1: k_local<T, 1><<<bgrid(A, g), b, 0, st>>>(A.self_d, s, stats); break;
+        case 2: k_local<T, 2><<<bgrid(A, g), b, 0, st>>>(A.self_d, s, stats); break;
+        default: k_local<T, 3><<<bgrid(A, g), b, 0, st>>>(A.self_d, s, stats); break;
+    }
+}
+
 // One RHS evaluation of stage s (1 = the current state m0).  control:
 // 0 none, 1 step controller, 2 statistics only (k_control after the final
 // kernel; implies per-block statistics).
@@ -2640,8 +2804,7 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
         launch_inv_x(A, s, stats, st);
     } else {
         if (s >= 2) k_stage<T><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s);
-        if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats);
-        else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, stats);
+        launch_local(A, s, stats, st);
     }
     if (control) k_control<T><<<bgrid(A, dim3(1)), dim3(kThreads), 0, st>>>(A.self_d, (int)gpart, control == 2);
     return hipGetLastError();
@@ -2810,8 +2973,7 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
             if (A.demag) {
                 launch_inv_x(A, s, 1, st);
             } else {
-                if (A.region) k_local<T, true><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1);
-                else k_local<T, false><<<bgrid(A, dim3(gpart)), dim3(kThreads), 0, st>>>(A.self_d, s, 1);
+                launch_local(A, s, 1, st);
             }
             break;
         case 5:

@@ -45,13 +45,55 @@ class Case:
     wave_freq: float = 0.0
     wave_phase: float = 0.0
     wave_mask: np.ndarray | None = None
+    # further terms in the oracle Problem's form (cubic, kc*_cell, dmi_i,
+    # dmi_b, mel, j, stt_degree, stt_beta, slon_p, slon_lambda, slon_eps, sot,
+    # oersted); plan() maps them onto the plan descriptor's fields
+    terms: dict | None = None
 
     @property
     def n(self):
         return self.nx * self.ny * self.nz
 
+    def _desc_terms(self):
+        """Plan-descriptor fields (fullmag_fdm.h) and per-cell arrays of `terms`."""
+        t = self.terms or {}
+        d, cells = {}, {}
+        if "cubic" in t:
+            k1, k2, k3, a1, a2 = t["cubic"]
+            d.update(has_cubic_anisotropy=1, cubic_Kc1=k1, cubic_Kc2=k2, cubic_Kc3=k3, cubic_axis1=tuple(a1),
+                     cubic_axis2=tuple(a2))
+        for q in ("kc1", "kc2", "kc3"):
+            if t.get(q + "_cell") is not None:
+                cells[q + "_field"] = t[q + "_cell"]
+        if t.get("dmi_i"):
+            d.update(has_interfacial_dmi=1, dmi_D_interfacial=t["dmi_i"])
+        if t.get("dmi_b"):
+            d.update(has_bulk_dmi=1, dmi_D_bulk=t["dmi_b"])
+        if "mel" in t:
+            d.update(has_magnetoelastic=1, mel_b1=t["mel"][0], mel_b2=t["mel"][1], mel_strain=tuple(t["mel"][2]))
+        j = t.get("j", (0.0, 0.0, 0.0))
+        d.update(current_density_x=j[0], current_density_y=j[1], current_density_z=j[2],
+                 stt_degree=t.get("stt_degree", 0.0), stt_beta=t.get("stt_beta", 0.0))
+        sp = t.get("slon_p", (0.0, 0.0, 0.0))
+        d.update(stt_p_x=sp[0], stt_p_y=sp[1], stt_p_z=sp[2], stt_lambda=t.get("slon_lambda", 0.0),
+                 stt_epsilon_prime=t.get("slon_eps", 0.0))
+        if "sot" in t:
+            je, dl, fl, sg, th = t["sot"]
+            d.update(has_sot=1, sot_je=je, sot_xi_dl=dl, sot_xi_fl=fl, sot_sigma=tuple(sg), sot_thickness=th)
+        if "oersted" in t:
+            o = tuple(t["oersted"]) + (0, 0.0, 0.0, 0.0, 0.0, 0.0)[len(t["oersted"]) - 4:]
+            d.update(has_oersted_cylinder=1, oersted_current=o[0], oersted_radius=o[1], oersted_center=tuple(o[2]),
+                     oersted_axis=tuple(o[3]), oersted_time_dep_kind=int(o[4]), oersted_time_dep_freq=o[5],
+                     oersted_time_dep_phase=o[6], oersted_time_dep_offset=o[7], oersted_time_dep_t_on=o[8],
+                     oersted_time_dep_t_off=o[9])
+        return d, cells
+
     def plan(self, precision="double", integrator="dp45", **kw):
         from fullmag_amd.backend import FdmPlan
+        d, cells = self._desc_terms()
+        if d:
+            kw["extra"] = {**d, **kw.get("extra", {})}
+        kw.update(cells)
         return FdmPlan(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
                        a_ex=self.a_ex, alpha=self.alpha, m0=self.m0, gamma=self.gamma, precision=precision,
                        integrator=integrator, exchange=self.exchange, demag=self.demag,
@@ -71,7 +113,7 @@ class Case:
                        dt_max=self.dt_max, headroom=self.headroom, ms_cell=self.ms_cell, ku1_cell=self.ku1_cell,
                        ku2_cell=self.ku2_cell, axis_cell=self.axis_cell, wave_kind=self.wave_kind,
                        wave_amp=self.wave_amp, wave_t0=self.wave_t0, wave_t1=self.wave_t1, wave_freq=self.wave_freq,
-                       wave_phase=self.wave_phase, wave_mask=self.wave_mask)
+                       wave_phase=self.wave_phase, wave_mask=self.wave_mask, **(self.terms or {}))
 
 
 def masked_3x3(demag: bool) -> Case:

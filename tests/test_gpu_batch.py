@@ -58,7 +58,7 @@ def test_batch_second_advance_continues_and_max_steps_caps():
 def test_batch_reports_bad_plan():
     from fullmag_amd.backend import FdmBatch, FdmError
     plans = _plans("single")
-    plans[3].extra = {"has_cubic_anisotropy": 1}
+    plans[3].extra = {"boundary_correction": 1}
     with pytest.raises(FdmError) as e:
         FdmBatch(plans)
     assert "system 3" in str(e.value)
