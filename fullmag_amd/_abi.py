@@ -212,6 +212,8 @@ class PlanExt(ctypes.Structure):
         ("adaptive_rtol", ctypes.c_double),
         ("adaptive_growth_limit", ctypes.c_double),
         ("adaptive_shrink_limit", ctypes.c_double),
+        ("thermal_dt", ctypes.c_double),
+        ("thermal_seed", ctypes.c_uint64),
     ]
 
 

@@ -65,6 +65,7 @@ class FdmPlan:
     kc1_field: np.ndarray | None = None      # per-cell cubic Kc1..Kc3 (reference ABI kc*_field)
     kc2_field: np.ndarray | None = None
     kc3_field: np.ndarray | None = None
+    temperature: float = 0.0                 # K; thermal field (fdm_fields.rs:818-905)
     # plan extension (include/fullmag_fdm_ext.h)
     ms_field: np.ndarray | None = None       # per-cell Ms
     axis_field: np.ndarray | None = None     # (n,3) per-cell uniaxial axis
@@ -78,11 +79,14 @@ class FdmPlan:
     rtol: float = 0.0
     growth: float = 0.0
     shrink: float = 0.0
+    thermal_dt: float = 0.0                  # noise dt; 0 = the dt of each step attempt
+    thermal_seed: int = 0                    # counter-generator seed; 0 = 42
     extra: dict = field(default_factory=dict)  # raw PlanDesc fields (for error tests)
 
     def uses_ext(self) -> bool:
         return (self.ms_field is not None or self.axis_field is not None or self.wave != 0
-                or self.rtol > 0 or self.growth > 0 or self.shrink > 0)
+                or self.rtol > 0 or self.growth > 0 or self.shrink > 0 or self.thermal_dt > 0
+                or self.thermal_seed != 0)
 
 
 _INTEGRATORS = {"heun": abi.INTEGRATOR_HEUN, "dp45": abi.INTEGRATOR_DP45, "rk45": abi.INTEGRATOR_DP45,
@@ -158,6 +162,7 @@ def marshal_plan(plan: FdmPlan):
     desc.adaptive_dt_min = plan.dt_min
     desc.adaptive_dt_max = plan.dt_max
     desc.adaptive_headroom = plan.headroom
+    desc.temperature = plan.temperature
     for k, v in plan.extra.items():
         setattr(desc, k, v)
     ext = None
@@ -182,6 +187,7 @@ def marshal_plan(plan: FdmPlan):
             ext.field_mask = wm.ctypes.data_as(abi.c_u8_p)
         ext.adaptive_rtol, ext.adaptive_growth_limit = plan.rtol, plan.growth
         ext.adaptive_shrink_limit = plan.shrink
+        ext.thermal_dt, ext.thermal_seed = plan.thermal_dt, plan.thermal_seed
     return desc, ext, keep
 
 

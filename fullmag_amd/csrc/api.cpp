@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -42,6 +43,8 @@ template <class T>
 hipError_t enqueue_eval(const Args<T> &A, int s, int stats, hipStream_t st);
 template <class T>
 hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st);
+template <class T>
+hipError_t enqueue_k1(const Args<T> &A, hipStream_t st);
 template <class T>
 hipError_t enqueue_control(const Args<T> &A, int observe, hipStream_t st);
 template <class T>
@@ -80,6 +83,8 @@ struct fullmag_fdm_backend {
     int nstage = 0;
     bool adaptive = false;
     bool multistep = false;  // ABM3: RHS history in k_all slots kHistSlot.. (Ctl::abm_*)
+    bool thermal = false;    // thermal field: RHS depends on Ctl::step and the attempt's dt
+    bool fsal = false;       // DP45: k1 is the last stage of the previous step
     fm::Args<float> af{};
     fm::Args<double> ad{};
     std::vector<void *> allocs;
@@ -288,7 +293,6 @@ int select_device(Backend *h) {
 
 std::string unsupported_terms(const fullmag_fdm_plan_desc *p) {
     std::vector<std::string> bad;
-    if (p->temperature > 0.0) bad.push_back("thermal field");
     if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE) bad.push_back("boundary correction tiers");
     if (p->has_demag_boundary_corr) bad.push_back("sparse demag boundary correction");
     std::string s;
@@ -451,7 +455,30 @@ bool pull_ctl(Backend *h) {
 }
 
 // k1 of the current state and its statistics (stage-1 evaluation).
+// With a thermal field the evaluation takes the noise of the current counter
+// at the dt of the last accepted step (the CPU engine's fields between
+// steps), and the FSAL stage of DP45, whose noise belongs to the previous
+// counter, is kept: it is copied to the stage-2 slot, which is free between
+// steps, and back.
 bool eval_current(Backend *h, bool observables) {
+    double dt_keep = 0.0;
+    bool keep_k1 = false;
+    if (h->thermal) {
+        if (!pull_ctl(h)) return false;
+        dt_keep = h->ctl_h->dt;
+        h->ctl_h->dt = h->ctl_h->dt_used;
+        if (!push_ctl(h)) return false;
+        keep_k1 = h->fsal && !h->ctl_h->k1_redo;
+    }
+    auto k1_copy = [&](bool save) {
+        return with_args(h, [&](auto &A) {
+            const size_t v = 3 * (size_t)h->n * sizeof(*A.k_all);
+            auto *k1 = A.k_all + (size_t)h->ctl_h->parity * 3 * h->n, *k2 = A.k_all + 2 * 3 * h->n;
+            return hip_ok(h, hipMemcpyAsync(save ? k2 : k1, save ? k1 : k2, v, hipMemcpyDeviceToDevice, h->stream),
+                          "keep k1");
+        });
+    };
+    if (keep_k1 && !k1_copy(true)) return false;
     const bool ok = with_args(h, [&](auto &A) {
         auto B = A;
         if (observables) {
@@ -463,7 +490,11 @@ bool eval_current(Backend *h, bool observables) {
         }
         return hip_ok(h, fm::enqueue_eval(B, 1, 2, h->stream), "launch (evaluation)");
     });
-    if (!ok || !pull_ctl(h)) return false;
+    if (!ok || (keep_k1 && !k1_copy(false)) || !pull_ctl(h)) return false;
+    if (h->thermal) {
+        h->ctl_h->dt = dt_keep;
+        if (!push_ctl(h)) return false;
+    }
     if (observables) h->obs_fresh = true;
     return true;
 }
@@ -597,6 +628,16 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     A.two_mu0inv = (T)(2.0 / kMu0);
     A.cell_vol = p->grid.dx * p->grid.dy * p->grid.dz;
     if (!parse_further_terms(h, A, p)) return false;
+    if (p->temperature > 0.0) {
+        // fdm_fields.rs:818-905: sigma^2 = 2 alpha kB T / (gamma (1 + alpha^2) mu0 Ms V dt)
+        constexpr double kKb = 1.380649e-23;
+        A.has_th = 1;
+        A.th_pre = 2.0 * alpha * kKb * p->temperature / (gamma * (1.0 + alpha * alpha) * kMu0 * A.cell_vol);
+        const bool v5 = x && x->struct_size >= sizeof(fullmag_fdm_plan_ext);
+        A.th_dt = v5 && x->thermal_dt > 0.0 ? x->thermal_dt : 0.0;
+        A.th_seed = v5 && x->thermal_seed ? x->thermal_seed : 42ull;
+        A.xt = 1;
+    }
     if (x) {
         if (x->adaptive_rtol < 0 || x->adaptive_shrink_limit < 0) {
             h->err = "invalid adaptive tolerances in plan extension";
@@ -629,6 +670,8 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     for (int s = 0; s <= fm::kMaxStages; ++s) A.stage_nk[s] = fm::method_stage_nk(A.method, s);
     h->adaptive = A.method == fm::M_DP45 || A.method == fm::M_RK23;
     h->multistep = A.method == fm::M_ABM3;
+    h->thermal = A.has_th != 0;
+    h->fsal = A.method == fm::M_DP45;
 
     // geometry
     if (p->active_mask) {
@@ -868,12 +911,13 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
     if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) return false;
     if (!hip_ok(h, hipHostMalloc((void **)&h->ctl_h, sizeof(fm::Ctl), hipHostMallocDefault), "pinned ctl"))
         return false;
-    if (x && x->struct_size < sizeof(fullmag_fdm_plan_ext))
+    if (x && x->struct_size < offsetof(fullmag_fdm_plan_ext, thermal_dt))
         return (h->err = "fullmag_fdm_plan_ext.struct_size too small", false);
     const bool ok = h->fp64 ? create_typed<double>(h, p, x) : create_typed<float>(h, p, x);
     if (!ok) return false;
     std::memset(h->ctl_h, 0, sizeof(fm::Ctl));
     h->ctl_h->dt = 1e-13;
+    h->ctl_h->k1_redo = 1;
     if (!push_ctl(h)) return false;
     if (!eval_current(h, true)) return false;
     h->ok = true;
@@ -1007,6 +1051,7 @@ int fullmag_fdm_backend_step(fullmag_fdm_backend *h, double dt, fullmag_fdm_step
             // replaced by the controller, so an interrupt leaves m untouched
             bool interrupted = false;
             const bool ok = with_args(h, [&](auto &A) {
+                if (!hip_ok(h, fm::enqueue_k1(A, h->stream), "launch (k1)")) return false;
                 for (int s = 2; s < h->nstage; ++s) {
                     if (!hip_ok(h, fm::enqueue_eval(A, s, 0, h->stream), "launch")) return false;
                     if (h->poll(h->poll_user)) {
@@ -1230,8 +1275,10 @@ static int upload_common(fullmag_fdm_backend *h, const double *d, const float *f
         // trajectory (ExchangeLlgState::reset_abm_history, fdm_state.rs:172)
         h->ctl_h->abm_count = 0;
         h->ctl_h->abm_dt = 0.0;
-        if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
     }
+    // nor does the FSAL stage (k_fsal is dropped with the state)
+    h->ctl_h->k1_redo = 1;
+    if (!push_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
     // the first stage of the next step needs k1 of the new state
     if (!eval_current(h, true)) return FULLMAG_FDM_ERR_CUDA;
     return FULLMAG_FDM_OK;
@@ -1934,7 +1981,7 @@ using Batch = fullmag_fdm_batch;
 template <class T>
 std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
     return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
-            A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt};
+            A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt, A.has_th};
 }
 
 template <class T>

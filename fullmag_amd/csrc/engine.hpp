@@ -108,6 +108,8 @@ struct Ctl {
     int loop;          // 1 = loop mode, 0 = single step driven by the host
     int abm_count;     // ABM3: history values since the last restart (0..3)
     int abm_head;      // ABM3: f_n is in k_all slot kHistSlot + abm_head
+    int k1_redo;       // DP45 with a thermal field: k1 must be evaluated afresh
+                       // (no FSAL stage: after create, upload or a rejection)
 };
 
 template <class T>
@@ -208,6 +210,10 @@ struct Args {
     int oe_kind;            // envelope: 0 constant, 1 sin(2 pi f t + phase) + offset, 2 pulse [t_on, t_off)
     double oe_freq, oe_phase, oe_offset, oe_t_on, oe_t_off;
     double cell_vol;        // dx dy dz
+    int has_th;             // thermal field: sigma = sqrt(th_pre / (Ms dt)), dt = th_dt or the trial dt;
+    double th_pre;          // 2 alpha kB T / (gamma (1 + alpha^2) mu0 V)
+    double th_dt;           // 0 = the dt of the step attempt in flight
+    unsigned long long th_seed;  // counter = accepted steps (Ctl::step)
     // step control
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
     int method;             // index into the constant method table

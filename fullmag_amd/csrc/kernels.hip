@@ -473,6 +473,36 @@ __device__ __forceinline__ T wave_factor(const Args<T> &A, double t) {
     }
 }
 
+// Thermal field of one cell, fdm_fields.rs:818-905: the SplitMix64
+// finaliser of a key of (seed, counter, cell, stream) gives four uniforms in
+// (0, 1], Box-Muller three Gaussians.  The counter is the number of accepted
+// steps (one per step call, shared by all stages and attempts).
+__device__ __forceinline__ unsigned long long th_mix(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double th_uniform(unsigned long long seed, unsigned long long step, unsigned long long cell,
+                                             unsigned long long stream) {
+    const unsigned long long key = seed + step * 0x9E3779B97F4A7C15ull + cell * 0x517CC1B727220A95ull +
+                                   stream * 0x6C62272E07BB0142ull;
+    return ((double)(th_mix(key) >> 11) + 1.0) / ((double)(1ull << 53) + 1.0);
+}
+template <class T>
+__device__ __forceinline__ void thermal_field(const Args<T> &A, int cell, T ms_i, T h[3]) {
+    const double dt = A.th_dt > 0.0 ? A.th_dt : A.ctl->dt;
+    if (!(dt > 0.0)) return;
+    const double sig = sqrt(A.th_pre / ((double)ms_i * dt));
+    const unsigned long long st = A.ctl->step, c = (unsigned long long)cell;
+    const double u1 = fmax(th_uniform(A.th_seed, st, c, 0), 1e-300), u2 = th_uniform(A.th_seed, st, c, 1);
+    const double u3 = fmax(th_uniform(A.th_seed, st, c, 2), 1e-300), u4 = th_uniform(A.th_seed, st, c, 3);
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    const double tp = 2.0 * 3.14159265358979323846;
+    h[0] += (T)(sig * r1 * cos(tp * u2));
+    h[1] += (T)(sig * r1 * sin(tp * u2));
+    h[2] += (T)(sig * r2 * cos(tp * u4));
+}
+
 // Time-dependent factors of one stage: Zeeman waveform and Oersted current
 // envelope (native/backends/fdm/src/demag_fp64.cu:628-650 forms).
 template <class T>
@@ -695,6 +725,7 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) h[c] += hcub[c] + hdmi[c] + (A.oe_field ? tf.oe * g.oe[c] : T(0));
+        if (A.has_th) thermal_field(A, cell, ms_i, h);
     }
     // LLG: k = -gb (m x H + alpha m x (m x H))
     const T pr[3] = {m[1] * h[2] - m[2] * h[1], m[2] * h[0] - m[0] * h[2], m[0] * h[1] - m[1] * h[0]};
@@ -2181,6 +2212,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     } else {
         c->dt_next = 0.0;
     }
+    c->k1_redo = accept ? 0 : 1;
     if (!accept) {
         c->rejected += 1;
         c->accepted = 0;
@@ -2791,6 +2823,43 @@ static void launch_local(const Args<T> &A, int s, int stats, hipStream_t st) {
     }
 }
 
+// Closes (open = 0) or reopens (open = 1) the k1 evaluation of a thermal
+// trial: bit 1 of Ctl::done makes every kernel return at once.  k1 is
+// needed by DP45 only without a valid FSAL stage and by ABM3 only in its
+// Heun start-up phase.
+template <class T>
+__global__ void k_k1_gate(const Args<T> *__restrict__ AP, int open) {
+    if (threadIdx.x != 0) return;
+    const Args<T> &A = AP[blockIdx.y];
+    Ctl *c = A.ctl;
+    if (open) {
+        c->done &= ~2;
+        return;
+    }
+    const bool need = A.method == M_DP45 ? c->k1_redo != 0 : A.method == M_ABM3 ? c->abm_count < 3 : true;
+    if (!need) c->done |= 2;
+}
+
+// With a thermal field the RHS of a state depends on the step counter and
+// on the dt of the attempt, so k1 is evaluated at the start of the attempt
+// wherever the CPU engine's integrators evaluate it (fdm_integrators.rs):
+// every attempt for Heun, RK4, RK23 and the ABM3 start-up, for DP45 the
+// first attempt after create/upload and every retry (otherwise the FSAL
+// stage of the previous step is k1).  Without a thermal field k1 is the
+// previous step's last stage and nothing is launched.
+template <class T>
+hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st);
+template <class T>
+hipError_t enqueue_k1(const Args<T> &A, hipStream_t st) {
+    if (!A.has_th) return hipSuccess;
+    const bool gated = A.method == M_DP45 || A.method == M_ABM3;
+    if (gated) k_k1_gate<T><<<bgrid(A, dim3(1)), dim3(64), 0, st>>>(A.self_d, 0);
+    const hipError_t e = enqueue_eval(A, 1, 0, st);
+    if (e != hipSuccess) return e;
+    if (gated) k_k1_gate<T><<<bgrid(A, dim3(1)), dim3(64), 0, st>>>(A.self_d, 1);
+    return hipGetLastError();
+}
+
 // One RHS evaluation of stage s (1 = the current state m0).  control:
 // 0 none, 1 step controller, 2 statistics only (k_control after the final
 // kernel; implies per-block statistics).
@@ -2815,6 +2884,10 @@ hipError_t enqueue_eval(const Args<T> &A, int s, int control, hipStream_t st) {
 // launch.
 template <class T>
 hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
+    {
+        const hipError_t e = enqueue_k1(A, st);
+        if (e != hipSuccess) return e;
+    }
     if (!A.demag) {
         for (int s = 2; s <= nstage; ++s) {
             const hipError_t e = enqueue_eval(A, s, s == nstage ? 1 : 0, st);
@@ -2932,6 +3005,7 @@ template const char *configure<FM_REAL>(Args<FM_REAL> &, size_t, int, int);
 template long long num_partials<FM_REAL>(const Args<FM_REAL> &);
 template hipError_t enqueue_eval<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
 template hipError_t enqueue_trial<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
+template hipError_t enqueue_k1<FM_REAL>(const Args<FM_REAL> &, hipStream_t);
 template hipError_t enqueue_control<FM_REAL>(const Args<FM_REAL> &, int, hipStream_t);
 template long long adj_blocks<FM_REAL>(const Args<FM_REAL> &);
 template hipError_t enqueue_adj_stage<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, hipStream_t);

@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 4
+#define FULLMAG_FDM_EXT_VERSION 5
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -42,6 +42,11 @@ typedef struct {
     double        adaptive_rtol;          /* as fullmag_fdm_backend_set_adaptive_ext     */
     double        adaptive_growth_limit;
     double        adaptive_shrink_limit;
+    /* version 5: thermal field of the plan's temperature (the CPU engine's
+     * FdmProblem::thermal_dt / thermal_seed, fdm_problem.rs:27-35).  A struct
+     * of the version-4 size (ending at adaptive_shrink_limit) is accepted. */
+    double        thermal_dt;             /* s; 0 = the dt of each step attempt          */
+    uint64_t      thermal_seed;           /* counter-generator seed; 0 = 42              */
 } fullmag_fdm_plan_ext;
 
 /* create() with the extension struct (ext may be NULL: same as create()). */

@@ -28,6 +28,9 @@
 //   magnetoelastic ....... crates/fullmag-engine/src/magnetoelastic.rs:52-75,155-176
 //   Oersted cylinder ..... crates/fullmag-engine/src/fdm_fields.rs:1049-1150 (geometry);
 //                          time envelopes native/backends/fdm/src/demag_fp64.cu:628-650
+//   thermal field ........ crates/fullmag-engine/src/fdm_fields.rs:818-905 (counter-based
+//                          SplitMix64 keys, Box-Muller; counter advanced per step call,
+//                          fdm_problem.rs:225-273)
 //   Zhang-Li / Slonczewski / SOT torques
 //                          crates/fullmag-engine/src/fdm_fields.rs:1446-1680, added to
 //                          every RHS as native/backends/fdm/src/llg_fp64.cu:58-205 does
@@ -380,6 +383,10 @@ typedef struct {
     double oe_current, oe_radius, oe_center[3], oe_axis[3];
     int oe_kind;
     double oe_freq, oe_phase, oe_offset, oe_t_on, oe_t_off;
+    // thermal field: temperature (K), noise dt (0: the dt of the step in
+    // flight), seed of the counter-based generator
+    double temperature, thermal_dt;
+    uint64_t thermal_seed;
 } orc_desc;
 
 typedef struct {
@@ -402,6 +409,8 @@ struct Problem {
     std::vector<double> ms_c, ku1_c, ku2_c, axis_c, kc1_c, kc2_c, kc3_c;
     std::vector<uint8_t> wmask;
     std::vector<V3> oe_prof;  // Oersted field at the full current, per cell
+    uint64_t th_step = 0;     // thermal counter: step calls so far
+    double th_dt_cur = 0.0;   // dt of the step attempt in flight
     double t_eval = 0.0;  // time at which the Zeeman waveform is evaluated
     // demag workspace (fdm_fft.rs:64-131)
     int px = 0, py = 0, pz = 0;
@@ -700,6 +709,41 @@ void torque_add(const Problem &p, const V3 *m, V3 *k) {
     }
 }
 
+// Thermal field, fdm_fields.rs:818-905: per cell three Gaussians from four
+// uniforms of the SplitMix64 finaliser applied to a key of (seed, counter,
+// cell, stream); sigma = sqrt(2 alpha kB T / (gamma (1 + alpha^2) mu0 Ms V dt)).
+uint64_t splitmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double counter_uniform(uint64_t seed, uint64_t step, uint64_t cell, uint64_t stream) {
+    const uint64_t key = seed + step * 0x9E3779B97F4A7C15ull + cell * 0x517CC1B727220A95ull +
+                         stream * 0x6C62272E07BB0142ull;
+    return ((double)(splitmix64(key) >> 11) + 1.0) / ((double)(1ull << 53) + 1.0);
+}
+void thermal_add(const Problem &p, V3 *h) {
+    const double dt = p.d.thermal_dt > 0.0 ? p.d.thermal_dt : p.th_dt_cur;
+    if (!(p.d.temperature > 0.0) || !(dt > 0.0)) return;
+    constexpr double KB = 1.380649e-23;
+    const double a = p.d.alpha, g0 = p.d.gamma * (1.0 + a * a), vol = p.d.dx * p.d.dy * p.d.dz;
+    const uint64_t seed = p.d.thermal_seed ? p.d.thermal_seed : 42;
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        if (!p.active(s)) continue;
+        const double sig = std::sqrt(2.0 * a * KB * p.d.temperature / (g0 * MU0 * p.ms_at(s) * vol * dt));
+        const uint64_t c = (uint64_t)s;
+        const double u1 = std::max(counter_uniform(seed, p.th_step, c, 0), 1e-300);
+        const double u2 = counter_uniform(seed, p.th_step, c, 1);
+        const double u3 = std::max(counter_uniform(seed, p.th_step, c, 2), 1e-300);
+        const double u4 = counter_uniform(seed, p.th_step, c, 3);
+        const double r1 = std::sqrt(-2.0 * std::log(u1)), r2 = std::sqrt(-2.0 * std::log(u3));
+        h[s].c[0] += sig * r1 * std::cos(2.0 * PI * u2);
+        h[s].c[1] += sig * r1 * std::sin(2.0 * PI * u2);
+        h[s].c[2] += sig * r2 * std::cos(2.0 * PI * u4);
+    }
+}
+
 // fdm_fields.rs:1153-1183; the Oersted envelope and the torques of
 // torque_add as the reference's GPU engine applies them.
 void effective_field(Problem &p, const V3 *m, V3 *h) {
@@ -710,6 +754,7 @@ void effective_field(Problem &p, const V3 *m, V3 *h) {
     ext_add(p, h);
     uni_add(p, m, h);
     cubic_add(p, m, h);
+    thermal_add(p, h);
     dmi_add(p, m, h);
     oe_add(p, h);
 }
@@ -784,6 +829,7 @@ void observables(Problem &p, const V3 *m, orc_report *r) {
     r->e_dmi = energy_field_form(p, m, p.hs.data(), -0.5);
     for (size_t s = 0; s < n; ++s) p.heff[s] = v_add(p.heff[s], p.hs[s]);
     mel_add(p, m, p.heff.data());
+    thermal_add(p, p.heff.data());
     oe_add(p, p.heff.data());
     r->e_mel = 0.0;
     if (p.d.has_mel) {
@@ -1017,8 +1063,9 @@ int orc_abm_count(void *h) { return static_cast<Problem *>(h)->abm_count; }
 void orc_set_time(void *h, double t) { static_cast<Problem *>(h)->t_eval = t; }
 
 // fdm_integrators.rs:84-150
-int orc_heun_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+static int heun_impl(void *h, double *m, double *t, double dt, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
+    p->th_dt_cur = dt;
     const size_t n = p->n;
     V3 *mm = reinterpret_cast<V3 *>(m);
     std::copy(mm, mm + n, p->m0.begin());
@@ -1041,8 +1088,9 @@ int orc_heun_step(void *h, double *m, double *t, double dt, orc_report *rep) {
 }
 
 // fdm_integrators.rs:155-264
-int orc_rk4_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+static int rk4_impl(void *h, double *m, double *t, double dt, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
+    p->th_dt_cur = dt;
     const size_t n = p->n;
     V3 *mm = reinterpret_cast<V3 *>(m);
     std::copy(mm, mm + n, p->m0.begin());
@@ -1073,7 +1121,7 @@ int orc_rk4_step(void *h, double *m, double *t, double dt, orc_report *rep) {
 }
 
 // Dormand-Prince 5(4) with FSAL, fdm_integrators.rs:429-680,828-870.
-int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) {
+static int rk45_impl(void *h, double *m, double *t, double dt_in, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
     const orc_desc &c = p->d;
     const size_t n = p->n;
@@ -1095,6 +1143,7 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
     uint64_t rejected = 0;
     const double t0 = *t;
     for (;;) {
+        p->th_dt_cur = dt;
         p->t_eval = t0;
         if (p->fsal_valid) {
             std::copy(p->kfsal.begin(), p->kfsal.end(), K[0].begin());
@@ -1183,7 +1232,7 @@ int orc_rk45_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
 // dt*(-5/72 k1 + 1/12 k2 + 1/9 k3 - 1/8 k4) with k4 at the new state, and
 // the controller exponent 1/3.  The waveform is evaluated at the stage
 // times t0, t0 + dt/2, t0 + 3dt/4, t0 + dt.
-int orc_rk23_step(void *h, double *m, double *t, double dt_in, orc_report *rep) {
+static int rk23_impl(void *h, double *m, double *t, double dt_in, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
     const orc_desc &c = p->d;
     const size_t n = p->n;
@@ -1196,6 +1245,7 @@ int orc_rk23_step(void *h, double *m, double *t, double dt_in, orc_report *rep) 
     uint64_t rejected = 0;
     const double t0 = *t;
     for (;;) {
+        p->th_dt_cur = dt;
         p->t_eval = t0;
         rhs_eval(*p, m0, K[0].data());
 #pragma omp parallel for schedule(static)
@@ -1267,13 +1317,14 @@ static void abm_push(Problem &p, std::vector<V3> &f, double dt) {
 // the step is a Heun step and pushes f(m_new); afterwards the
 // Adams-Bashforth predictor, f* = f(pred) and the Adams-Moulton corrector,
 // pushing f*.
-int orc_abm3_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+static int abm3_impl(void *h, double *m, double *t, double dt, orc_report *rep) {
     Problem *p = static_cast<Problem *>(h);
+    p->th_dt_cur = dt;
     const size_t n = p->n;
     V3 *mm = reinterpret_cast<V3 *>(m);
     std::vector<V3> f(n);
     if (p->abm_count < 3) {
-        orc_heun_step(p, m, t, dt, rep);
+        heun_impl(p, m, t, dt, rep);
         p->t_eval = *t;
         rhs_eval(*p, mm, f.data());
         abm_push(*p, f, dt);
@@ -1305,6 +1356,39 @@ int orc_abm3_step(void *h, double *m, double *t, double dt, orc_report *rep) {
     p->fsal_valid = false;
     return 0;
 }
+
+// One step call each; the thermal counter advances after every call
+// (FdmProblem::step_with_buffers, fdm_problem.rs:225-247).
+int orc_heun_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    const int r = heun_impl(h, m, t, dt, rep);
+    ++static_cast<Problem *>(h)->th_step;
+    return r;
+}
+int orc_rk4_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    const int r = rk4_impl(h, m, t, dt, rep);
+    ++static_cast<Problem *>(h)->th_step;
+    return r;
+}
+int orc_rk45_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    const int r = rk45_impl(h, m, t, dt, rep);
+    ++static_cast<Problem *>(h)->th_step;
+    return r;
+}
+int orc_rk23_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    const int r = rk23_impl(h, m, t, dt, rep);
+    ++static_cast<Problem *>(h)->th_step;
+    return r;
+}
+int orc_abm3_step(void *h, double *m, double *t, double dt, orc_report *rep) {
+    const int r = abm3_impl(h, m, t, dt, rep);
+    ++static_cast<Problem *>(h)->th_step;
+    return r;
+}
+
+uint64_t orc_thermal_step(void *h) { return static_cast<Problem *>(h)->th_step; }
+
+// SplitMix64 finaliser (for pinning against its published outputs).
+uint64_t orc_splitmix64(uint64_t z) { return splitmix64(z); }
 
 // Runner loop, cpu_reference.rs:622-639: dt_step = min(dt, until - t); the
 // next dt is the integrator's suggestion.  integrator: 1 Heun, 2 DP45, 3 ABM3,

@@ -63,6 +63,7 @@ class _Desc(ctypes.Structure):
         ("oe_center", ctypes.c_double * 3), ("oe_axis", ctypes.c_double * 3), ("oe_kind", ctypes.c_int),
         ("oe_freq", ctypes.c_double), ("oe_phase", ctypes.c_double), ("oe_offset", ctypes.c_double),
         ("oe_t_on", ctypes.c_double), ("oe_t_off", ctypes.c_double),
+        ("temperature", ctypes.c_double), ("thermal_dt", ctypes.c_double), ("thermal_seed", ctypes.c_uint64),
     ]
 
 
@@ -110,6 +111,10 @@ def lib():
         L.orc_reset_abm.argtypes = [ctypes.c_void_p]
         L.orc_abm_count.argtypes = [ctypes.c_void_p]
         L.orc_abm_count.restype = ctypes.c_int
+        L.orc_thermal_step.argtypes = [ctypes.c_void_p]
+        L.orc_thermal_step.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64]
+        L.orc_splitmix64.restype = ctypes.c_uint64
         L.orc_set_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         for nm in ("orc_heun_step", "orc_rk4_step", "orc_rk45_step", "orc_rk23_step", "orc_abm3_step"):
             getattr(L, nm).argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.POINTER(Report)]
@@ -226,6 +231,9 @@ class Problem:
     slon_eps: float = 0.0
     sot: tuple | None = None            # (je, xi_dl, xi_fl, sigma[3], thickness)
     oersted: tuple | None = None        # (current, radius, center[3], axis[3], kind, freq, phase, offset, t_on, t_off)
+    temperature: float = 0.0            # K
+    thermal_dt: float = 0.0             # s; 0 = the dt of each step
+    thermal_seed: int = 0               # 0 = 42
 
 
 class Oracle:
@@ -297,6 +305,7 @@ class Oracle:
             d.oe_kind = int(o[4]) if len(o) > 4 else 0
             if len(o) > 5:
                 d.oe_freq, d.oe_phase, d.oe_offset, d.oe_t_on, d.oe_t_off = o[5:10]
+        d.temperature, d.thermal_dt, d.thermal_seed = pb.temperature, pb.thermal_dt, pb.thermal_seed
         self.pb = pb
         self.n = pb.nx * pb.ny * pb.nz
         self.h = L.orc_create(ctypes.byref(d))
@@ -355,6 +364,10 @@ class Oracle:
         """Drop the ABM3 history (the next three steps are Heun start-up steps)."""
         lib().orc_reset_abm(self.h)
 
+    def thermal_step(self):
+        """Step calls so far (the thermal field's counter)."""
+        return lib().orc_thermal_step(self.h)
+
     def abm_count(self):
         """RHS values in the ABM3 history since the last restart (0..3)."""
         return lib().orc_abm_count(self.h)
@@ -383,3 +396,8 @@ class Oracle:
         if trace:
             res["trace"] = tr[:2 * n].reshape(n, 2)
         return res
+
+
+def splitmix64(z):
+    """SplitMix64 finaliser of the oracle's thermal generator."""
+    return lib().orc_splitmix64(z)

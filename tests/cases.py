@@ -47,7 +47,8 @@ class Case:
     wave_mask: np.ndarray | None = None
     # further terms in the oracle Problem's form (cubic, kc*_cell, dmi_i,
     # dmi_b, mel, j, stt_degree, stt_beta, slon_p, slon_lambda, slon_eps, sot,
-    # oersted); plan() maps them onto the plan descriptor's fields
+    # oersted, temperature, thermal_dt, thermal_seed); plan() maps them onto
+    # the plan descriptor's and the plan extension's fields
     terms: dict | None = None
 
     @property
@@ -94,6 +95,9 @@ class Case:
         if d:
             kw["extra"] = {**d, **kw.get("extra", {})}
         kw.update(cells)
+        for q in ("temperature", "thermal_dt", "thermal_seed"):
+            if q in (self.terms or {}):
+                kw.setdefault(q, self.terms[q])
         return FdmPlan(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
                        a_ex=self.a_ex, alpha=self.alpha, m0=self.m0, gamma=self.gamma, precision=precision,
                        integrator=integrator, exchange=self.exchange, demag=self.demag,

@@ -283,7 +283,7 @@ def test_fp32_field_copy_and_snapshot_precision():
 def test_rejected_plans_report_errors():
     from fullmag_amd.backend import FdmBackend, FdmError
     case = thin_film_8x6()
-    for extra in ({"temperature": 300.0}, {"boundary_correction": 1}, {"has_demag_boundary_corr": 1}):
+    for extra in ({"boundary_correction": 1}, {"has_demag_boundary_corr": 1}):
         with pytest.raises(FdmError) as ei:
             FdmBackend(case.plan(extra=extra))
         assert "not supported" in str(ei.value)

@@ -129,8 +129,8 @@ def test_plans_with_unusable_terms_are_rejected():
         with pytest.raises(FdmError):
             FdmBackend(_case(t).plan())
     with pytest.raises(FdmError) as e:
-        FdmBackend(_case("cubic").plan(extra={"temperature": 300.0}))
-    assert "thermal" in str(e.value)
+        FdmBackend(_case("cubic").plan(extra={"boundary_correction": 2}))
+    assert "boundary" in str(e.value)
 
 
 def test_adjoint_rejects_further_terms():

@@ -275,3 +275,65 @@ def test_energy_total_includes_every_term():
     parts = r["e_ex"] + r["e_demag"] + r["e_ext"] + r["e_ani"] + r["e_cub"] + r["e_dmi"] + r["e_mel"]
     assert abs(r["e_total"] - parts) <= 1e-12 * abs(parts)
     assert all(r[k] != 0.0 for k in ("e_cub", "e_dmi", "e_mel"))
+
+
+# ---------------------------------------------------------------- thermal --
+# The thermal field (fdm_fields.rs:818-905) is pinned by: the SplitMix64
+# finaliser against the published output sequence of its reference
+# implementation (state 1234567, increments of 0x9E3779B97F4A7C15); the
+# sample statistics of the Box-Muller normals (zero mean, the stated
+# variance with scipy's Boltzmann constant, Gaussian kurtosis, no correlation
+# between components); sigma ~ dt^-1/2; the counter semantics (one value per
+# step call, shared by all stages, fdm_problem.rs:225-273).
+
+def test_splitmix64_published_sequence():
+    st = 1234567
+    out = []
+    for _ in range(5):
+        st = (st + 0x9E3779B97F4A7C15) % 2 ** 64
+        out.append(O.splitmix64(st))
+    assert out == [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+                   16408922859458223821]
+
+
+def _thermal(nx=64, ny=64, nz=4, **kw):
+    base = dict(dx=3e-9, dy=3e-9, dz=3e-9, ms=8e5, alpha=0.1, temperature=300.0, thermal_dt=1e-13)
+    base.update(kw)
+    return _problem(nx, ny, nz, **base)
+
+
+def test_thermal_field_statistics():
+    orc = _thermal()
+    h = orc.fields(np.tile([0.0, 0.0, 1.0], (64 * 64 * 4, 1)))["h_eff"]
+    a, gamma, vol, dt = 0.1, 2.211e5, 27e-27, 1e-13
+    sigma = np.sqrt(2 * a * C.k * 300.0 / (gamma * (1 + a * a) * MU0 * 8e5 * vol * dt))
+    x = h / sigma
+    n = x.size
+    assert abs(x.mean()) < 4 / np.sqrt(n)
+    assert abs(x.var() - 1.0) < 0.03
+    kurt = np.mean(x ** 4) / np.mean(x ** 2) ** 2
+    assert abs(kurt - 3.0) < 0.1
+    c = np.corrcoef(x.T)
+    assert np.abs(c - np.eye(3)).max() < 0.03
+
+
+def test_thermal_counter_and_dt_scaling():
+    m = np.tile([0.0, 0.0, 1.0], (64 * 64 * 4, 1))
+    a = _thermal()
+    b = _thermal(thermal_dt=4e-13)
+    ha, hb = a.fields(m)["h_eff"], b.fields(m)["h_eff"]
+    assert np.allclose(hb, 0.5 * ha, rtol=1e-13, atol=0)
+    assert np.array_equal(a.fields(m)["h_eff"], ha)                 # same counter, same field
+    c = _thermal(thermal_seed=43)
+    assert np.abs(np.corrcoef(c.fields(m)["h_eff"].ravel(), ha.ravel())[0, 1]) < 0.03
+    # one counter value per step call (also through rejections)
+    small = _thermal(nx=4, ny=4, nz=1, exchange=False, atol=1e-12)
+    mm = _random_m(16, 3)
+    assert small.thermal_step() == 0
+    t, rep = small.step("rk45", mm, 0.0, 1e-11)
+    assert rep["rejected"] >= 1 and small.thermal_step() == 1
+    small.step("heun", mm, t, 1e-13)
+    assert small.thermal_step() == 2
+    h2 = small.fields(mm)["h_eff"]
+    small.step("rk4", mm, t, 1e-13)
+    assert not np.array_equal(small.fields(mm)["h_eff"], h2)
