@@ -66,6 +66,17 @@ class FdmPlan:
     kc2_field: np.ndarray | None = None
     kc3_field: np.ndarray | None = None
     temperature: float = 0.0                 # K; thermal field (fdm_fields.rs:818-905)
+    # boundary-correction tier (fullmag_fdm.h:222-258): 0 none, 1 volume (T0),
+    # 2 full (T1); face_links / deltas: six per-cell arrays in the order
+    # +x, -x, +y, -y, +z, -z; demag_corr: (targets, sources (count, stencil),
+    # tensors (count, stencil, 6) as xx xy xz yy yz zz)
+    boundary_tier: int = 0
+    phi_floor: float = 0.0
+    delta_min: float = 0.0
+    volume_fraction: np.ndarray | None = None
+    face_links: tuple | None = None
+    deltas: tuple | None = None
+    demag_corr: tuple | None = None
     # plan extension (include/fullmag_fdm_ext.h)
     ms_field: np.ndarray | None = None       # per-cell Ms
     axis_field: np.ndarray | None = None     # (n,3) per-cell uniaxial axis
@@ -163,6 +174,29 @@ def marshal_plan(plan: FdmPlan):
     desc.adaptive_dt_max = plan.dt_max
     desc.adaptive_headroom = plan.headroom
     desc.temperature = plan.temperature
+    desc.boundary_correction = plan.boundary_tier
+    desc.boundary_phi_floor, desc.boundary_delta_min = plan.phi_floor, plan.delta_min
+    if plan.volume_fraction is not None:
+        a = np.ascontiguousarray(np.asarray(plan.volume_fraction, dtype=np.float64).reshape(-1))
+        keep.append(a)
+        desc.volume_fraction, desc.volume_fraction_len = _dp(a), a.size
+    for prefix, arrays in (("face_link_", plan.face_links), ("delta_", plan.deltas)):
+        if arrays is not None:
+            for suffix, arr in zip(("xp", "xm", "yp", "ym", "zp", "zm"), arrays):
+                a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1))
+                keep.append(a)
+                setattr(desc, prefix + suffix, _dp(a))
+    if plan.demag_corr is not None:
+        tg, src, ten = plan.demag_corr
+        tg = np.ascontiguousarray(np.asarray(tg, dtype=np.int32).reshape(-1))
+        src = np.ascontiguousarray(np.asarray(src, dtype=np.int32).reshape(tg.size, -1))
+        ten = np.ascontiguousarray(np.asarray(ten, dtype=np.float64).reshape(-1))
+        keep += [tg, src, ten]
+        desc.has_demag_boundary_corr = 1
+        desc.demag_corr_target_idx = tg.ctypes.data_as(abi.c_i32_p)
+        desc.demag_corr_source_idx = src.ctypes.data_as(abi.c_i32_p)
+        desc.demag_corr_tensor = _dp(ten)
+        desc.demag_corr_target_count, desc.demag_corr_stencil_size = tg.size, src.shape[1]
     for k, v in plan.extra.items():
         setattr(desc, k, v)
     ext = None

@@ -291,13 +291,134 @@ int select_device(Backend *h) {
     return FULLMAG_FDM_OK;
 }
 
-std::string unsupported_terms(const fullmag_fdm_plan_desc *p) {
-    std::vector<std::string> bad;
-    if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE) bad.push_back("boundary correction tiers");
-    if (p->has_demag_boundary_corr) bad.push_back("sparse demag boundary correction");
-    std::string s;
-    for (size_t i = 0; i < bad.size(); ++i) s += (i ? ", " : "") + bad[i];
-    return s;
+// Boundary-correction tiers (fullmag_fdm.h:222-258; uploaded as
+// native/backends/fdm/src/api.cpp:436-476 and context.cu:722-800).  The
+// volume fraction phi weights the demag source (phi Ms m) and every cell's
+// energy; T0 (VOLUME with all six face-link arrays) and T1 (FULL with all six
+// boundary distances) replace the exchange stencil by per-face weights
+// computed here: T0 f / max(phi, floor) for a face with link f > 0 inside the
+// box, T1 2 d / (d + max(delta_opposite, delta_min)) / max(phi, floor) for an
+// active neighbour inside the box (exchange_t0_fp64.cu:52-165,
+// exchange_t1_fp64.cu:62-198; z faces only for nz > 1; cells with phi <= 0
+// get none).  Without the stencil arrays the standard stencil stays, as in
+// the reference.  The sparse demag correction becomes a per-target list
+// (repeated targets add up; inactive targets and negative indices are
+// skipped).
+template <class T>
+bool parse_boundary(Backend *h, fm::Args<T> &A, const fullmag_fdm_plan_desc *p, const fullmag_fdm_plan_ext *x,
+                    double ms) {
+    const uint64_t n = h->n;
+    A.ms_pack = A.ms_cell;
+    const int tier = (int)p->boundary_correction;
+    if (tier == FULLMAG_FDM_BOUNDARY_NONE) {
+        if (p->has_demag_boundary_corr && p->demag_corr_target_count > 0)
+            return (h->err = "sparse demag boundary correction needs a boundary_correction tier with volume_fraction",
+                    false);
+        return true;
+    }
+    if (tier != FULLMAG_FDM_BOUNDARY_VOLUME && tier != FULLMAG_FDM_BOUNDARY_FULL)
+        return (h->err = "boundary_correction must be 0 (none), 1 (volume) or 2 (full)", false);
+    if (!p->volume_fraction || p->volume_fraction_len != n)
+        return (h->err = "boundary_correction needs volume_fraction with cell_count values", false);
+    for (uint64_t i = 0; i < n; ++i)
+        if (!(p->volume_fraction[i] >= 0.0 && p->volume_fraction[i] <= 1.0))
+            return (h->err = "volume_fraction must lie in [0, 1]", false);
+    const double floor_ = p->boundary_phi_floor > 0.0 ? p->boundary_phi_floor : 0.05;
+    const double dmin = p->boundary_delta_min > 0.0 ? p->boundary_delta_min
+                                                    : 0.1 * std::min({p->grid.dx, p->grid.dy, p->grid.dz});
+    auto upload = [&](const void *src, size_t bytes, const char *what) -> void * {
+        void *d = dalloc(h, bytes);
+        if (!d || hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            h->err = std::string("upload ") + what;
+            return nullptr;
+        }
+        return d;
+    };
+    const double *phi = p->volume_fraction;
+    std::vector<T> ph(phi, phi + n), mp(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double msi = x && x->ms_field && x->ms_field[i] > 0.0 ? x->ms_field[i] : ms;
+        mp[i] = (T)(msi * phi[i]);
+    }
+    if (!(A.bc_phi = static_cast<const T *>(upload(ph.data(), n * sizeof(T), "volume_fraction"))) ||
+        !(A.ms_pack = static_cast<const T *>(upload(mp.data(), n * sizeof(T), "volume_fraction"))))
+        return false;
+    const double *fl[6] = {p->face_link_xm, p->face_link_xp, p->face_link_ym,
+                           p->face_link_yp, p->face_link_zm, p->face_link_zp};
+    const double *dl[6] = {p->delta_xm, p->delta_xp, p->delta_ym, p->delta_yp, p->delta_zm, p->delta_zp};
+    bool faces = true, dists = true;
+    for (int q = 0; q < 6; ++q) {
+        faces = faces && fl[q];
+        dists = dists && dl[q];
+    }
+    const int st = tier == FULLMAG_FDM_BOUNDARY_VOLUME && faces ? 1 : tier == FULLMAG_FDM_BOUNDARY_FULL && dists ? 2 : 0;
+    if (st) {
+        const long nx = h->nx, ny = h->ny, nz = h->nz;
+        const double d[3] = {p->grid.dx, p->grid.dy, p->grid.dz};
+        const long stride[3] = {1, nx, nx * ny}, ext[3] = {nx, ny, nz};
+        std::vector<T> w(6 * n, T(0));
+        for (long i = 0; i < (long)n; ++i) {
+            if (!(phi[i] > 0.0)) continue;
+            const double inv = 1.0 / std::max(phi[i], floor_);
+            const long c[3] = {i % nx, (i / nx) % ny, i / (nx * ny)};
+            for (int a = 0; a < 3; ++a) {
+                if (a == 2 && nz == 1) continue;
+                for (int side = 0; side < 2; ++side) {  // 0: -a neighbour, 1: +a neighbour
+                    const int q = 2 * a + side;
+                    if (side == 0 ? c[a] == 0 : c[a] + 1 >= ext[a]) continue;
+                    double v;
+                    if (st == 1) {
+                        v = fl[q][i] > 0.0 ? fl[q][i] : 0.0;
+                    } else {
+                        const long j = side == 0 ? i - stride[a] : i + stride[a];
+                        if (h->has_mask && !h->mask_h[j]) continue;
+                        v = 2.0 * d[a] / (d[a] + std::max(dl[2 * a + 1 - side][i], dmin));
+                    }
+                    w[(size_t)q * n + i] = (T)(v * inv);
+                }
+            }
+        }
+        if (!(A.bc_w = static_cast<const T *>(upload(w.data(), w.size() * sizeof(T), "boundary stencil")))) return false;
+    }
+    if (p->has_demag_boundary_corr && p->demag_corr_target_count > 0) {
+        const uint64_t tc = p->demag_corr_target_count, ss = p->demag_corr_stencil_size;
+        if (!p->demag_corr_target_idx || !p->demag_corr_source_idx || !p->demag_corr_tensor || ss == 0)
+            return (h->err = "sparse demag boundary correction: target, source and tensor arrays are required", false);
+        std::vector<int> cnt(n + 1, 0);
+        auto used = [&](uint64_t t) {
+            const int32_t ti = p->demag_corr_target_idx[t];
+            return ti >= 0 && (!h->has_mask || h->mask_h[ti]);
+        };
+        for (uint64_t t = 0; t < tc; ++t) {
+            const int32_t ti = p->demag_corr_target_idx[t];
+            if (ti >= (int64_t)n) return (h->err = "sparse demag boundary correction: target index out of range", false);
+            for (uint64_t j = 0; j < ss; ++j)
+                if (p->demag_corr_source_idx[t * ss + j] >= (int64_t)n)
+                    return (h->err = "sparse demag boundary correction: source index out of range", false);
+            if (!used(t)) continue;
+            for (uint64_t j = 0; j < ss; ++j) cnt[ti + 1] += p->demag_corr_source_idx[t * ss + j] >= 0;
+        }
+        for (uint64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+        std::vector<int> src(std::max(cnt[n], 1)), fill(cnt.begin(), cnt.end() - 1);
+        std::vector<T> ten(6 * (size_t)std::max(cnt[n], 1));
+        for (uint64_t t = 0; t < tc; ++t) {
+            if (!used(t)) continue;
+            const int32_t ti = p->demag_corr_target_idx[t];
+            for (uint64_t j = 0; j < ss; ++j) {
+                const int32_t sj = p->demag_corr_source_idx[t * ss + j];
+                if (sj < 0) continue;
+                const int k = fill[ti]++;
+                src[k] = sj;
+                for (int c = 0; c < 6; ++c) ten[6 * (size_t)k + c] = (T)p->demag_corr_tensor[6 * (t * ss + j) + c];
+            }
+        }
+        if (!(A.dc_ptr = static_cast<const int *>(upload(cnt.data(), cnt.size() * sizeof(int), "demag correction"))) ||
+            !(A.dc_src = static_cast<const int *>(upload(src.data(), src.size() * sizeof(int), "demag correction"))) ||
+            !(A.dc_n = static_cast<const T *>(upload(ten.data(), ten.size() * sizeof(T), "demag correction"))))
+            return false;
+    }
+    A.xt = 1;
+    return true;
 }
 
 // Cubic anisotropy, DMI, magnetoelastic coupling, Zhang-Li / Slonczewski /
@@ -771,6 +892,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.wave_mask = d;
     }
     A.lut_pre = p->enable_exchange ? (T)(2.0 / (kMu0 * ms)) : (T)0;
+    if (!parse_boundary(h, A, p, x, ms)) return false;
 
     // state buffers
     A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));
@@ -892,8 +1014,6 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
         p->integrator != FULLMAG_FDM_INTEGRATOR_HEUN && p->integrator != FULLMAG_FDM_INTEGRATOR_RK23 &&
         p->integrator != FULLMAG_FDM_INTEGRATOR_ABM3)
         return (h->err = "integrator not supported by fdm_gpu_native (supported: heun, dp45, rk4, rk23, abm3)", false);
-    const std::string bad = unsupported_terms(p);
-    if (!bad.empty()) return (h->err = "terms not supported by fdm_gpu_native: " + bad, false);
     h->nx = g.nx;
     h->ny = g.ny;
     h->nz = g.nz;

@@ -179,6 +179,7 @@ struct Args {
     double e_pre;           // mu0*Ms*V (energy prefactor)
     // per-cell maps (NULL = the uniform value above)
     const T *ms_cell;       // Ms per cell
+    const T *ms_pack;       // demag source weight per cell (Ms, or phi Ms with a boundary tier)
     const T *ku1_cell, *ku2_cell;
     const T *axis_cell;     // 3 x ncell SoA, unit vectors
     T two_a_mu0;            // 2A/mu0 (exchange prefactor numerator)
@@ -214,6 +215,17 @@ struct Args {
     double th_pre;          // 2 alpha kB T / (gamma (1 + alpha^2) mu0 V)
     double th_dt;           // 0 = the dt of the step attempt in flight
     unsigned long long th_seed;  // counter = accepted steps (Ctl::step)
+    // boundary-correction tier: exchange stencil weights per cell in the
+    // neighbour order -x, +x, -y, +y, -z, +z (6 x ncell, the 1/max(phi,
+    // floor) normalisation included; NULL = standard stencil), volume
+    // fraction (energy weights; NULL = 1), sparse demag correction per
+    // target cell: sources dc_src[dc_ptr[c] .. dc_ptr[c+1]) with symmetric
+    // tensors dc_n (6 per source, xx xy xz yy yz zz)
+    const T *bc_w;
+    const T *bc_phi;
+    const int *dc_ptr;
+    const int *dc_src;
+    const T *dc_n;
     // step control
     double atol, rtol, dt_min, dt_max, headroom, growth, shrink;
     int method;             // index into the constant method table

@@ -537,6 +537,9 @@ struct CellIn {
     T ms, ku1, ku2, ax[3];
     T kc[3];          // cubic constants (further terms)
     T oe[3];          // Oersted field at the full current (further terms)
+    T bw[6];          // boundary-tier exchange weights (further terms)
+    T phi;            // volume fraction (further terms)
+    T hc[3];          // sparse demag correction (further terms)
     T m0[3];          // state at the start of the step (mixed tolerance)
     double ev[3];     // sum of the error terms before the final stage (Args::errv)
     unsigned char act, nact[6], wm;
@@ -579,6 +582,26 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 #pragma unroll
             for (int c = 0; c < 3; ++c) g.oe[c] = A.oe_field[c * N + cell];
         }
+        if (A.bc_w) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) g.bw[q] = A.bc_w[q * N + cell];
+        }
+        g.phi = A.bc_phi ? A.bc_phi[cell] : T(1);
+        g.hc[0] = g.hc[1] = g.hc[2] = T(0);
+        if (A.dc_ptr) {
+            // demag_boundary_fp64.cu:23-76: H -= dN . (phi Ms y) over the
+            // cell's correction sources
+            const int b = A.dc_ptr[cell], e = A.dc_ptr[cell + 1];
+            for (int j = b; j < e; ++j) {
+                const int sc = A.dc_src[j];
+                const T w = A.ms_pack[sc];
+                const T M0 = w * yin[sc], M1 = w * yin[N + sc], M2 = w * yin[2 * N + sc];
+                const T *t = A.dc_n + 6 * (size_t)j;
+                g.hc[0] -= t[0] * M0 + t[1] * M1 + t[2] * M2;
+                g.hc[1] -= t[1] * M0 + t[3] * M1 + t[4] * M2;
+                g.hc[2] -= t[2] * M0 + t[4] * M1 + t[5] * M2;
+            }
+        }
     }
     if (LUT) {
         g.rc = A.region[cell];
@@ -601,7 +624,7 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 // Local field terms, LLG torque, and (optionally) statistics for one cell.
 template <class T, int FT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
-                                             const CellIn<T> &g, T *__restrict__ kout, const T hd[3], TimeF<T> tf,
+                                             const CellIn<T> &g, T *__restrict__ kout, const T hd_in[3], TimeF<T> tf,
                                              bool stats, bool err, Acc &acc, T *__restrict__ hrec,
                                              T kret[3] = nullptr, T *__restrict__ kpush = nullptr) {
     constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
@@ -624,7 +647,25 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     const T *m = g.m;
     const T ms_i = g.ms;
     T hex[3];
-    if (!LUT) {
+    bool tiered = false;
+    if constexpr (XT) tiered = A.bc_w != nullptr;
+    if (tiered) {
+        // boundary-corrected stencils (exchange_t0_fp64.cu:52-165,
+        // exchange_t1_fp64.cu:62-198): per-face weights prepared on the host
+        // from the face links or boundary distances; neighbours as stored
+        const T ww[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
+        T aw[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) aw[q] = g.bw[q] * ww[q] * (LUT ? A.lut[g.rc * 256u + g.rn[q]] : T(1));
+        const T pre = LUT ? (A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre) : (A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T acc2 = T(0);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) acc2 += aw[q] * (g.v[q][c] - m[c]);
+            hex[c] = pre * acc2;
+        }
+    } else if (!LUT) {
         const T pre = A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -673,6 +714,11 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         hani[0] = g.ax[0] * coef;
         hani[1] = g.ax[1] * coef;
         hani[2] = g.ax[2] * coef;
+    }
+    T hd[3] = {hd_in[0], hd_in[1], hd_in[2]};
+    if constexpr (XT) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hd[c] += g.hc[c];
     }
     T h[3];
 #pragma unroll
@@ -812,7 +858,8 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         auto sq = [](const T *f) {
             return (double)f[0] * (double)f[0] + (double)f[1] * (double)f[1] + (double)f[2] * (double)f[2];
         };
-        const double ep = A.ms_cell ? A.mu0_vol * (double)ms_i : A.e_pre;
+        double ep = A.ms_cell ? A.mu0_vol * (double)ms_i : A.e_pre;
+        if constexpr (XT) ep *= (double)g.phi;  // boundary tier: cell volume phi V
         acc.v[ST_EEX] += -0.5 * ep * dotd(hex);
         acc.v[ST_EDEMAG] += -0.5 * ep * dotd(hd);
         acc.v[ST_EEXT] += -1.0 * ep * dotd(hext);
@@ -820,7 +867,7 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         if constexpr (XT) {
             acc.v[ST_ECUB] += -0.5 * ep * dotd(hcub);
             acc.v[ST_EDMI] += -0.5 * ep * dotd(hdmi);
-            acc.v[ST_EMEL] += e_mel;
+            acc.v[ST_EMEL] += e_mel * (double)g.phi;
         }
         acc.v[ST_HEFF2] = fmax(acc.v[ST_HEFF2], sq(h));
         acc.v[ST_HD2] = fmax(acc.v[ST_HD2], sq(hd));
@@ -1035,7 +1082,7 @@ __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, i
 #pragma unroll
             for (int c = 0; c < 3; ++c) in.kv[j][q][c] = kp[c * N + cell];
         }
-        in.msv[j] = A.ms_cell ? A.ms_cell[cell] : A.ms;
+        in.msv[j] = A.ms_pack ? A.ms_pack[cell] : A.ms;
     }
 }
 

@@ -387,6 +387,21 @@ typedef struct {
     // flight), seed of the counter-based generator
     double temperature, thermal_dt;
     uint64_t thermal_seed;
+    // boundary-correction tier: 0 none, 1 face-link weighted exchange (T0),
+    // 2 intersection-distance stencil (T1); per-cell volume fraction phi
+    // (required for a tier), face link fractions and distances to the
+    // boundary in the order +x, -x, +y, -y, +z, -z (NULL = absent), phi
+    // floor and distance minimum (0 = 0.05 and 0.1 min(dx, dy, dz))
+    int bc_tier;
+    double bc_phi_floor, bc_delta_min;
+    const double *bc_phi;
+    const double *bc_face[6];
+    const double *bc_delta[6];
+    // sparse demag correction (with a tier): dc_count targets, dc_stencil
+    // sources each (-1 = none), tensors [xx, xy, xz, yy, yz, zz] per pair
+    uint32_t dc_count, dc_stencil;
+    const int32_t *dc_target, *dc_source;
+    const double *dc_tensor;
 } orc_desc;
 
 typedef struct {
@@ -409,6 +424,10 @@ struct Problem {
     std::vector<double> ms_c, ku1_c, ku2_c, axis_c, kc1_c, kc2_c, kc3_c;
     std::vector<uint8_t> wmask;
     std::vector<V3> oe_prof;  // Oersted field at the full current, per cell
+    std::vector<double> phi;                  // volume fraction (tier > 0)
+    std::vector<double> face[6], dlt[6];      // T0 face links, T1 distances
+    std::vector<int32_t> dc_t, dc_s;          // sparse demag correction
+    std::vector<double> dc_n;
     uint64_t th_step = 0;     // thermal counter: step calls so far
     double th_dt_cur = 0.0;   // dt of the step attempt in flight
     double t_eval = 0.0;  // time at which the Zeeman waveform is evaluated
@@ -430,6 +449,8 @@ struct Problem {
     double abm_dt = 0.0;
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
+    // magnetic volume fraction of a cell (1 without a tier)
+    double phi_at(size_t i) const { return phi.empty() ? 1.0 : phi[i]; }
     double ms_at(size_t i) const { return ms_c.empty() ? d.ms : ms_c[i]; }
     double wave(double t) const {
         switch (d.wave_kind) {
@@ -447,6 +468,30 @@ struct Problem {
     }
 };
 
+// Sparse demag correction, native/backends/fdm/src/demag_boundary_fp64.cu:23-76:
+// H(target) -= sum over its sources of dN . (phi Ms m)(source), dN symmetric
+// [xx, xy, xz, yy, yz, zz]; negative indices are skipped; repeated targets
+// add up; inactive targets get no field.
+void demag_correction_add(const Problem &p, const V3 *m, V3 *h) {
+    const size_t S = p.dc_t.empty() ? 0 : p.dc_s.size() / p.dc_t.size();
+    for (size_t t = 0; t < p.dc_t.size(); ++t) {
+        const int32_t ti = p.dc_t[t];
+        if (ti < 0 || !p.active((size_t)ti)) continue;
+        double c[3] = {0.0, 0.0, 0.0};
+        for (size_t j = 0; j < S; ++j) {
+            const int32_t si = p.dc_s[t * S + j];
+            if (si < 0) continue;
+            const double *N = &p.dc_n[6 * (t * S + j)];
+            const double w = p.phi_at((size_t)si) * p.ms_at((size_t)si);
+            const double M[3] = {w * m[si].c[0], w * m[si].c[1], w * m[si].c[2]};
+            c[0] -= N[0] * M[0] + N[1] * M[1] + N[2] * M[2];
+            c[1] -= N[1] * M[0] + N[3] * M[1] + N[4] * M[2];
+            c[2] -= N[2] * M[0] + N[4] * M[1] + N[5] * M[2];
+        }
+        for (int k = 0; k < 3; ++k) h[ti].c[k] += c[k];
+    }
+}
+
 void demag_add(Problem &p, const V3 *m, V3 *h) {
     const int nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
     std::fill(p.bmx.begin(), p.bmx.end(), cplx(0.0));
@@ -458,7 +503,8 @@ void demag_add(Problem &p, const V3 *m, V3 *h) {
                 const size_t s = ((size_t)z * ny + y) * nx + x;
                 const size_t t = ((size_t)z * p.py + y) * p.px + x;
                 if (!p.active(s)) continue;
-                const double ms = p.ms_at(s);
+                // with a tier the source is phi Ms m (demag_fp64.cu:81-86)
+                const double ms = p.ms_at(s) * p.phi_at(s);
                 p.bmx[t] = m[s].c[0] * ms;
                 p.bmy[t] = m[s].c[1] * ms;
                 p.bmz[t] = m[s].c[2] * ms;
@@ -486,9 +532,73 @@ void demag_add(Problem &p, const V3 *m, V3 *h) {
         h[s].c[1] += p.bhy[t].real() * norm;
         h[s].c[2] += p.bhz[t].real() * norm;
     }
+    demag_correction_add(p, m, h);
+}
+
+// Boundary-corrected exchange, native/backends/fdm/src/exchange_t0_fp64.cu:52-165
+// (T0) and exchange_t1_fp64.cu:62-198 (T1), with the stiffness applied once
+// (the reference kernels multiply by A in the sum and again in the prefactor
+// when there is no region table) and the cell's own Ms.  Cells with phi <= 0
+// or m = 0 get no exchange field; the sum is normalised by max(phi, floor).
+// T0: sum over faces with link fraction f > 0 inside the box of
+// f (m_j - m_i) / d^2.  T1: the neighbour on the -a side (inside the box,
+// nonzero m) has weight 2 / (d (d + delta_{+a})), the one on the +a side
+// 2 / (d (d + delta_{-a})), with delta clamped to delta_min.  z faces only
+// for nz > 1.
+void exchange_tier_add(const Problem &p, const V3 *m, V3 *h, int tier) {
+    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    const double d[3] = {p.d.dx, p.d.dy, p.d.dz};
+    const double floor_ = p.d.bc_phi_floor > 0.0 ? p.d.bc_phi_floor : 0.05;
+    const double dmin = p.d.bc_delta_min > 0.0 ? p.d.bc_delta_min : 0.1 * std::min({p.d.dx, p.d.dy, p.d.dz});
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < (long)p.n; ++s) {
+        const double phi0 = p.phi[s];
+        if (phi0 <= 0.0) continue;
+        const V3 c = m[s];
+        if (c.c[0] == 0.0 && c.c[1] == 0.0 && c.c[2] == 0.0) continue;
+        const long xyz[3] = {s % nx, (s / nx) % ny, s / (nx * ny)};
+        const long ext[3] = {nx, ny, nz};
+        const long stride[3] = {1, nx, nx * ny};
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int a = 0; a < 3; ++a) {
+            if (a == 2 && nz == 1) continue;
+            for (int side = 0; side < 2; ++side) {  // 0: +a neighbour, 1: -a neighbour
+                const bool inside = side == 0 ? xyz[a] + 1 < ext[a] : xyz[a] > 0;
+                if (!inside) continue;
+                const long q = side == 0 ? s + stride[a] : s - stride[a];
+                double w;
+                if (tier == 1) {
+                    const double f = p.face[2 * a + side][s];
+                    if (!(f > 0.0)) continue;
+                    w = f / (d[a] * d[a]);
+                } else {
+                    const V3 v = m[q];
+                    if (v.c[0] == 0.0 && v.c[1] == 0.0 && v.c[2] == 0.0) continue;
+                    const double opp = std::max(p.dlt[2 * a + (1 - side)][s], dmin);
+                    w = 2.0 / (d[a] * (d[a] + opp));
+                }
+                for (int k = 0; k < 3; ++k) acc[k] += w * (m[q].c[k] - c.c[k]);
+            }
+        }
+        const double pre = 2.0 * p.d.a_ex / (MU0 * p.ms_at(s) * std::max(phi0, floor_));
+        for (int k = 0; k < 3; ++k) h[s].c[k] += pre * acc[k];
+    }
+}
+
+// The stencil tier in effect: T0 needs all six face-link arrays, T1 all six
+// distance arrays (exchange_fp64.cu:162-205); otherwise the standard
+// exchange runs.
+int exchange_tier(const Problem &p) {
+    if (p.d.bc_tier == 1 && !p.face[0].empty()) return 1;
+    if (p.d.bc_tier >= 2 && !p.dlt[0].empty()) return 2;
+    return 0;
 }
 
 void exchange_add(const Problem &p, const V3 *m, V3 *h) {
+    if (const int tier = exchange_tier(p)) {
+        exchange_tier_add(p, m, h, tier);
+        return;
+    }
     const double dx2 = p.d.dx * p.d.dx, dy2 = p.d.dy * p.d.dy, dz2 = p.d.dz * p.d.dz;
     const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
 #pragma omp parallel for schedule(static)
@@ -782,10 +892,11 @@ double max_norm(const V3 *v, size_t n) {
     return r;
 }
 
+// With a tier every cell's energy is weighted by its volume fraction.
 double energy_field_form(const Problem &p, const V3 *m, const V3 *h, double f) {
     const double vol = p.d.dx * p.d.dy * p.d.dz;
     double e = 0.0;
-    for (size_t s = 0; s < p.n; ++s) e += f * MU0 * p.ms_at(s) * v_dot(m[s], h[s]) * vol;
+    for (size_t s = 0; s < p.n; ++s) e += f * MU0 * p.ms_at(s) * p.phi_at(s) * v_dot(m[s], h[s]) * vol;
     return e;
 }
 
@@ -835,7 +946,7 @@ void observables(Problem &p, const V3 *m, orc_report *r) {
     if (p.d.has_mel) {
         const double vol = p.d.dx * p.d.dy * p.d.dz;
         for (size_t s = 0; s < n; ++s)
-            if (p.active(s)) r->e_mel += mel_density(p, m[s]) * vol;
+            if (p.active(s)) r->e_mel += mel_density(p, m[s]) * vol * p.phi_at(s);
     }
     r->max_heff = max_norm(p.heff.data(), n);
     for (size_t s = 0; s < n; ++s) p.rhs[s] = llg(p, m[s], p.heff[s]);
@@ -927,6 +1038,23 @@ void *orc_create(const orc_desc *d) {
     p->d.kc1_cell = p->d.kc2_cell = p->d.kc3_cell = nullptr;
     p->d.wave_mask = nullptr;
     if (d->has_oe && d->oe_radius > 0.0) oe_profile(*p);
+    if (d->bc_tier > 0 && d->bc_phi) {
+        p->phi.assign(d->bc_phi, d->bc_phi + p->n);
+        for (int q = 0; q < 6; ++q) {
+            if (d->bc_face[0] && d->bc_face[q]) p->face[q].assign(d->bc_face[q], d->bc_face[q] + p->n);
+            if (d->bc_delta[0] && d->bc_delta[q]) p->dlt[q].assign(d->bc_delta[q], d->bc_delta[q] + p->n);
+        }
+        if (d->dc_count && d->dc_stencil && d->dc_target && d->dc_source && d->dc_tensor) {
+            const size_t ns = (size_t)d->dc_count * d->dc_stencil;
+            p->dc_t.assign(d->dc_target, d->dc_target + d->dc_count);
+            p->dc_s.assign(d->dc_source, d->dc_source + ns);
+            p->dc_n.assign(d->dc_tensor, d->dc_tensor + 6 * ns);
+        }
+    }
+    p->d.bc_phi = nullptr;
+    for (int q = 0; q < 6; ++q) p->d.bc_face[q] = p->d.bc_delta[q] = nullptr;
+    p->d.dc_target = p->d.dc_source = nullptr;
+    p->d.dc_tensor = nullptr;
     const size_t n = p->n;
     p->m0.resize(n);
     p->ms.resize(n);

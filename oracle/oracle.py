@@ -64,6 +64,12 @@ class _Desc(ctypes.Structure):
         ("oe_freq", ctypes.c_double), ("oe_phase", ctypes.c_double), ("oe_offset", ctypes.c_double),
         ("oe_t_on", ctypes.c_double), ("oe_t_off", ctypes.c_double),
         ("temperature", ctypes.c_double), ("thermal_dt", ctypes.c_double), ("thermal_seed", ctypes.c_uint64),
+        ("bc_tier", ctypes.c_int), ("bc_phi_floor", ctypes.c_double), ("bc_delta_min", ctypes.c_double),
+        ("bc_phi", ctypes.POINTER(ctypes.c_double)), ("bc_face", ctypes.POINTER(ctypes.c_double) * 6),
+        ("bc_delta", ctypes.POINTER(ctypes.c_double) * 6),
+        ("dc_count", ctypes.c_uint32), ("dc_stencil", ctypes.c_uint32),
+        ("dc_target", ctypes.POINTER(ctypes.c_int32)), ("dc_source", ctypes.POINTER(ctypes.c_int32)),
+        ("dc_tensor", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -234,6 +240,17 @@ class Problem:
     temperature: float = 0.0            # K
     thermal_dt: float = 0.0             # s; 0 = the dt of each step
     thermal_seed: int = 0               # 0 = 42
+    # boundary-correction tier (fullmag_fdm.h:222-258): 0 none, 1 T0, 2 T1;
+    # volume fraction per cell, face links / distances as six arrays in the
+    # order +x, -x, +y, -y, +z, -z, and the sparse demag correction as
+    # (targets, sources (count, stencil), tensors (count, stencil, 6))
+    boundary_tier: int = 0
+    phi_floor: float = 0.0
+    delta_min: float = 0.0
+    volume_fraction: np.ndarray | None = None
+    face_links: tuple | None = None
+    deltas: tuple | None = None
+    demag_corr: tuple | None = None
 
 
 class Oracle:
@@ -306,6 +323,28 @@ class Oracle:
             if len(o) > 5:
                 d.oe_freq, d.oe_phase, d.oe_offset, d.oe_t_on, d.oe_t_off = o[5:10]
         d.temperature, d.thermal_dt, d.thermal_seed = pb.temperature, pb.thermal_dt, pb.thermal_seed
+        d.bc_tier, d.bc_phi_floor, d.bc_delta_min = pb.boundary_tier, pb.phi_floor, pb.delta_min
+
+        def f64(a):
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+            self._keep.append(a)
+            return _ptr(a)
+        if pb.volume_fraction is not None:
+            d.bc_phi = f64(pb.volume_fraction)
+        for nm, arrays in (("bc_face", pb.face_links), ("bc_delta", pb.deltas)):
+            if arrays is not None:
+                dst = getattr(d, nm)
+                for q in range(6):
+                    dst[q] = f64(arrays[q])
+        if pb.demag_corr is not None:
+            tg, src, ten = pb.demag_corr
+            tg = np.ascontiguousarray(np.asarray(tg, dtype=np.int32).ravel())
+            src = np.ascontiguousarray(np.asarray(src, dtype=np.int32).reshape(len(tg), -1))
+            self._keep += [tg, src]
+            d.dc_count, d.dc_stencil = len(tg), src.shape[1]
+            d.dc_target = tg.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            d.dc_source = src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            d.dc_tensor = f64(ten)
         self.pb = pb
         self.n = pb.nx * pb.ny * pb.nz
         self.h = L.orc_create(ctypes.byref(d))

@@ -47,8 +47,10 @@ class Case:
     wave_mask: np.ndarray | None = None
     # further terms in the oracle Problem's form (cubic, kc*_cell, dmi_i,
     # dmi_b, mel, j, stt_degree, stt_beta, slon_p, slon_lambda, slon_eps, sot,
-    # oersted, temperature, thermal_dt, thermal_seed); plan() maps them onto
-    # the plan descriptor's and the plan extension's fields
+    # oersted, temperature, thermal_dt, thermal_seed, boundary_tier,
+    # phi_floor, delta_min, volume_fraction, face_links, deltas, demag_corr);
+    # plan() maps them onto the plan descriptor's and the plan extension's
+    # fields
     terms: dict | None = None
 
     @property
@@ -95,7 +97,8 @@ class Case:
         if d:
             kw["extra"] = {**d, **kw.get("extra", {})}
         kw.update(cells)
-        for q in ("temperature", "thermal_dt", "thermal_seed"):
+        for q in ("temperature", "thermal_dt", "thermal_seed", "boundary_tier", "phi_floor", "delta_min",
+                  "volume_fraction", "face_links", "deltas", "demag_corr"):
             if q in (self.terms or {}):
                 kw.setdefault(q, self.terms[q])
         return FdmPlan(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
@@ -172,3 +175,54 @@ def rel_err(a, b):
     b = np.asarray(b, dtype=np.float64)
     s = np.abs(b).max()
     return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def disk_boundary(nx, ny, nz, dx, dy, dz, radius, sub=8):
+    """Boundary-tier data of a disk of `radius` centred in an nx x ny grid
+    (every layer alike): volume fraction from sub x sub samples per cell, face
+    link fractions from sub samples along each x/y face (z faces 1 between
+    occupied layers), distances from the cell centre to the circle along
+    +-x / +-y (capped at one cell size; z: dz).  Returns (mask, phi,
+    face_links, deltas), arrays in the order +x, -x, +y, -y, +z, -z."""
+    cx, cy = nx * dx / 2, ny * dy / 2
+    u = (np.arange(sub) + 0.5) / sub
+    X = (np.arange(nx)[None, :, None, None] + u[None, None, None, :]) * dx - cx
+    Y = (np.arange(ny)[:, None, None, None] + u[None, None, :, None]) * dy - cy
+    phi2 = ((X ** 2 + Y ** 2) <= radius ** 2).mean(axis=(2, 3))
+    xc = (np.arange(nx) + 0.5) * dx - cx
+    yc = (np.arange(ny) + 0.5) * dy - cy
+    xf = (np.arange(nx + 1)) * dx - cx            # x faces
+    yf = (np.arange(ny + 1)) * dy - cy
+    fx = ((xf[None, :, None] ** 2 + ((np.arange(ny)[:, None, None] + u[None, None, :]) * dy - cy) ** 2)
+          <= radius ** 2).mean(axis=2)            # (ny, nx+1)
+    fy = ((yf[:, None, None] ** 2 + ((np.arange(nx)[None, :, None] + u[None, None, :]) * dx - cx) ** 2)
+          <= radius ** 2).mean(axis=2)            # (ny+1, nx)
+    xp, xm = fx[:, 1:].copy(), fx[:, :-1].copy()
+    xp[:, -1] = 0.0
+    xm[:, 0] = 0.0
+    yp, ym = fy[1:, :].copy(), fy[:-1, :].copy()
+    yp[-1, :] = 0.0
+    ym[0, :] = 0.0
+    # distance to the circle along each axis from the centre (inside cells)
+    Xc, Yc = np.meshgrid(xc, yc)
+    r2 = radius ** 2
+    sx = np.sqrt(np.maximum(r2 - Yc ** 2, 0.0))
+    sy = np.sqrt(np.maximum(r2 - Xc ** 2, 0.0))
+    dxp = np.clip(sx - Xc, 0.0, dx)
+    dxm = np.clip(Xc + sx, 0.0, dx)
+    dyp = np.clip(sy - Yc, 0.0, dy)
+    dym = np.clip(Yc + sy, 0.0, dy)
+
+    def stack(a):
+        return np.repeat(a[None], nz, axis=0).ravel()
+    phi = stack(phi2)
+    mask = (phi > 0).astype(np.uint8)
+    zp = np.ones((nz, ny, nx))
+    zm = np.ones((nz, ny, nx))
+    zp[-1] = 0.0
+    zm[0] = 0.0
+    zp, zm = zp.ravel() * (phi > 0), zm.ravel() * (phi > 0)
+    links = (stack(xp), stack(xm), stack(yp), stack(ym), zp, zm)
+    dzs = np.full(nx * ny * nz, dz)
+    deltas = (stack(dxp), stack(dxm), stack(dyp), stack(dym), dzs, dzs)
+    return mask, phi, links, deltas

@@ -283,10 +283,12 @@ def test_fp32_field_copy_and_snapshot_precision():
 def test_rejected_plans_report_errors():
     from fullmag_amd.backend import FdmBackend, FdmError
     case = thin_film_8x6()
-    for extra in ({"boundary_correction": 1}, {"has_demag_boundary_corr": 1}):
+    for extra, msg in (({"boundary_correction": 3}, "boundary_correction must be"),
+                       ({"boundary_correction": 1}, "needs volume_fraction"),
+                       ({"has_demag_boundary_corr": 1, "demag_corr_target_count": 2}, "needs a boundary_correction")):
         with pytest.raises(FdmError) as ei:
             FdmBackend(case.plan(extra=extra))
-        assert "not supported" in str(ei.value)
+        assert msg in str(ei.value)
     bad = case.plan()
     bad.mask = np.ones(5, dtype=np.uint8)
     with pytest.raises(FdmError) as ei:

@@ -130,7 +130,7 @@ def test_plans_with_unusable_terms_are_rejected():
             FdmBackend(_case(t).plan())
     with pytest.raises(FdmError) as e:
         FdmBackend(_case("cubic").plan(extra={"boundary_correction": 2}))
-    assert "boundary" in str(e.value)
+    assert "volume_fraction" in str(e.value)
 
 
 def test_adjoint_rejects_further_terms():
