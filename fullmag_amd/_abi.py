@@ -214,6 +214,8 @@ class PlanExt(ctypes.Structure):
         ("adaptive_shrink_limit", ctypes.c_double),
         ("thermal_dt", ctypes.c_double),
         ("thermal_seed", ctypes.c_uint64),
+        ("periodic_axes", ctypes.c_uint32),
+        ("periodic_images", ctypes.c_uint32 * 3),
     ]
 
 

@@ -92,12 +92,14 @@ class FdmPlan:
     shrink: float = 0.0
     thermal_dt: float = 0.0                  # noise dt; 0 = the dt of each step attempt
     thermal_seed: int = 0                    # counter-generator seed; 0 = 42
+    periodic: tuple = (False, False, False)  # periodic x, y, z (fdm_types.rs:94-132)
+    periodic_images: tuple = (0, 0, 0)       # images per periodic axis; 0 = 10
     extra: dict = field(default_factory=dict)  # raw PlanDesc fields (for error tests)
 
     def uses_ext(self) -> bool:
         return (self.ms_field is not None or self.axis_field is not None or self.wave != 0
                 or self.rtol > 0 or self.growth > 0 or self.shrink > 0 or self.thermal_dt > 0
-                or self.thermal_seed != 0)
+                or self.thermal_seed != 0 or any(self.periodic))
 
 
 _INTEGRATORS = {"heun": abi.INTEGRATOR_HEUN, "dp45": abi.INTEGRATOR_DP45, "rk45": abi.INTEGRATOR_DP45,
@@ -222,6 +224,8 @@ def marshal_plan(plan: FdmPlan):
         ext.adaptive_rtol, ext.adaptive_growth_limit = plan.rtol, plan.growth
         ext.adaptive_shrink_limit = plan.shrink
         ext.thermal_dt, ext.thermal_seed = plan.thermal_dt, plan.thermal_seed
+        ext.periodic_axes = sum(1 << a for a in range(3) if plan.periodic[a])
+        ext.periodic_images[:] = [int(v) for v in plan.periodic_images]
     return desc, ext, keep
 
 

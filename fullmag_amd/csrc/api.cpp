@@ -63,6 +63,8 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
 bool yfast_factors(int py, int *r1, int *r2);
 bool zfast_factors(int pz, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
+bool newell_periodic_spectrum(int nx, int ny, int nz, double dx, double dy, double dz, const int per[3],
+                              const int img[3], std::vector<double> &out, std::string *err);
 bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz, std::vector<double> &out,
                           std::string *err);
 }  // namespace fm
@@ -191,8 +193,16 @@ struct TuneEntry {
 std::mutex g_tune_mu;
 std::vector<TuneEntry> g_tune;
 
-void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int *rows, int *xfast) {
-    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method};
+// Kernel instantiation of a plan (further terms, thermal field, region
+// table, demag): plans that differ in it keep their own tuned tile, so that
+// a plan created alone uses the tile its batch group was tuned to.
+template <class T>
+int tune_feat(const fm::Args<T> &A) {
+    return (A.xt ? 1 : 0) | (A.has_th ? 2 : 0) | (A.region ? 4 : 0) | (A.demag ? 8 : 0);
+}
+
+void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int feat, int *rows, int *xfast) {
+    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method, feat};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const auto &e : g_tune)
         if (e.key == key) {
@@ -201,8 +211,8 @@ void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int *rows, int 
         }
 }
 
-void tuned_store(bool fp64, int nx, int ny, int nz, int method, int rows, int xfast) {
-    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method};
+void tuned_store(bool fp64, int nx, int ny, int nz, int method, int feat, int rows, int xfast) {
+    const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method, feat};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto &e : g_tune)
         if (e.key == key) {
@@ -754,7 +764,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         constexpr double kKb = 1.380649e-23;
         A.has_th = 1;
         A.th_pre = 2.0 * alpha * kKb * p->temperature / (gamma * (1.0 + alpha * alpha) * kMu0 * A.cell_vol);
-        const bool v5 = x && x->struct_size >= sizeof(fullmag_fdm_plan_ext);
+        const bool v5 = x && x->struct_size >= offsetof(fullmag_fdm_plan_ext, periodic_axes);
         A.th_dt = v5 && x->thermal_dt > 0.0 ? x->thermal_dt : 0.0;
         A.th_seed = v5 && x->thermal_seed ? x->thermal_seed : 42ull;
         A.xt = 1;
@@ -892,6 +902,24 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.wave_mask = d;
     }
     A.lut_pre = p->enable_exchange ? (T)(2.0 / (kMu0 * ms)) : (T)0;
+    // periodic boundaries (plan extension version 6; the CPU engine's
+    // FdmBoundaryPolicy with demag_image_counts, default 10 per axis)
+    int per[3] = {0, 0, 0}, img[3] = {10, 10, 10};
+    if (x && x->struct_size >= sizeof(fullmag_fdm_plan_ext) && x->periodic_axes) {
+        if (x->periodic_axes & ~7u) return (h->err = "periodic_axes: only bits 0 (x), 1 (y), 2 (z)", false);
+        if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE)
+            return (h->err = "periodic boundaries cannot be combined with boundary-correction tiers", false);
+        if (p->demag_kernel_xx_spectrum)
+            return (h->err = "periodic boundaries need the engine's own demag kernel (plan spectra are open-boundary)",
+                    false);
+        for (int a = 0; a < 3; ++a) {
+            per[a] = (x->periodic_axes >> a) & 1;
+            if (x->periodic_images[a]) img[a] = (int)x->periodic_images[a];
+            if (per[a] && img[a] > 1000) return (h->err = "periodic_images above 1000", false);
+        }
+        A.pbc = (int)x->periodic_axes;
+        A.xt = 1;
+    }
     if (!parse_boundary(h, A, p, x, ms)) return false;
 
     // state buffers
@@ -922,7 +950,19 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         std::vector<double> red;
         const double *spec[6] = {p->demag_kernel_xx_spectrum, p->demag_kernel_yy_spectrum, p->demag_kernel_zz_spectrum,
                                  p->demag_kernel_xy_spectrum, p->demag_kernel_xz_spectrum, p->demag_kernel_yz_spectrum};
-        if (spec[0]) {
+        if (A.pbc) {
+            std::vector<double> gs;
+            if (!fm::newell_periodic_spectrum(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz, per, img, gs, &e)) {
+                h->err = e;
+                return false;
+            }
+            std::vector<T> gt(gs.begin(), gs.end());
+            T *gd = static_cast<T *>(dalloc(h, gt.size() * sizeof(T)));
+            if (!gd) return hip_ok(h, hipErrorOutOfMemory, "alloc periodic tensor");
+            if (!hip_ok(h, hipMemcpy(gd, gt.data(), gt.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))
+                return false;
+            A.gtens = gd;
+        } else if (spec[0]) {
             if (!fm::reduce_given_spectra(spec, p->demag_kernel_spectrum_len, nx, ny, nz, red, &e)) {
                 h->err = e;
                 return false;
@@ -930,10 +970,10 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         } else {
             red = cached_newell(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz);
         }
-        std::vector<T> rt(red.size());
+        std::vector<T> rt(std::max<size_t>(red.size(), 1));
         if (nz == 1) {
             for (size_t i = 0; i < red.size(); ++i) rt[i] = (T)red[i];
-        } else {
+        } else if (!red.empty()) {
             // 3-D: [kx][kz][ky][component], the six components of a
             // frequency contiguous for the z pass
             const size_t X = nx + 1, Y = ny + 1, Z = nz + 1;
@@ -983,7 +1023,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             lds = 65536;
         h->lds_limit = (size_t)lds - 1024;
         int rows_req = 0, xfast_req = -1;
-        tuned_lookup(h->fp64, nx, ny, nz, A.method, &rows_req, &xfast_req);
+        tuned_lookup(h->fp64, nx, ny, nz, A.method, tune_feat(A), &rows_req, &xfast_req);
         const char *ce = fm::configure(A, h->lds_limit, rows_req, xfast_req);
         if (rows_req && ce && *ce) ce = fm::configure(A, h->lds_limit);
         if (ce && *ce) {
@@ -1766,7 +1806,7 @@ int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out
     h->graph_tried = false;
     *h->ctl_h = saved;
     if (!push_ctl(h) || !eval_current(h, false)) return FULLMAG_FDM_ERR_CUDA;
-    tuned_store(h->fp64, (int)h->nx, (int)h->ny, (int)h->nz, A.method, A.rows_per_wg, A.xfast ? 1 : 0);
+    tuned_store(h->fp64, (int)h->nx, (int)h->ny, (int)h->nz, A.method, tune_feat(A), A.rows_per_wg, A.xfast ? 1 : 0);
     if (out_rows) *out_rows = A.rows_per_wg;
     if (out_fs) *out_fs = A.xfast ? 1 : 0;
     return FULLMAG_FDM_OK;
@@ -2184,7 +2224,7 @@ bool group_tune(Batch *b, Batch::Group &g) {
     // k1 of the initial states with the chosen tile (as a create with it gives)
     for (size_t m : g.members)
         if (!eval_current(b->sys[m], false)) return (b->err = "batch: evaluation failed", false);
-    tuned_store(h0->fp64, (int)h0->nx, (int)h0->ny, (int)h0->nz, A0.method, best_rows, best_fs);
+    tuned_store(h0->fp64, (int)h0->nx, (int)h0->ny, (int)h0->nz, A0.method, tune_feat(A0), best_rows, best_fs);
     return true;
 }
 

@@ -223,6 +223,12 @@ struct Args {
     // tensors dc_n (6 per source, xx xy xz yy yz zz)
     const T *bc_w;
     const T *bc_phi;
+    // periodic boundaries: bit a set = axis a wraps (neighbours of the
+    // further-term instantiations); gtens = complex Newell spectrum of the
+    // periodic kernel (thin film [c][kx][ky], 3-D [kx][kz][ky][c]; NULL =
+    // the reduced real spectrum ntens)
+    int pbc;
+    const T *gtens;
     const int *dc_ptr;
     const int *dc_src;
     const T *dc_n;

@@ -552,9 +552,24 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
     constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
     const int N = (int)A.ncell;
     const int sy = A.nx, sz = A.nx * A.ny;
-    const int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
-                             y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
-                             z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
+    int nb[6] = {x > 0 ? cell - 1 : cell, x < A.nx - 1 ? cell + 1 : cell,
+                 y > 0 ? cell - sy : cell, y < A.ny - 1 ? cell + sy : cell,
+                 z > 0 ? cell - sz : cell, z < A.nz - 1 ? cell + sz : cell};
+    if constexpr (XT) {
+        // periodic axes wrap (neighbor_index, fdm_types.rs:119-132)
+        if (A.pbc & 1) {
+            if (x == 0) nb[0] = cell + A.nx - 1;
+            if (x == A.nx - 1) nb[1] = cell - (A.nx - 1);
+        }
+        if (A.pbc & 2) {
+            if (y == 0) nb[2] = cell + (A.ny - 1) * sy;
+            if (y == A.ny - 1) nb[3] = cell - (A.ny - 1) * sy;
+        }
+        if (A.pbc & 4) {
+            if (z == 0) nb[4] = cell + (A.nz - 1) * sz;
+            if (z == A.nz - 1) nb[5] = cell - (A.nz - 1) * sz;
+        }
+    }
     g.act = A.mask ? A.mask[cell] : 1;
 #pragma unroll
     for (int q = 0; q < 6; ++q) g.nact[q] = A.mask ? A.mask[nb[q]] : 1;
@@ -1464,6 +1479,20 @@ __global__ void __launch_bounds__(kThreads) k_y(const Args<T> *__restrict__ AP) 
         res = lds_fft<T, false>(b0, b1, A.fy, tw, 3, ny, py);
         const long long cs = (long long)nkx * (ny + 1);
         const T *N = A.ntens + (long long)kx * (ny + 1);
+        if (A.gtens) {
+            // complex spectrum of a periodic kernel: [c][kx][ky], all ky
+            const C2<T> *G = reinterpret_cast<const C2<T> *>(A.gtens) + (long long)kx * py;
+            const long long gs = (long long)nkx * py;
+            for (int ky = threadIdx.x; ky < py; ky += blockDim.x) {
+                const C2<T> gxx = G[ky], gyy = G[gs + ky], gzz = G[2 * gs + ky], gxy = G[3 * gs + ky];
+                const C2<T> mx = res[ky], my = res[py + ky], mz = res[2 * py + ky];
+                const C2<T> hx = cmul(gxx, mx), hx2 = cmul(gxy, my), hy = cmul(gxy, mx), hy2 = cmul(gyy, my);
+                const C2<T> hz = cmul(gzz, mz);
+                res[ky] = {-(hx.x + hx2.x), -(hx.y + hx2.y)};
+                res[py + ky] = {-(hy.x + hy2.x), -(hy.y + hy2.y)};
+                res[2 * py + ky] = {-hz.x, -hz.y};
+            }
+        } else
         for (int ky = threadIdx.x; ky < py; ky += blockDim.x) {
             const bool lo = ky <= ny;
             const int kr = lo ? ky : py - ky;
@@ -1617,6 +1646,22 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
         const int t = i / pz, kz = i - t * pz;
         const int ky = ky0 + t;
         if (ky >= py) continue;
+        if (A.gtens) {
+            // complex spectrum of a periodic kernel: [kx][kz][ky][c]
+            const C2<T> *G = reinterpret_cast<const C2<T> *>(A.gtens) + (((size_t)kx * pz + kz) * py + ky) * 6;
+            C2<T> &ex = res[(0 * TY + t) * pz + kz];
+            C2<T> &ey = res[(1 * TY + t) * pz + kz];
+            C2<T> &ez = res[(2 * TY + t) * pz + kz];
+            const C2<T> mx = ex, my = ey, mz = ez;
+            auto dot3 = [](C2<T> a, C2<T> u, C2<T> b, C2<T> v, C2<T> c, C2<T> w) {
+                const C2<T> p = cmul(a, u), q = cmul(b, v), r = cmul(c, w);
+                return C2<T>{-(p.x + q.x + r.x), -(p.y + q.y + r.y)};
+            };
+            ex = dot3(G[0], mx, G[3], my, G[4], mz);
+            ey = dot3(G[3], mx, G[1], my, G[5], mz);
+            ez = dot3(G[4], mx, G[5], my, G[2], mz);
+            continue;
+        }
         const bool ylo = ky <= ny, zlo = kz <= nz;
         const int kyr = ylo ? ky : py - ky, kzr = zlo ? kz : pz - kz;
         const T sy = ylo ? T(1) : T(-1), sz = zlo ? T(1) : T(-1);
@@ -2646,7 +2691,7 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
     A.yfast = 0;
     {
         int r1 = 0, r2 = 0;
-        if (A.nz == 1 && A.tw4y && yfast_shape(A.py, r1, r2)) A.yfast = A.py;
+        if (A.nz == 1 && A.tw4y && !A.gtens && yfast_shape(A.py, r1, r2)) A.yfast = A.py;
     }
     A.ylfast = 0;
     A.zfast = 0;
@@ -2655,7 +2700,8 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
         int r1 = 0, r2 = 0;
         if (A.tw4y && A.tw4y_f && yfast_shape(A.py, r1, r2) && set_lds_yl<T>(A.py, lds_limit) == hipSuccess)
             A.ylfast = A.py;
-        if (A.tw4z && zfast_shape(A.pz, r1, r2) && set_lds_zf<T>(A.pz, lds_limit) == hipSuccess) A.zfast = A.pz;
+        if (A.tw4z && !A.gtens && zfast_shape(A.pz, r1, r2) && set_lds_zf<T>(A.pz, lds_limit) == hipSuccess)
+            A.zfast = A.pz;
     }
     if (!A.yfast && !A.ylfast && lds_y(A) > lds_limit) return "padded y length does not fit the LDS of one work-group";
     if (A.nz > 1 && !A.zfast) {

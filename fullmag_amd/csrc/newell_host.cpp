@@ -16,6 +16,7 @@
 // is kept: thin film 4 x (nx+1)(ny+1) values, 3-D 6 x (nx+1)(ny+1)(nz+1).
 // Transform: per axis, a complex FFT of each (parity-extended) line of the
 // first-octant data, keeping the lower half of the output.
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -258,6 +259,132 @@ std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, d
         }
     }
     return out;
+}
+
+// Spectrum with periodic axes (crates/fullmag-engine/src/fdm_fft.rs:140-229,
+// 611-717).  On a periodic axis of n cells the reference convolves
+// circularly over n with the truncated image sum
+//   K(i) = sum_{m=-I..I} N(i + m n),  i in [0, n)
+// (images on both sides of the base offset i >= 0, so K is not even and its
+// spectrum is complex).  The engine keeps its zero-padded length 2n on that
+// axis and places K there so that the padded linear convolution of the n
+// data points equals the circular one: padded index p < n holds K(p),
+// p > n holds K(p - n) (offset p - 2n = (p - n) - n), index n is never used.
+// Open axes are as in the open tensor; a thin film (nz == 1) keeps its 2-D
+// transform, with the z images summed when z is periodic.  Output, complex
+// interleaved: thin film [c][kx][ky] (c = xx,yy,zz,xy), 3-D [kx][kz][ky][c]
+// (c = xx,yy,zz,xy,xz,yz), kx in [0, nx], ky in [0, 2ny), kz in [0, 2nz).
+bool newell_periodic_spectrum(int nx, int ny, int nz, double dx, double dy, double dz, const int per[3],
+                              const int img[3], std::vector<double> &out, std::string *err) {
+    const int n[3] = {nx, ny, nz};
+    const bool film = nz == 1;
+    int ext[3], I[3];
+    double cells = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        I[a] = per[a] ? img[a] : 0;
+        ext[a] = n[a] * (I[a] + 1);
+        cells *= ext[a];
+    }
+    if (cells > (double)(1 << 26)) {
+        *err = "periodic image sum too large (cells x (images + 1) per periodic axis above 2^26)";
+        return false;
+    }
+    const std::vector<double> oct = newell_octant(ext[0], ext[1], ext[2], dx, dy, dz);
+    const size_t on = (size_t)ext[0] * ext[1] * ext[2];
+    const int ncomp = film ? 4 : 6;
+    const int px = 2 * nx, py = 2 * ny, pz = film ? 1 : 2 * nz;
+    const int P[3] = {px, py, pz};
+    // base offset of padded index p on axis a (INT_MIN: unused index)
+    auto base = [&](int a, int p) -> int {
+        if (a == 2 && film) return 0;
+        if (p < n[a]) return p;
+        if (p == n[a]) return INT_MIN;
+        return per[a] ? p - n[a] : p - 2 * n[a];
+    };
+    // parity of the odd axes per component: xx,yy,zz even; xy odd in x,y;
+    // xz odd in x,z; yz odd in y,z
+    const int odd[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
+    const size_t PP = (size_t)px * py * pz;
+    std::vector<cd> k2((size_t)ncomp * PP, cd(0.0));
+    parallel_for(pz, [&](long lo, long hi) {
+        for (long pk = lo; pk < hi; ++pk)
+            for (int pj = 0; pj < py; ++pj)
+                for (int pi = 0; pi < px; ++pi) {
+                    const int b[3] = {base(0, pi), base(1, pj), base(2, (int)pk)};
+                    if (b[0] == INT_MIN || b[1] == INT_MIN || b[2] == INT_MIN) continue;
+                    double sum[6] = {0, 0, 0, 0, 0, 0};
+                    const int step_z = film ? 1 : n[2];
+                    for (int mz = -I[2]; mz <= I[2]; ++mz)
+                        for (int my = -I[1]; my <= I[1]; ++my)
+                            for (int mx = -I[0]; mx <= I[0]; ++mx) {
+                                const int o[3] = {b[0] + mx * n[0], b[1] + my * n[1], b[2] + mz * step_z};
+                                const size_t id = ((size_t)std::abs(o[2]) * ext[1] + std::abs(o[1])) * ext[0] +
+                                                  std::abs(o[0]);
+                                for (int c = 0; c < ncomp; ++c) {
+                                    double s = 1.0;
+                                    for (int a = 0; a < 3; ++a)
+                                        if (odd[c][a] && o[a] < 0) s = -s;
+                                    sum[c] += s * oct[c * on + id];
+                                }
+                            }
+                    const size_t q = ((size_t)pk * py + pj) * px + pi;
+                    for (int c = 0; c < ncomp; ++c) k2[c * PP + q] = sum[c];
+                }
+    });
+    // complex FFT along x (kx in [0, nx] kept), y and z (all kept)
+    const int nkx = nx + 1;
+    std::vector<cd> sx((size_t)ncomp * nkx * py * pz);
+    {
+        LineFft f(px);
+        parallel_for((long)ncomp * py * pz, [&](long lo, long hi) {
+            std::vector<cd> a(px), t(px);
+            for (long l = lo; l < hi; ++l) {
+                const long c = l / ((long)py * pz), r = l % ((long)py * pz);
+                for (int i = 0; i < px; ++i) a[i] = k2[c * PP + (size_t)r * px + i];
+                f.run(a.data(), t.data());
+                for (int k = 0; k < nkx; ++k) sx[((size_t)c * py * pz + r) * nkx + k] = a[k];
+            }
+        });
+    }
+    // layout now [c][z][y][kx]; y transform
+    {
+        LineFft f(py);
+        parallel_for((long)ncomp * pz * nkx, [&](long lo, long hi) {
+            std::vector<cd> a(py), t(py);
+            for (long l = lo; l < hi; ++l) {
+                const long c = l / ((long)pz * nkx), r = l % ((long)pz * nkx), z = r / nkx, kx = r % nkx;
+                auto at = [&](int y) -> cd & { return sx[(((size_t)c * pz + z) * py + y) * nkx + kx]; };
+                for (int y = 0; y < py; ++y) a[y] = at(y);
+                f.run(a.data(), t.data());
+                for (int y = 0; y < py; ++y) at(y) = a[y];
+            }
+        });
+    }
+    if (pz > 1) {
+        LineFft f(pz);
+        parallel_for((long)ncomp * py * nkx, [&](long lo, long hi) {
+            std::vector<cd> a(pz), t(pz);
+            for (long l = lo; l < hi; ++l) {
+                const long c = l / ((long)py * nkx), r = l % ((long)py * nkx), y = r / nkx, kx = r % nkx;
+                auto at = [&](int z) -> cd & { return sx[(((size_t)c * pz + z) * py + y) * nkx + kx]; };
+                for (int z = 0; z < pz; ++z) a[z] = at(z);
+                f.run(a.data(), t.data());
+                for (int z = 0; z < pz; ++z) at(z) = a[z];
+            }
+        });
+    }
+    out.assign(2 * sx.size(), 0.0);
+    for (int c = 0; c < ncomp; ++c)
+        for (int z = 0; z < pz; ++z)
+            for (int y = 0; y < py; ++y)
+                for (int kx = 0; kx < nkx; ++kx) {
+                    const cd v = sx[(((size_t)c * pz + z) * py + y) * nkx + kx];
+                    const size_t o = film ? ((size_t)c * nkx + kx) * py + y
+                                          : (((size_t)kx * pz + z) * py + y) * ncomp + c;
+                    out[2 * o] = v.real();
+                    out[2 * o + 1] = v.imag();
+                }
+    return true;
 }
 
 // Reduced spectrum from reference-format spectra (complex interleaved over the

@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 5
+#define FULLMAG_FDM_EXT_VERSION 6
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -47,6 +47,13 @@ typedef struct {
      * of the version-4 size (ending at adaptive_shrink_limit) is accepted. */
     double        thermal_dt;             /* s; 0 = the dt of each step attempt          */
     uint64_t      thermal_seed;           /* counter-generator seed; 0 = 42              */
+    /* version 6: periodic boundaries of the CPU engine (FdmBoundaryPolicy,
+     * fdm_types.rs:94-132; demag by truncated image sums,
+     * fdm_fft.rs:140-229,611-717).  Bit 0/1/2: x/y/z periodic; images per
+     * periodic axis (0 = the CPU engine's default 10).  Smaller structs
+     * (versions 4 and 5) are accepted. */
+    uint32_t      periodic_axes;
+    uint32_t      periodic_images[3];
 } fullmag_fdm_plan_ext;
 
 /* create() with the extension struct (ext may be NULL: same as create()). */

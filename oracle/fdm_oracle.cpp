@@ -402,6 +402,10 @@ typedef struct {
     uint32_t dc_count, dc_stencil;
     const int32_t *dc_target, *dc_source;
     const double *dc_tensor;
+    // periodic boundaries (FdmBoundaryPolicy, fdm_types.rs:94-132): bit 0/1/2
+    // = x/y/z periodic; demag image counts per axis (0 = 10)
+    uint32_t periodic;
+    uint32_t images[3];
 } orc_desc;
 
 typedef struct {
@@ -449,6 +453,15 @@ struct Problem {
     double abm_dt = 0.0;
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
+    bool per(int a) const { return (d.periodic >> a) & 1u; }
+    // neighbor_index, fdm_types.rs:119-132: wrap on periodic axes, clamp
+    // otherwise
+    long nbr(int a, long i, long delta) const {
+        const long n = a == 0 ? d.nx : a == 1 ? d.ny : d.nz;
+        const long j = i + delta;
+        if (per(a)) return (j % n + n) % n;
+        return j < 0 ? 0 : j > n - 1 ? n - 1 : j;
+    }
     // magnetic volume fraction of a cell (1 without a tier)
     double phi_at(size_t i) const { return phi.empty() ? 1.0 : phi[i]; }
     double ms_at(size_t i) const { return ms_c.empty() ? d.ms : ms_c[i]; }
@@ -467,6 +480,44 @@ struct Problem {
         }
     }
 };
+
+// Periodic demag kernel by truncated images, fdm_fft.rs:611-717: on a
+// periodic axis the workspace has n points (no padding) and
+// K(i) = sum_{m=-I..I} N_open(i + m n) for i in [0, n), read from the open
+// kernel of the grid extended to n (1 + 2 I) cells (its padded array indexed
+// with wrap-around); open axes keep 2n padded points and m = 0.
+void periodic_kernels(const Problem &p, double *out) {
+    const int n[3] = {(int)p.d.nx, (int)p.d.ny, (int)p.d.nz};
+    int I[3], en[3], ep[3];
+    for (int a = 0; a < 3; ++a) {
+        I[a] = p.per(a) ? (p.d.images[a] ? (int)p.d.images[a] : 10) : 0;
+        en[a] = n[a] * (1 + 2 * I[a]);
+        ep[a] = 2 * en[a];
+    }
+    const size_t EP = (size_t)ep[0] * ep[1] * ep[2];
+    std::vector<double> open(6 * EP);
+    newell_kernels(en[0], en[1], en[2], p.d.dx, p.d.dy, p.d.dz, open.data());
+    const int px = p.px, py = p.py, pz = p.pz;
+    const size_t P = (size_t)px * py * pz;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < pz; ++k)
+        for (int j = 0; j < py; ++j)
+            for (int i = 0; i < px; ++i) {
+                double sum[6] = {0, 0, 0, 0, 0, 0};
+                for (int mz = -I[2]; mz <= I[2]; ++mz)
+                    for (int my = -I[1]; my <= I[1]; ++my)
+                        for (int mx = -I[0]; mx <= I[0]; ++mx) {
+                            auto wrap = [](long g, long e) { return (size_t)(((g % e) + e) % e); };
+                            const size_t ei = wrap(i + (long)mx * n[0], ep[0]);
+                            const size_t ej = wrap(j + (long)my * n[1], ep[1]);
+                            const size_t ek = wrap(k + (long)mz * n[2], ep[2]);
+                            const size_t src = ei + (size_t)ep[0] * (ej + (size_t)ep[1] * ek);
+                            for (int c = 0; c < 6; ++c) sum[c] += open[c * EP + src];
+                        }
+                const size_t dst = (size_t)i + (size_t)px * (j + (size_t)py * k);
+                for (int c = 0; c < 6; ++c) out[c * P + dst] = sum[c];
+            }
+}
 
 // Sparse demag correction, native/backends/fdm/src/demag_boundary_fp64.cu:23-76:
 // H(target) -= sum over its sources of dN . (phi Ms m)(source), dN symmetric
@@ -600,7 +651,7 @@ void exchange_add(const Problem &p, const V3 *m, V3 *h) {
         return;
     }
     const double dx2 = p.d.dx * p.d.dx, dy2 = p.d.dy * p.d.dy, dz2 = p.d.dz * p.d.dz;
-    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    const long nx = p.d.nx, ny = p.d.ny;
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) {
         if (!p.active(s)) continue;
@@ -611,10 +662,10 @@ void exchange_add(const Problem &p, const V3 *m, V3 *h) {
             const size_t q = ((size_t)zi * ny + yi) * nx + xi;
             return p.active(q) ? m[q] : c;
         };
-        // fdm_types.rs:126-132 non-periodic clamp
-        const V3 xm = sample(std::max(x - 1, 0L), y, z), xp = sample(std::min(x + 1, nx - 1), y, z);
-        const V3 ym = sample(x, std::max(y - 1, 0L), z), yp = sample(x, std::min(y + 1, ny - 1), z);
-        const V3 zm = sample(x, y, std::max(z - 1, 0L)), zp = sample(x, y, std::min(z + 1, nz - 1));
+        // fdm_types.rs:119-132 clamp or periodic wrap
+        const V3 xm = sample(p.nbr(0, x, -1), y, z), xp = sample(p.nbr(0, x, 1), y, z);
+        const V3 ym = sample(x, p.nbr(1, y, -1), z), yp = sample(x, p.nbr(1, y, 1), z);
+        const V3 zm = sample(x, y, p.nbr(2, z, -1)), zp = sample(x, y, p.nbr(2, z, 1));
         for (int k = 0; k < 3; ++k)
             h[s].c[k] += pre * ((xp.c[k] - 2.0 * c.c[k] + xm.c[k]) / dx2 + (yp.c[k] - 2.0 * c.c[k] + ym.c[k]) / dy2 +
                                 (zp.c[k] - 2.0 * c.c[k] + zm.c[k]) / dz2);
@@ -687,7 +738,7 @@ void cubic_add(const Problem &p, const V3 *m, V3 *h) {
 // (zero on empty cells); H_i = 2D/(mu0 Ms) (dz mz... ), H_b = -2D/(mu0 Ms) curl m.
 void dmi_add(const Problem &p, const V3 *m, V3 *h) {
     if (p.d.dmi_i == 0.0 && p.d.dmi_b == 0.0) return;
-    const long nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
+    const long nx = p.d.nx, ny = p.d.ny;
     const double i2x = 1.0 / (2.0 * p.d.dx), i2y = 1.0 / (2.0 * p.d.dy), i2z = 1.0 / (2.0 * p.d.dz);
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) {
@@ -695,9 +746,9 @@ void dmi_add(const Problem &p, const V3 *m, V3 *h) {
         const double ms = std::max(p.ms_at(s), 1e-30);
         const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
         auto at = [&](long xi, long yi, long zi) { return m[((size_t)zi * ny + yi) * nx + xi]; };
-        const V3 xm = at(std::max(x - 1, 0L), y, z), xp = at(std::min(x + 1, nx - 1), y, z);
-        const V3 ym = at(x, std::max(y - 1, 0L), z), yp = at(x, std::min(y + 1, ny - 1), z);
-        const V3 zm = at(x, y, std::max(z - 1, 0L)), zp = at(x, y, std::min(z + 1, nz - 1));
+        const V3 xm = at(p.nbr(0, x, -1), y, z), xp = at(p.nbr(0, x, 1), y, z);
+        const V3 ym = at(x, p.nbr(1, y, -1), z), yp = at(x, p.nbr(1, y, 1), z);
+        const V3 zm = at(x, y, p.nbr(2, z, -1)), zp = at(x, y, p.nbr(2, z, 1));
         if (p.d.dmi_i != 0.0) {
             const double pf = 2.0 * p.d.dmi_i / (MU0 * ms);
             h[s].c[0] += pf * (xp.c[2] - xm.c[2]) * i2x;
@@ -796,8 +847,13 @@ void torque_add(const Problem &p, const V3 *m, V3 *k) {
             V3 du = {{0.0, 0.0, 0.0}};
             for (int a = 0; a < 3; ++a) {
                 const double u = b * p.d.j[a];
-                if (u > 0.0 && pos[a] > 0) du = v_add(du, v_scale(v_add(mm, v_scale(m[s - st[a]], -1.0)), u / dd[a]));
-                else if (u < 0.0 && pos[a] + 1 < len[a]) du = v_add(du, v_scale(v_add(m[s + st[a]], v_scale(mm, -1.0)), u / dd[a]));
+                // upwind difference; a box face contributes nothing unless
+                // the axis is periodic (fdm_fields.rs:1290-1330)
+                const long prev = s + (p.nbr(a, pos[a], -1) - pos[a]) * st[a];
+                const long next = s + (p.nbr(a, pos[a], 1) - pos[a]) * st[a];
+                if (u > 0.0 && (p.per(a) || pos[a] > 0)) du = v_add(du, v_scale(v_add(mm, v_scale(m[prev], -1.0)), u / dd[a]));
+                else if (u < 0.0 && (p.per(a) || pos[a] + 1 < len[a]))
+                    du = v_add(du, v_scale(v_add(m[next], v_scale(mm, -1.0)), u / dd[a]));
             }
             const V3 c = v_cross(mm, du), dc = v_cross(mm, c);
             k[s] = v_add(k[s], v_add(v_scale(dc, -1.0), v_scale(c, -beta)));
@@ -1063,15 +1119,16 @@ void *orc_create(const orc_desc *d) {
     p->rhs.resize(n);
     for (auto &k : p->k) k.resize(n);
     if (d->demag) {
-        p->px = 2 * d->nx;
-        p->py = 2 * d->ny;
-        p->pz = 2 * d->nz;
+        p->px = p->per(0) ? d->nx : 2 * d->nx;
+        p->py = p->per(1) ? d->ny : 2 * d->ny;
+        p->pz = p->per(2) ? d->nz : 2 * d->nz;
         p->P = (size_t)p->px * p->py * p->pz;
         p->fx = FftPlan(p->px);
         p->fy = FftPlan(p->py);
         p->fz = FftPlan(p->pz);
         std::vector<double> nk(6 * p->P);
-        newell_kernels(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, nk.data());
+        if (d->periodic) periodic_kernels(*p, nk.data());
+        else newell_kernels(d->nx, d->ny, d->nz, d->dx, d->dy, d->dz, nk.data());
         std::vector<cplx> *dst[6] = {&p->kxx, &p->kyy, &p->kzz, &p->kxy, &p->kxz, &p->kyz};
         for (int c = 0; c < 6; ++c) {
             dst[c]->resize(p->P);

@@ -70,6 +70,7 @@ class _Desc(ctypes.Structure):
         ("dc_count", ctypes.c_uint32), ("dc_stencil", ctypes.c_uint32),
         ("dc_target", ctypes.POINTER(ctypes.c_int32)), ("dc_source", ctypes.POINTER(ctypes.c_int32)),
         ("dc_tensor", ctypes.POINTER(ctypes.c_double)),
+        ("periodic", ctypes.c_uint32), ("images", ctypes.c_uint32 * 3),
     ]
 
 
@@ -251,6 +252,10 @@ class Problem:
     face_links: tuple | None = None
     deltas: tuple | None = None
     demag_corr: tuple | None = None
+    # periodic boundaries (fdm_types.rs:94-132) and demag images per axis
+    # (0 = the CPU engine's default 10)
+    periodic: tuple = (False, False, False)
+    periodic_images: tuple = (0, 0, 0)
 
 
 class Oracle:
@@ -324,6 +329,8 @@ class Oracle:
                 d.oe_freq, d.oe_phase, d.oe_offset, d.oe_t_on, d.oe_t_off = o[5:10]
         d.temperature, d.thermal_dt, d.thermal_seed = pb.temperature, pb.thermal_dt, pb.thermal_seed
         d.bc_tier, d.bc_phi_floor, d.bc_delta_min = pb.boundary_tier, pb.phi_floor, pb.delta_min
+        d.periodic = sum(1 << a for a in range(3) if pb.periodic[a])
+        d.images[:] = [int(v) for v in pb.periodic_images]
 
         def f64(a):
             a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
