@@ -216,6 +216,7 @@ class PlanExt(ctypes.Structure):
         ("thermal_seed", ctypes.c_uint64),
         ("periodic_axes", ctypes.c_uint32),
         ("periodic_images", ctypes.c_uint32 * 3),
+        ("layer_z", c_double_p),
     ]
 
 

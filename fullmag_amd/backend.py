@@ -94,12 +94,13 @@ class FdmPlan:
     thermal_seed: int = 0                    # counter-generator seed; 0 = 42
     periodic: tuple = (False, False, False)  # periodic x, y, z (fdm_types.rs:94-132)
     periodic_images: tuple = (0, 0, 0)       # images per periodic axis; 0 = 10
+    layer_z: np.ndarray | None = None        # multilayer: z centre of each plane (m)
     extra: dict = field(default_factory=dict)  # raw PlanDesc fields (for error tests)
 
     def uses_ext(self) -> bool:
         return (self.ms_field is not None or self.axis_field is not None or self.wave != 0
                 or self.rtol > 0 or self.growth > 0 or self.shrink > 0 or self.thermal_dt > 0
-                or self.thermal_seed != 0 or any(self.periodic))
+                or self.thermal_seed != 0 or any(self.periodic) or self.layer_z is not None)
 
 
 _INTEGRATORS = {"heun": abi.INTEGRATOR_HEUN, "dp45": abi.INTEGRATOR_DP45, "rk45": abi.INTEGRATOR_DP45,
@@ -226,6 +227,10 @@ def marshal_plan(plan: FdmPlan):
         ext.thermal_dt, ext.thermal_seed = plan.thermal_dt, plan.thermal_seed
         ext.periodic_axes = sum(1 << a for a in range(3) if plan.periodic[a])
         ext.periodic_images[:] = [int(v) for v in plan.periodic_images]
+        if plan.layer_z is not None:
+            lz = np.ascontiguousarray(np.asarray(plan.layer_z, dtype=np.float64).reshape(-1))
+            keep.append(lz)
+            ext.layer_z = _dp(lz)
     return desc, ext, keep
 
 

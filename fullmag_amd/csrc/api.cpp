@@ -65,6 +65,7 @@ bool zfast_factors(int pz, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
 bool newell_periodic_spectrum(int nx, int ny, int nz, double dx, double dy, double dz, const int per[3],
                               const int img[3], std::vector<double> &out, std::string *err);
+std::vector<double> newell_layer_spectrum(int nx, int ny, double dx, double dy, double dz, double s);
 bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int ny, int nz, std::vector<double> &out,
                           std::string *err);
 }  // namespace fm
@@ -905,7 +906,9 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     // periodic boundaries (plan extension version 6; the CPU engine's
     // FdmBoundaryPolicy with demag_image_counts, default 10 per axis)
     int per[3] = {0, 0, 0}, img[3] = {10, 10, 10};
-    if (x && x->struct_size >= sizeof(fullmag_fdm_plan_ext) && x->periodic_axes) {
+    const bool v6 = x && x->struct_size >= offsetof(fullmag_fdm_plan_ext, layer_z);
+    const double *layer_z = x && x->struct_size >= sizeof(fullmag_fdm_plan_ext) ? x->layer_z : nullptr;
+    if (v6 && x->periodic_axes) {
         if (x->periodic_axes & ~7u) return (h->err = "periodic_axes: only bits 0 (x), 1 (y), 2 (z)", false);
         if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE)
             return (h->err = "periodic boundaries cannot be combined with boundary-correction tiers", false);
@@ -918,6 +921,19 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             if (per[a] && img[a] > 1000) return (h->err = "periodic_images above 1000", false);
         }
         A.pbc = (int)x->periodic_axes;
+        A.xt = 1;
+    }
+    if (layer_z) {
+        if (nz < 2 || nz > fm::kMaxLayers)
+            return (h->err = "layer_z needs 2.." + std::to_string(fm::kMaxLayers) + " planes (grid nz)", false);
+        for (uint32_t k = 0; k < nz; ++k)
+            if (!std::isfinite(layer_z[k])) return (h->err = "layer_z must be finite", false);
+        if (per[2]) return (h->err = "layer_z cannot be combined with a periodic z axis", false);
+        if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE)
+            return (h->err = "layer_z cannot be combined with boundary-correction tiers", false);
+        if (p->demag_kernel_xx_spectrum)
+            return (h->err = "layer_z needs the engine's own demag kernel (plan spectra are a uniform grid)", false);
+        A.layered = 1;
         A.xt = 1;
     }
     if (!parse_boundary(h, A, p, x, ms)) return false;
@@ -950,7 +966,25 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         std::vector<double> red;
         const double *spec[6] = {p->demag_kernel_xx_spectrum, p->demag_kernel_yy_spectrum, p->demag_kernel_zz_spectrum,
                                  p->demag_kernel_xy_spectrum, p->demag_kernel_xz_spectrum, p->demag_kernel_yz_spectrum};
-        if (A.pbc) {
+        if (layer_z) {
+            // pair kernels of every (destination, source) plane; the self
+            // pairs are the film kernel (shift 0)
+            const int L = nz;
+            const size_t ps = (size_t)(nx + 1) * (ny + 1);
+            std::vector<T> lk((size_t)L * L * 6 * ps);
+            for (int l = 0; l < L; ++l)
+                for (int q = 0; q < L; ++q) {
+                    const std::vector<double> k = fm::newell_layer_spectrum(nx, ny, p->grid.dx, p->grid.dy, p->grid.dz,
+                                                                            layer_z[l] - layer_z[q]);
+                    for (size_t i = 0; i < 6 * ps; ++i) lk[(size_t)(l * L + q) * 6 * ps + i] = (T)k[i];
+                }
+            T *ld = static_cast<T *>(dalloc(h, lk.size() * sizeof(T)));
+            if (!ld) return hip_ok(h, hipErrorOutOfMemory, "alloc layer kernels");
+            if (!hip_ok(h, hipMemcpy(ld, lk.data(), lk.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))
+                return false;
+            A.lmix = ld;
+            A.inv_p = 1.0 / ((double)A.px * A.py);
+        } else if (A.pbc) {
             std::vector<double> gs;
             if (!fm::newell_periodic_spectrum(nx, ny, nz, p->grid.dx, p->grid.dy, p->grid.dz, per, img, gs, &e)) {
                 h->err = e;

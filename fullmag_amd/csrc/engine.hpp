@@ -31,6 +31,7 @@ constexpr int kMaxK = 5;            // k-vectors in one stage combination (DP45 
 constexpr int kMaxErr = 6;          // terms of the embedded error estimate (DP45)
 constexpr int kHist = 8;            // StageRow::kidx kHist + r names the ABM3 history value f_{n-r}
 constexpr int kHistSlot = 4;        // first k_all slot of the ABM3 history ring
+constexpr int kMaxLayers = 8;       // planes of a multilayer plan
 
 // Per-block partial layout (doubles).
 enum StatSlot {
@@ -229,6 +230,13 @@ struct Args {
     // the reduced real spectrum ntens)
     int pbc;
     const T *gtens;
+    // multilayer: the nz planes are single-cell layers at arbitrary heights,
+    // coupled only by demag through the pair kernels lmix
+    // [dst][src][c][kx][ky] (c = xx,yy,zz,xy real, xz,yz imaginary parts;
+    // kx <= nx, ky <= ny; NULL without demag or on a uniform grid);
+    // layered = 1: no exchange/DMI/torque neighbours along z
+    const T *lmix;
+    int layered;
     const int *dc_ptr;
     const int *dc_src;
     const T *dc_n;

@@ -569,6 +569,8 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
             if (z == 0) nb[4] = cell + (A.nz - 1) * sz;
             if (z == A.nz - 1) nb[5] = cell - (A.nz - 1) * sz;
         }
+        // multilayer: the planes are separate bodies (no z neighbours)
+        if (A.layered) nb[4] = nb[5] = cell;
     }
     g.act = A.mask ? A.mask[cell] : 1;
 #pragma unroll
@@ -1684,6 +1686,56 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
         const int zz = rest % nz, c = rest / nz;
         const int ky = ky0 + t;
         if (ky < py) S[(((long long)c * A.nzs + zz) * nkx + kx) * ly + ky] = res[(c * TY + t) * pz + zz];
+    }
+}
+
+// Multilayer demag in the spectral plane (crates/fullmag-engine/src/
+// multilayer.rs:166-260): after the x and y transforms of every plane, each
+// (kx, ky) column holds the L planes' spectra; plane l receives
+// H_l = -sum_q K_{l q} M_q with the pair kernels of the layer heights.  One
+// thread per column, the column's M in registers.
+template <class T>
+__global__ void __launch_bounds__(kThreads) k_lmix(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
+    if (A.ctl->done) return;
+    const int py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, L = A.nz;
+    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long long)nkx * py) return;
+    const int kx = (int)(col / py), ky = (int)(col - (long long)kx * py);
+    const bool lo = ky <= ny;
+    const int kr = lo ? ky : py - ky;
+    const T sy = lo ? T(1) : T(-1);
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec);
+    const size_t cs = (size_t)A.nzs * nkx * ly;
+    C2<T> mv[3][kMaxLayers];
+#pragma unroll
+    for (int q = 0; q < kMaxLayers; ++q)
+        if (q < L) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mv[c][q] = S[c * cs + ((size_t)q * nkx + kx) * ly + ky];
+        }
+    const size_t ps = (size_t)nkx * (ny + 1);
+    const size_t at = (size_t)kx * (ny + 1) + kr;
+#pragma unroll
+    for (int l = 0; l < kMaxLayers; ++l) {
+        if (l >= L) continue;
+        T h[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int q = 0; q < kMaxLayers; ++q) {
+            if (q >= L) continue;
+            const T *K = A.lmix + (size_t)(l * L + q) * 6 * ps + at;
+            const T xx = K[0], yy = K[ps], zz = K[2 * ps], xy = sy * K[3 * ps], xz = K[4 * ps], yz = sy * K[5 * ps];
+            const C2<T> mx = mv[0][q], my = mv[1][q], mz = mv[2][q];
+            // xz and yz are i times the stored values
+            h[0] += xx * mx.x + xy * my.x - xz * mz.y;
+            h[1] += xx * mx.y + xy * my.y + xz * mz.x;
+            h[2] += xy * mx.x + yy * my.x - yz * mz.y;
+            h[3] += xy * mx.y + yy * my.y + yz * mz.x;
+            h[4] += zz * mz.x - xz * mx.y - yz * my.y;
+            h[5] += zz * mz.y + xz * mx.x + yz * my.x;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S[c * cs + ((size_t)l * nkx + kx) * ly + ky] = C2<T>{-h[2 * c], -h[2 * c + 1]};
     }
 }
 
@@ -2899,7 +2951,12 @@ static void launch_yz(const Args<T> &A, hipStream_t st) {
         else k_y<T, 0><<<bgrid(A, dim3(A.nkx)), dim3(kThreads), lds_y(A), st>>>(A.self_d);
     } else {
         launch_y3(A, false, st);
-        launch_z3(A, st);
+        if (A.lmix) {
+            const unsigned g = (unsigned)(((long long)A.nkx * A.py + kThreads - 1) / kThreads);
+            k_lmix<T><<<bgrid(A, dim3(g)), dim3(kThreads), 0, st>>>(A.self_d);
+        } else {
+            launch_z3(A, st);
+        }
         launch_y3(A, true, st);
     }
 }

@@ -261,6 +261,78 @@ std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, d
     return out;
 }
 
+// Thin-layer pair kernel of the multilayer runtime
+// (crates/fullmag-fdm-demag/src/newell.rs:528-660, shifted_kernel.rs:24-46):
+// the tensor between two single-cell-thick layers whose centres are `s`
+// apart along z (s = z_destination - z_source) is N(i dx, j dy, s), from the
+// 27-point second difference of f/g sampled at z in {s - dz, s, s + dz}, or
+// the point dipole where i >= 40, j >= 40 or i^2 + j^2 >= 40^2 (the
+// reference decides on the integer offsets, not on s).  In x and y the
+// components keep their parity (xz odd in x, yz odd in y), so the 2-D
+// spectrum is real for xx, yy, zz, xy and imaginary for xz, yz.  Output:
+// 6 x nkx x (ny+1) values [c][kx][ky]: the real parts of xx,yy,zz,xy and
+// the imaginary parts of xz, yz.
+std::vector<double> newell_layer_spectrum(int nx, int ny, double dx, double dy, double dz, double s) {
+    const int fx = std::min(nx, kFar) + 2, fy = std::min(ny, kFar) + 2, fz = 3;
+    const size_t fl = (size_t)fx * fy * fz;
+    std::vector<double> pv[6];
+    for (auto &v : pv) v.resize(fl);
+    for (int k = 0; k < fz; ++k)
+        for (int j = 0; j < fy; ++j)
+            for (int i = 0; i < fx; ++i) {
+                const double x = (i - 1) * dx, y = (j - 1) * dy, z = (k - 1) * dz + s;
+                const size_t id = (size_t)k * fy * fx + (size_t)j * fx + i;
+                pv[0][id] = pot_f(x, y, z);
+                pv[1][id] = pot_f(y, x, z);
+                pv[2][id] = pot_f(z, y, x);
+                pv[3][id] = pot_g(x, y, z);
+                pv[4][id] = pot_g(x, z, y);
+                pv[5][id] = pot_g(y, z, x);
+            }
+    const size_t n = (size_t)nx * ny;
+    std::vector<double> tens(6 * n);
+    const double vol = dx * dy * dz, den = 4.0 * kPi * vol;
+    parallel_for(ny, [&](long lo, long hi) {
+        for (long j = lo; j < hi; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t o = (size_t)j * nx + i;
+                if (i >= kFar || j >= kFar || (long)i * i + j * j >= (long)kFar * kFar) {
+                    const double x = i * dx, y = j * dy, z = s;
+                    const double r2 = x * x + y * y + z * z, r = std::sqrt(r2);
+                    const double r3 = r2 * r, r5 = r2 * r2 * r;
+                    auto dia = [&](double a) { return (1.0 / r3 - 3.0 * a * a / (r3 * r2)) / (4.0 * kPi) * vol; };
+                    auto off = [&](double a, double b) { return -3.0 * a * b / (4.0 * kPi * r5) * vol; };
+                    tens[o] = dia(x);
+                    tens[n + o] = dia(y);
+                    tens[2 * n + o] = dia(z);
+                    tens[3 * n + o] = off(x, y);
+                    tens[4 * n + o] = off(x, z);
+                    tens[5 * n + o] = off(y, z);
+                } else {
+                    for (int c = 0; c < 6; ++c) tens[c * n + o] = second_difference(pv[c], fx, fy, i, (int)j, 0) / den;
+                }
+            }
+    });
+    const int nkx = nx + 1, nky = ny + 1;
+    const size_t red = (size_t)nkx * nky;
+    std::vector<double> out(6 * red);
+    const int par[6][2] = {{1, 1}, {1, 1}, {1, 1}, {-1, -1}, {-1, 1}, {1, -1}};
+    for (int c = 0; c < 6; ++c) {
+        std::vector<cd> a(n);
+        for (size_t i = 0; i < n; ++i) a[i] = tens[c * n + i];
+        std::vector<cd> b((size_t)nkx * ny);
+        axis_transform(a, b, nx, ny, 1, 1, nx, 0, 1, nkx, 0, par[c][0]);
+        std::vector<cd> d((size_t)nkx * nky);
+        axis_transform(b, d, ny, nkx, 1, nkx, 1, 0, nkx, 1, 0, par[c][1]);
+        for (int kx = 0; kx < nkx; ++kx)
+            for (int ky = 0; ky < nky; ++ky) {
+                const cd v = d[(size_t)ky * nkx + kx];
+                out[c * red + (size_t)kx * nky + ky] = c < 4 ? v.real() : v.imag();
+            }
+    }
+    return out;
+}
+
 // Spectrum with periodic axes (crates/fullmag-engine/src/fdm_fft.rs:140-229,
 // 611-717).  On a periodic axis of n cells the reference convolves
 // circularly over n with the truncated image sum

@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 6
+#define FULLMAG_FDM_EXT_VERSION 7
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -54,6 +54,12 @@ typedef struct {
      * (versions 4 and 5) are accepted. */
     uint32_t      periodic_axes;
     uint32_t      periodic_images[3];
+    /* version 7: multilayer stack (crates/fullmag-engine/src/multilayer.rs,
+     * shifted kernels crates/fullmag-fdm-demag/src/shifted_kernel.rs): the
+     * grid's nz planes (2..8) are separate single-cell layers whose centres
+     * lie at layer_z[k] (m); they couple only through demag, each plane keeps
+     * its own exchange.  NULL = uniform grid.  Smaller structs accepted. */
+    const double *layer_z;
 } fullmag_fdm_plan_ext;
 
 /* create() with the extension struct (ext may be NULL: same as create()). */

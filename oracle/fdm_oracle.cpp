@@ -406,6 +406,9 @@ typedef struct {
     // = x/y/z periodic; demag image counts per axis (0 = 10)
     uint32_t periodic;
     uint32_t images[3];
+    // multilayer (multilayer.rs): the nz planes are single-cell layers with
+    // centres at layer_z[k], coupled only by demag (NULL = uniform grid)
+    const double *layer_z;
 } orc_desc;
 
 typedef struct {
@@ -432,6 +435,8 @@ struct Problem {
     std::vector<double> face[6], dlt[6];      // T0 face links, T1 distances
     std::vector<int32_t> dc_t, dc_s;          // sparse demag correction
     std::vector<double> dc_n;
+    std::vector<double> lz;                   // multilayer plane heights
+    std::vector<std::vector<cplx>> lk;        // pair kernel spectra [(dst*L + src)*6 + c]
     uint64_t th_step = 0;     // thermal counter: step calls so far
     double th_dt_cur = 0.0;   // dt of the step attempt in flight
     double t_eval = 0.0;  // time at which the Zeeman waveform is evaluated
@@ -454,11 +459,13 @@ struct Problem {
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
     bool per(int a) const { return (d.periodic >> a) & 1u; }
+    bool layered() const { return !lz.empty(); }
     // neighbor_index, fdm_types.rs:119-132: wrap on periodic axes, clamp
     // otherwise
     long nbr(int a, long i, long delta) const {
         const long n = a == 0 ? d.nx : a == 1 ? d.ny : d.nz;
         const long j = i + delta;
+        if (a == 2 && layered()) return i;  // separate layers: no z neighbours
         if (per(a)) return (j % n + n) % n;
         return j < 0 ? 0 : j > n - 1 ? n - 1 : j;
     }
@@ -519,6 +526,139 @@ void periodic_kernels(const Problem &p, double *out) {
             }
 }
 
+// Multilayer demag, crates/fullmag-engine/src/multilayer.rs:129-260: every
+// layer (one plane of nx x ny cells) is padded to (2nx, 2ny, 2) and
+// transformed; the destination's spectrum is the sum over sources of the
+// pair kernel times the source's, negated and transformed back.  Pair
+// kernels: the exact self kernel for a layer with itself, otherwise the
+// shifted kernel of newell.rs:528-660 / shifted_kernel.rs:24-46 for
+// s = z_destination - z_source: N(i dx, j dy, s) from the 27-point stencil
+// of f/g at z in {s - dz, s, s + dz} (point dipole where i or j >= 40 or
+// i^2 + j^2 >= 40^2), placed with x/y parity at z index 0.
+void shifted_layer_kernel(int nx, int ny, double dx, double dy, double dz, double s, double *out) {
+    const int px = 2 * nx, py = 2 * ny, pz = 2;
+    const size_t P = (size_t)px * py * pz;
+    const int fsx = std::min(nx, ASYMPTOTIC_DISTANCE) + 2, fsy = std::min(ny, ASYMPTOTIC_DISTANCE) + 2, fsz = 3;
+    const size_t flen = (size_t)fsx * fsy * fsz;
+    std::vector<double> fxx(flen), fyy(flen), fzz(flen), gxy(flen), gxz(flen), gyz(flen);
+    for (int k = 0; k < fsz; ++k)
+        for (int j = 0; j < fsy; ++j)
+            for (int i = 0; i < fsx; ++i) {
+                const double x = (i - 1) * dx, y = (j - 1) * dy, z = (k - 1) * dz + s;
+                const size_t id = (size_t)k * fsy * fsx + (size_t)j * fsx + i;
+                fxx[id] = newell_f(x, y, z);
+                fyy[id] = newell_f(y, x, z);
+                fzz[id] = newell_f(z, y, x);
+                gxy[id] = newell_g(x, y, z);
+                gxz[id] = newell_g(x, z, y);
+                gyz[id] = newell_g(y, z, x);
+            }
+    std::memset(out, 0, sizeof(double) * 6 * P);
+    const double hvol = dx * dy * dz;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double v[6];
+            const long d2 = (long)i * i + (long)j * j;
+            if (i >= ASYMPTOTIC_DISTANCE || j >= ASYMPTOTIC_DISTANCE ||
+                d2 >= (long)ASYMPTOTIC_DISTANCE * ASYMPTOTIC_DISTANCE) {
+                const double x = i * dx, y = j * dy;
+                v[0] = asym_diag(x, y, s, hvol);
+                v[1] = asym_diag(y, x, s, hvol);
+                v[2] = asym_diag(s, y, x, hvol);
+                v[3] = asym_off(x, y, s, hvol);
+                v[4] = asym_off(x, s, y, hvol);
+                v[5] = asym_off(y, s, x, hvol);
+            } else {
+                v[0] = stencil27(i, j, 0, fxx, fsx, fsy, hvol);
+                v[1] = stencil27(i, j, 0, fyy, fsx, fsy, hvol);
+                v[2] = stencil27(i, j, 0, fzz, fsx, fsy, hvol);
+                v[3] = stencil27(i, j, 0, gxy, fsx, fsy, hvol);
+                v[4] = stencil27(i, j, 0, gxz, fsx, fsy, hvol);
+                v[5] = stencil27(i, j, 0, gyz, fsx, fsy, hvol);
+            }
+            const int xs_n = i == 0 ? 1 : 2, ys_n = j == 0 ? 1 : 2;
+            const int xi[2] = {i, px - i}, yi[2] = {j, py - j};
+            const double sg[2] = {1.0, -1.0};
+            for (int a = 0; a < xs_n; ++a)
+                for (int b = 0; b < ys_n; ++b) {
+                    const size_t q = (size_t)yi[b] * px + xi[a];
+                    out[q] = v[0];
+                    out[P + q] = v[1];
+                    out[2 * P + q] = v[2];
+                    out[3 * P + q] = v[3] * sg[a] * sg[b];
+                    out[4 * P + q] = v[4] * sg[a];
+                    out[5 * P + q] = v[5] * sg[b];
+                }
+        }
+}
+
+void layer_kernels(Problem &p) {
+    const int nx = p.d.nx, ny = p.d.ny, L = p.d.nz;
+    const size_t P = (size_t)p.px * p.py * p.pz;
+    std::vector<double> self(6 * P), k(6 * P);
+    newell_kernels(nx, ny, 1, p.d.dx, p.d.dy, p.d.dz, self.data());
+    p.lk.assign((size_t)L * L * 6, {});
+    for (int dst = 0; dst < L; ++dst)
+        for (int src = 0; src < L; ++src) {
+            const double *kr = self.data();
+            if (dst != src) {
+                shifted_layer_kernel(nx, ny, p.d.dx, p.d.dy, p.d.dz, p.lz[dst] - p.lz[src], k.data());
+                kr = k.data();
+            }
+            for (int c = 0; c < 6; ++c) {
+                auto &v = p.lk[((size_t)dst * L + src) * 6 + c];
+                v.resize(P);
+                for (size_t i = 0; i < P; ++i) v[i] = cplx(kr[c * P + i], 0.0);
+                fft3(v.data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, false);
+            }
+        }
+}
+
+void demag_layers_add(Problem &p, const V3 *m, V3 *h) {
+    const int nx = p.d.nx, ny = p.d.ny, L = p.d.nz;
+    const size_t P = (size_t)p.px * p.py * p.pz, A = (size_t)nx * ny;
+    std::vector<std::vector<cplx>> M((size_t)3 * L, std::vector<cplx>(P));
+    for (int l = 0; l < L; ++l) {
+        for (int c = 0; c < 3; ++c) std::fill(M[3 * l + c].begin(), M[3 * l + c].end(), cplx(0.0));
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t s = (size_t)l * A + (size_t)y * nx + x, t = (size_t)y * p.px + x;
+                if (!p.active(s)) continue;
+                const double ms = p.ms_at(s);
+                for (int c = 0; c < 3; ++c) M[3 * l + c][t] = m[s].c[c] * ms;
+            }
+        for (int c = 0; c < 3; ++c) fft3(M[3 * l + c].data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, false);
+    }
+    std::vector<cplx> hx(P), hy(P), hz(P);
+    const double norm = 1.0 / (double)P;
+    for (int dst = 0; dst < L; ++dst) {
+        std::fill(hx.begin(), hx.end(), cplx(0.0));
+        std::fill(hy.begin(), hy.end(), cplx(0.0));
+        std::fill(hz.begin(), hz.end(), cplx(0.0));
+        for (int src = 0; src < L; ++src) {
+            const auto *K = &p.lk[((size_t)dst * L + src) * 6];
+            const auto &mx = M[3 * src], &my = M[3 * src + 1], &mz = M[3 * src + 2];
+            for (size_t i = 0; i < P; ++i) {
+                hx[i] += K[0][i] * mx[i] + K[3][i] * my[i] + K[4][i] * mz[i];
+                hy[i] += K[3][i] * mx[i] + K[1][i] * my[i] + K[5][i] * mz[i];
+                hz[i] += K[4][i] * mx[i] + K[5][i] * my[i] + K[2][i] * mz[i];
+            }
+        }
+        for (auto *b : {&hx, &hy, &hz}) {
+            for (auto &v : *b) v = -v;
+            fft3(b->data(), p.px, p.py, p.pz, p.fx, p.fy, p.fz, true);
+        }
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t s = (size_t)dst * A + (size_t)y * nx + x, t = (size_t)y * p.px + x;
+                if (!p.active(s)) continue;
+                h[s].c[0] += hx[t].real() * norm;
+                h[s].c[1] += hy[t].real() * norm;
+                h[s].c[2] += hz[t].real() * norm;
+            }
+    }
+}
+
 // Sparse demag correction, native/backends/fdm/src/demag_boundary_fp64.cu:23-76:
 // H(target) -= sum over its sources of dN . (phi Ms m)(source), dN symmetric
 // [xx, xy, xz, yy, yz, zz]; negative indices are skipped; repeated targets
@@ -544,6 +684,10 @@ void demag_correction_add(const Problem &p, const V3 *m, V3 *h) {
 }
 
 void demag_add(Problem &p, const V3 *m, V3 *h) {
+    if (p.layered()) {
+        demag_layers_add(p, m, h);
+        return;
+    }
     const int nx = p.d.nx, ny = p.d.ny, nz = p.d.nz;
     std::fill(p.bmx.begin(), p.bmx.end(), cplx(0.0));
     std::fill(p.bmy.begin(), p.bmy.end(), cplx(0.0));
@@ -846,6 +990,7 @@ void torque_add(const Problem &p, const V3 *m, V3 *k) {
             const double dd[3] = {p.d.dx, p.d.dy, p.d.dz};
             V3 du = {{0.0, 0.0, 0.0}};
             for (int a = 0; a < 3; ++a) {
+                if (a == 2 && p.layered()) continue;
                 const double u = b * p.d.j[a];
                 // upwind difference; a box face contributes nothing unless
                 // the axis is periodic (fdm_fields.rs:1290-1330)
@@ -1118,7 +1263,18 @@ void *orc_create(const orc_desc *d) {
     p->hs.resize(n);
     p->rhs.resize(n);
     for (auto &k : p->k) k.resize(n);
-    if (d->demag) {
+    if (d->layer_z) p->lz.assign(d->layer_z, d->layer_z + d->nz);
+    p->d.layer_z = nullptr;
+    if (d->demag && p->layered()) {
+        p->px = 2 * d->nx;
+        p->py = 2 * d->ny;
+        p->pz = 2;
+        p->P = (size_t)p->px * p->py * p->pz;
+        p->fx = FftPlan(p->px);
+        p->fy = FftPlan(p->py);
+        p->fz = FftPlan(p->pz);
+        layer_kernels(*p);
+    } else if (d->demag) {
         p->px = p->per(0) ? d->nx : 2 * d->nx;
         p->py = p->per(1) ? d->ny : 2 * d->ny;
         p->pz = p->per(2) ? d->nz : 2 * d->nz;

@@ -71,6 +71,7 @@ class _Desc(ctypes.Structure):
         ("dc_target", ctypes.POINTER(ctypes.c_int32)), ("dc_source", ctypes.POINTER(ctypes.c_int32)),
         ("dc_tensor", ctypes.POINTER(ctypes.c_double)),
         ("periodic", ctypes.c_uint32), ("images", ctypes.c_uint32 * 3),
+        ("layer_z", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -256,6 +257,8 @@ class Problem:
     # (0 = the CPU engine's default 10)
     periodic: tuple = (False, False, False)
     periodic_images: tuple = (0, 0, 0)
+    # multilayer: z centre of each plane (single-cell layers coupled by demag)
+    layer_z: np.ndarray | None = None
 
 
 class Oracle:
@@ -336,6 +339,8 @@ class Oracle:
             a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
             self._keep.append(a)
             return _ptr(a)
+        if pb.layer_z is not None:
+            d.layer_z = f64(pb.layer_z)
         if pb.volume_fraction is not None:
             d.bc_phi = f64(pb.volume_fraction)
         for nm, arrays in (("bc_face", pb.face_links), ("bc_delta", pb.deltas)):
