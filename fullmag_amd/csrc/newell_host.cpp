@@ -378,9 +378,10 @@ bool newell_periodic_spectrum(int nx, int ny, int nz, double dx, double dy, doub
     const int odd[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
     const size_t PP = (size_t)px * py * pz;
     std::vector<cd> k2((size_t)ncomp * PP, cd(0.0));
-    parallel_for(pz, [&](long lo, long hi) {
-        for (long pk = lo; pk < hi; ++pk)
-            for (int pj = 0; pj < py; ++pj)
+    parallel_for((long)pz * py, [&](long lo, long hi) {
+        for (long l = lo; l < hi; ++l) {
+            const long pk = l / py;
+            const int pj = (int)(l % py);
                 for (int pi = 0; pi < px; ++pi) {
                     const int b[3] = {base(0, pi), base(1, pj), base(2, (int)pk)};
                     if (b[0] == INT_MIN || b[1] == INT_MIN || b[2] == INT_MIN) continue;
@@ -402,6 +403,7 @@ bool newell_periodic_spectrum(int nx, int ny, int nz, double dx, double dy, doub
                     const size_t q = ((size_t)pk * py + pj) * px + pi;
                     for (int c = 0; c < ncomp; ++c) k2[c * PP + q] = sum[c];
                 }
+        }
     });
     // complex FFT along x (kx in [0, nx] kept), y and z (all kept)
     const int nkx = nx + 1;
