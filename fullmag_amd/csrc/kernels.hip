@@ -45,6 +45,34 @@ struct C2 {
     T x, y;
 };
 
+// Global-memory accesses.  The buffers reach the kernels through the
+// argument record in device memory, so the compiler cannot infer their
+// address space and emits FLAT instructions, which also count against the
+// LDS wait counter: every wait for an LDS result would then also wait for
+// all outstanding memory loads.  These helpers access through address
+// space 1 (global) pointers.
+template <class T>
+using gp = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ T ldg(const T *p) {
+    return *(gp<const T>)p;
+}
+template <class T>
+__device__ __forceinline__ C2<T> ldg(const C2<T> *p) {
+    const gp<const T> q = (gp<const T>)p;
+    return {q[0], q[1]};
+}
+template <class T>
+__device__ __forceinline__ void stg(T *p, T v) {
+    *(gp<T>)p = v;
+}
+template <class T>
+__device__ __forceinline__ void stg(C2<T> *p, C2<T> v) {
+    const gp<T> q = (gp<T>)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+
 template <class T>
 __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
@@ -1551,16 +1579,16 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     const long long cs = (long long)nkx * (ny + 1);
     const T *NT = A.ntens + (long long)kx * (ny + 1);
     const int other = c == 2 ? 2 : (c ^ 1);
+    // raw values here, the sign of N_xy on the upper half at the multiply,
+    // so that all loads are in flight together
     T na[R1], nb[R1];
     if (t < R2) {
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
             const int k = t + R2 * k1;
-            const bool lo = k <= ny;
-            const int kr = lo ? k : N - k;
-            na[k1] = NT[c * cs + kr];
-            const T x = NT[3 * cs + kr];
-            nb[k1] = c == 2 ? T(0) : (lo ? x : -x);
+            const int kr = k <= ny ? k : N - k;
+            na[k1] = ldg(NT + c * cs + kr);
+            nb[k1] = c == 2 ? T(0) : ldg(NT + 3 * cs + kr);
         }
     }
     // ---- forward step 1: thread t1 < R1 (py = 2 ny: the upper half of
@@ -1568,10 +1596,10 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     C2<T> v[R2];
     if (t < R1) {
 #pragma unroll
-        for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? S[t + R1 * n2] : C2<T>{T(0), T(0)};
+        for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? ldg(S + t + R1 * n2) : C2<T>{T(0), T(0)};
         DftReg<T, R2, false>::run(v);
 #pragma unroll
-        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
     }
@@ -1596,7 +1624,8 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         for (int k1 = 0; k1 < R1; ++k1) {
             const int k = t + R2 * k1;
             const C2<T> ms = ex[c * N + k], mo = ex[other * N + k];
-            u[k1] = {-(na[k1] * ms.x + nb[k1] * mo.x), -(na[k1] * ms.y + nb[k1] * mo.y)};
+            const T b = k <= ny ? nb[k1] : -nb[k1];
+            u[k1] = {-(na[k1] * ms.x + b * mo.x), -(na[k1] * ms.y + b * mo.y)};
         }
     }
     __syncthreads();
@@ -1605,7 +1634,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         DftReg<T, R1, true>::run(u);
 #pragma unroll
         for (int t1 = 1; t1 < R1; ++t1) {
-            C2<T> w = tw4[t * R1 + t1];
+            C2<T> w = ldg(tw4 + t * R1 + t1);
             w.y = -w.y;
             u[t1] = cmul(u[t1], w);
         }
@@ -1619,7 +1648,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         for (int k2 = 0; k2 < R2; ++k2) v[k2] = tb[t * P2 + k2];
         DftReg<T, R2, true>::run(v);
 #pragma unroll
-        for (int n2 = 0; n2 < R2 / 2; ++n2) S[t + R1 * n2] = v[n2];
+        for (int n2 = 0; n2 < R2 / 2; ++n2) stg(S + t + R1 * n2, v[n2]);
     }
 }
 
