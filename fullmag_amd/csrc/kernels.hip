@@ -54,6 +54,10 @@ struct C2 {
 template <class T>
 using gp = __attribute__((address_space(1))) T *;
 template <class T>
+__device__ __forceinline__ gp<T> G(T *p) {
+    return (gp<T>)p;
+}
+template <class T>
 __device__ __forceinline__ T ldg(const T *p) {
     return *(gp<const T>)p;
 }
@@ -345,9 +349,9 @@ __device__ __forceinline__ int kslot(const Args<T> &A, const Method &M, int j, i
 // ABM3 start-up: the RHS of the new state is also pushed into the history
 // (the slot that becomes f_n); NULL otherwise.
 template <class T>
-__device__ __forceinline__ T *stage_push(const Args<T> &A, const Method &M, int s) {
+__device__ __forceinline__ gp<T> stage_push(const Args<T> &A, const Method &M, int s) {
     if (M.abm != 2 || s != M.nstage) return nullptr;
-    return A.k_all + (size_t)(kHistSlot + (A.ctl->abm_head + 1) % 3) * 3 * (size_t)A.ncell;
+    return G(A.k_all) + (size_t)(kHistSlot + (A.ctl->abm_head + 1) % 3) * 3 * (size_t)A.ncell;
 }
 
 // Intermediate stage states alternate between two buffers, so that a fused
@@ -359,28 +363,28 @@ __device__ __forceinline__ size_t rec_slot(const Args<T> &A, long long stride) {
 }
 
 template <class T>
-__device__ __forceinline__ T *stage_buf(const Args<T> &A, int s) {
-    return A.ybuf + (size_t)(s & 1) * 3 * (size_t)A.ncell;
+__device__ __forceinline__ gp<T> stage_buf(const Args<T> &A, int s) {
+    return G(A.ybuf) + (size_t)(s & 1) * 3 * (size_t)A.ncell;
 }
 
 template <class T>
-__device__ __forceinline__ const T *stage_input(const Args<T> &A, const Method &M, int s, int p) {
+__device__ __forceinline__ gp<const T> stage_input(const Args<T> &A, const Method &M, int s, int p) {
     const size_t n3 = 3 * (size_t)A.ncell;
-    if (s <= 1) return A.m_all + p * n3;
-    if (s == M.nstage) return A.m_all + (1 - p) * n3;
+    if (s <= 1) return G(A.m_all) + p * n3;
+    if (s == M.nstage) return G(A.m_all) + (1 - p) * n3;
     return stage_buf(A, s);
 }
 
 // Where stage s writes its state (s >= 2).
 template <class T>
-__device__ __forceinline__ T *stage_output(const Args<T> &A, const Method &M, int s, int p) {
-    return s == M.nstage ? A.m_all + (size_t)(1 - p) * 3 * (size_t)A.ncell : stage_buf(A, s);
+__device__ __forceinline__ gp<T> stage_output(const Args<T> &A, const Method &M, int s, int p) {
+    return s == M.nstage ? G(A.m_all) + (size_t)(1 - p) * 3 * (size_t)A.ncell : stage_buf(A, s);
 }
 
 template <class T>
-__device__ __forceinline__ T *stage_rhs(const Args<T> &A, const Method &M, int s, int p) {
+__device__ __forceinline__ gp<T> stage_rhs(const Args<T> &A, const Method &M, int s, int p) {
     const int slot = s <= 1 ? p : kslot(A, M, s, p);
-    return A.k_all + (size_t)slot * 3 * (size_t)A.ncell;
+    return G(A.k_all) + (size_t)slot * 3 * (size_t)A.ncell;
 }
 
 // y_s at `cell` (s >= 2 combines and stores; s <= 1 reads m0).
@@ -388,7 +392,7 @@ template <class T>
 __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, int s, int p, double dt, long long cell,
                                             T v[3]) {
     const long long N = A.ncell;
-    const T *m0 = A.m_all + p * 3 * N;
+    const gp<const T> m0 = G(A.m_all) + p * 3 * N;
     if (s <= 1) {
         v[0] = m0[cell];
         v[1] = m0[N + cell];
@@ -403,7 +407,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
     const bool efin = A.errv && s == M.nstage;
     for (int q = 0; q < R.nk; ++q) {
-        const T *k = A.k_all + kslot(A, M, R.kidx[q], p) * 3 * N;
+        const gp<const T> k = G(A.k_all) + kslot(A, M, R.kidx[q], p) * 3 * N;
         const double w = R.a[q];
         a0 += w * (double)k[cell];
         a1 += w * (double)k[N + cell];
@@ -416,9 +420,9 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
         }
     }
     if (efin) {
-        A.errv[cell] = e0;
-        A.errv[N + cell] = e1;
-        A.errv[2 * N + cell] = e2;
+        G(A.errv)[cell] = e0;
+        G(A.errv)[N + cell] = e1;
+        G(A.errv)[2 * N + cell] = e2;
     }
     const double f = dt * R.scale;
     double x = (double)m0[cell] + a0 * f;
@@ -433,7 +437,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     } else {
         x = y = z = 0.0;
     }
-    T *out = stage_output(A, M, s, p);
+    const gp<T> out = stage_output(A, M, s, p);
     out[cell] = (T)x;
     out[N + cell] = (T)y;
     out[2 * N + cell] = (T)z;
@@ -442,7 +446,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
         ry[cell] = (T)x;
         ry[N + cell] = (T)y;
         ry[2 * N + cell] = (T)z;
-        A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+        G(A.rec_vn)[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
     }
     v[0] = (T)x;
     v[1] = (T)y;
@@ -480,7 +484,7 @@ __device__ void block_reduce_store(Acc &a, double *out) {
         const int i = threadIdx.x;
         double x = red[i][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x = slot_is_max(i) ? fmax(x, red[i][w]) : x + red[i][w];
-        out[i] = x;
+        out[i] = x;  // out may be LDS (k_control): generic access
     }
 }
 
@@ -576,7 +580,7 @@ struct CellIn {
 
 template <class T, int FT>
 __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, int p, int cell, int x, int y, int z,
-                                            const T *__restrict__ yin, bool err, CellIn<T> &g) {
+                                            gp<const T> __restrict__ yin, bool err, CellIn<T> &g) {
     constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
     const int N = (int)A.ncell;
     const int sy = A.nx, sz = A.nx * A.ny;
@@ -600,48 +604,48 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
         // multilayer: the planes are separate bodies (no z neighbours)
         if (A.layered) nb[4] = nb[5] = cell;
     }
-    g.act = A.mask ? A.mask[cell] : 1;
+    g.act = A.mask ? G(A.mask)[cell] : 1;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) g.nact[q] = A.mask ? A.mask[nb[q]] : 1;
+    for (int q = 0; q < 6; ++q) g.nact[q] = A.mask ? G(A.mask)[nb[q]] : 1;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         g.m[c] = yin[c * N + cell];
 #pragma unroll
         for (int q = 0; q < 6; ++q) g.v[q][c] = yin[c * N + nb[q]];
     }
-    g.ms = A.ms_cell ? A.ms_cell[cell] : A.ms;
+    g.ms = A.ms_cell ? G(A.ms_cell)[cell] : A.ms;
     if (A.has_uni) {
-        g.ku1 = A.ku1_cell ? A.ku1_cell[cell] : A.ku1_u;
-        g.ku2 = A.ku2_cell ? A.ku2_cell[cell] : A.ku2_u;
+        g.ku1 = A.ku1_cell ? G(A.ku1_cell)[cell] : A.ku1_u;
+        g.ku2 = A.ku2_cell ? G(A.ku2_cell)[cell] : A.ku2_u;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) g.ax[c] = A.axis_cell ? A.axis_cell[c * N + cell] : A.axis[c];
+        for (int c = 0; c < 3; ++c) g.ax[c] = A.axis_cell ? G(A.axis_cell)[c * N + cell] : A.axis[c];
     }
-    g.wm = A.wave_mask ? A.wave_mask[cell] : 1;
+    g.wm = A.wave_mask ? G(A.wave_mask)[cell] : 1;
     if constexpr (XT) {
         if (A.has_cub) {
-            g.kc[0] = A.kc1_cell ? A.kc1_cell[cell] : A.kc1_u;
-            g.kc[1] = A.kc2_cell ? A.kc2_cell[cell] : A.kc2_u;
-            g.kc[2] = A.kc3_cell ? A.kc3_cell[cell] : A.kc3_u;
+            g.kc[0] = A.kc1_cell ? G(A.kc1_cell)[cell] : A.kc1_u;
+            g.kc[1] = A.kc2_cell ? G(A.kc2_cell)[cell] : A.kc2_u;
+            g.kc[2] = A.kc3_cell ? G(A.kc3_cell)[cell] : A.kc3_u;
         }
         if (A.oe_field) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g.oe[c] = A.oe_field[c * N + cell];
+            for (int c = 0; c < 3; ++c) g.oe[c] = G(A.oe_field)[c * N + cell];
         }
         if (A.bc_w) {
 #pragma unroll
-            for (int q = 0; q < 6; ++q) g.bw[q] = A.bc_w[q * N + cell];
+            for (int q = 0; q < 6; ++q) g.bw[q] = G(A.bc_w)[q * N + cell];
         }
-        g.phi = A.bc_phi ? A.bc_phi[cell] : T(1);
+        g.phi = A.bc_phi ? G(A.bc_phi)[cell] : T(1);
         g.hc[0] = g.hc[1] = g.hc[2] = T(0);
         if (A.dc_ptr) {
             // demag_boundary_fp64.cu:23-76: H -= dN . (phi Ms y) over the
             // cell's correction sources
-            const int b = A.dc_ptr[cell], e = A.dc_ptr[cell + 1];
+            const int b = G(A.dc_ptr)[cell], e = G(A.dc_ptr)[cell + 1];
             for (int j = b; j < e; ++j) {
-                const int sc = A.dc_src[j];
-                const T w = A.ms_pack[sc];
+                const int sc = G(A.dc_src)[j];
+                const T w = G(A.ms_pack)[sc];
                 const T M0 = w * yin[sc], M1 = w * yin[N + sc], M2 = w * yin[2 * N + sc];
-                const T *t = A.dc_n + 6 * (size_t)j;
+                const gp<const T> t = G(A.dc_n) + 6 * (size_t)j;
                 g.hc[0] -= t[0] * M0 + t[1] * M1 + t[2] * M2;
                 g.hc[1] -= t[1] * M0 + t[3] * M1 + t[4] * M2;
                 g.hc[2] -= t[2] * M0 + t[4] * M1 + t[5] * M2;
@@ -649,17 +653,17 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
         }
     }
     if (LUT) {
-        g.rc = A.region[cell];
+        g.rc = G(A.region)[cell];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) g.rn[q] = A.region[nb[q]];
+        for (int q = 0; q < 6; ++q) g.rn[q] = G(A.region)[nb[q]];
     }
     if (err) {
         // every adaptive tableau has Method::efuse: the final stage
         // combination left the other error terms' sum in errv
 #pragma unroll
-        for (int c = 0; c < 3; ++c) g.ev[c] = A.errv[c * N + cell];
+        for (int c = 0; c < 3; ++c) g.ev[c] = G(A.errv)[c * N + cell];
         if (A.rtol > 0.0) {
-            const T *m0 = A.m_all + (size_t)p * 3 * N;
+            const gp<const T> m0 = G(A.m_all) + (size_t)p * 3 * N;
 #pragma unroll
             for (int c = 0; c < 3; ++c) g.m0[c] = m0[c * N + cell];
         }
@@ -669,9 +673,9 @@ __device__ __forceinline__ void gather_cell(const Args<T> &A, const Method &M, i
 // Local field terms, LLG torque, and (optionally) statistics for one cell.
 template <class T, int FT>
 __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, int cell, double dt,
-                                             const CellIn<T> &g, T *__restrict__ kout, const T hd_in[3], TimeF<T> tf,
-                                             bool stats, bool err, Acc &acc, T *__restrict__ hrec,
-                                             T kret[3] = nullptr, T *__restrict__ kpush = nullptr) {
+                                             const CellIn<T> &g, gp<T> __restrict__ kout, const T hd_in[3], TimeF<T> tf,
+                                             bool stats, bool err, Acc &acc, gp<T> __restrict__ hrec,
+                                             T kret[3] = nullptr, gp<T> __restrict__ kpush = nullptr) {
     constexpr bool LUT = (FT & 1) != 0, XT = (FT & 2) != 0;
     const T wf = tf.wave;
     const int N = (int)A.ncell;
@@ -683,9 +687,9 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         if (kret) kret[0] = kret[1] = kret[2] = T(0);
         if (hrec) hrec[cell] = hrec[N + cell] = hrec[2 * N + cell] = T(0);
         if (A.obs_hex) {
-            A.obs_hex[cell] = A.obs_hex[N + cell] = A.obs_hex[2 * N + cell] = T(0);
-            A.obs_hd[cell] = A.obs_hd[N + cell] = A.obs_hd[2 * N + cell] = T(0);
-            A.obs_heff[cell] = A.obs_heff[N + cell] = A.obs_heff[2 * N + cell] = T(0);
+            G(A.obs_hex)[cell] = G(A.obs_hex)[N + cell] = G(A.obs_hex)[2 * N + cell] = T(0);
+            G(A.obs_hd)[cell] = G(A.obs_hd)[N + cell] = G(A.obs_hd)[2 * N + cell] = T(0);
+            G(A.obs_heff)[cell] = G(A.obs_heff)[N + cell] = G(A.obs_heff)[2 * N + cell] = T(0);
         }
         return;
     }
@@ -701,7 +705,7 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
         const T ww[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
         T aw[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) aw[q] = g.bw[q] * ww[q] * (LUT ? A.lut[g.rc * 256u + g.rn[q]] : T(1));
+        for (int q = 0; q < 6; ++q) aw[q] = g.bw[q] * ww[q] * (LUT ? G(A.lut)[g.rc * 256u + g.rn[q]] : T(1));
         const T pre = LUT ? (A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre) : (A.ms_cell ? A.two_a_mu0 / ms_i : A.ex_pre);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -725,7 +729,7 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     } else {
         T aij[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) aij[q] = A.lut[g.rc * 256u + g.rn[q]];
+        for (int q = 0; q < 6; ++q) aij[q] = G(A.lut)[g.rc * 256u + g.rn[q]];
         const T ww[6] = {A.idx2, A.idx2, A.idy2, A.idy2, A.idz2, A.idz2};
         const T pre = A.ms_cell ? A.two_mu0inv / ms_i : A.lut_pre;
 #pragma unroll
@@ -892,9 +896,9 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
     if (A.obs_hex) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            A.obs_hex[c * N + cell] = hex[c];
-            A.obs_hd[c * N + cell] = hd[c];
-            A.obs_heff[c * N + cell] = h[c];
+            G(A.obs_hex)[c * N + cell] = hex[c];
+            G(A.obs_hd)[c * N + cell] = hd[c];
+            G(A.obs_heff)[c * N + cell] = h[c];
         }
     }
     if (stats) {
@@ -986,7 +990,7 @@ __device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
             }
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + (k2 * R1 + t)));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) tt[k2 * F::P1 + t] = v[k2];
         }
@@ -1020,7 +1024,7 @@ __device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
             DftReg<T, R1, true>::run(u);
 #pragma unroll
             for (int t1 = 1; t1 < R1; ++t1) {
-                C2<T> w = tw4[t * R1 + t1];
+                C2<T> w = ldg(tw4 + (t * R1 + t1));
                 w.y = -w.y;
                 u[t1] = cmul(u[t1], w);
             }
@@ -1109,7 +1113,7 @@ __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, i
     const int ncl = A.rows_per_wg * nx;
     const float rnx = 1.0f / (float)nx;
     const int N = (int)A.ncell, bd = blockDim.x;
-    const T *__restrict__ m0 = A.m_all + (size_t)p * 3 * N;
+    const gp<const T> __restrict__ m0 = G(A.m_all) + (size_t)p * 3 * N;
     const StageRow &R = M.row[NK > 0 ? s : 2];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
@@ -1123,11 +1127,11 @@ __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, i
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
             if (q == skip) continue;
-            const T *kp = A.k_all + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N;
+            const gp<const T> kp = G(A.k_all) + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N;
 #pragma unroll
             for (int c = 0; c < 3; ++c) in.kv[j][q][c] = kp[c * N + cell];
         }
-        in.msv[j] = A.ms_pack ? A.ms_pack[cell] : A.ms;
+        in.msv[j] = A.ms_pack ? G(A.ms_pack)[cell] : A.ms;
     }
 }
 
@@ -1144,7 +1148,7 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
     const StageRow &R = M.row[NK > 0 ? s : 2];
     const double f = dt * R.scale;
     const int sb = FAST ? A.xsb : px;
-    T *__restrict__ yout = NK > 0 ? stage_output(A, M, s, p) : nullptr;
+    const gp<T> __restrict__ yout = NK > 0 ? stage_output(A, M, s, p) : nullptr;
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         const int i = base + j * bd + (int)threadIdx.x;
@@ -1184,9 +1188,9 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
                     e2 += w * (double)in.kv[j][q][2];
                 }
                 const int cell = c0 + x;
-                A.errv[cell] = e0;
-                A.errv[N + cell] = e1;
-                A.errv[2 * N + cell] = e2;
+                G(A.errv)[cell] = e0;
+                G(A.errv)[N + cell] = e1;
+                G(A.errv)[2 * N + cell] = e2;
             }
             const double xx = (double)in.mv[j][0] + a0 * f, yy = (double)in.mv[j][1] + a1 * f,
                          zz = (double)in.mv[j][2] + a2 * f;
@@ -1205,7 +1209,7 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
                     ry[cell] = v[0];
                     ry[N + cell] = v[1];
                     ry[2 * N + cell] = v[2];
-                    A.rec_vn[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
+                    G(A.rec_vn)[rec_slot(A, A.rec_ss) + (size_t)(s - 2) * N + cell] = (T)sqrt(n2);
                 }
             }
         }
@@ -1272,7 +1276,7 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
         const C2<T> zk = zq[kx], zm = zq[kx == 0 ? 0 : px - kx];
         const C2<T> X = (l & 1) ? C2<T>{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)}
                                 : C2<T>{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
-        S[off + kx * ly] = X;
+        stg(S + off + kx * ly, X);
     }
 }
 
@@ -1302,7 +1306,7 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
             const int ic = i < tot ? i : tot - 1;
             const int kk = small_div<false>(ic, nlp, rnl), l = ic - kk * nlp;
             const int off = l < nl ? xr.spec[l] : -1;
-            v[j] = off >= 0 ? S[off + kk * ly] : C2<T>{T(0), T(0)};
+            v[j] = off >= 0 ? ldg(S + off + kk * ly) : C2<T>{T(0), T(0)};
         }
 #pragma unroll
         for (int j = 0; j < CZ; ++j) {
@@ -1331,7 +1335,7 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
 // optional statistics and error norm.  x_local_load issues the loads of one
 // chunk of cells, x_local_put consumes it (and returns the chunk's k).
 template <class T, int FT, int R1, int R2>
-__device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, int p, const T *yin,
+__device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, int p, gp<const T> yin,
                                              const XRows &xr, int base, bool err, CellIn<T> *g) {
     constexpr bool FAST = R1 > 0;
     constexpr int CH = Chunk<T>::value;
@@ -1357,9 +1361,9 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
     const float rnx = 1.0f / (float)nx;
-    T *kout = stage_rhs(A, M, s, p);
-    T *kpush = stage_push(A, M, s);
-    T *hrec = A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
+    const gp<T> kout = stage_rhs(A, M, s, p);
+    const gp<T> kpush = stage_push(A, M, s);
+    const gp<T> hrec = A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
     const T scale = T(A.inv_p);
     const int sb = FAST ? A.xsb : px;
 #pragma unroll
@@ -1386,7 +1390,7 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
                                         const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
     constexpr int CH = Chunk<T>::value;
     const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
-    const T *yin = stage_input(A, M, s, p);
+    const gp<const T> yin = stage_input(A, M, s, p);
     for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
         CellIn<T> g[CH];
         x_local_load<T, FT, R1, R2>(A, M, p, yin, xr, base, err, g);
@@ -2020,7 +2024,7 @@ template <class T>
 __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T> &J, const Method &M, int j,
                                                int cell, int x, int y, int z, const T hdT[3], T wf, double amp[3]) {
     const int N = (int)A.ncell;
-    if (A.mask && !A.mask[cell]) return;
+    if (A.mask && !G(A.mask)[cell]) return;
     const T *yv = J.rec_y + (size_t)(j - 1) * 3 * N;
     const T *Hv = J.rec_h + (size_t)(j - 1) * 3 * N;
     const double *a = J.cot_k + (size_t)j * 3 * N;
@@ -2029,7 +2033,7 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
     const double av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
     const double bi[3] = {J.beta[cell], J.beta[N + cell], J.beta[2 * N + cell]};
     const double hd[3] = {(double)hdT[0], (double)hdT[1], (double)hdT[2]};
-    const double ms_i = A.ms_cell ? (double)A.ms_cell[cell] : (double)A.ms;
+    const double ms_i = A.ms_cell ? (double)G(A.ms_cell)[cell] : (double)A.ms;
     const double gb = (double)A.gamma_bar, al = (double)A.alpha;
     double Hxa[3], mxH[3], mxHxa[3], axm[3], Hxaxm[3];
     cross3(H, av, Hxa);
@@ -2053,9 +2057,9 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         const int o = nb[q];
-        const bool use = o != cell && (!A.mask || A.mask[o]);
+        const bool use = o != cell && (!A.mask || G(A.mask)[o]);
         if (!use) continue;
-        const double inv_ms_o = A.ms_cell ? 1.0 / (double)A.ms_cell[o] : inv_ms_i;
+        const double inv_ms_o = A.ms_cell ? 1.0 / (double)G(A.ms_cell)[o] : inv_ms_i;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             r[c] += c_ex * wq[q] * (J.beta[c * N + o] * inv_ms_o - bi[c] * inv_ms_i);
@@ -2072,11 +2076,11 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
     if (A.has_uni) {
         double u[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[c] = A.axis_cell ? (double)A.axis_cell[c * N + cell] : (double)A.axis[c];
+        for (int c = 0; c < 3; ++c) u[c] = A.axis_cell ? (double)G(A.axis_cell)[c * N + cell] : (double)A.axis[c];
         double c1 = (double)A.uni_c1, c2 = (double)A.uni_c2;
         if (A.ku1_cell || A.ku2_cell || A.ms_cell) {
-            const double k1 = A.ku1_cell ? (double)A.ku1_cell[cell] : (double)A.ku1_u;
-            const double k2 = A.ku2_cell ? (double)A.ku2_cell[cell] : (double)A.ku2_u;
+            const double k1 = A.ku1_cell ? (double)G(A.ku1_cell)[cell] : (double)A.ku1_u;
+            const double k2 = A.ku2_cell ? (double)G(A.ku2_cell)[cell] : (double)A.ku2_u;
             c1 = (double)A.two_mu0inv * k1 * inv_ms_i;
             c2 = 2.0 * (double)A.two_mu0inv * k2 * inv_ms_i;
         }
@@ -2096,7 +2100,7 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
 #pragma unroll
         for (int c = 0; c < 3; ++c) g += -bi[c] * (hex[c] + hani[c]) * inv_ms_i + m[c] * hd[c];
         J.grad_ms[cell] += g;
-        if (A.wave_kind && (!A.wave_mask || A.wave_mask[cell])) {
+        if (A.wave_kind && (!A.wave_mask || G(A.wave_mask)[cell])) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) amp[c] += bi[c] * (double)wf;
         }
@@ -2338,7 +2342,7 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
         CellIn<T> g;
         gather_cell<T, FT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
         compute_cell<T, FT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, tf, stats != 0, err, acc,
-                             A.rec_h ? A.rec_h + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
+                             A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
                              nullptr, stage_push(A, M, s));
     }
     if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
