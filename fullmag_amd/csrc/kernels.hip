@@ -1797,10 +1797,10 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
         if (t < R1) {
             C2<T> v[R2];
 #pragma unroll
-            for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? S[t + R1 * n2] : C2<T>{T(0), T(0)};
+            for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? ldg(S + t + R1 * n2) : C2<T>{T(0), T(0)};
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
         }
@@ -1812,18 +1812,18 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
             DftReg<T, R1, false>::run(u);
             if (on) {
 #pragma unroll
-                for (int k1 = 0; k1 < R1; ++k1) S[t + R2 * k1] = u[k1];
+                for (int k1 = 0; k1 < R1; ++k1) stg(S + t + R2 * k1, u[k1]);
             }
         }
     } else {
         if (t < R2) {
             C2<T> u[R1];
 #pragma unroll
-            for (int k1 = 0; k1 < R1; ++k1) u[k1] = S[t + R2 * k1];
+            for (int k1 = 0; k1 < R1; ++k1) u[k1] = ldg(S + t + R2 * k1);
             DftReg<T, R1, true>::run(u);
 #pragma unroll
             for (int t1 = 1; t1 < R1; ++t1) {
-                C2<T> w = tw4[t * R1 + t1];
+                C2<T> w = ldg(tw4 + t * R1 + t1);
                 w.y = -w.y;
                 u[t1] = cmul(u[t1], w);
             }
@@ -1838,7 +1838,7 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
             DftReg<T, R2, true>::run(v);
             if (on) {
 #pragma unroll
-                for (int n2 = 0; n2 < R2 / 2; ++n2) S[t + R1 * n2] = v[n2];
+                for (int n2 = 0; n2 < R2 / 2; ++n2) stg(S + t + R1 * n2, v[n2]);
             }
         }
     }
@@ -1879,7 +1879,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int n2 = 0; n2 < R2 / 2; ++n2) raw[c][n2] = S[(size_t)(c * nzs + t + R1 * n2) * zs];
+            for (int n2 = 0; n2 < R2 / 2; ++n2) raw[c][n2] = ldg(S + (size_t)(c * nzs + t + R1 * n2) * zs);
     }
     // forward, one component at a time through one transpose buffer: thread
     // t < R2 ends with kz = t + R2 k1 of every component
@@ -1892,7 +1892,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? raw[c][n2] : C2<T>{T(0), T(0)};
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], tw4[k2 * R1 + t]);
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) L(k2 * P1 + t) = v[k2];
         }
@@ -1916,7 +1916,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             const bool zlo = kz <= nz;
             const int kzr = zlo ? kz : 2 * nz - kz;
             const T sz = zlo ? T(1) : T(-1);
-            const T *N = NT + (size_t)kzr * (ny + 1) * 6;
+            const gp<const T> N = G(NT) + (size_t)kzr * (ny + 1) * 6;
             const T nxx = N[0], nyy = N[1], nzz = N[2];
             const T nxy = sy * N[3], nxz = sz * N[4], nyz = sy * sz * N[5];
             const C2<T> mx = x[0][k1], my = x[1][k1], mz = x[2][k1];
@@ -1929,7 +1929,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             DftReg<T, R1, true>::run(x[c]);
 #pragma unroll
             for (int t1 = 1; t1 < R1; ++t1) {
-                C2<T> w = tw4[t * R1 + t1];
+                C2<T> w = ldg(tw4 + t * R1 + t1);
                 w.y = -w.y;
                 x[c][t1] = cmul(x[c][t1], w);
             }
@@ -1951,7 +1951,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             DftReg<T, R2, true>::run(v);
             if (on) {
 #pragma unroll
-                for (int n2 = 0; n2 < R2 / 2; ++n2) S[(size_t)(c * nzs + t + R1 * n2) * zs] = v[n2];
+                for (int n2 = 0; n2 < R2 / 2; ++n2) stg(S + (size_t)(c * nzs + t + R1 * n2) * zs, v[n2]);
             }
         }
         if (c < 2) __syncthreads();
