@@ -1403,9 +1403,10 @@ template <class T, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
-    if (ctl->done) return;
-    const int p = ctl->parity;
+    // the control fields in one round trip, then the early exit
+    const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
+    if (done) return;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.xsb;
@@ -1424,9 +1425,10 @@ template <class T, int FT, int R1, int R2>
 __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
-    if (ctl->done) return;
-    const int p = ctl->parity;
+    // the control fields in one round trip, then the early exit
+    const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
+    if (done) return;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.xsb;
@@ -1465,9 +1467,10 @@ template <class T, int FT, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
-    if (ctl->done) return;
-    const int p = ctl->parity;
+    // the control fields in one round trip, then the early exit
+    const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
+    if (done) return;
     const Method &M = method_for(A);
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.xsb;
@@ -1857,7 +1860,8 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
 template <class T, int R1, int R2, int UB>
 __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    // checked once the loads below are issued (they only read)
+    const int done = A.ctl->done;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
     const int ntile = (py + UB - 1) / UB;
@@ -1881,6 +1885,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
 #pragma unroll
             for (int n2 = 0; n2 < R2 / 2; ++n2) raw[c][n2] = ldg(S + (size_t)(c * nzs + t + R1 * n2) * zs);
     }
+    if (done) return;
     // forward, one component at a time through one transpose buffer: thread
     // t < R2 ends with kz = t + R2 k1 of every component
     C2<T> x[3][R1];
@@ -2325,9 +2330,10 @@ template <class T, int FT>
 __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     Ctl *ctl = A.ctl;
-    if (ctl->done) return;
-    const int p = ctl->parity;
+    // the control fields in one round trip, then the early exit
+    const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
+    if (done) return;
     const Method &M = method_for(A);
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool err = M.adaptive && s == M.nstage;
