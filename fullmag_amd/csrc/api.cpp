@@ -937,6 +937,38 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.xt = 1;
     }
     if (!parse_boundary(h, A, p, x, ms)) return false;
+    // activity of each cell and of its six stencil neighbours in one byte,
+    // with the kernels' neighbour rule (box clamp, periodic wrap, no z
+    // neighbours in a multilayer stack)
+    if (h->has_mask) {
+        const long nx_ = (long)h->nx, ny_ = (long)h->ny, nz_ = (long)h->nz, sxy = nx_ * ny_;
+        std::vector<uint8_t> nm(h->n);
+        for (long i = 0; i < (long)h->n; ++i) {
+            const long xi = i % nx_, yi = (i / nx_) % ny_, zi = i / sxy;
+            long nb[6] = {xi > 0 ? i - 1 : i,        xi < nx_ - 1 ? i + 1 : i,  yi > 0 ? i - nx_ : i,
+                          yi < ny_ - 1 ? i + nx_ : i, zi > 0 ? i - sxy : i,       zi < nz_ - 1 ? i + sxy : i};
+            if (A.pbc & 1) {
+                if (xi == 0) nb[0] = i + nx_ - 1;
+                if (xi == nx_ - 1) nb[1] = i - (nx_ - 1);
+            }
+            if (A.pbc & 2) {
+                if (yi == 0) nb[2] = i + (ny_ - 1) * nx_;
+                if (yi == ny_ - 1) nb[3] = i - (ny_ - 1) * nx_;
+            }
+            if (A.pbc & 4) {
+                if (zi == 0) nb[4] = i + (nz_ - 1) * sxy;
+                if (zi == nz_ - 1) nb[5] = i - (nz_ - 1) * sxy;
+            }
+            if (A.layered) nb[4] = nb[5] = i;
+            uint8_t b = h->mask_h[i] ? 1 : 0;
+            for (int q = 0; q < 6; ++q) b |= (uint8_t)((h->mask_h[nb[q]] ? 1 : 0) << (q + 1));
+            nm[i] = b;
+        }
+        auto *d = static_cast<uint8_t *>(dalloc(h, h->n));
+        if (!d || !hip_ok(h, hipMemcpy(d, nm.data(), h->n, hipMemcpyHostToDevice), "upload neighbour mask"))
+            return d ? false : hip_ok(h, hipErrorOutOfMemory, "alloc neighbour mask");
+        A.nmask = d;
+    }
 
     // state buffers
     A.m_all = static_cast<T *>(dalloc(h, 2 * 3 * n * sizeof(T)));

@@ -146,6 +146,8 @@ struct Args {
     const void *tw4z;       // four-step twiddles of the 3-D z pass (W_pz^{t1 k2} at [k2*R1 + t1])
     int zfast;              // 3-D: 0 = Stockham z pass, else pz of the register four-step z pass
     const uint8_t *mask;    // NULL = all active
+    const uint8_t *nmask;   // with a mask: bit 0 = the cell, bit 1+q = its neighbour q
+                            // (-x,+x,-y,+y,-z,+z as the stencil takes it) active
     const uint32_t *region; // NULL = uniform exchange
     const T *lut;           // 256 x 256 A_ij when region != NULL
     Ctl *ctl;
