@@ -42,8 +42,8 @@ HBM_PEAK_GBS = 8000.0
 # the passes and their commands are recorded in
 # profiles/film512_dp45_hbm_traffic.md and profiles/granular3d_dp45_profile.md
 MEASURED_TRAFFIC_FP32 = {
-    "film512": {0: 16.12e6, 1: 17.42e6, 4: 13.64e6, 6: 31.83e6},
-    "granular3d": {0: 252.5e6, 1: 303.3e6, 2: 658.9e6, 3: 303.3e6, 4: 340.3e6, 6: 669.7e6},
+    "film512": {0: 16.11e6, 1: 17.40e6, 4: 13.62e6, 6: 32.62e6},
+    "granular3d": {0: 252.5e6, 1: 303.3e6, 2: 658.9e6, 3: 303.3e6, 4: 345.3e6, 6: 683.4e6},
 }
 
 NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xx"}
