@@ -2361,12 +2361,15 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
 // one work-group.  observe != 0: only store the statistics.
 template <class T>
 __device__ void control_block(const Args<T> &A, int nparts, int observe) {
-    Ctl *c = A.ctl;
+    // the control block, the partials and the recording counter are device
+    // allocations: global accesses, which do not share the LDS wait counter
+    // (a generic store there made every later LDS read wait for it)
+    const gp<Ctl> c = G(A.ctl);
     Acc acc;
     acc.init();
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
         // written by the previous launch: the kernel boundary made them visible
-        const double *q = A.partials + (long long)i * kStatSlots;
+        const gp<const double> q = G((const double *)A.partials) + (long long)i * kStatSlots;
 #pragma unroll
         for (int k = 0; k < kStatSlots; ++k) {
             const double v = q[k];
@@ -2405,7 +2408,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
         return;
     }
     for (int k = 0; k < kStatSlots; ++k) c->stats[k] = tot[k];
-    if (A.rec_step) *A.rec_step += 1;
+    if (A.rec_step) *G(A.rec_step) += 1;
     c->t += dt;
     c->step += 1;
     c->parity ^= 1;

@@ -468,10 +468,9 @@ bool reduce_given_spectra(const double *const spec[6], uint64_t len, int nx, int
                           std::vector<double> &out, std::string *err) {
     const uint64_t px = 2ull * nx, py = 2ull * ny, pz3 = 2ull * nz;
     const bool film = nz == 1;
-    uint64_t pz;
-    if (film && len == 2 * px * py) pz = 1;
-    else if (len == 2 * px * py * pz3) pz = pz3;
-    else {
+    // a film accepts the 2-D form (pz = 1) or the 3-D form; only the kz = 0
+    // plane is sampled either way
+    if (!(film && len == 2 * px * py) && len != 2 * px * py * pz3) {
         *err = "demag_kernel_spectrum_len mismatch: expected " + std::to_string(2 * px * py * pz3) +
                (film ? " or " + std::to_string(2 * px * py) : std::string()) + ", got " + std::to_string(len);
         return false;
