@@ -56,6 +56,7 @@ class FdmPlan:
     regions: np.ndarray | None = None   # uint32 (n,)
     exchange_lut: np.ndarray | None = None  # (256*256,) float64
     spectra: dict | None = None         # {"xx": interleaved complex f64, ...}
+    spectra_len: int | None = None      # demag_kernel_spectrum_len (None: the arrays' length)
     atol: float = 0.0                   # 0 -> engine default 1e-5
     dt_min: float = 0.0
     dt_max: float = 0.0
@@ -165,7 +166,7 @@ def marshal_plan(plan: FdmPlan):
                 keep.append(a)
                 setattr(desc, f"demag_kernel_{nm}_spectrum", _dp(a))
                 ln = a.size
-        desc.demag_kernel_spectrum_len = ln
+        desc.demag_kernel_spectrum_len = ln if plan.spectra_len is None else plan.spectra_len
     for nm in ("ku1_field", "ku2_field", "kc1_field", "kc2_field", "kc3_field"):
         a = getattr(plan, nm)
         if a is not None:
@@ -395,6 +396,24 @@ class FdmBackend:
             return np.frombuffer(bytes(buf), dtype=dt).reshape(3, d.cell_count).copy()
         finally:
             self.lib.fullmag_fdm_field_snapshot_destroy(s)
+
+    def preview_snapshot(self, name, pnx, pny, pnz, z_origin=0, z_stride=1) -> np.ndarray:
+        """Asynchronous preview API; returns the AoS payload as (cells, 3) in
+        the state precision."""
+        s = self.lib.fullmag_fdm_backend_begin_preview_snapshot(self.h, _OBS[name], pnx, pny, pnz, z_origin,
+                                                                z_stride)
+        if not s:
+            self._check(abi.ERR_INTERNAL)
+        try:
+            p, ln, d = ctypes.c_void_p(), ctypes.c_uint64(), abi.SnapshotDesc()
+            rc = self.lib.fullmag_fdm_preview_snapshot_wait(s, ctypes.byref(p), ctypes.byref(ln), ctypes.byref(d))
+            if rc != abi.OK:
+                raise FdmError(rc, "preview snapshot wait failed")
+            dt = np.float64 if d.scalar_type == abi.SNAPSHOT_F64 else np.float32
+            buf = (ctypes.c_char * ln.value).from_address(p.value)
+            return np.frombuffer(bytes(buf), dtype=dt).reshape(d.cell_count, 3).copy()
+        finally:
+            self.lib.fullmag_fdm_preview_snapshot_destroy(s)
 
     def upload_m(self, m: np.ndarray):
         a = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))

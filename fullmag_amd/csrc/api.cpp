@@ -60,6 +60,9 @@ template <class T>
 hipError_t adj_configure(const Args<T> &A);
 template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
+template <class T>
+hipError_t enqueue_preview(const T *field, long long ncell, const PreviewGeom &g, void *out, int out_fp64,
+                           hipStream_t st);
 bool yfast_factors(int py, int *r1, int *r2);
 bool zfast_factors(int pz, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
@@ -80,6 +83,12 @@ struct fullmag_fdm_backend {
     bool ok = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    // readout stream (non-blocking, default priority): device-to-host copies
+    // of snapshots run here, so the compute stream's next step does not
+    // queue behind them
+    hipStream_t io_stream = nullptr;
+    void *preview_d = nullptr;  // preview bins of copy_field_preview_* (grown on demand)
+    size_t preview_cap = 0;
     bool fp64 = false;
     uint32_t nx = 0, ny = 0, nz = 0;
     uint64_t n = 0;
@@ -97,6 +106,7 @@ struct fullmag_fdm_backend {
     bool has_mask = false;
     double hext[3] = {0, 0, 0};
     bool has_ext = false;
+    std::vector<uint8_t> wave_mask_h;  // cells of the Zeeman waveform (empty: all magnetic cells)
     fullmag_fdm_interrupt_poll_fn poll = nullptr;
     void *poll_user = nullptr;
     hipGraph_t graph = nullptr;
@@ -127,17 +137,29 @@ struct fullmag_fdm_backend {
     } adj;
 };
 
+// Asynchronous readouts (reference AsyncFieldSnapshot/AsyncPreviewSnapshot,
+// native/backends/fdm/src/context.cu:1271-1745): the payload is staged on
+// the device on the compute stream (a device-to-device copy of the field, or
+// the preview bins), `ready` is recorded there, and the device-to-host copy
+// into pinned memory runs on the backend's readout stream after `ready`,
+// ending with `done`.  The snapshot owns its staging buffer, pinned payload
+// and events, so it stays valid after the backend is destroyed.
 struct fullmag_fdm_field_snapshot {
-    void *host = nullptr;  // pinned
+    void *host = nullptr;     // pinned payload
+    void *staging = nullptr;  // device copy of the field
     uint64_t bytes = 0;
     fullmag_fdm_snapshot_desc desc{};
-    hipEvent_t ev = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
     bool waited = false;
 };
 
 struct fullmag_fdm_preview_snapshot {
-    std::vector<unsigned char> host;
+    void *host = nullptr;     // pinned payload
+    void *staging = nullptr;  // device preview bins
+    uint64_t bytes = 0;
     fullmag_fdm_snapshot_desc desc{};
+    hipEvent_t ready = nullptr, done = nullptr;
+    bool waited = false;
 };
 
 namespace {
@@ -153,6 +175,11 @@ bool hip_ok(Backend *h, hipError_t e, const char *what) {
     if (e == hipSuccess) return true;
     h->err = std::string(what) + ": " + hipGetErrorString(e);
     return false;
+}
+
+// step + count without wrap-around (a caller may pass UINT64_MAX as "no limit")
+uint64_t sat_add(uint64_t step, uint64_t count) {
+    return count > std::numeric_limits<uint64_t>::max() - step ? std::numeric_limits<uint64_t>::max() : step + count;
 }
 
 int fail(Backend *h, int code, const std::string &msg) {
@@ -895,6 +922,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
     if (x && x->field_mask) {
         std::vector<uint8_t> wm(x->field_mask, x->field_mask + n);
         for (auto &v : wm) v = v ? 1 : 0;
+        h->wave_mask_h = wm;
         auto *d = static_cast<uint8_t *>(dalloc(h, n));
         if (!d || hipMemcpy(d, wm.data(), n, hipMemcpyHostToDevice) != hipSuccess) {
             h->err = "upload field_mask";
@@ -903,6 +931,27 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.wave_mask = d;
     }
     A.lut_pre = p->enable_exchange ? (T)(2.0 / (kMu0 * ms)) : (T)0;
+    // plan-supplied demag spectra (native/backends/fdm/src/api.cpp:207,362-388):
+    // a nonzero demag_kernel_spectrum_len selects them, with every component
+    // pointer present and the length of the 3-D form, or for nz == 1 of the
+    // thin-film 2-D form; checked whether or not demag is enabled
+    if (p->demag_kernel_spectrum_len != 0) {
+        const double *given[6] = {p->demag_kernel_xx_spectrum, p->demag_kernel_yy_spectrum,
+                                  p->demag_kernel_zz_spectrum, p->demag_kernel_xy_spectrum,
+                                  p->demag_kernel_xz_spectrum, p->demag_kernel_yz_spectrum};
+        for (const double *g : given)
+            if (!g)
+                return (h->err = "demag kernel spectra pointers must all be present when demag_kernel_spectrum_len "
+                                 "is set",
+                        false);
+        const uint64_t p3 = 2ull * (2ull * nx) * (2ull * ny) * (2ull * nz), p2 = 2ull * (2ull * nx) * (2ull * ny);
+        const uint64_t len = p->demag_kernel_spectrum_len;
+        if (len != p3 && !(nz == 1 && len == p2))
+            return (h->err = "demag_kernel_spectrum_len mismatch: expected " + std::to_string(p3) + " (3D)" +
+                             (nz == 1 ? " or " + std::to_string(p2) + " (thin-film 2D)" : std::string()) + ", got " +
+                             std::to_string(len),
+                    false);
+    }
     // periodic boundaries (plan extension version 6; the CPU engine's
     // FdmBoundaryPolicy with demag_image_counts, default 10 per axis)
     int per[3] = {0, 0, 0}, img[3] = {10, 10, 10};
@@ -912,7 +961,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         if (x->periodic_axes & ~7u) return (h->err = "periodic_axes: only bits 0 (x), 1 (y), 2 (z)", false);
         if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE)
             return (h->err = "periodic boundaries cannot be combined with boundary-correction tiers", false);
-        if (p->demag_kernel_xx_spectrum)
+        if (p->demag_kernel_spectrum_len != 0)
             return (h->err = "periodic boundaries need the engine's own demag kernel (plan spectra are open-boundary)",
                     false);
         for (int a = 0; a < 3; ++a) {
@@ -931,7 +980,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         if (per[2]) return (h->err = "layer_z cannot be combined with a periodic z axis", false);
         if (p->boundary_correction != FULLMAG_FDM_BOUNDARY_NONE)
             return (h->err = "layer_z cannot be combined with boundary-correction tiers", false);
-        if (p->demag_kernel_xx_spectrum)
+        if (p->demag_kernel_spectrum_len != 0)
             return (h->err = "layer_z needs the engine's own demag kernel (plan spectra are a uniform grid)", false);
         A.layered = 1;
         A.xt = 1;
@@ -1028,7 +1077,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
             if (!hip_ok(h, hipMemcpy(gd, gt.data(), gt.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))
                 return false;
             A.gtens = gd;
-        } else if (spec[0]) {
+        } else if (p->demag_kernel_spectrum_len != 0) {
             if (!fm::reduce_given_spectra(spec, p->demag_kernel_spectrum_len, nx, ny, nz, red, &e)) {
                 h->err = e;
                 return false;
@@ -1134,7 +1183,9 @@ bool create_impl(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_p
     h->fp64 = p->precision == FULLMAG_FDM_PRECISION_DOUBLE;
     if (!hip_ok(h, h->fp64 ? fm::upload_methods<double>() : fm::upload_methods<float>(), "method table"))
         return false;
-    if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) return false;
+    if (!hip_ok(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream") ||
+        !hip_ok(h, hipStreamCreateWithFlags(&h->io_stream, hipStreamNonBlocking), "readout stream"))
+        return false;
     if (!hip_ok(h, hipHostMalloc((void **)&h->ctl_h, sizeof(fm::Ctl), hipHostMallocDefault), "pinned ctl"))
         return false;
     if (x && x->struct_size < offsetof(fullmag_fdm_plan_ext, thermal_dt))
@@ -1156,14 +1207,42 @@ bool download_soa(Backend *h, const void *dev, std::vector<T> &out) {
     return hip_ok(h, hipMemcpy(out.data(), dev, out.size() * sizeof(T), hipMemcpyDeviceToHost), "download");
 }
 
+// Waveform factor f(t) of the time-dependent Zeeman term, as the kernels
+// evaluate it (FULLMAG_FDM_WAVE_*).
+double wave_factor_host(int kind, double t0, double t1, double freq, double phase, double t) {
+    constexpr double kPi = 3.14159265358979323846;
+    switch (kind) {
+        case fm::WAVE_RAMP: {
+            const double u = (t - t0) / (t1 - t0);
+            return u < 0.0 ? 0.0 : u > 1.0 ? 1.0 : u;
+        }
+        case fm::WAVE_SINC: {
+            const double x = 2.0 * kPi * freq * (t - t0);
+            return x == 0.0 ? 1.0 : std::sin(x) / x;
+        }
+        case fm::WAVE_SINE: return std::sin(2.0 * kPi * freq * t + phase);
+        default: return 0.0;
+    }
+}
+
 // Field as AoS double on the host.
 bool field_aos(Backend *h, fullmag_fdm_observable obs, std::vector<double> &out) {
     const uint64_t n = h->n;
     out.assign(3 * n, 0.0);
     if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
-        for (uint64_t i = 0; i < n; ++i)
-            if (h->has_ext && (!h->has_mask || h->mask_h[i]))
-                for (int c = 0; c < 3; ++c) out[3 * i + c] = h->hext[c];
+        // H_ext(t) = H0 + f(t) H1 (H1 on the waveform's cells) at the time of
+        // the current state, the field the kernels add and price in the
+        // Zeeman energy
+        double h1[3] = {0, 0, 0};
+        const double f = with_args(h, [&](auto &A) {
+            for (int c = 0; c < 3; ++c) h1[c] = A.wave_amp[c];
+            return wave_factor_host(A.wave_kind, A.wave_t0, A.wave_t1, A.wave_freq, A.wave_phase, h->ctl_h->t);
+        });
+        for (uint64_t i = 0; i < n; ++i) {
+            if (h->has_mask && !h->mask_h[i]) continue;
+            const bool wave = f != 0.0 && (h->wave_mask_h.empty() || h->wave_mask_h[i]);
+            for (int c = 0; c < 3; ++c) out[3 * i + c] = (h->has_ext ? h->hext[c] : 0.0) + (wave ? f * h1[c] : 0.0);
+        }
         return true;
     }
     const void *src = nullptr;
@@ -1190,20 +1269,30 @@ bool field_aos(Backend *h, fullmag_fdm_observable obs, std::vector<double> &out)
 
 bool valid_observable(int o) { return o >= FULLMAG_FDM_OBSERVABLE_M && o <= FULLMAG_FDM_OBSERVABLE_H_EFF; }
 
-// Bin averages, reference context.cu:254-317 binning rule.
-void preview_bins(Backend *h, const std::vector<double> &f, uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0,
-                  uint32_t zs, std::vector<double> &out) {
+// Preview bins of H_ext (reference context.cu:1530-1600: the external field
+// is not a device field, so its preview is formed on the host): the mean over
+// each bin (the bin rule of context.cu:254-317) of H0 on magnetic cells plus
+// f(t) H1 on the waveform's cells, FP64 sums in z, y, x order, AoS output.
+// Writes only `out` (no allocation).
+void hext_bins(Backend *h, uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, double *out) {
     const uint32_t fx = h->nx, fy = h->ny, fz = h->nz;
-    out.assign((size_t)pnx * pny * pnz * 3, 0.0);
+    double h1[3] = {0, 0, 0};
+    const double f = with_args(h, [&](auto &A) {
+        for (int c = 0; c < 3; ++c) h1[c] = A.wave_amp[c];
+        return wave_factor_host(A.wave_kind, A.wave_t0, A.wave_t1, A.wave_freq, A.wave_phase, h->ctl_h->t);
+    });
+    auto range = [](uint32_t p, uint32_t full, uint32_t np, uint32_t &b, uint32_t &e) {
+        b = (uint32_t)((uint64_t)p * full / np);
+        e = (uint32_t)((uint64_t)(p + 1) * full / np);
+        if (e <= b) e = b + 1;
+        if (e > full) e = full;
+    };
     for (uint32_t pz = 0; pz < pnz; ++pz)
         for (uint32_t py = 0; py < pny; ++py)
             for (uint32_t px = 0; px < pnx; ++px) {
-                uint32_t xs = (uint32_t)((uint64_t)px * fx / pnx), xe = (uint32_t)((uint64_t)(px + 1) * fx / pnx);
-                if (xe <= xs) xe = xs + 1;
-                if (xe > fx) xe = fx;
-                uint32_t ys = (uint32_t)((uint64_t)py * fy / pny), ye = (uint32_t)((uint64_t)(py + 1) * fy / pny);
-                if (ye <= ys) ye = ys + 1;
-                if (ye > fy) ye = fy;
+                uint32_t xs, xe, ys, ye;
+                range(px, fx, pnx, xs, xe);
+                range(py, fy, pny, ys, ye);
                 uint32_t zs0 = z0 + pz * zs;
                 if (zs0 >= fz) zs0 = fz - 1;
                 uint32_t ze = z0 + (pz + 1) * zs;
@@ -1214,11 +1303,14 @@ void preview_bins(Backend *h, const std::vector<double> &f, uint32_t pnx, uint32
                     for (uint32_t y = ys; y < ye; ++y)
                         for (uint32_t x = xs; x < xe; ++x) {
                             const size_t i = ((size_t)z * fy + y) * fx + x;
-                            for (int c = 0; c < 3; ++c) a[c] += f[3 * i + c];
                             cnt += 1.0;
+                            if (h->has_mask && !h->mask_h[i]) continue;
+                            const bool wave = f != 0.0 && (h->wave_mask_h.empty() || h->wave_mask_h[i]);
+                            for (int c = 0; c < 3; ++c)
+                                a[c] += (h->has_ext ? h->hext[c] : 0.0) + (wave ? f * h1[c] : 0.0);
                         }
                 const size_t o = ((size_t)pz * pny + py) * pnx + px;
-                for (int c = 0; c < 3; ++c) out[3 * o + c] = cnt > 0 ? a[c] / cnt : 0.0;
+                for (int c = 0; c < 3; ++c) out[3 * o + c] = a[c] / cnt;
             }
 }
 
@@ -1337,40 +1429,134 @@ int fullmag_fdm_backend_copy_field_f32(fullmag_fdm_backend *h, fullmag_fdm_obser
     return FULLMAG_FDM_OK;
 }
 
-static int preview_common(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny, uint32_t pnz,
-                          uint32_t z0, uint32_t zs, uint64_t len, std::vector<double> &out) {
+// Device address of an observable's SoA field in the state precision, the
+// fields evaluated at the current state first if needed.  H_EXT has no
+// device copy (it is built on the host).
+static bool field_device_src(fullmag_fdm_backend *h, fullmag_fdm_observable obs, const void **src) {
+    if (obs == FULLMAG_FDM_OBSERVABLE_M) {
+        *src = with_args(h, [&](auto &A) -> const void * { return A.m_all + (size_t)h->ctl_h->parity * 3 * h->n; });
+        return true;
+    }
+    if (!h->obs_fresh && !eval_current(h, true)) return false;
+    *src = h->obs_base[obs == FULLMAG_FDM_OBSERVABLE_H_EX ? 0 : obs == FULLMAG_FDM_OBSERVABLE_H_DEMAG ? 1 : 2];
+    return true;
+}
+
+// Preview bins of a device field into device memory (AoS, FP64 or FP32), on
+// the compute stream.
+static bool preview_launch(fullmag_fdm_backend *h, const void *src, const fm::PreviewGeom &g, void *out, int out_fp64,
+                           hipStream_t st) {
+    const hipError_t e =
+        h->fp64 ? fm::enqueue_preview<double>(static_cast<const double *>(src), (long long)h->n, g, out, out_fp64, st)
+                : fm::enqueue_preview<float>(static_cast<const float *>(src), (long long)h->n, g, out, out_fp64, st);
+    return hip_ok(h, e, "preview binning");
+}
+
+static int preview_check(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny, uint32_t pnz,
+                         uint32_t zs) {
     if (!h) return FULLMAG_FDM_ERR_INVALID;
     if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
     if (!valid_observable(obs) || pnx == 0 || pny == 0 || pnz == 0 || zs == 0)
         return fail(h, FULLMAG_FDM_ERR_INVALID, "invalid preview request");
-    if (len != (uint64_t)pnx * pny * pnz * 3) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview out_len mismatch");
+    return FULLMAG_FDM_OK;
+}
+
+// Synchronous preview (copy_field_preview_*): binned on the device, only the
+// bins cross to the host.  H_EXT is binned on the host from its field.
+static int preview_copy(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny, uint32_t pnz,
+                        uint32_t z0, uint32_t zs, void *out, uint64_t len, bool out_fp64) {
+    const int rc = preview_check(h, obs, pnx, pny, pnz, zs);
+    if (rc != FULLMAG_FDM_OK) return rc;
+    if (!out) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview output is NULL");
+    const uint64_t cnt = (uint64_t)pnx * pny * pnz;
+    if (len != cnt * 3) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview out_len mismatch");
     hipSetDevice(h->device);
-    std::vector<double> f;
-    if (!field_aos(h, obs, f)) return FULLMAG_FDM_ERR_CUDA;
-    preview_bins(h, f, pnx, pny, pnz, z0, zs, out);
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        if (out_fp64) {
+            hext_bins(h, pnx, pny, pnz, z0, zs, static_cast<double *>(out));
+        } else {
+            std::vector<double> v(len);
+            hext_bins(h, pnx, pny, pnz, z0, zs, v.data());
+            for (size_t i = 0; i < v.size(); ++i) static_cast<float *>(out)[i] = (float)v[i];
+        }
+        return FULLMAG_FDM_OK;
+    }
+    const void *src = nullptr;
+    if (!field_device_src(h, obs, &src)) return FULLMAG_FDM_ERR_CUDA;
+    const size_t bytes = (size_t)len * (out_fp64 ? sizeof(double) : sizeof(float));
+    if (bytes > h->preview_cap) {
+        if (!hip_ok(h, hipStreamSynchronize(h->stream), "stream sync")) return FULLMAG_FDM_ERR_CUDA;
+        if (h->preview_d) hipFree(h->preview_d);
+        h->preview_d = nullptr;
+        h->preview_cap = 0;
+        if (!hip_ok(h, hipMalloc(&h->preview_d, bytes), "preview buffer")) return FULLMAG_FDM_ERR_CUDA;
+        h->preview_cap = bytes;
+    }
+    const fm::PreviewGeom g{h->nx, h->ny, h->nz, pnx, pny, pnz, z0, zs};
+    if (!preview_launch(h, src, g, h->preview_d, out_fp64 ? 1 : 0, h->stream) ||
+        !hip_ok(h, hipMemcpyAsync(out, h->preview_d, bytes, hipMemcpyDeviceToHost, h->stream), "preview download") ||
+        !hip_ok(h, hipStreamSynchronize(h->stream), "stream sync"))
+        return FULLMAG_FDM_ERR_CUDA;
     return FULLMAG_FDM_OK;
 }
 
 int fullmag_fdm_backend_copy_field_preview_f64(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx,
                                                uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, double *out,
                                                uint64_t len) {
-    if (!out) return FULLMAG_FDM_ERR_INVALID;
-    std::vector<double> v;
-    const int rc = preview_common(h, obs, pnx, pny, pnz, z0, zs, len, v);
-    if (rc == FULLMAG_FDM_OK) std::memcpy(out, v.data(), v.size() * sizeof(double));
-    return rc;
+    return preview_copy(h, obs, pnx, pny, pnz, z0, zs, out, len, true);
 }
 
 int fullmag_fdm_backend_copy_field_preview_f32(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx,
                                                uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, float *out,
                                                uint64_t len) {
-    if (!out) return FULLMAG_FDM_ERR_INVALID;
-    std::vector<double> v;
-    const int rc = preview_common(h, obs, pnx, pny, pnz, z0, zs, len, v);
-    if (rc == FULLMAG_FDM_OK)
-        for (size_t i = 0; i < v.size(); ++i) out[i] = (float)v[i];
-    return rc;
+    return preview_copy(h, obs, pnx, pny, pnz, z0, zs, out, len, false);
 }
+
+}  // extern "C"
+
+namespace {
+
+// Releases what a snapshot holds (either kind).
+template <class S>
+void snapshot_release(S *s) {
+    if (!s) return;
+    if (s->done) hipEventSynchronize(s->done);
+    if (s->ready) hipEventDestroy(s->ready);
+    if (s->done) hipEventDestroy(s->done);
+    if (s->staging) hipFree(s->staging);
+    if (s->host) hipHostFree(s->host);
+    delete s;
+}
+
+// Schedules the staged payload's download: `ready` on the compute stream,
+// the copy on the readout stream after it, `done` behind the copy.
+template <class S>
+bool snapshot_schedule(fullmag_fdm_backend *h, S *s) {
+    return hip_ok(h, hipEventCreateWithFlags(&s->ready, hipEventDisableTiming), "snapshot event") &&
+           hip_ok(h, hipEventCreateWithFlags(&s->done, hipEventDisableTiming), "snapshot event") &&
+           hip_ok(h, hipEventRecord(s->ready, h->stream), "snapshot event") &&
+           hip_ok(h, hipStreamWaitEvent(h->io_stream, s->ready, 0), "snapshot stream wait") &&
+           hip_ok(h, hipMemcpyAsync(s->host, s->staging, s->bytes, hipMemcpyDeviceToHost, h->io_stream),
+                  "snapshot download") &&
+           hip_ok(h, hipEventRecord(s->done, h->io_stream), "snapshot event");
+}
+
+template <class S>
+int snapshot_wait(S *s, const void **data, uint64_t *len, fullmag_fdm_snapshot_desc *desc) {
+    if (!s) return FULLMAG_FDM_ERR_INVALID;
+    if (s->done && !s->waited) {
+        if (hipEventSynchronize(s->done) != hipSuccess) return FULLMAG_FDM_ERR_CUDA;
+        s->waited = true;
+    }
+    if (data) *data = s->host;
+    if (len) *len = s->bytes;
+    if (desc) *desc = s->desc;
+    return FULLMAG_FDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 fullmag_fdm_field_snapshot *fullmag_fdm_backend_begin_field_snapshot(fullmag_fdm_backend *h,
                                                                      fullmag_fdm_observable obs) {
@@ -1387,39 +1573,30 @@ fullmag_fdm_field_snapshot *fullmag_fdm_backend_begin_field_snapshot(fullmag_fdm
     s->desc.component_count = 3;
     s->desc.scalar_bytes = (uint32_t)sb;
     s->desc.scalar_type = h->fp64 ? FULLMAG_FDM_SNAPSHOT_SCALAR_F64 : FULLMAG_FDM_SNAPSHOT_SCALAR_F32;
-    if (hipHostMalloc(&s->host, s->bytes, hipHostMallocDefault) != hipSuccess) {
-        delete s;
-        h->err = "pinned snapshot allocation failed";
+    if (!hip_ok(h, hipHostMalloc(&s->host, s->bytes, hipHostMallocDefault), "pinned snapshot allocation")) {
+        snapshot_release(s);
         return nullptr;
     }
     if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        // built on the host: complete at once
         std::vector<double> f;
-        field_aos(h, obs, f);
+        if (!field_aos(h, obs, f)) {
+            snapshot_release(s);
+            return nullptr;
+        }
         for (uint64_t i = 0; i < h->n; ++i)
             for (int c = 0; c < 3; ++c) {
                 if (h->fp64) static_cast<double *>(s->host)[c * h->n + i] = f[3 * i + c];
                 else static_cast<float *>(s->host)[c * h->n + i] = (float)f[3 * i + c];
             }
+        s->waited = true;
         return s;
     }
-    const void *src;
-    if (obs == FULLMAG_FDM_OBSERVABLE_M) {
-        src = with_args(h, [&](auto &A) -> const void * { return A.m_all + (size_t)h->ctl_h->parity * 3 * h->n; });
-    } else {
-        if (!h->obs_fresh && !eval_current(h, true)) {
-            hipHostFree(s->host);
-            delete s;
-            return nullptr;
-        }
-        src = h->obs_base[obs == FULLMAG_FDM_OBSERVABLE_H_EX ? 0 : obs == FULLMAG_FDM_OBSERVABLE_H_DEMAG ? 1 : 2];
-    }
-    if (hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess ||
-        hipMemcpyAsync(s->host, src, s->bytes, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
-        hipEventRecord(s->ev, h->stream) != hipSuccess) {
-        h->err = "snapshot copy failed";
-        if (s->ev) hipEventDestroy(s->ev);
-        hipHostFree(s->host);
-        delete s;
+    const void *src = nullptr;
+    if (!field_device_src(h, obs, &src) || !hip_ok(h, hipMalloc(&s->staging, s->bytes), "snapshot staging") ||
+        !hip_ok(h, hipMemcpyAsync(s->staging, src, s->bytes, hipMemcpyDeviceToDevice, h->stream), "snapshot staging") ||
+        !snapshot_schedule(h, s)) {
+        snapshot_release(s);
         return nullptr;
     }
     return s;
@@ -1429,63 +1606,54 @@ fullmag_fdm_preview_snapshot *fullmag_fdm_backend_begin_preview_snapshot(fullmag
                                                                          fullmag_fdm_observable obs, uint32_t pnx,
                                                                          uint32_t pny, uint32_t pnz, uint32_t z0,
                                                                          uint32_t zs) {
-    if (!h) return nullptr;
-    std::vector<double> v;
-    if (preview_common(h, obs, pnx, pny, pnz, z0, zs, (uint64_t)pnx * pny * pnz * 3, v) != FULLMAG_FDM_OK)
-        return nullptr;
+    if (preview_check(h, obs, pnx, pny, pnz, zs) != FULLMAG_FDM_OK) return nullptr;
+    hipSetDevice(h->device);
     auto *s = new (std::nothrow) fullmag_fdm_preview_snapshot();
     if (!s) return nullptr;
     const uint64_t cells = (uint64_t)pnx * pny * pnz;
+    const size_t sb = h->fp64 ? 8 : 4;
+    s->bytes = cells * 3 * sb;
     s->desc.cell_count = cells;
     s->desc.component_count = 3;
-    if (h->fp64) {
-        s->desc.scalar_bytes = 8;
-        s->desc.scalar_type = FULLMAG_FDM_SNAPSHOT_SCALAR_F64;
-        s->host.resize(v.size() * 8);
-        std::memcpy(s->host.data(), v.data(), s->host.size());
-    } else {
-        s->desc.scalar_bytes = 4;
-        s->desc.scalar_type = FULLMAG_FDM_SNAPSHOT_SCALAR_F32;
-        s->host.resize(v.size() * 4);
-        float *f = reinterpret_cast<float *>(s->host.data());
-        for (size_t i = 0; i < v.size(); ++i) f[i] = (float)v[i];
+    s->desc.scalar_bytes = (uint32_t)sb;
+    s->desc.scalar_type = h->fp64 ? FULLMAG_FDM_SNAPSHOT_SCALAR_F64 : FULLMAG_FDM_SNAPSHOT_SCALAR_F32;
+    if (!hip_ok(h, hipHostMalloc(&s->host, s->bytes, hipHostMallocDefault), "pinned preview allocation")) {
+        snapshot_release(s);
+        return nullptr;
+    }
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        std::vector<double> v(cells * 3);
+        hext_bins(h, pnx, pny, pnz, z0, zs, v.data());
+        for (size_t i = 0; i < v.size(); ++i) {
+            if (h->fp64) static_cast<double *>(s->host)[i] = v[i];
+            else static_cast<float *>(s->host)[i] = (float)v[i];
+        }
+        s->waited = true;
+        return s;
+    }
+    const void *src = nullptr;
+    const fm::PreviewGeom g{h->nx, h->ny, h->nz, pnx, pny, pnz, z0, zs};
+    if (!field_device_src(h, obs, &src) || !hip_ok(h, hipMalloc(&s->staging, s->bytes), "preview staging") ||
+        !preview_launch(h, src, g, s->staging, h->fp64 ? 1 : 0, h->stream) || !snapshot_schedule(h, s)) {
+        snapshot_release(s);
+        return nullptr;
     }
     return s;
 }
 
 int fullmag_fdm_field_snapshot_wait(fullmag_fdm_field_snapshot *s, const void **data, uint64_t *len,
                                     fullmag_fdm_snapshot_desc *desc) {
-    if (!s) return FULLMAG_FDM_ERR_INVALID;
-    if (s->ev && !s->waited) {
-        if (hipEventSynchronize(s->ev) != hipSuccess) return FULLMAG_FDM_ERR_CUDA;
-        s->waited = true;
-    }
-    if (data) *data = s->host;
-    if (len) *len = s->bytes;
-    if (desc) *desc = s->desc;
-    return FULLMAG_FDM_OK;
+    return snapshot_wait(s, data, len, desc);
 }
 
 int fullmag_fdm_preview_snapshot_wait(fullmag_fdm_preview_snapshot *s, const void **data, uint64_t *len,
                                       fullmag_fdm_snapshot_desc *desc) {
-    if (!s) return FULLMAG_FDM_ERR_INVALID;
-    if (data) *data = s->host.data();
-    if (len) *len = s->host.size();
-    if (desc) *desc = s->desc;
-    return FULLMAG_FDM_OK;
+    return snapshot_wait(s, data, len, desc);
 }
 
-void fullmag_fdm_field_snapshot_destroy(fullmag_fdm_field_snapshot *s) {
-    if (!s) return;
-    if (s->ev) {
-        hipEventSynchronize(s->ev);
-        hipEventDestroy(s->ev);
-    }
-    if (s->host) hipHostFree(s->host);
-    delete s;
-}
+void fullmag_fdm_field_snapshot_destroy(fullmag_fdm_field_snapshot *s) { snapshot_release(s); }
 
-void fullmag_fdm_preview_snapshot_destroy(fullmag_fdm_preview_snapshot *s) { delete s; }
+void fullmag_fdm_preview_snapshot_destroy(fullmag_fdm_preview_snapshot *s) { snapshot_release(s); }
 
 static int upload_common(fullmag_fdm_backend *h, const double *d, const float *f, uint64_t len) {
     if (!h || (!d && !f)) return FULLMAG_FDM_ERR_INVALID;
@@ -1566,8 +1734,11 @@ void fullmag_fdm_backend_destroy(fullmag_fdm_backend *h) {
     if (h->graph) hipGraphDestroy(h->graph);
     for (void *p : h->allocs) hipFree(p);
     for (void *p : h->adj.bufs) hipFree(p);
+    if (h->io_stream) hipStreamSynchronize(h->io_stream);
+    if (h->preview_d) hipFree(h->preview_d);
     if (h->ctl_h) hipHostFree(h->ctl_h);
     if (h->stream) hipStreamDestroy(h->stream);
+    if (h->io_stream) hipStreamDestroy(h->io_stream);
     delete h;
 }
 
@@ -1612,7 +1783,7 @@ int fullmag_fdm_backend_advance(fullmag_fdm_backend *h, double t_end, uint64_t m
     const double dmin = h->fp64 ? h->ad.dt_min : h->af.dt_min, dmax = h->fp64 ? h->ad.dt_max : h->af.dt_max;
     c.loop = 1;
     c.t_end = t_end;
-    c.step_end = c.step + max_steps;
+    c.step_end = sat_add(c.step, max_steps);
     c.dt_carry = dt;
     double d = std::min(dt, t_end - c.t);
     if (h->adaptive) d = std::min(std::max(d, dmin), dmax);
@@ -1662,8 +1833,12 @@ int fullmag_fdm_backend_advance(fullmag_fdm_backend *h, double t_end, uint64_t m
         }
         if (rc != FULLMAG_FDM_OK) break;
         launched += nl;
-        hipMemcpyAsync(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream);
-        hipEventRecord(ev[cur], h->stream);
+        if (!hip_ok(h, hipMemcpyAsync(&flag_h[cur], h->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, h->stream),
+                    "control readback") ||
+            !hip_ok(h, hipEventRecord(ev[cur], h->stream), "event record")) {
+            rc = FULLMAG_FDM_ERR_CUDA;
+            break;
+        }
         if (pending) {
             if (hipEventSynchronize(ev[1 - cur]) != hipSuccess) {
                 rc = fail(h, FULLMAG_FDM_ERR_CUDA, "event sync");
@@ -1990,8 +2165,11 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     D.t0 = c.t;
     D.parity0 = c.parity;
     const size_t n3 = 3 * n;
-    hipMemcpyAsync(D.cp, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream);
-    hipMemsetAsync(D.obj_part, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
+    if (!hip_ok(h, hipMemcpyAsync(D.cp, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream),
+                "adjoint checkpoint") ||
+        !hip_ok(h, hipMemsetAsync(D.obj_part, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream),
+                "adjoint objective clear"))
+        return FULLMAG_FDM_ERR_CUDA;
     c.loop = 1;
     c.t_end = std::numeric_limits<double>::infinity();
     c.step_end = c.step + steps;
@@ -2003,7 +2181,9 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     std::vector<double> tn(steps + 1);
     tn[0] = D.t0;
     for (uint64_t s = 1; s <= steps; ++s) tn[s] = tn[s - 1] + D.dt;
-    hipMemcpyAsync(D.tn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+    if (!hip_ok(h, hipMemcpyAsync(D.tn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                "adjoint step times"))
+        return FULLMAG_FDM_ERR_CUDA;
     fm::AdjArgs<T> J = adj_args<T>(h);
     J.obj_from = (long long)(c.step + D.first);
     if (!push_ctl(h) || !hip_ok(h, hipStreamSynchronize(h->stream), "adjoint setup")) return FULLMAG_FDM_ERR_CUDA;
@@ -2018,9 +2198,11 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     for (uint64_t s = 1; s <= steps && ok; ++s) {
         ok = g ? hip_ok(h, hipGraphLaunch(g, h->stream), "adjoint forward graph") : one_step();
         par ^= 1;
-        if (s % D.seg == 0 && s < steps)
-            hipMemcpyAsync(static_cast<char *>(D.cp) + (s / D.seg) * v3, A.m_all + (size_t)par * n3, v3,
-                           hipMemcpyDeviceToDevice, h->stream);
+        if (ok && s % D.seg == 0 && s < steps)
+            ok = hip_ok(h,
+                        hipMemcpyAsync(static_cast<char *>(D.cp) + (s / D.seg) * v3, A.m_all + (size_t)par * n3, v3,
+                                       hipMemcpyDeviceToDevice, h->stream),
+                        "adjoint checkpoint");
     }
     if (g) hipGraphExecDestroy(g);
     if (!ok) return FULLMAG_FDM_ERR_CUDA;
@@ -2037,7 +2219,9 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     c.loop = 0;
     c.done = 0;
     if (!push_ctl(h) || !pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
-    hipMemcpyAsync(D.m_end, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream);
+    if (!hip_ok(h, hipMemcpyAsync(D.m_end, A.m_all + (size_t)c.parity * n3, v3, hipMemcpyDeviceToDevice, h->stream),
+                "adjoint final state"))
+        return FULLMAG_FDM_ERR_CUDA;
     D.ctl_end = c;
     D.ready = true;
     if (out_j) *out_j = jsum * D.obj_scale;
@@ -2050,9 +2234,10 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
     const uint64_t n = h->n;
     const int ns = h->nstage;
     const size_t n3 = 3 * n, v3 = n3 * sizeof(T);
-    hipMemsetAsync(D.grad_ms, 0, n * sizeof(double), h->stream);
-    hipMemsetAsync(D.grad_amp, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream);
-    hipMemsetAsync(D.cot_m, 0, 3 * n * sizeof(double), h->stream);
+    if (!hip_ok(h, hipMemsetAsync(D.grad_ms, 0, n * sizeof(double), h->stream), "adjoint clear") ||
+        !hip_ok(h, hipMemsetAsync(D.grad_amp, 0, (size_t)D.blocks * 3 * sizeof(double), h->stream), "adjoint clear") ||
+        !hip_ok(h, hipMemsetAsync(D.cot_m, 0, 3 * n * sizeof(double), h->stream), "adjoint clear"))
+        return FULLMAG_FDM_ERR_CUDA;
     const fm::AdjArgs<T> J = adj_args<T>(h);
     // recording form of the arguments: slot = device step index mod seg
     fm::Args<T> B = A;
@@ -2099,17 +2284,17 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
         c.dt = D.dt;
         c.done = 0;
         const char *cpk = static_cast<const char *>(D.cp) + k * v3;
-        hipMemcpyAsync(A.m_all, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
-        hipMemcpyAsync(D.rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream);
-        ok = push_ctl(h) && hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n0, 0, h->stream), "adjoint step") &&
+        ok = hip_ok(h, hipMemcpyAsync(A.m_all, cpk, v3, hipMemcpyDeviceToDevice, h->stream), "adjoint restart") &&
+             hip_ok(h, hipMemcpyAsync(D.rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream), "adjoint restart") &&
+             push_ctl(h) && hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n0, 0, h->stream), "adjoint step") &&
              hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint recompute (k1)");
         for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rec, rec_step);
         if (!ok) break;
         // the recompute ends with the device loop marked done; the y/z passes
         // that the reverse sweep shares with the forward step return early
         // while it is set
-        hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream);
-        ok = hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n1 - 1, 0, h->stream), "adjoint step");
+        ok = hip_ok(h, hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream), "adjoint loop flag") &&
+             hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n1 - 1, 0, h->stream), "adjoint step");
         for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rev, rev_step);
     }
     if (g_rec) hipGraphExecDestroy(g_rec);
@@ -2127,8 +2312,10 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
     if (le != hipSuccess) return fail(h, FULLMAG_FDM_ERR_CUDA, std::string("device error: ") + hipGetErrorString(le));
     // restore the final state of the forward run and its k1
     c = D.ctl_end;
-    hipMemcpyAsync(A.m_all + (size_t)c.parity * n3, D.m_end, v3, hipMemcpyDeviceToDevice, h->stream);
-    if (!push_ctl(h) || !eval_current(h, false)) return FULLMAG_FDM_ERR_CUDA;
+    if (!hip_ok(h, hipMemcpyAsync(A.m_all + (size_t)c.parity * n3, D.m_end, v3, hipMemcpyDeviceToDevice, h->stream),
+                "adjoint final state") ||
+        !push_ctl(h) || !eval_current(h, false))
+        return FULLMAG_FDM_ERR_CUDA;
     if (grad_ms)
         for (uint64_t i = 0; i < n; ++i) grad_ms[i] = gms[i];
     if (grad_amp) {
@@ -2206,8 +2393,12 @@ using Batch = fullmag_fdm_batch;
 // Launch geometry that must agree for systems to share launches.
 template <class T>
 std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
+    // the y/z stage of a launch is chosen from the group's first member:
+    // multilayer mixing, periodic (complex) spectra and the z-neighbour rule
+    // must agree as well
     return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
-            A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt, A.has_th};
+            A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt, A.has_th, A.lmix != nullptr, A.layered, A.pbc,
+            A.gtens != nullptr};
 }
 
 template <class T>
@@ -2388,7 +2579,7 @@ int fullmag_fdm_batch_advance(fullmag_fdm_batch *b, double t_end, uint64_t max_s
         const double dmin = h->fp64 ? h->ad.dt_min : h->af.dt_min, dmax = h->fp64 ? h->ad.dt_max : h->af.dt_max;
         c.loop = 1;
         c.t_end = t_end;
-        c.step_end = c.step + max_steps;
+        c.step_end = sat_add(c.step, max_steps);
         c.dt_carry = dt0;
         double d = std::min(dt0, t_end - c.t);
         if (h->adaptive) d = std::min(std::max(d, dmin), dmax);
@@ -2415,8 +2606,11 @@ int fullmag_fdm_batch_advance(fullmag_fdm_batch *b, double t_end, uint64_t max_s
             active = true;
             if (!(g.fp64 ? group_launch<double>(b, g) : group_launch<float>(b, g))) rc = FULLMAG_FDM_ERR_CUDA;
             for (size_t m : g.members)
-                hipMemcpyAsync(&b->flags[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream);
-            hipEventRecord(g.ev, g.stream);
+                if (hipMemcpyAsync(&b->flags[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream) !=
+                    hipSuccess)
+                    rc = FULLMAG_FDM_ERR_CUDA;
+            if (hipEventRecord(g.ev, g.stream) != hipSuccess) rc = FULLMAG_FDM_ERR_CUDA;
+            if (rc != FULLMAG_FDM_OK && b->err.empty()) b->err = "batch: control readback failed";
         }
         if (!active || rc != FULLMAG_FDM_OK) break;
         for (auto &g : b->groups) {

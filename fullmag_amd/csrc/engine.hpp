@@ -273,6 +273,13 @@ struct AdjArgs {
     double dt;                        // step size
 };
 
+// Field grid and preview request of the preview binning kernel.
+struct PreviewGeom {
+    uint32_t fx, fy, fz;     // field grid
+    uint32_t pnx, pny, pnz;  // preview grid
+    uint32_t z0, zs;         // z origin and z cells per preview plane
+};
+
 // Method indices in the constant table.
 // M_ABM3S is the start-up table of M_ABM3 (selected on the device while the
 // history holds fewer than three values).

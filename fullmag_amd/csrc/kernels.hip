@@ -465,9 +465,12 @@ __device__ __forceinline__ bool slot_is_max(int i) {
     return i == ST_ERR || i == ST_HEFF2 || i == ST_HD2 || i == ST_RHS2;
 }
 
-// Block-wide reduction of Acc into out[0..kStatSlots) by thread 0 (fixed
-// order, so deterministic).  Must be called by all threads.
-__device__ void block_reduce_store(Acc &a, double *out) {
+// Block-wide reduction of Acc in a fixed order (deterministic): thread
+// i < kStatSlots returns the total of slot i, the others 0.  Must be called
+// by all threads.  The two stores below keep the destination's address
+// space explicit: per-work-group partials go to global memory, the
+// controller's totals to LDS.
+__device__ double block_reduce_slot(Acc &a) {
     __shared__ double red[kStatSlots][kThreads / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -480,12 +483,25 @@ __device__ void block_reduce_store(Acc &a, double *out) {
         if (lane == 0) red[i][wid] = x;
     }
     __syncthreads();
+    double x = 0.0;
     if (threadIdx.x < kStatSlots) {
         const int i = threadIdx.x;
-        double x = red[i][0];
+        x = red[i][0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x = slot_is_max(i) ? fmax(x, red[i][w]) : x + red[i][w];
-        out[i] = x;  // out may be LDS (k_control): generic access
     }
+    return x;
+}
+
+// Totals into a work-group's partial record (device memory).
+__device__ void block_reduce_global(Acc &a, double *out) {
+    const double x = block_reduce_slot(a);
+    if (threadIdx.x < kStatSlots) G(out)[threadIdx.x] = x;
+}
+
+// Totals into an LDS array of the calling kernel.
+__device__ void block_reduce_lds(Acc &a, double (&tot)[kStatSlots]) {
+    const double x = block_reduce_slot(a);
+    if (threadIdx.x < kStatSlots) tot[threadIdx.x] = x;
 }
 
 // Waveform value f(t) of the time-dependent Zeeman term.
@@ -1446,7 +1462,7 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     Acc acc;
     acc.init();
     x_local<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, res, stats != 0, err, acc);
-    if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
+    if (stats || err) block_reduce_global(acc, A.partials + (long long)blockIdx.x * kStatSlots);
 }
 
 // Waves per SIMD the joined x pass is compiled for: FP32 with the uniform
@@ -2354,7 +2370,7 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
                              A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
                              nullptr, stage_push(A, M, s));
     }
-    if (stats || err) block_reduce_store(acc, A.partials + (long long)blockIdx.x * kStatSlots);
+    if (stats || err) block_reduce_global(acc, A.partials + (long long)blockIdx.x * kStatSlots);
 }
 
 // Reduction of all per-block partials and the step controller, executed by
@@ -2377,7 +2393,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
         }
     }
     __shared__ double tot[kStatSlots];
-    block_reduce_store(acc, tot);
+    block_reduce_lds(acc, tot);
     __syncthreads();
     if (threadIdx.x != 0) return;
     const Method &M = method_for(A);
@@ -3259,5 +3275,63 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st) {
 }
 
 template hipError_t enqueue_kind<FM_REAL>(const Args<FM_REAL> &, int, int, hipStream_t);
+
+// Preview bins of a SoA field (reference downsample_field_preview_kernel,
+// native/backends/fdm/src/context.cu:254-317): preview cell (px, py, pz)
+// averages the box x in [px fx/pnx, (px+1) fx/pnx), y likewise, z in
+// [z0 + pz zs, z0 + (pz+1) zs), each range at least one cell and clipped to
+// the grid; FP64 sums in z, y, x order; AoS output [x0,y0,z0, x1,...].  One
+// thread per preview cell.
+template <class T, class O>
+__global__ void __launch_bounds__(kThreads) k_preview(const T *__restrict__ f, long long n, PreviewGeom g,
+                                                      O *__restrict__ out) {
+    const unsigned long long cnt = (unsigned long long)g.pnx * g.pny * g.pnz;
+    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const uint32_t px = (uint32_t)(i % g.pnx), py = (uint32_t)((i / g.pnx) % g.pny),
+                   pz = (uint32_t)(i / ((unsigned long long)g.pnx * g.pny));
+    auto range = [](uint32_t p, uint32_t full, uint32_t np, uint32_t &b, uint32_t &e) {
+        b = (uint32_t)(((unsigned long long)p * full) / np);
+        e = (uint32_t)(((unsigned long long)(p + 1) * full) / np);
+        if (e <= b) e = b + 1;
+        if (e > full) e = full;
+    };
+    uint32_t xb, xe, yb, ye;
+    range(px, g.fx, g.pnx, xb, xe);
+    range(py, g.fy, g.pny, yb, ye);
+    uint32_t zb = g.z0 + pz * g.zs;
+    if (zb >= g.fz) zb = g.fz - 1;
+    uint32_t ze = g.z0 + (pz + 1) * g.zs;
+    if (ze <= zb) ze = zb + 1;
+    if (ze > g.fz) ze = g.fz;
+    const gp<const T> F = G(f);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, c = 0.0;
+    for (uint32_t z = zb; z < ze; ++z)
+        for (uint32_t y = yb; y < ye; ++y)
+            for (uint32_t x = xb; x < xe; ++x) {
+                const long long q = ((long long)z * g.fy + y) * g.fx + x;
+                a0 += (double)F[q];
+                a1 += (double)F[n + q];
+                a2 += (double)F[2 * n + q];
+                c += 1.0;
+            }
+    const gp<O> o = G(out);
+    o[3 * i] = (O)(a0 / c);
+    o[3 * i + 1] = (O)(a1 / c);
+    o[3 * i + 2] = (O)(a2 / c);
+}
+
+template <class T>
+hipError_t enqueue_preview(const T *field, long long ncell, const PreviewGeom &g, void *out, int out_fp64,
+                           hipStream_t st) {
+    const unsigned long long cnt = (unsigned long long)g.pnx * g.pny * g.pnz;
+    if (cnt == 0 || g.fx == 0 || g.fy == 0 || g.fz == 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((cnt + kThreads - 1) / kThreads));
+    if (out_fp64) k_preview<T, double><<<grid, dim3(kThreads), 0, st>>>(field, ncell, g, static_cast<double *>(out));
+    else k_preview<T, float><<<grid, dim3(kThreads), 0, st>>>(field, ncell, g, static_cast<float *>(out));
+    return hipGetLastError();
+}
+
+template hipError_t enqueue_preview<FM_REAL>(const FM_REAL *, long long, const PreviewGeom &, void *, int, hipStream_t);
 
 }  // namespace fm

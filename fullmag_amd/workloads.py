@@ -26,6 +26,40 @@ def film_with_hole(seed: int = 42, n: int = 512) -> dict:
                 dt_max=1e-10, headroom=0.8)
 
 
+def xorshift_uniform(seed: int, count: int) -> np.ndarray:
+    """count values in [-1, 1] from the planner's xorshift64 stream
+    (crates/fullmag-plan/src/util.rs:118-147: shifts 13/7/17, value
+    state / u64::MAX * 2 - 1), without its rejection and normalisation."""
+    if seed == 0:
+        raise ValueError("xorshift64 seed 0 never leaves the zero state")
+    mask = (1 << 64) - 1
+    s = seed & mask
+    out = np.empty(count)
+    for i in range(count):
+        s ^= (s << 13) & mask
+        s ^= s >> 7
+        s ^= (s << 17) & mask
+        out[i] = (s / float(mask)) * 2.0 - 1.0
+    return out
+
+
+def texture_relax(seed: int = 1, nx: int = 200, ny: int = 50) -> dict:
+    """Plain small relaxation: nx x ny x 1 cells of 2.5 x 2.5 x 3 nm, Ms
+    8.0e5 A/m, A 1.3e-11 J/m, alpha 0.5, no anisotropy, exchange + Newell
+    demag + Zeeman with B = 0; m0 = normalise(normalise(1, 0.25, 0.1) + n_i)
+    with per-component noise n uniform in [-0.01, 0.01] from the seeded
+    xorshift64 stream (cell-major, x y z per cell); DP45 with rtol 1e-6,
+    atol 1e-5, FP64."""
+    n = nx * ny
+    base = np.array([1.0, 0.25, 0.1])
+    base /= np.linalg.norm(base)
+    m0 = base[None, :] + 0.01 * xorshift_uniform(seed, 3 * n).reshape(n, 3)
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    return dict(nx=nx, ny=ny, nz=1, dx=2.5e-9, dy=2.5e-9, dz=3e-9, ms=8.0e5, a_ex=1.3e-11, alpha=0.5, m0=m0,
+                integrator="dp45", precision="double", h_ext=(0.0, 0.0, 0.0), atol=1e-5, rtol=1e-6, dt_min=1e-18,
+                dt_max=1e-10, headroom=0.8)
+
+
 def voronoi_axes(nx: int, ny: int, nz: int, cell: float, grains: int, seed: int):
     """Per-cell easy axes of a seeded Voronoi grain structure: `grains`
     sites uniform in the box, each cell takes the axis of the nearest site

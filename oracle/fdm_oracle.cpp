@@ -22,6 +22,9 @@
 //                           history: crates/fullmag-engine/src/fdm_state.rs:204-262
 //   error norm ........... crates/fullmag-engine/src/fdm_integrators.rs:828-870
 //   neighbour clamp ...... crates/fullmag-engine/src/fdm_types.rs:126-132
+//   region exchange table  native/backends/fdm/src/exchange_fp64.cu:80-114 (per-neighbour
+//                          A_ij = lut[r_i * 256 + r_j] from the region ids); the table the
+//                          runner builds: crates/fullmag-runner/src/native_fdm.rs:215-238
 //   cubic anisotropy ..... crates/fullmag-engine/src/fdm_fields.rs:648-705 (Kc1, Kc2);
 //                          Kc3 term native/backends/fdm/src/demag_fp64.cu:301-330
 //   interfacial/bulk DMI . crates/fullmag-engine/src/fdm_fields.rs:707-816
@@ -409,6 +412,11 @@ typedef struct {
     // multilayer (multilayer.rs): the nz planes are single-cell layers with
     // centres at layer_z[k], coupled only by demag (NULL = uniform grid)
     const double *layer_z;
+    // exchange region table (exchange_fp64.cu:80-114): region id per cell
+    // (< 256) and the 256 x 256 stiffness table A_ij (J/m); NULL = the
+    // uniform stiffness a_ex
+    const uint32_t *region;
+    const double *exchange_lut;
 } orc_desc;
 
 typedef struct {
@@ -436,6 +444,8 @@ struct Problem {
     std::vector<int32_t> dc_t, dc_s;          // sparse demag correction
     std::vector<double> dc_n;
     std::vector<double> lz;                   // multilayer plane heights
+    std::vector<uint32_t> reg;                // exchange region ids (empty: uniform stiffness)
+    std::vector<double> lut;                  // 256 x 256 region stiffness table
     std::vector<std::vector<cplx>> lk;        // pair kernel spectra [(dst*L + src)*6 + c]
     uint64_t th_step = 0;     // thermal counter: step calls so far
     double th_dt_cur = 0.0;   // dt of the step attempt in flight
@@ -458,6 +468,9 @@ struct Problem {
     double abm_dt = 0.0;
 
     bool active(size_t i) const { return !has_mask || mask[i] != 0; }
+    // exchange stiffness between cells i and j: A_{r_i r_j} from the region
+    // table (exchange_fp64.cu:93-104), else the uniform a_ex
+    double stiff(size_t i, size_t j) const { return reg.empty() ? d.a_ex : lut[(size_t)reg[i] * 256 + reg[j]]; }
     bool per(int a) const { return (d.periodic >> a) & 1u; }
     bool layered() const { return !lz.empty(); }
     // neighbor_index, fdm_types.rs:119-132: wrap on periodic axes, clamp
@@ -772,10 +785,11 @@ void exchange_tier_add(const Problem &p, const V3 *m, V3 *h, int tier) {
                     const double opp = std::max(p.dlt[2 * a + (1 - side)][s], dmin);
                     w = 2.0 / (d[a] * (d[a] + opp));
                 }
+                w *= p.stiff(s, q);
                 for (int k = 0; k < 3; ++k) acc[k] += w * (m[q].c[k] - c.c[k]);
             }
         }
-        const double pre = 2.0 * p.d.a_ex / (MU0 * p.ms_at(s) * std::max(phi0, floor_));
+        const double pre = 2.0 / (MU0 * p.ms_at(s) * std::max(phi0, floor_));
         for (int k = 0; k < 3; ++k) h[s].c[k] += pre * acc[k];
     }
 }
@@ -796,6 +810,29 @@ void exchange_add(const Problem &p, const V3 *m, V3 *h) {
     }
     const double dx2 = p.d.dx * p.d.dx, dy2 = p.d.dy * p.d.dy, dz2 = p.d.dz * p.d.dz;
     const long nx = p.d.nx, ny = p.d.ny;
+    if (!p.reg.empty()) {
+        // region table form (exchange_fp64.cu:80-114): H = 2/(mu0 Ms) sum_j
+        // A_{r_i r_j} (m_j - m_i)/d^2 over the clamped (or wrapped)
+        // neighbours; an inactive neighbour contributes nothing
+#pragma omp parallel for schedule(static)
+        for (long s = 0; s < (long)p.n; ++s) {
+            if (!p.active(s)) continue;
+            const long x = s % nx, y = (s / nx) % ny, z = s / (nx * ny);
+            const long nb[6] = {(z * ny + y) * nx + p.nbr(0, x, -1), (z * ny + y) * nx + p.nbr(0, x, 1),
+                                (z * ny + p.nbr(1, y, -1)) * nx + x, (z * ny + p.nbr(1, y, 1)) * nx + x,
+                                (p.nbr(2, z, -1) * ny + y) * nx + x, (p.nbr(2, z, 1) * ny + y) * nx + x};
+            const double w[6] = {1.0 / dx2, 1.0 / dx2, 1.0 / dy2, 1.0 / dy2, 1.0 / dz2, 1.0 / dz2};
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < 6; ++q) {
+                if (!p.active(nb[q])) continue;
+                const double a = p.stiff(s, nb[q]) * w[q];
+                for (int k = 0; k < 3; ++k) acc[k] += a * (m[nb[q]].c[k] - m[s].c[k]);
+            }
+            const double pre = 2.0 / (MU0 * p.ms_at(s));
+            for (int k = 0; k < 3; ++k) h[s].c[k] += pre * acc[k];
+        }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < (long)p.n; ++s) {
         if (!p.active(s)) continue;
@@ -1265,6 +1302,17 @@ void *orc_create(const orc_desc *d) {
     for (auto &k : p->k) k.resize(n);
     if (d->layer_z) p->lz.assign(d->layer_z, d->layer_z + d->nz);
     p->d.layer_z = nullptr;
+    if (d->region && d->exchange_lut) {
+        p->reg.assign(d->region, d->region + n);
+        for (uint32_t r : p->reg)
+            if (r >= 256) {
+                delete p;
+                return nullptr;
+            }
+        p->lut.assign(d->exchange_lut, d->exchange_lut + 256 * 256);
+    }
+    p->d.region = nullptr;
+    p->d.exchange_lut = nullptr;
     if (d->demag && p->layered()) {
         p->px = 2 * d->nx;
         p->py = 2 * d->ny;

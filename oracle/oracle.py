@@ -72,6 +72,7 @@ class _Desc(ctypes.Structure):
         ("dc_tensor", ctypes.POINTER(ctypes.c_double)),
         ("periodic", ctypes.c_uint32), ("images", ctypes.c_uint32 * 3),
         ("layer_z", ctypes.POINTER(ctypes.c_double)),
+        ("region", ctypes.POINTER(ctypes.c_uint32)), ("exchange_lut", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -259,6 +260,10 @@ class Problem:
     periodic_images: tuple = (0, 0, 0)
     # multilayer: z centre of each plane (single-cell layers coupled by demag)
     layer_z: np.ndarray | None = None
+    # exchange region table (exchange_fp64.cu:80-114): region id per cell and
+    # the 256 x 256 stiffness table (both or neither)
+    regions: np.ndarray | None = None
+    exchange_lut: np.ndarray | None = None
 
 
 class Oracle:
@@ -341,6 +346,11 @@ class Oracle:
             return _ptr(a)
         if pb.layer_z is not None:
             d.layer_z = f64(pb.layer_z)
+        if pb.regions is not None:
+            rg = np.ascontiguousarray(np.asarray(pb.regions, dtype=np.uint32).ravel())
+            self._keep.append(rg)
+            d.region = rg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            d.exchange_lut = f64(pb.exchange_lut)
         if pb.volume_fraction is not None:
             d.bc_phi = f64(pb.volume_fraction)
         for nm, arrays in (("bc_face", pb.face_links), ("bc_delta", pb.deltas)):

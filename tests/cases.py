@@ -98,7 +98,8 @@ class Case:
             kw["extra"] = {**d, **kw.get("extra", {})}
         kw.update(cells)
         for q in ("temperature", "thermal_dt", "thermal_seed", "boundary_tier", "phi_floor", "delta_min",
-                  "volume_fraction", "face_links", "deltas", "demag_corr", "periodic", "periodic_images", "layer_z"):
+                  "volume_fraction", "face_links", "deltas", "demag_corr", "periodic", "periodic_images", "layer_z",
+                  "regions", "exchange_lut"):
             if q in (self.terms or {}):
                 kw.setdefault(q, self.terms[q])
         return FdmPlan(nx=self.nx, ny=self.ny, nz=self.nz, dx=self.dx, dy=self.dy, dz=self.dz, ms=self.ms,
