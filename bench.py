@@ -36,15 +36,38 @@ METRIC = "Mcell-steps/s (RK45, exch+demag) at 512²×1, fp32 vs fp64 at stated e
 METRIC_3D = "Mcell-steps/s (RK45, exch+demag+anisotropy+ramp) at 256²×64, fp32 vs fp64 at stated error"
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# (FP32, the stage-2 launches that the kernel timings use; FETCH_SIZE scaled
-# by 2.0, WRITE_SIZE by 1.0, the factors a plain 256 MiB device copy gives);
-# the passes and their commands are recorded in
-# profiles/film512_dp45_hbm_traffic.md and profiles/granular3d_dp45_profile.md
-MEASURED_TRAFFIC_FP32 = {
-    "film512": {0: 16.11e6, 1: 17.40e6, 4: 13.62e6, 6: 32.62e6},
-    "granular3d": {0: 252.5e6, 1: 303.3e6, 2: 658.9e6, 3: 303.3e6, 4: 345.3e6, 6: 683.4e6},
-}
+# Measured HBM traffic per launch: the rocprofv3 FETCH_SIZE / WRITE_SIZE
+# summaries under profiles/ (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the
+# factors of a plain 256 MiB device copy), read from the "total MB" column of
+# the FP32 stage-2 instantiation of each kernel kind.  They are the values
+# recorded when those profiles were taken; bench.py cannot read hardware
+# counters itself.
+TRAFFIC_PROFILES = {"film512": "film512_dp45_hbm_traffic.md", "granular3d": "granular3d_dp45_profile.md"}
+TRAFFIC_ROWS = {0: ("k_fwd_x<float", ", 1>`"), 1: ("k_yf<float", ">`"), 2: ("k_zf<float", ">`"),
+                4: ("k_inv_x<float", ">`"), 6: ("k_xx<float", ", 2>`")}
+
+
+def recorded_traffic(workload, kind):
+    """(bytes per launch, source) from the committed profile summary, or
+    (None, reason)."""
+    name = TRAFFIC_PROFILES.get(workload)
+    row = TRAFFIC_ROWS.get(kind)
+    if not name or not row:
+        return None, "no recorded profile"
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None, f"profiles/{name} missing"
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if cells and cells[0].startswith("`" + row[0]) and cells[0].endswith(row[1]) and "KiB" not in cells[0]:
+            try:
+                return float(cells[-1]) * 1e6, f"profiles/{name} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, recorded)"
+            except ValueError:
+                continue
+    return None, f"no row for {row[0]} in profiles/{name}"
+
 
 NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xx"}
 NAMES_3D = {0: "k_fwd_x", 1: "k_y (forward)", 2: "k_z", 3: "k_y (inverse)", 4: "k_inv_x", 6: "k_xx"}
@@ -151,27 +174,100 @@ def run_gpu(wl, precision, steps, warmup):
     }
 
 
-def accuracy(wl, horizons):
-    """FP32 against FP64 at the same simulated times, each precision with its
-    own adaptive dt sequence (the final step of each leg is clamped to the
-    horizon): max_i |m_fp32 - m_fp64| and the relative total-energy error."""
+def accuracy(wl, horizons, oracle_horizon):
+    """Error of the FP32 engine against FP64 at stated times.
+
+    matched_dt: the FP64 engine runs with its own controller, and its
+    accepted dt sequence is replayed (atol 1: every step accepted, so every
+    dt is taken as given) by the FP32 engine, by the FP64 engine from m0
+    perturbed by 1e-12 (the control: how fast a roundoff-sized difference
+    grows on this trajectory) and, up to `oracle_horizon`, by the FP64 CPU
+    oracle (the parity statement of the FP64 form); max_i |dm| and the
+    relative total-energy error at each horizon.
+    independent_dt: each precision with its own adaptive dt sequence."""
+    from fullmag_amd.backend import FdmBackend, FdmPlan
+    kw = dict(wl.kw)
+    be64 = FdmBackend(FdmPlan(precision="double", **kw))
+    dts, ref, dt = [], [], 1e-13
+    for h in horizons:
+        while True:
+            r = be64.advance(h, 1, dt)
+            if r["steps_taken"] == 0:
+                break
+            dts.append(r["dt_seconds"])
+            dt = r["suggested_next_dt"] or dt
+            if r["time_seconds"] >= h:
+                break
+        ref.append((len(dts), be64.field("m"), r["total_energy_joules"], r["time_seconds"]))
+    be64.close()
+
+    def replay(precision, m0, upto):
+        b = FdmBackend(FdmPlan(**{**kw, "precision": precision, "atol": 1.0, "m0": m0}))
+        out, done = [], 0
+        for nsteps, _, _, _ in ref[:upto]:
+            st = None
+            for d in dts[done:nsteps]:
+                st = b.step(d)
+            done = nsteps
+            out.append((b.field("m"), st["total_energy_joules"]))
+        b.close()
+        return out
+
+    rng = np.random.default_rng(5)
+    mp = kw["m0"] + 1e-12 * rng.standard_normal(kw["m0"].shape)
+    r32 = replay("single", kw["m0"], len(ref))
+    rpt = replay("double", mp, len(ref))
+    rows = []
+    for i, (nsteps, m64, e64, t) in enumerate(ref):
+        rows.append({"t_s": horizons[i], "steps": nsteps,
+                     "fp32_vs_fp64_max_dm": float(np.abs(r32[i][0] - m64).max()),
+                     "fp32_vs_fp64_energy_rel": float(abs(r32[i][1] - e64) / abs(e64)),
+                     "fp64_perturbed_1e-12_vs_fp64_max_dm": float(np.abs(rpt[i][0] - m64).max())})
+    # the FP64 engine against the CPU oracle on the same dt sequence
+    k = max(i for i, h in enumerate(horizons) if h <= oracle_horizon) if oracle_horizon >= horizons[0] else -1
+    if k >= 0:
+        from oracle.oracle import Oracle, Problem
+        orc = Oracle(Problem(kw["nx"], kw["ny"], kw["nz"], kw["dx"], kw["dy"], kw["dz"], kw["ms"], kw["a_ex"],
+                             kw["alpha"], h_ext=kw.get("h_ext"), mask=kw.get("mask"), atol=1.0))
+        m = orc.prepare(kw["m0"])
+        t, done = 0.0, 0
+        t0 = time.perf_counter()
+        for i in range(k + 1):
+            for d in dts[done:ref[i][0]]:
+                t, rep = orc.step("rk45", m, t, d)
+            done = ref[i][0]
+            rows[i]["fp64_vs_oracle_max_dm"] = float(np.abs(m - ref[i][1]).max())
+            rows[i]["fp64_vs_oracle_energy_rel"] = float(abs(rep["e_total"] - ref[i][2]) / abs(rep["e_total"]))
+        rows[k]["oracle_cpu_s"] = round(time.perf_counter() - t0, 1)
+    indep = []
     runs = {}
     for p in ("single", "double"):
         be = wl.backend(p)
-        dt, rows = 1e-13, []
+        dt, res = 1e-13, []
         for t_end in horizons:
             r = be.advance(t_end, 10 ** 7, dt)
             dt = r["suggested_next_dt"] or dt
-            rows.append((be.field("m"), r["total_energy_joules"], r["time_seconds"]))
-        runs[p] = rows
+            res.append((be.field("m"), r["total_energy_joules"]))
+        runs[p] = res
         be.close()
-    out = []
     for i, t_end in enumerate(horizons):
-        m32, e32, _ = runs["single"][i]
-        m64, e64, _ = runs["double"][i]
-        out.append({"t_s": t_end, "max_dm_fp32_vs_fp64": float(np.abs(m32 - m64).max()),
-                    "energy_rel_err_fp32_vs_fp64": float(abs(e32 - e64) / abs(e64))})
-    return out
+        (m32, e32), (m64, e64) = runs["single"][i], runs["double"][i]
+        indep.append({"t_s": t_end, "max_dm_fp32_vs_fp64": float(np.abs(m32 - m64).max()),
+                      "energy_rel_err_fp32_vs_fp64": float(abs(e32 - e64) / abs(e64))})
+    return {"dt_source": "the FP64 engine's own DP45 controller (atol 1e-5); replays take each dt as given",
+            "matched_dt": rows, "independent_dt": indep}
+
+
+def physical_cores():
+    """Physical cores of the host (distinct core, socket pairs of lscpu -p),
+    or None when lscpu is unavailable."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    pairs = {tuple(ln.split(",")) for ln in out.splitlines() if ln and not ln.startswith("#")}
+    return len(pairs) or None
 
 
 def cpu_baseline(wl, seconds):
@@ -199,6 +295,7 @@ def cpu_baseline(wl, seconds):
                 return steps, el
 
     threads = num_threads()
+    phys = physical_cores()
     steps, el = timed(seconds, 200)
     set_num_threads(1)
     steps1, el1 = timed(seconds / 3, 50)
@@ -206,6 +303,9 @@ def cpu_baseline(wl, seconds):
     return {"value": wl.n * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} accepted RK45 steps (after 1 untimed) of the same {wl.nx}x{wl.ny}x{wl.nz} "
                       f"problem, FP64, {el:.1f} s wall on {threads} threads",
+            "host_physical_cores": phys,
+            "threads_note": "OpenMP threads = the process's CPU share on the GPU host (OMP_NUM_THREADS); "
+                            "host_physical_cores counts the whole host",
             "single_thread_value": wl.n * steps1 / el1 / 1e6,
             "single_thread_sample": f"{steps1} further steps on 1 thread, {el1:.1f} s wall"}
 
@@ -338,7 +438,7 @@ def roofline(wl, r32):
                             "algorithmic_bytes": kb[k], "GBs": round(kb[k] / (kt[k] * 1e-6) / 1e9, 1)}
     return {"bound": "hbm", "kernel": wl.names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": MEASURED_TRAFFIC_FP32.get(wl.name, {}).get(dom),
+            "traffic": recorded_traffic(wl.name, dom)[0], "traffic_source": recorded_traffic(wl.name, dom)[1],
             "bytes_per_launch": kb[dom], "avg_launch_us": round(us, 3), "per_kernel": per,
             "controller_us": round(kt[5], 3)}
 
@@ -357,6 +457,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--oracle-horizon", type=float, default=1e-12,
+                    help="film: simulated time over which the CPU oracle replays the FP64 engine's dt sequence")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 plain device copies of 256 MiB (scale for FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -423,7 +525,8 @@ def main():
         "roofline": roofline(wl, r32),
     }
     if not a.no_accuracy:
-        out["accuracy"] = accuracy(wl, [1e-12, 5e-12, 20e-12] if wl.film else [1e-12, 5e-12])
+        out["accuracy"] = accuracy(wl, [1e-12, 5e-12, 20e-12] if wl.film else [1e-12, 5e-12],
+                                   a.oracle_horizon if (wl.film and not a.no_cpu) else 0.0)
     if not a.no_cpu and wl.film:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -226,7 +226,8 @@ std::vector<TuneEntry> g_tune;
 // a plan created alone uses the tile its batch group was tuned to.
 template <class T>
 int tune_feat(const fm::Args<T> &A) {
-    return (A.xt ? 1 : 0) | (A.has_th ? 2 : 0) | (A.region ? 4 : 0) | (A.demag ? 8 : 0);
+    return (A.xt ? 1 : 0) | (A.has_th ? 2 : 0) | (A.region ? 4 : 0) | (A.demag ? 8 : 0) | (A.lmix ? 16 : 0) |
+           (A.layered ? 32 : 0) | (A.pbc << 6) | (A.gtens ? 512 : 0);
 }
 
 void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int feat, int *rows, int *xfast) {
