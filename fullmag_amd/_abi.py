@@ -297,6 +297,8 @@ EXT_SYMBOLS = {
     "fullmag_fdm_batch_system": (_H, [_B, ctypes.c_uint64]),
     "fullmag_fdm_batch_last_error": (ctypes.c_char_p, [_B]),
     "fullmag_fdm_batch_destroy": (None, [_B]),
+    "fullmag_fdm_backend_copy_field_preview_f64_noalloc": (
+        ctypes.c_int, [_H, ctypes.c_int] + [ctypes.c_uint32] * 5 + [c_double_p, ctypes.c_uint64]),
 }
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

@@ -63,6 +63,7 @@ hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
 template <class T>
 hipError_t enqueue_preview(const T *field, long long ncell, const PreviewGeom &g, void *out, int out_fp64,
                            hipStream_t st);
+hipError_t preview_scratch(int device, void **ptr);
 bool yfast_factors(int py, int *r1, int *r2);
 bool zfast_factors(int pz, int *r1, int *r2);
 std::vector<double> newell_reduced_spectrum(int nx, int ny, int nz, double dx, double dy, double dz);
@@ -1511,6 +1512,39 @@ int fullmag_fdm_backend_copy_field_preview_f32(fullmag_fdm_backend *h, fullmag_f
                                                uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, float *out,
                                                uint64_t len) {
     return preview_copy(h, obs, pnx, pny, pnz, z0, zs, out, len, false);
+}
+
+// copy_field_preview_f64 without any device allocation: the bins are staged
+// in the library's fixed per-GPU array (kPreviewScratchBytes) instead of the
+// handle's grown-on-demand buffer.  Same kernel, same result.
+int fullmag_fdm_backend_copy_field_preview_f64_noalloc(fullmag_fdm_backend *h, fullmag_fdm_observable obs,
+                                                       uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0,
+                                                       uint32_t zs, double *out, uint64_t len) {
+    const int rc = preview_check(h, obs, pnx, pny, pnz, zs);
+    if (rc != FULLMAG_FDM_OK) return rc;
+    if (!out) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview output is NULL");
+    const uint64_t cnt = (uint64_t)pnx * pny * pnz;
+    if (len != cnt * 3) return fail(h, FULLMAG_FDM_ERR_INVALID, "preview out_len mismatch");
+    const uint64_t bytes = len * sizeof(double);
+    if (bytes > fm::kPreviewScratchBytes)
+        return fail(h, FULLMAG_FDM_ERR_INVALID,
+                    "preview of " + std::to_string(cnt) + " cells exceeds the allocation-free staging array (" +
+                        std::to_string(fm::kPreviewScratchBytes / (3 * sizeof(double))) + " cells)");
+    hipSetDevice(h->device);
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        hext_bins(h, pnx, pny, pnz, z0, zs, out);
+        return FULLMAG_FDM_OK;
+    }
+    void *scr = nullptr;
+    if (!hip_ok(h, fm::preview_scratch(h->device, &scr), "preview staging array")) return FULLMAG_FDM_ERR_CUDA;
+    const void *src = nullptr;
+    if (!field_device_src(h, obs, &src)) return FULLMAG_FDM_ERR_CUDA;
+    const fm::PreviewGeom g{h->nx, h->ny, h->nz, pnx, pny, pnz, z0, zs};
+    if (!preview_launch(h, src, g, scr, 1, h->stream) ||
+        !hip_ok(h, hipMemcpyAsync(out, scr, bytes, hipMemcpyDeviceToHost, h->stream), "preview download") ||
+        !hip_ok(h, hipStreamSynchronize(h->stream), "stream sync"))
+        return FULLMAG_FDM_ERR_CUDA;
+    return FULLMAG_FDM_OK;
 }
 
 }  // extern "C"

@@ -273,6 +273,10 @@ struct AdjArgs {
     double dt;                        // step size
 };
 
+// Size of the fixed device staging array of the allocation-free preview
+// entry point (one per GPU): FP64 AoS bins of up to 87381 preview cells.
+constexpr unsigned long long kPreviewScratchBytes = 2ull << 20;
+
 // Field grid and preview request of the preview binning kernel.
 struct PreviewGeom {
     uint32_t fx, fy, fz;     // field grid

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include "engine.hpp"
 
 #ifndef FM_REAL
@@ -3333,5 +3334,35 @@ hipError_t enqueue_preview(const T *field, long long ncell, const PreviewGeom &g
 }
 
 template hipError_t enqueue_preview<FM_REAL>(const FM_REAL *, long long, const PreviewGeom &, void *, int, hipStream_t);
+
+#ifdef FM_PRIMARY
+// Staging array of the allocation-free preview entry point.  Each GPU holds
+// its own copy until the process ends.  In host code the name `scratch` is
+// only a stand-in: the device address of the current GPU's copy comes from
+// hipGetSymbolAddress, looked up once per device and kept.
+static __device__ __align__(256) unsigned char scratch[kPreviewScratchBytes];
+
+hipError_t preview_scratch(int device, void **ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        // the device must be current: the symbol resolves to its copy
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device) return hipErrorInvalidDevice;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+#endif
 
 }  // namespace fm

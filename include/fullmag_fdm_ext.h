@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 7
+#define FULLMAG_FDM_EXT_VERSION 8
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -201,6 +201,25 @@ int fullmag_fdm_batch_advance(fullmag_fdm_batch *batch, double t_end, uint64_t m
 fullmag_fdm_backend *fullmag_fdm_batch_system(fullmag_fdm_batch *batch, uint64_t index);
 const char *fullmag_fdm_batch_last_error(const fullmag_fdm_batch *batch);
 void fullmag_fdm_batch_destroy(fullmag_fdm_batch *batch);
+
+/*
+ * No counterpart in the reference.  fullmag_fdm_backend_copy_field_preview_f64
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving or preview loops, harnesses that forbid hipMalloc):
+ * the bins are staged in a fixed device array of 2 MiB per GPU
+ * (FULLMAG_FDM_PREVIEW_NOALLOC_MAX_CELLS preview cells), which the library
+ * holds until the process ends; the call allocates nothing, on the device or
+ * otherwise.  Larger previews return FULLMAG_FDM_ERR_INVALID with
+ * last_error set.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (several threads, or handles on the same device) race
+ * and must be serialised by the caller.  Arguments, result and errors are
+ * otherwise those of copy_field_preview_f64.
+ */
+#define FULLMAG_FDM_PREVIEW_NOALLOC_MAX_CELLS 87381u
+int fullmag_fdm_backend_copy_field_preview_f64_noalloc(fullmag_fdm_backend *handle, fullmag_fdm_observable observable,
+                                                       uint32_t preview_nx, uint32_t preview_ny, uint32_t preview_nz,
+                                                       uint32_t z_origin, uint32_t z_stride, double *out_xyz,
+                                                       uint64_t out_len);
 
 #ifdef __cplusplus
 }
