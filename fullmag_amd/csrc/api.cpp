@@ -2433,7 +2433,7 @@ std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
     // must agree as well
     return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
             A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt, A.has_th, A.lmix != nullptr, A.layered, A.pbc,
-            A.gtens != nullptr};
+            A.gtens != nullptr, A.xpf};
 }
 
 template <class T>

@@ -142,6 +142,8 @@ struct Args {
     int yfast;              // 0 = Stockham y pass, else py of the register four-step y pass
     const void *tw4x;       // four-step twiddles of the x passes (W_px^{t1 k2} at [k2*R1 + t1])
     int xfast;              // 0 = Stockham x passes, else px of the register four-step x passes
+    int xpf;                // 1 = film joined x pass with its inputs prefetched into LDS (k_xxp)
+    int xpf_off;            // byte offset of the prefetch area in the x passes' dynamic LDS
     int ylfast;             // 3-D: 0 = Stockham y passes, else py of the register four-step y-line passes
     const void *tw4z;       // four-step twiddles of the 3-D z pass (W_pz^{t1 k2} at [k2*R1 + t1])
     int zfast;              // 3-D: 0 = Stockham z pass, else pz of the register four-step z pass

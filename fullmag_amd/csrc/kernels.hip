@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include "engine.hpp"
 
@@ -99,9 +100,14 @@ static __constant__ Method c_methods[M_COUNT];
 // Tableau of the step in flight: ABM3 runs its Heun start-up table until
 // the history holds three values (Ctl::abm_count changes only in the
 // controller, after every kernel of the step has read it).
+// The index is the same for every lane; readfirstlane tells the compiler
+// so, and the table fields then come through scalar loads (otherwise the
+// control-block read makes the index a vector value, and every field read
+// of the table became a dependent vector load with a full memory wait).
 template <class T>
 __device__ __forceinline__ const Method &method_for(const Args<T> &A) {
-    return c_methods[A.method == M_ABM3 && A.ctl->abm_count < 3 ? M_ABM3S : A.method];
+    const int idx = A.method == M_ABM3 && A.ctl->abm_count < 3 ? M_ABM3S : A.method;
+    return c_methods[__builtin_amdgcn_readfirstlane(idx)];
 }
 
 extern __shared__ __align__(16) unsigned char fm_smem[];
@@ -992,8 +998,44 @@ struct FourStep {
     static constexpr int TB = R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2;
 };
 
-template <class T, int R1, int R2>
-__device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in, int sb) {
+// Exchange with the neighbouring lane of a pair (lanes 2j, 2j+1) through a
+// DPP quad permutation [1,0,3,2] (a VALU move, no LDS round trip).
+__device__ __forceinline__ float pair_swap(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pair_swap(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// R-point DFT shared by a lane pair: lane e of the pair holds x[e + 2n]
+// (n < R/2) and returns X[k + e R/2] (k < R/2): an R/2-point DFT per lane,
+// the twiddle W_R^{e k} and one radix-2 butterfly across the pair.  Both
+// lanes of a pair must be active.
+template <class T, int R, bool INV>
+__device__ __forceinline__ void dft_pair(C2<T> *v, int e) {
+    constexpr int Q = R / 2;
+    DftReg<T, Q, INV>::run(v);
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+        const T c = T(TwHolder<R>::tw.c[k]);
+        const T sn = INV ? T(TwHolder<R>::tw.s[k]) : T(-TwHolder<R>::tw.s[k]);
+        const C2<T> w = cmul(v[k], C2<T>{c, sn});
+        v[k] = e ? w : v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const C2<T> o = {pair_swap(v[k].x), pair_swap(v[k].y)};
+        v[k] = e ? csub(o, v[k]) : cadd(v[k], o);
+    }
+}
+
+// One lane per line DFT (FP64: the pair exchange doubles the register
+// traffic of the FP64 values and measured slower).
+template <class T, int R1, int R2, bool TL>
+__device__ void fs_forward1(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in, int sb) {
     using F = FourStep<R1, R2>;
     const int spb = blockDim.x / F::TS;  // sequences per round
     for (int base = 0; base < nseq; base += spb) {
@@ -1010,7 +1052,7 @@ __device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
             }
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + (k2 * R1 + t)));
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], TL ? tw4[k2 * R1 + t] : ldg(tw4 + (k2 * R1 + t)));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) tt[k2 * F::P1 + t] = v[k2];
         }
@@ -1027,9 +1069,8 @@ __device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
     }
 }
 
-// Inverse of fs_forward (unnormalised); only outputs n < n_out are written.
-template <class T, int R1, int R2>
-__device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out, int sb) {
+template <class T, int R1, int R2, bool TL>
+__device__ void fs_inverse1(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out, int sb) {
     using F = FourStep<R1, R2>;
     const int spb = blockDim.x / F::TS;
     for (int base = 0; base < nseq; base += spb) {
@@ -1044,7 +1085,7 @@ __device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
             DftReg<T, R1, true>::run(u);
 #pragma unroll
             for (int t1 = 1; t1 < R1; ++t1) {
-                C2<T> w = ldg(tw4 + (t * R1 + t1));
+                C2<T> w = TL ? tw4[t * R1 + t1] : ldg(tw4 + (t * R1 + t1));
                 w.y = -w.y;
                 u[t1] = cmul(u[t1], w);
             }
@@ -1061,6 +1102,98 @@ __device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
             for (int n2 = 0; n2 < R2; ++n2) {
                 const int n = t + R1 * n2;
                 if (n < n_out) b[n] = v[n2];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Register four-step FFT, two lanes per line DFT in FP32 (64 lanes per
+// sequence at R1 = R2 = 32, so three sequences keep three of a
+// work-group's four SIMDs busy; the x passes of the 512 x 512 film 20-30 %
+// faster); FP64 keeps one lane per line DFT.  TL: tw4 is in LDS (else
+// global memory).
+template <class T, int R1, int R2, bool TL = false>
+__device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_in, int sb) {
+    if constexpr (sizeof(T) == 8) {
+        fs_forward1<T, R1, R2, TL>(buf, tb, tw4, nseq, n_in, sb);
+        return;
+    }
+    using F = FourStep<R1, R2>;
+    constexpr int L = 2 * F::TS;               // lanes per sequence
+    const int spb = blockDim.x / L;            // sequences per round
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + threadIdx.x / L, r = threadIdx.x % L, t = r >> 1, e = r & 1;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * L;
+        C2<T> *b = buf + sq * sb;
+        C2<T> *tt = tb + (sq - base) * F::TB;
+        if (on && t < R1) {
+            // R2-point DFT over n2 of x[t + R1 n2]: this lane takes n2 = e + 2n
+            C2<T> v[R2 / 2];
+#pragma unroll
+            for (int n = 0; n < R2 / 2; ++n) {
+                const int i = t + R1 * (e + 2 * n);
+                v[n] = i < n_in ? b[i] : C2<T>{T(0), T(0)};
+            }
+            dft_pair<T, R2, false>(v, e);
+#pragma unroll
+            for (int k = 0; k < R2 / 2; ++k) {
+                const int k2 = k + e * (R2 / 2);
+                const C2<T> w = TL ? tw4[k2 * R1 + t] : ldg(tw4 + (k2 * R1 + t));
+                tt[k2 * F::P1 + t] = k2 ? cmul(v[k], w) : v[k];
+            }
+        }
+        __syncthreads();
+        if (on && t < R2) {
+            C2<T> u[R1 / 2];
+#pragma unroll
+            for (int n = 0; n < R1 / 2; ++n) u[n] = tt[t * F::P1 + e + 2 * n];
+            dft_pair<T, R1, false>(u, e);
+#pragma unroll
+            for (int k = 0; k < R1 / 2; ++k) b[t + R2 * (k + e * (R1 / 2))] = u[k];
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse of fs_forward (unnormalised); only outputs n < n_out are written.
+template <class T, int R1, int R2, bool TL = false>
+__device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4, int nseq, int n_out, int sb) {
+    if constexpr (sizeof(T) == 8) {
+        fs_inverse1<T, R1, R2, TL>(buf, tb, tw4, nseq, n_out, sb);
+        return;
+    }
+    using F = FourStep<R1, R2>;
+    constexpr int L = 2 * F::TS;
+    const int spb = blockDim.x / L;
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + threadIdx.x / L, r = threadIdx.x % L, t = r >> 1, e = r & 1;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * L;
+        C2<T> *b = buf + sq * sb;
+        C2<T> *tt = tb + (sq - base) * F::TB;
+        if (on && t < R2) {
+            C2<T> u[R1 / 2];
+#pragma unroll
+            for (int n = 0; n < R1 / 2; ++n) u[n] = b[t + R2 * (e + 2 * n)];
+            dft_pair<T, R1, true>(u, e);
+#pragma unroll
+            for (int k = 0; k < R1 / 2; ++k) {
+                const int t1 = k + e * (R1 / 2);
+                C2<T> w = TL ? tw4[t * R1 + t1] : ldg(tw4 + (t * R1 + t1));
+                w.y = -w.y;
+                tt[t1 * F::P2 + t] = t1 ? cmul(u[k], w) : u[k];
+            }
+        }
+        __syncthreads();
+        if (on && t < R1) {
+            C2<T> v[R2 / 2];
+#pragma unroll
+            for (int n = 0; n < R2 / 2; ++n) v[n] = tt[t * F::P2 + e + 2 * n];
+            dft_pair<T, R2, true>(v, e);
+#pragma unroll
+            for (int k = 0; k < R2 / 2; ++k) {
+                const int i = t + R1 * (k + e * (R2 / 2));
+                if (i < n_out) b[i] = v[k];
             }
         }
         __syncthreads();
@@ -1268,7 +1401,7 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
 // Zero-padded forward FFT of the packed sequences in b0 and the unpacking of
 // each sequence into the half spectra of its two real lines,
 // X = (Z[k] + conj Z[-k])/2 and (Z[k] - conj Z[-k])/(2i), y-lines contiguous.
-template <class T, int R1, int R2>
+template <class T, int R1, int R2, bool TL = false>
 __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
                                                 const C2<T> *tw) {
     constexpr bool FAST = R1 > 0;
@@ -1277,7 +1410,8 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
     const C2<T> *res;
     const int sb = FAST ? A.xsb : px;
     if constexpr (FAST) {
-        fs_forward<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
+        // TL: `tw` holds the four-step twiddles in LDS
+        fs_forward<T, R1, R2, TL>(b0, b1, TL ? tw : reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
         res = b0;
     } else {
         res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
@@ -1508,6 +1642,240 @@ __global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(co
     __syncthreads();
     x_pack_stage<T, R1, R2, NK>(A, M, s + 1, p, dt, xr, b0);
     x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
+}
+
+// ---------------------------------------------- film joined x pass, LDS prefetch --
+// On the 512 x 512 film every launch is one round of work-groups (one per
+// CU, one wave per SIMD), so k_xx lasts one work-group's dependent chain:
+// spectrum loads -> inverse FFT -> stencil loads -> local pass -> stage
+// combination loads -> forward FFT -> stores.  k_xxp issues every global
+// read of the launch at its start: the spectrum into registers, and the
+// stencil rows, the activity bytes and the stage combination's m0 and k
+// vectors straight into LDS with global->LDS loads (no VGPRs held), so the
+// three round trips overlap.  The values, and hence the arithmetic and the
+// results, are those of k_xx.  Film (nz = 1), two rows per work-group,
+// nx a multiple of 256, uniform exchange without further terms, one-step
+// methods.
+//
+// LDS after the FFT buffers (floats unless noted):
+//   Y  [4][3][nx]  stencil rows y0-1 .. y0+2 (clamped to the grid) of y_s
+//   M  [3][2][nx]  m0 of the two rows
+//   K  [NK][3][2][nx] the k-vectors of stage s+1's combination (the slot of
+//                  k_s is filled by the local pass)
+//   TW [2nx] complex  the four-step twiddles of the x FFTs
+//   NM [2][nx] bytes  activity bytes of the two rows (with a mask)
+constexpr int kXpfRows = 2;
+
+template <class T>
+__host__ __device__ constexpr size_t xpf_floats(int nx, int nk) {
+    return (size_t)(4 * 3 + 3 * kXpfRows + nk * 3 * kXpfRows + 4) * nx;
+}
+
+// Moves n floats (n a multiple of 256; src 16-byte aligned) from global src
+// to LDS dst with direct global->LDS loads of the calling wave: lane l of
+// load j moves floats 256 j + 4 l .. + 3.  dst must be wave-uniform.
+__device__ __forceinline__ void dma_floats(const float *src, float *dst, int n) {
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < n; j += 256)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + j + 4 * lane),
+                                         (__attribute__((address_space(3))) void *)(dst + j), 16, 0, 0);
+}
+// The same for n bytes (n a multiple of 1024).
+__device__ __forceinline__ void dma_bytes(const uint8_t *src, uint8_t *dst, int n) {
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < n; j += 1024)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src + j + 16 * lane),
+                                         (__attribute__((address_space(3))) void *)(dst + j), 16, 0, 0);
+}
+
+// CellIn of a film cell from the prefetched rows (the loads gather_cell<T,0>
+// makes, with the film's neighbour rule: x and y clamped to the grid, z
+// neighbours the cell itself).
+template <class T>
+__device__ __forceinline__ void gather_film_lds(const Args<T> &A, int cell, int x, int rr, const T *Y,
+                                                const uint8_t *NM, CellIn<T> &g) {
+    const int nx = A.nx, N = (int)A.ncell;
+    const unsigned bits = A.nmask ? (unsigned)NM[rr * nx + x] : 0x7Fu;
+    g.act = bits & 1u;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g.nact[q] = (bits >> (q + 1)) & 1u;
+    const int xm = x > 0 ? x - 1 : x, xp = x < nx - 1 ? x + 1 : x;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T *r0 = Y + (rr * 3 + c) * nx, *r1 = r0 + 3 * nx, *r2 = r1 + 3 * nx;
+        g.m[c] = r1[x];
+        g.v[0][c] = r1[xm];
+        g.v[1][c] = r1[xp];
+        g.v[2][c] = r0[x];
+        g.v[3][c] = r2[x];
+        g.v[4][c] = r1[x];
+        g.v[5][c] = r1[x];
+    }
+    g.ms = A.ms_cell ? G(A.ms_cell)[cell] : A.ms;
+    if (A.has_uni) {
+        g.ku1 = A.ku1_cell ? G(A.ku1_cell)[cell] : A.ku1_u;
+        g.ku2 = A.ku2_cell ? G(A.ku2_cell)[cell] : A.ku2_u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g.ax[c] = A.axis_cell ? G(A.axis_cell)[c * N + cell] : A.axis[c];
+    }
+    g.wm = A.wave_mask ? G(A.wave_mask)[cell] : 1;
+}
+
+template <class T, int R1, int R2, int NK>
+__global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP, int s) {
+    static_assert(sizeof(T) == 4, "FP32 only");
+    constexpr int CH = Chunk<T>::value;
+    constexpr int PX = R1 * R2, NX = PX / 2, NKX = NX + 1;
+    constexpr int NLP = 2 * ((3 * kXpfRows + 1) / 2);           // packed lines (lane pairs)
+    constexpr int TOT = NLP * NKX;                              // spectrum values of the work-group
+    constexpr int NV = (TOT + kThreads - 1) / kThreads;         // per thread
+    const Args<T> &A = AP[blockIdx.y];
+    Ctl *ctl = A.ctl;
+    const int done = ctl->done, p = ctl->parity;
+    const double dt = ctl->dt;
+    if (done) return;
+    const Method &M = method_for(A);
+    const int N = (int)A.ncell, sb = A.xsb, ly = A.ly;
+    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
+    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
+    T *Y = reinterpret_cast<T *>(fm_smem + A.xpf_off);
+    T *Mz = Y + 12 * NX;
+    T *K = Mz + 3 * kXpfRows * NX;
+    T *TW = K + NK * 3 * kXpfRows * NX;
+    uint8_t *NM = reinterpret_cast<uint8_t *>(TW + 2 * PX);
+    __shared__ XRows xr;
+    const int r0 = (int)xcd_block(blockIdx.x, gridDim.x) * kXpfRows;
+    x_rows(A, r0, xr);
+    // the work-group's rows are y0, y0 + 1 of the film (nrows even)
+    const int y0 = r0, c0 = r0 * NX;
+    // stage s+1's row: which of its k-vectors is k_s (made here), and every
+    // source address of the prefetch, resolved before any load is issued
+    // (their method-table reads would otherwise wait behind the spectrum)
+    const StageRow &R = M.row[s + 1];
+    int qs = -1;
+    const T *ksrc[NK];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+        if (R.kidx[q] == s) qs = q;
+        ksrc[q] = A.k_all + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N + c0;
+    }
+    const T *ysrc = (const T *)stage_input(A, M, s, p);
+    const T *msrc = A.m_all + (size_t)p * 3 * N + c0;
+    __builtin_amdgcn_s_waitcnt(0);
+    // 1. spectrum values into registers (as x_inverse_load: line fastest)
+    const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
+    constexpr int NL = 3 * kXpfRows;
+    const float rnl = 1.0f / (float)NLP;
+    C2<T> v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const int i = j * kThreads + (int)threadIdx.x;
+        const int ic = i < TOT ? i : TOT - 1;
+        const int kk = small_div<false>(ic, NLP, rnl), l = ic - kk * NLP;
+        const int rr = l / 3, c = l - 3 * rr;  // line l = 3 rr + c, as XRows::spec
+        const int off = l < NL ? (c * A.nkx) * ly + y0 + rr : -1;
+        v[j] = off >= 0 ? ldg(S + off + kk * ly) : C2<T>{T(0), T(0)};
+    }
+    // 2. everything else of the launch into LDS, spread over the waves
+    {
+        const int wv = threadIdx.x >> 6, nw = kThreads / 64;
+        int item = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int row = min(max(y0 - 1 + k, 0), A.ny - 1);
+            for (int c = 0; c < 3; ++c, ++item)
+                if (item % nw == wv) dma_floats(ysrc + c * N + row * NX, Y + (k * 3 + c) * NX, NX);
+        }
+        for (int c = 0; c < 3; ++c, ++item)
+            if (item % nw == wv) dma_floats(msrc + c * N, Mz + c * kXpfRows * NX, kXpfRows * NX);
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            if (q == qs) continue;
+            for (int c = 0; c < 3; ++c, ++item)
+                if (item % nw == wv) dma_floats(ksrc[q] + c * N, K + (q * 3 + c) * kXpfRows * NX, kXpfRows * NX);
+        }
+        if (A.nmask && item % nw == wv) dma_bytes(A.nmask + c0, NM, kXpfRows * NX);
+        ++item;
+        if (item % nw == wv) dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
+    }
+    // 3. Hermitian rebuild of the packed sequences, inverse FFT
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const int i = j * kThreads + (int)threadIdx.x;
+        const C2<T> b = {__shfl_xor(v[j].x, 1, 64), __shfl_xor(v[j].y, 1, 64)};
+        if (i >= TOT || (threadIdx.x & 1)) continue;
+        const int kk = small_div<false>(i, NLP, rnl), l = i - kk * NLP;
+        const C2<T> a = v[j];
+        C2<T> *q = b0 + (l >> 1) * sb;
+        q[kk] = {a.x - b.y, a.y + b.x};
+        if (kk > 0 && kk < NX) q[PX - kk] = {a.x + b.y, b.x - a.y};
+    }
+    // the prefetched rows and twiddles have arrived (every wave waits for
+    // its own global->LDS loads; the barrier publishes them to the
+    // work-group)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const C2<T> *tw = reinterpret_cast<const C2<T> *>(TW);
+    fs_inverse<T, R1, R2, true>(b0, b1, tw, A.nseq_x, NX, sb);
+    // 5. local pass of stage s from the LDS rows; k_s also into its K slot
+    const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
+    {
+        Acc acc;
+        const gp<T> kout = stage_rhs(A, M, s, p);
+        const gp<T> hrec = A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
+        const T scale = T(A.inv_p);
+        constexpr int NCL = kXpfRows * NX;
+        for (int base = 0; base < NCL; base += CH * kThreads) {
+            CellIn<T> g[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int i = base + j * kThreads + (int)threadIdx.x;
+                const int ic = i < NCL ? i : NCL - 1;
+                const int rr = ic / NX, x = ic - rr * NX;
+                gather_film_lds(A, c0 + ic, x, rr, Y, NM, g[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int i = base + j * kThreads + (int)threadIdx.x;
+                if (i >= NCL) continue;
+                const int rr = i / NX, x = i - rr * NX;
+                T hd[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int l = 3 * rr + c;
+                    const C2<T> zv = b0[(l >> 1) * sb + x];
+                    hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
+                }
+                T kr[3];
+                compute_cell<T, 0>(A, M, c0 + i, dt, g[j], kout, hd, tf, false, false, acc, hrec, kr, nullptr);
+                if (qs >= 0)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) K[(qs * 3 + c) * kXpfRows * NX + i] = kr[c];
+            }
+        }
+    }
+    __syncthreads();
+    // 6. combination of stage s+1 from LDS, Ms*y packed, forward FFT, store
+    {
+        constexpr int NCL = kXpfRows * NX;
+        for (int base = 0; base < NCL; base += CH * kThreads) {
+            PackIn<T, NK> in;
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int i = base + j * kThreads + (int)threadIdx.x;
+                const int ic = i < NCL ? i : NCL - 1;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) in.mv[j][c] = Mz[c * NCL + ic];
+#pragma unroll
+                for (int q = 0; q < NK; ++q)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) in.kv[j][q][c] = K[(q * 3 + c) * NCL + ic];
+                in.msv[j] = A.ms_pack ? G(A.ms_pack)[c0 + ic] : A.ms;
+            }
+            x_pack_put<T, R1, R2, NK>(A, M, s + 1, p, dt, xr, base, in, b0);
+        }
+        x_pack_tail<T, R1, R2>(A, b0);
+    }
+    x_forward_store<T, R1, R2, true>(A, xr, b0, b1, tw);
 }
 
 // MODE 0: thin-film fused forward FFT + tensor multiply + inverse FFT.
@@ -2774,6 +3142,24 @@ static hipError_t set_lds_zf(int pz, size_t limit) {
     }
 }
 
+template <class T, int R1, int R2>
+static hipError_t set_lds_xxp_shape(size_t bytes) {
+    hipError_t e = set_lds<T>((const void *)k_xxp<T, R1, R2, 1>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 3>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 4>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 5>, bytes);
+    return e;
+}
+template <class T>
+static hipError_t set_lds_xxp(int xfast, size_t bytes) {
+    switch (xfast) {
+        case 512: return set_lds_xxp_shape<T, 16, 32>(bytes);
+        case 1024: return set_lds_xxp_shape<T, 32, 32>(bytes);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <class T>
 const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req) {
     if (!A.demag) return "";
@@ -2843,6 +3229,22 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
         return "cannot raise LDS limit (y)";
     if (A.nz > 1 && !A.zfast && set_lds<T>((const void *)k_z<T>, lds_z(A)) != hipSuccess)
         return "cannot raise LDS limit (z)";
+    // film joined x pass with its inputs prefetched into LDS (FP32; the
+    // environment variable FULLMAG_FDM_XPASS_PREFETCH=0 keeps k_xx)
+    A.xpf = 0;
+    A.xpf_off = 0;
+    if constexpr (sizeof(T) == 4) {
+        const char *env = std::getenv("FULLMAG_FDM_XPASS_PREFETCH");
+        const bool allowed = !(env && env[0] == '0');
+        const int off = (int)((lds_x(A) + 15) / 16 * 16);
+        const size_t bytes = (size_t)off + xpf_floats<T>(A.nx, kMaxK) * sizeof(T) + (size_t)kXpfRows * A.nx;
+        if (allowed && A.nz == 1 && (A.xfast == 512 || A.xfast == 1024) && A.rows_per_wg == kXpfRows &&
+            A.nrows % kXpfRows == 0 && !A.region && !A.xt && A.method != M_ABM3 && bytes <= lds_limit &&
+            set_lds_xxp<T>(A.xfast, bytes) == hipSuccess) {
+            A.xpf = 1;
+            A.xpf_off = off;
+        }
+    }
     return "";
 }
 
@@ -2922,9 +3324,28 @@ static void launch_xx_shape(const Args<T> &A, int s, hipStream_t st) {
         default: launch_xx_ft<T, 3, R1, R2>(A, s, st); break;
     }
 }
+template <class T, int R1, int R2>
+static void launch_xxp_shape(const Args<T> &A, int s, hipStream_t st) {
+    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    const size_t l = (size_t)A.xpf_off + xpf_floats<T>(A.nx, kMaxK) * sizeof(T) + (size_t)kXpfRows * A.nx;
+    switch (A.stage_nk[s + 1]) {
+        case 1: k_xxp<T, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 2: k_xxp<T, R1, R2, 2><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 3: k_xxp<T, R1, R2, 3><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        case 4: k_xxp<T, R1, R2, 4><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+        default: k_xxp<T, R1, R2, 5><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
+    }
+}
 // Fused end of stage s and start of stage s+1 (2 <= s < nstage).
 template <class T>
 static void launch_xx(const Args<T> &A, int s, hipStream_t st) {
+    if constexpr (sizeof(T) == 4) {
+        if (A.xpf) {
+            if (A.xfast == 512) launch_xxp_shape<T, 16, 32>(A, s, st);
+            else launch_xxp_shape<T, 32, 32>(A, s, st);
+            return;
+        }
+    }
     switch (A.xfast) {
         case 64: launch_xx_shape<T, 8, 8>(A, s, st); break;
         case 128: launch_xx_shape<T, 8, 16>(A, s, st); break;
