@@ -100,13 +100,24 @@ static __constant__ Method c_methods[M_COUNT];
 // Tableau of the step in flight: ABM3 runs its Heun start-up table until
 // the history holds three values (Ctl::abm_count changes only in the
 // controller, after every kernel of the step has read it).
+// Read-only view of the control block through the scalar cache, for the
+// kernels that do not write it (every kernel but the controller and the k1
+// gate): the scalar cache is invalidated at every launch, and the scalar
+// round trip counts on its own counter instead of waiting in line behind the
+// vector loads a kernel issues first.
+using CtlR = const __attribute__((address_space(4))) Ctl *;
+template <class T>
+__device__ __forceinline__ CtlR ctl_ro(const Args<T> &A) {
+    return (CtlR)A.ctl;
+}
+
 // The index is the same for every lane; readfirstlane tells the compiler
 // so, and the table fields then come through scalar loads (otherwise the
 // control-block read makes the index a vector value, and every field read
 // of the table became a dependent vector load with a full memory wait).
 template <class T>
 __device__ __forceinline__ const Method &method_for(const Args<T> &A) {
-    const int idx = A.method == M_ABM3 && A.ctl->abm_count < 3 ? M_ABM3S : A.method;
+    const int idx = A.method == M_ABM3 && ctl_ro(A)->abm_count < 3 ? M_ABM3S : A.method;
     return c_methods[__builtin_amdgcn_readfirstlane(idx)];
 }
 
@@ -347,8 +358,8 @@ __device__ __forceinline__ int kslot(const Args<T> &A, const Method &M, int j, i
     if (M.abm) {
         // ABM3 history ring: f_{n-r} for kidx kHist + r; the predictor's
         // RHS f* goes to the slot of f_{n-2}, which becomes f_n at the push
-        if (j >= kHist) return kHistSlot + (A.ctl->abm_head + 3 - (j - kHist)) % 3;
-        if (M.abm == 1 && j == 2) return kHistSlot + (A.ctl->abm_head + 1) % 3;
+        if (j >= kHist) return kHistSlot + (ctl_ro(A)->abm_head + 3 - (j - kHist)) % 3;
+        if (M.abm == 1 && j == 2) return kHistSlot + (ctl_ro(A)->abm_head + 1) % 3;
     }
     return j == 1 ? p : (j == M.nstage ? 1 - p : j);
 }
@@ -358,7 +369,7 @@ __device__ __forceinline__ int kslot(const Args<T> &A, const Method &M, int j, i
 template <class T>
 __device__ __forceinline__ gp<T> stage_push(const Args<T> &A, const Method &M, int s) {
     if (M.abm != 2 || s != M.nstage) return nullptr;
-    return G(A.k_all) + (size_t)(kHistSlot + (A.ctl->abm_head + 1) % 3) * 3 * (size_t)A.ncell;
+    return G(A.k_all) + (size_t)(kHistSlot + (ctl_ro(A)->abm_head + 1) % 3) * 3 * (size_t)A.ncell;
 }
 
 // Intermediate stage states alternate between two buffers, so that a fused
@@ -545,10 +556,10 @@ __device__ __forceinline__ double th_uniform(unsigned long long seed, unsigned l
 }
 template <class T>
 __device__ __forceinline__ void thermal_field(const Args<T> &A, int cell, T ms_i, T h[3]) {
-    const double dt = A.th_dt > 0.0 ? A.th_dt : A.ctl->dt;
+    const double dt = A.th_dt > 0.0 ? A.th_dt : ctl_ro(A)->dt;
     if (!(dt > 0.0)) return;
     const double sig = sqrt(A.th_pre / ((double)ms_i * dt));
-    const unsigned long long st = A.ctl->step, c = (unsigned long long)cell;
+    const unsigned long long st = ctl_ro(A)->step, c = (unsigned long long)cell;
     const double u1 = fmax(th_uniform(A.th_seed, st, c, 0), 1e-300), u2 = th_uniform(A.th_seed, st, c, 1);
     const double u3 = fmax(th_uniform(A.th_seed, st, c, 2), 1e-300), u4 = th_uniform(A.th_seed, st, c, 3);
     const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
@@ -576,8 +587,8 @@ __device__ __forceinline__ TimeF<T> time_factors(const Args<T> &A, double t) {
 }
 
 // Evaluation time of stage s of the trial step from the control block.
-template <class T>
-__device__ __forceinline__ double stage_time(const Args<T> &A, const Method &M, int s, const Ctl *ctl, double dt) {
+template <class T, class CP>
+__device__ __forceinline__ double stage_time(const Args<T> &A, const Method &M, int s, CP ctl, double dt) {
     return s <= 1 ? ctl->t : ctl->t + M.node[s] * dt;
 }
 
@@ -1460,7 +1471,10 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
             const int ic = i < tot ? i : tot - 1;
             const int kk = small_div<false>(ic, nlp, rnl), l = ic - kk * nlp;
             const int off = l < nl ? xr.spec[l] : -1;
-            v[j] = off >= 0 ? ldg(S + off + kk * ly) : C2<T>{T(0), T(0)};
+            // unconditional load (a conditional one becomes a branch per
+            // load), zero for lines past the grid
+            const C2<T> w = ldg(S + (off >= 0 ? off + kk * ly : 0));
+            v[j] = off >= 0 ? w : C2<T>{T(0), T(0)};
         }
 #pragma unroll
         for (int j = 0; j < CZ; ++j) {
@@ -1556,7 +1570,7 @@ __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s
 template <class T, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
+    const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
     const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
@@ -1578,7 +1592,7 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
 template <class T, int FT, int R1, int R2>
 __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
+    const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
     const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
@@ -1620,7 +1634,7 @@ constexpr int xx_waves() {
 template <class T, int FT, int R1, int R2, int NK>
 __global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
+    const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
     const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
@@ -1730,12 +1744,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     constexpr int TOT = NLP * NKX;                              // spectrum values of the work-group
     constexpr int NV = (TOT + kThreads - 1) / kThreads;         // per thread
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
-    const int done = ctl->done, p = ctl->parity;
-    const double dt = ctl->dt;
-    if (done) return;
-    const Method &M = method_for(A);
-    const int N = (int)A.ncell, sb = A.xsb, ly = A.ly;
+    const int N = (int)A.ncell, sb = A.xsb, ly = A.ly, nkx = A.nkx;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + A.nseq_x * A.xsb;
     T *Y = reinterpret_cast<T *>(fm_smem + A.xpf_off);
@@ -1745,24 +1754,13 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     uint8_t *NM = reinterpret_cast<uint8_t *>(TW + 2 * PX);
     __shared__ XRows xr;
     const int r0 = (int)xcd_block(blockIdx.x, gridDim.x) * kXpfRows;
-    x_rows(A, r0, xr);
     // the work-group's rows are y0, y0 + 1 of the film (nrows even)
     const int y0 = r0, c0 = r0 * NX;
-    // stage s+1's row: which of its k-vectors is k_s (made here), and every
-    // source address of the prefetch, resolved before any load is issued
-    // (their method-table reads would otherwise wait behind the spectrum)
-    const StageRow &R = M.row[s + 1];
-    int qs = -1;
-    const T *ksrc[NK];
-#pragma unroll
-    for (int q = 0; q < NK; ++q) {
-        if (R.kidx[q] == s) qs = q;
-        ksrc[q] = A.k_all + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N + c0;
-    }
-    const T *ysrc = (const T *)stage_input(A, M, s, p);
-    const T *msrc = A.m_all + (size_t)p * 3 * N + c0;
-    __builtin_amdgcn_s_waitcnt(0);
-    // 1. spectrum values into registers (as x_inverse_load: line fastest)
+    // 1. the loads that do not depend on the control block go out first, so
+    // that their latency overlaps the control block's round trip: the
+    // spectrum values into registers (as x_inverse_load: line fastest), the
+    // twiddles and the activity bytes into LDS (the work-group's loads are
+    // wasted when the device loop has finished: the early exit follows)
     const C2<T> *S = reinterpret_cast<const C2<T> *>(A.spec);
     constexpr int NL = 3 * kXpfRows;
     const float rnl = 1.0f / (float)NLP;
@@ -1773,12 +1771,38 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
         const int ic = i < TOT ? i : TOT - 1;
         const int kk = small_div<false>(ic, NLP, rnl), l = ic - kk * NLP;
         const int rr = l / 3, c = l - 3 * rr;  // line l = 3 rr + c, as XRows::spec
-        const int off = l < NL ? (c * A.nkx) * ly + y0 + rr : -1;
-        v[j] = off >= 0 ? ldg(S + off + kk * ly) : C2<T>{T(0), T(0)};
+        static_assert(NL == NLP, "two rows pack into whole sequences");
+        // unconditional loads: a conditional one becomes a branch per load,
+        // and the compiler then re-reads the arguments inside each branch
+        v[j] = ldg(S + (c * nkx) * ly + y0 + rr + kk * ly);
     }
-    // 2. everything else of the launch into LDS, spread over the waves
+    const int wv = threadIdx.x >> 6, nw = kThreads / 64;
+    if (wv == nw - 1) dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
+    if (A.nmask && wv == nw - 2) dma_bytes(A.nmask + c0, NM, kXpfRows * NX);
+    // the control block through the scalar cache (this kernel does not
+    // write it; the scalar cache is invalidated at every launch): its round
+    // trip counts on the scalar counter and does not wait behind the
+    // spectrum loads, which count on the vector counter in issue order
+    const CtlR ctl = ctl_ro(A);
+    const int done = ctl->done, p = ctl->parity;
+    const double dt = ctl->dt;
+    if (done) return;
+    const Method &M = method_for(A);
+    x_rows(A, r0, xr);
+    // stage s+1's row: which of its k-vectors is k_s (made here), and the
+    // source addresses of the remaining prefetch
+    const StageRow &R = M.row[s + 1];
+    int qs = -1;
+    const T *ksrc[NK];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) {
+        if (R.kidx[q] == s) qs = q;
+        ksrc[q] = A.k_all + (size_t)kslot(A, M, R.kidx[q], p) * 3 * N + c0;
+    }
+    const T *ysrc = (const T *)stage_input(A, M, s, p);
+    const T *msrc = A.m_all + (size_t)p * 3 * N + c0;
+    // 2. the rest of the launch's inputs into LDS, spread over the waves
     {
-        const int wv = threadIdx.x >> 6, nw = kThreads / 64;
         int item = 0;
         for (int k = 0; k < 4; ++k) {
             const int row = min(max(y0 - 1 + k, 0), A.ny - 1);
@@ -1793,9 +1817,6 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
             for (int c = 0; c < 3; ++c, ++item)
                 if (item % nw == wv) dma_floats(ksrc[q] + c * N, K + (q * 3 + c) * kXpfRows * NX, kXpfRows * NX);
         }
-        if (A.nmask && item % nw == wv) dma_bytes(A.nmask + c0, NM, kXpfRows * NX);
-        ++item;
-        if (item % nw == wv) dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
     }
     // 3. Hermitian rebuild of the packed sequences, inverse FFT
 #pragma unroll
@@ -1883,7 +1904,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
 template <class T, int MODE>
 __global__ void __launch_bounds__(kThreads) k_y(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     const int py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
     C2<T> *b1 = b0 + 3 * py;
@@ -1955,7 +1976,7 @@ __global__ void __launch_bounds__(kThreads) k_y(const Args<T> *__restrict__ AP) 
 template <class T, int R1, int R2>
 __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     constexpr int TS = R1 > R2 ? R1 : R2;
     constexpr int N = R1 * R2;
     const int ny = A.ny, ly = A.ly, nkx = A.nkx;
@@ -2050,7 +2071,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     const int pz = A.pz, nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, TY = A.ty;
     const int ntile = (py + TY - 1) / TY;
     const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x % ntile) * TY;
@@ -2121,7 +2142,7 @@ __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) 
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_lmix(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     const int py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, L = A.nz;
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (long long)nkx * py) return;
@@ -2173,7 +2194,7 @@ __global__ void __launch_bounds__(kThreads) k_lmix(const Args<T> *__restrict__ A
 template <class T, int R1, int R2, bool INV>
 __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     constexpr int TS = R1 > R2 ? R1 : R2, LPB = kThreads / TS;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
@@ -2249,7 +2270,7 @@ template <class T, int R1, int R2, int UB>
 __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
     // checked once the loads below are issued (they only read)
-    const int done = A.ctl->done;
+    const int done = ctl_ro(A)->done;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
     const int ntile = (py + UB - 1) / UB;
@@ -2705,7 +2726,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
+    const CtlR ctl = ctl_ro(A);
     if (ctl->done) return;
     const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= A.ncell) return;
@@ -2717,7 +2738,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Args<T> *__restrict__ 
 template <class T, int FT>
 __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
-    Ctl *ctl = A.ctl;
+    const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
     const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
@@ -2829,7 +2850,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_control(const Args<T> *__restrict__ AP, int nparts, int observe) {
     const Args<T> &A = AP[blockIdx.y];
-    if (A.ctl->done) return;
+    if (ctl_ro(A)->done) return;
     control_block(A, nparts, observe);
 }
 
