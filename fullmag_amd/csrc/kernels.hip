@@ -1687,7 +1687,8 @@ __host__ __device__ constexpr size_t xpf_floats(int nx, int nk) {
 
 // Moves n floats (n a multiple of 256; src 16-byte aligned) from global src
 // to LDS dst with direct global->LDS loads of the calling wave: lane l of
-// load j moves floats 256 j + 4 l .. + 3.  dst must be wave-uniform.
+// load j moves floats 256 j + 4 l .. + 3.  dst must be wave-uniform and all
+// 64 lanes of the wave active (an absent lane leaves its 16 bytes unset).
 __device__ __forceinline__ void dma_floats(const float *src, float *dst, int n) {
     const int lane = threadIdx.x & 63;
     for (int j = 0; j < n; j += 256)
@@ -1786,7 +1787,11 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     const CtlR ctl = ctl_ro(A);
     const int done = ctl->done, p = ctl->parity;
     const double dt = ctl->dt;
-    if (done) return;
+    if (done) {
+        // the global->LDS loads land before the work-group's LDS is released
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        return;
+    }
     const Method &M = method_for(A);
     x_rows(A, r0, xr);
     // stage s+1's row: which of its k-vectors is k_s (made here), and the
@@ -1976,9 +1981,20 @@ __global__ void __launch_bounds__(kThreads) k_y(const Args<T> *__restrict__ AP) 
 template <class T, int R1, int R2>
 __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
-    if (ctl_ro(A)->done) return;
     constexpr int TS = R1 > R2 ? R1 : R2;
     constexpr int N = R1 * R2;
+    // the twiddle table comes into LDS with the launch's first loads, moved
+    // by the first wave (a global->LDS load moves 256 floats with all 64
+    // lanes: the table must be a multiple of that, and the work-group at
+    // least one full wave)
+    // (measured: FP64 26.6 us against 27.7 on the 512 x 512 film; FP32 is
+    // faster with the exit check first and the twiddles from global memory,
+    // 11.4 us against 13.1)
+    constexpr int TWF = 2 * N * (int)sizeof(T) / 4;
+    constexpr bool TWL = sizeof(T) == 8 && TWF % 256 == 0 && 3 * TS >= 64;
+    if constexpr (!TWL) {
+        if (ctl_ro(A)->done) return;
+    }
     const int ny = A.ny, ly = A.ly, nkx = A.nkx;
     const int kx = blockIdx.x;
     const int c = threadIdx.x / TS, t = threadIdx.x - c * TS;
@@ -1988,15 +2004,16 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     constexpr int P1 = R1 + 1, P2 = R2 + 1;                    // padded LDS rows
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);  // transpose block per component
     C2<T> *tb = lds + c * TB;
+    C2<T> *twl = lds + (3 * TB + 1) / 2 * 2;                   // 16-byte aligned
 
-    // ---- tensor values of this thread's frequencies, loaded before any
-    // use so their latency overlaps the forward transform: component c
-    // needs N_cc and (for x, y) N_xy with the sign of ky's half
+    // ---- the loads of the launch (FP64: before the exit check; they only
+    // read, and the check's scalar round trip then overlaps them): the
+    // tensor values of this thread's frequencies (component c needs N_cc and
+    // for x, y N_xy, the sign of ky's half applied at the multiply), the
+    // forward transform's inputs and the twiddle table
     const long long cs = (long long)nkx * (ny + 1);
     const T *NT = A.ntens + (long long)kx * (ny + 1);
     const int other = c == 2 ? 2 : (c ^ 1);
-    // raw values here, the sign of N_xy on the upper half at the multiply,
-    // so that all loads are in flight together
     T na[R1], nb[R1];
     if (t < R2) {
 #pragma unroll
@@ -2004,18 +2021,37 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
             const int k = t + R2 * k1;
             const int kr = k <= ny ? k : N - k;
             na[k1] = ldg(NT + c * cs + kr);
-            nb[k1] = c == 2 ? T(0) : ldg(NT + 3 * cs + kr);
+            const T b = ldg(NT + 3 * cs + kr);
+            nb[k1] = c == 2 ? T(0) : b;
         }
     }
-    // ---- forward step 1: thread t1 < R1 (py = 2 ny: the upper half of
+    // forward step 1 inputs: thread t1 < R1 (py = 2 ny: the upper half of
     // every thread's inputs is the zero padding)
     C2<T> v[R2];
     if (t < R1) {
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? ldg(S + t + R1 * n2) : C2<T>{T(0), T(0)};
-        DftReg<T, R2, false>::run(v);
+    }
+    if constexpr (TWL) {
+        if (threadIdx.x < 64)
+            dma_floats(reinterpret_cast<const float *>(tw4), reinterpret_cast<float *>(twl), TWF);
+    }
+    if constexpr (TWL) {
+        if (ctl_ro(A)->done) {
+            // the global->LDS loads land before the work-group's LDS is released
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            return;
+        }
+    }
+    // ---- forward step 1: R2-point DFTs, twiddles, transpose
+    if (t < R1) DftReg<T, R2, false>::run(v);
+    if constexpr (TWL) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+    if (t < R1) {
 #pragma unroll
-        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
+        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], TWL ? twl[k2 * R1 + t] : ldg(tw4 + k2 * R1 + t));
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
     }
@@ -2050,7 +2086,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         DftReg<T, R1, true>::run(u);
 #pragma unroll
         for (int t1 = 1; t1 < R1; ++t1) {
-            C2<T> w = ldg(tw4 + t * R1 + t1);
+            C2<T> w = TWL ? twl[t * R1 + t1] : ldg(tw4 + t * R1 + t1);
             w.y = -w.y;
             u[t1] = cmul(u[t1], w);
         }
@@ -3079,7 +3115,9 @@ template <class T, int R1, int R2>
 static size_t lds_yf() {
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
-    return 3ull * TB * sizeof(C2<T>);
+    // transpose blocks (the exchange of the components reuses them), then
+    // the twiddle table, 16-byte aligned
+    return ((3ull * TB + 1) / 2 * 2 + (size_t)R1 * R2) * sizeof(C2<T>);
 }
 
 template <class T, int FT, int R1, int R2>
