@@ -43,33 +43,35 @@ HBM_PEAK_GBS = 8000.0
 # recorded when those profiles were taken; bench.py cannot read hardware
 # counters itself.
 TRAFFIC_PROFILES = {"film512": "film512_dp45_hbm_traffic.md", "granular3d": "granular3d_dp45_profile.md"}
-TRAFFIC_ROWS = {0: ("k_fwd_x<float", ", 1>`"), 1: ("k_yf<float", ">`"), 2: ("k_zf<float", ">`"),
-                4: ("k_inv_x<float", ">`"), 6: ("k_xx<float", ", 2>`")}
+TRAFFIC_ROWS = {0: [("k_fwd_x<float", ", 1>`")], 1: [("k_yf<float", ">`")], 2: [("k_zf<float", ">`")],
+                4: [("k_inv_x<float", ">`")], 6: [("k_xxp<float", ", 2>`"), ("k_xx<float", ", 2>`")]}
 
 
 def recorded_traffic(workload, kind):
     """(bytes per launch, source) from the committed profile summary, or
     (None, reason)."""
     name = TRAFFIC_PROFILES.get(workload)
-    row = TRAFFIC_ROWS.get(kind)
-    if not name or not row:
+    rows = TRAFFIC_ROWS.get(kind)
+    if not name or not rows:
         return None, "no recorded profile"
     path = os.path.join(ROOT, "profiles", name)
     try:
         lines = open(path).read().splitlines()
     except OSError:
         return None, f"profiles/{name} missing"
-    for ln in lines:
-        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if cells and cells[0].startswith("`" + row[0]) and cells[0].endswith(row[1]) and "KiB" not in cells[0]:
-            try:
-                return float(cells[-1]) * 1e6, f"profiles/{name} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE, recorded)"
-            except ValueError:
-                continue
-    return None, f"no row for {row[0]} in profiles/{name}"
+    for row in rows:
+        for ln in lines:
+            cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+            if cells and cells[0].startswith("`" + row[0]) and cells[0].endswith(row[1]) and "KiB" not in cells[0]:
+                try:
+                    return float(cells[-1]) * 1e6, (f"profiles/{name} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE of "
+                                                    f"{cells[0].strip('`')}, recorded)")
+                except ValueError:
+                    continue
+    return None, f"no row for {rows[0][0]} in profiles/{name}"
 
 
-NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xx"}
+NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xxp (FP32 film joined x pass; k_xx in FP64)"}
 NAMES_3D = {0: "k_fwd_x", 1: "k_y (forward)", 2: "k_z", 3: "k_y (inverse)", 4: "k_inv_x", 6: "k_xx"}
 
 
