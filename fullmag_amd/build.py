@@ -61,8 +61,10 @@ def build(verbose: bool = False, incremental: bool = False) -> str:
         cmd = [hipcc()] + common + extra
         if src.endswith(".hip"):
             # packed FP32 instructions disabled: plain scalar FP32 only
+            # kernel arguments preloaded into SGPRs (the argument-record
+            # pointer then needs no load at kernel entry)
             cmd += ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-                    "-packed-fp32-ops", "-ffp-contract=fast"]
+                    "-packed-fp32-ops", "-ffp-contract=fast", "-mllvm", "-amdgpu-kernarg-preload-count=16"]
         else:
             # host-only translation units: HIP runtime API, no device code
             cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wno-unused-value"]
