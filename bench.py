@@ -351,15 +351,27 @@ def adjoint_bench(steps, fd_params, fd_h=1e-3):
         j, gd, ga = pr.value_and_grad(design, amp)
         t_grad = time.perf_counter() - t0
         grads[prec] = (j, gd, ga)
+        # the whole run recorded by the forward (checkpoint_interval = steps:
+        # 7 (nstage - 1) N values per step, 55 GB FP32 / 110 GB FP64 here)
+        tape = WaveguideProblem(steps=steps, window=base.window, precision=prec, checkpoint_interval=steps)
+        t0 = time.perf_counter()
+        j_t, gd_t, ga_t = tape.value_and_grad(design, amp)
+        t_tape = time.perf_counter() - t0
         out[prec] = {"objective": j, "forward_s": round(t_fwd, 3), "gradient_s": round(t_grad, 3),
                      "gradient_over_forward": round(t_grad / t_fwd, 3),
                      "forward_Mcell_steps_per_s": round(pr.n * steps / t_fwd / 1e6, 2),
-                     "checkpoint_interval": int(np.ceil(np.sqrt(steps)))}
+                     "checkpoint_interval": int(np.ceil(np.sqrt(steps))),
+                     "full_record_gradient_s": round(t_tape, 3),
+                     "full_record_gradient_over_forward": round(t_tape / t_fwd, 3),
+                     "full_record_equals_checkpointed": bool(j_t == j and np.array_equal(gd_t, gd) and ga_t == ga)}
+        if prec == "single":
+            st = WaveguideProblem(steps=steps, window=base.window, precision=prec, beta_state_precision=True)
+            grads["single_beta32"] = st.value_and_grad(design, amp)
     if fd_params:
         pr = WaveguideProblem(steps=steps, window=base.window, precision="double")
         dy, dx = pr.design_shape()
         cells = rng.choice(dy * dx, size=fd_params - 1, replace=False)
-        fd, adj32, adj64 = [], [], []
+        fd, adj32, adj64, adj32b = [], [], [], []
         for f in cells:
             iy, ix = int(f // dx), int(f % dx)
             p, m = design.copy(), design.copy()
@@ -372,6 +384,7 @@ def adjoint_bench(steps, fd_params, fd_h=1e-3):
             fd.append((jp - jm) / (2 * fd_h * design[iy, ix]))
             adj32.append(grads["single"][1][iy, ix])
             adj64.append(grads["double"][1][iy, ix])
+            adj32b.append(grads["single_beta32"][1][iy, ix])
         jp, b = pr.run(design, amp * (1 + fd_h))
         b.close()
         jm, b = pr.run(design, amp * (1 - fd_h))
@@ -379,7 +392,8 @@ def adjoint_bench(steps, fd_params, fd_h=1e-3):
         fd.append((jp - jm) / (2 * fd_h * amp))
         adj32.append(grads["single"][2])
         adj64.append(grads["double"][2])
-        fd, adj32, adj64 = map(np.array, (fd, adj32, adj64))
+        adj32b.append(grads["single_beta32"][2])
+        fd, adj32, adj64, adj32b = map(np.array, (fd, adj32, adj64, adj32b))
         sc = np.abs(fd).max()
         out["finite_differences"] = {
             "params": fd_params, "h_rel": fd_h, "fd_from": "FP64 engine forward",
@@ -388,6 +402,9 @@ def adjoint_bench(steps, fd_params, fd_h=1e-3):
             "grad_scale": float(sc),
             "per_param_rel_err_fp32": [float(v) for v in np.abs(adj32 - fd) / np.abs(fd)],
             "per_param_rel_err_fp64": [float(v) for v in np.abs(adj64 - fd) / np.abs(fd)],
+            "max_rel_err_fp32_adjoint_fp32_beta": float(np.abs(adj32b - fd).max() / sc),
+            "per_param_rel_err_fp32_fp32_beta": [float(v) for v in np.abs(adj32b - fd) / np.abs(fd)],
+            "fp32_note": "FP32 adjoint: field cotangent convolved in FP64 (default); *_fp32_beta: in FP32",
             "fd": [float(v) for v in fd]}
     return out
 

@@ -224,6 +224,9 @@ WAVE_NONE, WAVE_RAMP, WAVE_SINC, WAVE_SINE = 0, 1, 2, 3
 
 POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
 
+ADJOINT_BETA_STATE_PRECISION = 1
+
+
 class AdjointDesc(ctypes.Structure):
     """fullmag_fdm_adjoint_desc (include/fullmag_fdm_ext.h)."""
     _fields_ = [
@@ -232,7 +235,7 @@ class AdjointDesc(ctypes.Structure):
         ("dt", ctypes.c_double),
         ("objective_first_step", ctypes.c_uint64),
         ("objective_component", ctypes.c_uint32),
-        ("reserved0", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
         ("objective_weight", c_double_p),
         ("objective_weight_len", ctypes.c_uint64),
         ("checkpoint_interval", ctypes.c_uint64),

@@ -50,6 +50,7 @@ class WaveguideProblem:
     precision: str = "single"
     integrator: str = "rk4"
     checkpoint_interval: int = 0
+    beta_state_precision: bool = False
     m0: np.ndarray | None = None
     extra: dict = field(default_factory=dict)
 
@@ -114,7 +115,7 @@ class WaveguideProblem:
         `adjoint_backward`."""
         be = backend or FdmBackend(self.plan(ms_design, amplitude_t))
         j = be.adjoint_forward(self.steps, self.dt, self.weights(), self.first_step(), self.component,
-                               self.checkpoint_interval)
+                               self.checkpoint_interval, self.beta_state_precision)
         return j, be
 
     def value_and_grad(self, ms_design, amplitude_t):

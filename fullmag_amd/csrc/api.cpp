@@ -58,6 +58,9 @@ template <class T>
 hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStream_t st);
 template <class T>
 hipError_t adj_configure(const Args<T> &A);
+hipError_t enqueue_adj_stage_f64(const Args<float> &A, const Args<double> &F, const AdjArgs<float> &J, int j,
+                                 hipStream_t st);
+hipError_t adj_configure_f64(const Args<double> &F);
 template <class T>
 hipError_t enqueue_kind(const Args<T> &A, int kind, int s, hipStream_t st);
 template <class T>
@@ -135,7 +138,14 @@ struct fullmag_fdm_backend {
         int parity0 = 0;
         fm::Ctl ctl_end{};
         bool ready = false;
+        bool beta64 = false;     // FP32 system: field cotangent convolved in the FP64 context f64
+        bool last_rec = false;   // the forward recorded the last segment (no recompute of it)
     } adj;
+    // FP64 reduced Newell spectrum of an FP32 demag plan (kept up to 64 MB),
+    // and the FP64 demag context built from it on the first adjoint run
+    std::vector<double> red64;
+    fm::Args<double> f64{};
+    bool f64_ready = false;
 };
 
 // Asynchronous readouts (reference AsyncFieldSnapshot/AsyncPreviewSnapshot,
@@ -316,6 +326,29 @@ void *upload_twiddles(Backend *h, int n) {
     void *d = dalloc(h, tw.size() * sizeof(T));
     if (d && hipMemcpy(d, tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
     return d;
+}
+
+// Four-step twiddle table W_n^{t1 k2} at [k2*r1 + t1] of a register
+// four-step length n (none, and true, when n has no such factorisation);
+// swap: the table of the transposed factorisation n = r2*r1 (the forward
+// 3-D y-line pass runs with the factors exchanged).
+template <class T>
+bool four_step_twiddles(Backend *h, int n, const void **dst, bool zshape = false, bool swap = false) {
+    int r1 = 0, r2 = 0;
+    if (!(zshape ? fm::zfast_factors(n, &r1, &r2) : fm::yfast_factors(n, &r1, &r2))) return true;
+    if (swap) std::swap(r1, r2);
+    std::vector<T> t4(2 * (size_t)n);
+    for (int k2 = 0; k2 < r2; ++k2)
+        for (int t1 = 0; t1 < r1; ++t1) {
+            const double a = 2.0 * 3.14159265358979323846 * (double)((long)t1 * k2) / (double)n;
+            t4[2 * ((size_t)k2 * r1 + t1)] = (T)std::cos(a);
+            t4[2 * ((size_t)k2 * r1 + t1) + 1] = (T)(-std::sin(a));
+        }
+    void *d = dalloc(h, t4.size() * sizeof(T));
+    if (!d || hipMemcpy(d, t4.data(), t4.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return hip_ok(h, hipErrorOutOfMemory, "four-step twiddles");
+    *dst = d;
+    return true;
 }
 
 int select_device(Backend *h) {
@@ -1105,35 +1138,16 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         if (!hip_ok(h, hipMemcpy(nt, rt.data(), rt.size() * sizeof(T), hipMemcpyHostToDevice), "upload tensor"))
             return false;
         A.ntens = nt;
+        if (!h->fp64 && !A.lmix && !A.gtens && red.size() * sizeof(double) <= (64ull << 20)) h->red64 = red;
         const size_t spec_elems = 3ull * A.nzs * A.nkx * A.ly;
         A.spec = dalloc(h, spec_elems * 2 * sizeof(T));
         A.twx = upload_twiddles<T>(h, A.px);
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
-        // swap: the table of the transposed factorisation n = r2*r1 (the
-        // forward 3-D y-line pass runs with the factors exchanged)
-        auto four_step_twiddles = [&](int n, const void **dst, bool zshape = false, bool swap = false) -> bool {
-            int r1 = 0, r2 = 0;
-            if (!(zshape ? fm::zfast_factors(n, &r1, &r2) : fm::yfast_factors(n, &r1, &r2))) return true;
-            if (swap) std::swap(r1, r2);
-            // W_n^{t1 k2} stored at [k2*r1 + t1]
-            std::vector<T> t4(2 * (size_t)n);
-            for (int k2 = 0; k2 < r2; ++k2)
-                for (int t1 = 0; t1 < r1; ++t1) {
-                    const double a = 2.0 * 3.14159265358979323846 * (double)((long)t1 * k2) / (double)n;
-                    t4[2 * ((size_t)k2 * r1 + t1)] = (T)std::cos(a);
-                    t4[2 * ((size_t)k2 * r1 + t1) + 1] = (T)(-std::sin(a));
-                }
-            void *d = dalloc(h, t4.size() * sizeof(T));
-            if (!d || hipMemcpy(d, t4.data(), t4.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-                return hip_ok(h, hipErrorOutOfMemory, "four-step twiddles");
-            *dst = d;
-            return true;
-        };
-        if (!four_step_twiddles(A.py, &A.tw4y)) return false;
-        if (nz > 1 && !four_step_twiddles(A.py, &A.tw4y_f, false, true)) return false;
-        if (!four_step_twiddles(A.px, &A.tw4x)) return false;
-        if (nz > 1 && !four_step_twiddles(A.pz, &A.tw4z, true)) return false;
+        if (!four_step_twiddles<T>(h, A.py, &A.tw4y)) return false;
+        if (nz > 1 && !four_step_twiddles<T>(h, A.py, &A.tw4y_f, false, true)) return false;
+        if (!four_step_twiddles<T>(h, A.px, &A.tw4x)) return false;
+        if (nz > 1 && !four_step_twiddles<T>(h, A.pz, &A.tw4z, true)) return false;
         if (!A.spec || !A.twx || !A.twy || !A.twz) return hip_ok(h, hipErrorOutOfMemory, "alloc spectrum");
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)
@@ -2156,6 +2170,104 @@ fm::AdjArgs<T> adj_args(Backend *h) {
     return J;
 }
 
+// Recording form of the arguments: the trial step's kernels store the
+// stage states, fields and pre-normalisation norms in record slot
+// (device step index mod seg).
+template <class T>
+fm::Args<T> adj_record_args(Backend *h, const fm::Args<T> &A) {
+    const auto &D = h->adj;
+    const int ns = h->nstage;
+    fm::Args<T> B = A;
+    B.rec_y = static_cast<T *>(D.rec_y);
+    B.rec_h = static_cast<T *>(D.rec_h);
+    B.rec_vn = static_cast<T *>(D.rec_vn);
+    B.rec_step = D.step;
+    B.rec_seg = (long long)D.seg;
+    B.rec_sv = (long long)(ns - 1) * 3 * (long long)h->n;
+    B.rec_ss = (long long)(ns - 1) * (long long)h->n;
+    return B;
+}
+
+// FP64 demag context of an FP32 system (same grid, FP64 tensor, spectrum
+// and twiddles): the adjoint's field cotangent beta is convolved there.  The
+// cotangent's far field from a small source (a pulse region acting on
+// distant probe cells) is a small difference of FFT terms of the size of its
+// near field; in FP32 it keeps ~1e-7 of that near field, in FP64 ~1e-16.
+bool build_f64_context(Backend *h) {
+    if (h->f64_ready) return true;
+    const fm::Args<float> &A = h->af;
+    if (h->fp64 || !A.demag || A.lmix || A.gtens || h->red64.empty()) return false;
+    fm::Args<double> F{};
+    F.nx = A.nx;
+    F.ny = A.ny;
+    F.nz = A.nz;
+    F.ncell = A.ncell;
+    F.nrows = A.nrows;
+    F.nkx = A.nkx;
+    F.ly = A.ly;
+    F.nzs = A.nzs;
+    F.px = A.px;
+    F.py = A.py;
+    F.pz = A.pz;
+    F.inv_p = A.inv_p;
+    F.fx = A.fx;
+    F.fy = A.fy;
+    F.fz = A.fz;
+    F.demag = 1;
+    F.method = A.method;
+    F.ctl = A.ctl;
+    const int nx = (int)h->nx, ny = (int)h->ny, nz = (int)h->nz;
+    const std::vector<double> &red = h->red64;
+    std::vector<double> rt(red.size());
+    if (nz == 1) {
+        rt = red;
+    } else {
+        const size_t X = nx + 1, Y = ny + 1, Z = nz + 1;
+        for (size_t c = 0; c < 6; ++c)
+            for (size_t kx = 0; kx < X; ++kx)
+                for (size_t ky = 0; ky < Y; ++ky)
+                    for (size_t kz = 0; kz < Z; ++kz)
+                        rt[((kx * Z + kz) * Y + ky) * 6 + c] = red[((c * X + kx) * Y + ky) * Z + kz];
+    }
+    double *nt = static_cast<double *>(dalloc(h, rt.size() * sizeof(double)));
+    if (!nt || !hip_ok(h, hipMemcpy(nt, rt.data(), rt.size() * sizeof(double), hipMemcpyHostToDevice),
+                       "FP64 demag context"))
+        return false;
+    F.ntens = nt;
+    F.spec = dalloc(h, 3ull * F.nzs * F.nkx * F.ly * 2 * sizeof(double));
+    F.twx = upload_twiddles<double>(h, F.px);
+    F.twy = upload_twiddles<double>(h, F.py);
+    F.twz = F.pz > 1 ? upload_twiddles<double>(h, F.pz) : F.twy;
+    if (!F.spec || !F.twx || !F.twy || !F.twz) return hip_ok(h, hipErrorOutOfMemory, "FP64 demag context");
+    if (!four_step_twiddles<double>(h, F.py, &F.tw4y) ||
+        (nz > 1 && !four_step_twiddles<double>(h, F.py, &F.tw4y_f, false, true)) ||
+        !four_step_twiddles<double>(h, F.px, &F.tw4x) ||
+        (nz > 1 && !four_step_twiddles<double>(h, F.pz, &F.tw4z, true)))
+        return false;
+    const char *ce = fm::configure(F, h->lds_limit);
+    if (ce && *ce) return (h->err = std::string("FP64 demag context: ") + ce, false);
+    void *ad = dalloc(h, sizeof(fm::Args<double>));
+    if (!ad) return hip_ok(h, hipErrorOutOfMemory, "FP64 demag context");
+    F.self_d = static_cast<const fm::Args<double> *>(ad);
+    F.nsys = 1;
+    if (!hip_ok(h, hipMemcpy(ad, &F, sizeof(F), hipMemcpyHostToDevice), "FP64 demag context") ||
+        !hip_ok(h, fm::adj_configure_f64(F), "FP64 demag context LDS limits"))
+        return false;
+    h->f64 = F;
+    h->f64_ready = true;
+    return true;
+}
+
+// Reverse sweep of stage j in the system's precision or, for an FP32 system
+// with beta64, with the field cotangent convolved in the FP64 context.
+template <class T>
+hipError_t adj_stage(Backend *h, const fm::Args<T> &A, const fm::AdjArgs<T> &J, int j) {
+    if constexpr (sizeof(T) == 4) {
+        if (h->adj.beta64) return fm::enqueue_adj_stage_f64(A, h->f64, J, j, h->stream);
+    }
+    return fm::enqueue_adj_stage(A, J, j, h->stream);
+}
+
 template <class T>
 int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc *d, double *out_j) {
     adj_free(h);
@@ -2171,6 +2283,19 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     D.seg = std::min<uint64_t>(std::max<uint64_t>(D.seg, 1), steps);
     D.ncp = (steps + D.seg - 1) / D.seg;
     D.blocks = fm::adj_blocks(A);
+    // FP32 systems convolve the field cotangent in FP64 unless the caller
+    // asks for the state precision (or no FP64 context can be built)
+    D.beta64 = false;
+    if constexpr (sizeof(T) == 4) {
+        if (A.demag && !(d->flags & FULLMAG_FDM_ADJOINT_BETA_STATE_PRECISION)) {
+            if (!h->f64_ready) {
+                const std::string keep = h->err;
+                if (!build_f64_context(h) && h->err != keep) return FULLMAG_FDM_ERR_CUDA;
+            }
+            D.beta64 = h->f64_ready;
+            if (D.beta64) D.blocks = std::max<long long>(D.blocks, fm::num_partials(h->f64));
+        }
+    }
     const size_t v3 = 3 * n * sizeof(T), slots = (size_t)D.seg * (ns - 1) + 1;
     D.cp = adj_alloc(h, D.ncp * v3);
     D.rec_y = adj_alloc(h, slots * v3);
@@ -2227,11 +2352,33 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
         return hip_ok(h, fm::enqueue_trial(A, ns, h->stream), "adjoint forward step") &&
                hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective");
     };
+    // the last segment runs in recording form (the arguments of the
+    // backward's recompute), so that the backward starts with its record
+    // instead of recomputing it; with checkpoint_interval = steps the whole
+    // run is recorded and nothing is recomputed
+    fm::Args<T> B = adj_record_args(h, A);
+    if (!put_args(h, B, 2)) return FULLMAG_FDM_ERR_CUDA;
+    auto rec_step = [&]() {
+        return hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint forward step (recording)") &&
+               hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective");
+    };
+    const uint64_t n_last = (D.ncp - 1) * D.seg;
     hipGraphExec_t g = capture_graph(h, one_step);
+    hipGraphExec_t g_rec = capture_graph(h, rec_step);
     int par = D.parity0;
     bool ok = true;
     for (uint64_t s = 1; s <= steps && ok; ++s) {
-        ok = g ? hip_ok(h, hipGraphLaunch(g, h->stream), "adjoint forward graph") : one_step();
+        if (s - 1 == n_last) {
+            // entering the last segment at state n_last: the record starts
+            // with that state and its k1, as the backward's restart does
+            ok = hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n_last, 0, h->stream), "adjoint step") &&
+                 hip_ok(h, hipMemcpyAsync(D.rec_y, A.m_all + (size_t)par * n3, v3, hipMemcpyDeviceToDevice, h->stream),
+                        "adjoint record start") &&
+                 hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint forward (k1 record)");
+            if (!ok) break;
+        }
+        if (s - 1 >= n_last) ok = g_rec ? hip_ok(h, hipGraphLaunch(g_rec, h->stream), "adjoint forward graph") : rec_step();
+        else ok = g ? hip_ok(h, hipGraphLaunch(g, h->stream), "adjoint forward graph") : one_step();
         par ^= 1;
         if (ok && s % D.seg == 0 && s < steps)
             ok = hip_ok(h,
@@ -2240,6 +2387,7 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
                         "adjoint checkpoint");
     }
     if (g) hipGraphExecDestroy(g);
+    if (g_rec) hipGraphExecDestroy(g_rec);
     if (!ok) return FULLMAG_FDM_ERR_CUDA;
     std::vector<double> part((size_t)D.blocks * 3);
     if (!hip_ok(h, hipMemcpyAsync(part.data(), D.obj_part, part.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -2258,6 +2406,7 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
                 "adjoint final state"))
         return FULLMAG_FDM_ERR_CUDA;
     D.ctl_end = c;
+    D.last_rec = true;
     D.ready = true;
     if (out_j) *out_j = jsum * D.obj_scale;
     return FULLMAG_FDM_OK;
@@ -2274,21 +2423,13 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
         !hip_ok(h, hipMemsetAsync(D.cot_m, 0, 3 * n * sizeof(double), h->stream), "adjoint clear"))
         return FULLMAG_FDM_ERR_CUDA;
     const fm::AdjArgs<T> J = adj_args<T>(h);
-    // recording form of the arguments: slot = device step index mod seg
-    fm::Args<T> B = A;
-    B.rec_y = static_cast<T *>(D.rec_y);
-    B.rec_h = static_cast<T *>(D.rec_h);
-    B.rec_vn = static_cast<T *>(D.rec_vn);
-    B.rec_step = D.step;
-    B.rec_seg = J.seg;
-    B.rec_sv = J.sv;
-    B.rec_ss = J.ss;
+    fm::Args<T> B = adj_record_args(h, A);
     if (!put_args(h, B, 2)) return FULLMAG_FDM_ERR_CUDA;
     auto rec_step = [&]() { return hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint recompute"); };
     auto rev_step = [&]() {
         if (!hip_ok(h, fm::enqueue_adj_init(A, J, ns, h->stream), "adjoint init")) return false;
         for (int j = ns - 1; j >= 1; --j)
-            if (!hip_ok(h, fm::enqueue_adj_stage(A, J, j, h->stream), "adjoint stage")) return false;
+            if (!hip_ok(h, adj_stage(h, A, J, j), "adjoint stage")) return false;
         return hip_ok(h, fm::enqueue_adj_step(D.step, -1, 1, h->stream), "adjoint step");
     };
     hipGraphExec_t g_rec = capture_graph(h, rec_step);
@@ -2306,6 +2447,13 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
         if (!hip_ok(h, hipStreamSynchronize(h->stream), "adjoint sync")) {
             ok = false;
             break;
+        }
+        if (k == D.ncp - 1 && D.last_rec) {
+            // recorded by the forward run: reverse it directly
+            ok = hip_ok(h, hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream), "adjoint loop flag") &&
+                 hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n1 - 1, 0, h->stream), "adjoint step");
+            for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rev, rev_step);
+            continue;
         }
         // recompute the segment from its checkpoint, recording every stage
         c = D.ctl_end;
@@ -2351,6 +2499,8 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
                 "adjoint final state") ||
         !push_ctl(h) || !eval_current(h, false))
         return FULLMAG_FDM_ERR_CUDA;
+    // the recomputed segments reused the record slots of the last one
+    if (D.ncp > 1) D.last_rec = false;
     if (grad_ms)
         for (uint64_t i = 0; i < n; ++i) grad_ms[i] = gms[i];
     if (grad_amp) {

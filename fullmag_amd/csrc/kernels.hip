@@ -773,8 +773,13 @@ __device__ __forceinline__ void compute_cell(const Args<T> &A, const Method &M, 
 #pragma unroll
             for (int q = 0; q < 6; ++q) w[q] = g.nact[q] ? g.v[q][c] : m[c];
             const T mc = m[c];
-            const T lap = (w[1] - T(2) * mc + w[0]) * A.idx2 + (w[3] - T(2) * mc + w[2]) * A.idy2 +
-                          (w[5] - T(2) * mc + w[4]) * A.idz2;
+            // neighbour differences first: each is exact for neighbours
+            // within a factor of two of the cell's value (Sterbenz), so the
+            // second difference of a smooth field keeps its digits in FP32
+            // (w1 - 2m + w0 rounds 2m - w1 to ~1e-7 absolute, a large part
+            // of a long spin wave's second difference)
+            const T lap = ((w[1] - mc) + (w[0] - mc)) * A.idx2 + ((w[3] - mc) + (w[2] - mc)) * A.idy2 +
+                          ((w[5] - mc) + (w[4] - mc)) * A.idz2;
             hex[c] = pre * lap;
         }
     } else {
@@ -2486,7 +2491,8 @@ __device__ __forceinline__ void adj_beta_cell(const Args<T> &A, const AdjArgs<T>
 // state, field and demag values.
 template <class T>
 __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T> &J, const Method &M, int j,
-                                               int cell, int x, int y, int z, const T hdT[3], T wf, double amp[3]) {
+                                               int cell, int x, int y, int z, const double hd_in[3], T wf,
+                                               double amp[3]) {
     const int N = (int)A.ncell;
     if (A.mask && !G(A.mask)[cell]) return;
     const T *yv = J.rec_y + (size_t)(j - 1) * 3 * N;
@@ -2496,7 +2502,7 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
     const double H[3] = {(double)Hv[cell], (double)Hv[N + cell], (double)Hv[2 * N + cell]};
     const double av[3] = {a[cell], a[N + cell], a[2 * N + cell]};
     const double bi[3] = {J.beta[cell], J.beta[N + cell], J.beta[2 * N + cell]};
-    const double hd[3] = {(double)hdT[0], (double)hdT[1], (double)hdT[2]};
+    const double hd[3] = {hd_in[0], hd_in[1], hd_in[2]};
     const double ms_i = A.ms_cell ? (double)G(A.ms_cell)[cell] : (double)A.ms;
     const double gb = (double)A.gamma_bar, al = (double)A.alpha;
     double Hxa[3], mxH[3], mxHxa[3], axm[3], Hxaxm[3];
@@ -2689,7 +2695,7 @@ __global__ void __launch_bounds__(kThreads) k_adj_local(const Args<T> *__restric
     double amp[3] = {0.0, 0.0, 0.0};
     if (cell < N) {
         const int x = cell % A.nx, y = (cell / A.nx) % A.ny, z = cell / (A.nx * A.ny);
-        const T hd[3] = {T(0), T(0), T(0)};
+        const double hd[3] = {0.0, 0.0, 0.0};
         const T wf = A.wave_kind ? wave_factor(A, J.t_n + M.node[j] * J.dt) : T(0);
         adj_local_cell(A, J, M, j, cell, x, y, z, hd, wf, amp);
     }
@@ -2697,22 +2703,28 @@ __global__ void __launch_bounds__(kThreads) k_adj_local(const Args<T> *__restric
 }
 
 // With demag: beta of stage j for the work-group's x-rows, stored and
-// packed (unit Ms) two real rows per complex sequence, forward x FFT.
-template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+// packed (unit Ms) two real rows per complex sequence, forward x FFT.  TF is
+// the precision of the field cotangent's convolution: FP points to the
+// argument record of a demag context of that precision on the same grid
+// (the FP64 context of an FP32 system, or AP itself when TF == T); the rows
+// of a work-group and the transforms follow it.
+template <class T, class TF, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__ AP, const Args<TF> *__restrict__ FP,
+                                                     AdjArgs<T> J0, int j) {
     const Args<T> &A = AP[blockIdx.y];
+    const Args<TF> &F = FP[blockIdx.y];
     constexpr bool FAST = R1 > 0;
     const AdjArgs<T> J = adj_bind(J0);
-    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
-    C2<T> *tw = b1 + A.nseq_x * A.xsb;
+    C2<TF> *b0 = reinterpret_cast<C2<TF> *>(fm_smem);
+    C2<TF> *b1 = b0 + F.nseq_x * F.xsb;
+    C2<TF> *tw = b1 + F.nseq_x * F.xsb;
     __shared__ XRows xr;
-    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
-    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    x_rows(F, (int)xcd_block(blockIdx.x, gridDim.x) * F.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<TF>(tw, F.twx, F.px);
     __syncthreads();
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int rows = A.rows_per_wg, ncl = rows * nx, nl = 3 * rows, N = (int)A.ncell;
-    const int sb = FAST ? A.xsb : px;
+    const int px = FAST ? R1 * R2 : F.px, nx = px >> 1;
+    const int rows = F.rows_per_wg, ncl = rows * nx, nl = 3 * rows, N = (int)A.ncell;
+    const int sb = FAST ? F.xsb : px;
     const float rnx = 1.0f / (float)nx;
     for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
         const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
@@ -2726,47 +2738,49 @@ __global__ void __launch_bounds__(kThreads) k_adj_fx(const Args<T> *__restrict__
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            T *e = reinterpret_cast<T *>(&b0[(l >> 1) * sb + x]);
-            e[l & 1] = (T)b[c];
+            TF *e = reinterpret_cast<TF *>(&b0[(l >> 1) * sb + x]);
+            e[l & 1] = (TF)b[c];
         }
     }
     if (nl & 1)
-        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * sb + x].y = T(0);
-    x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
+        for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * sb + x].y = TF(0);
+    x_forward_store<TF, R1, R2>(F, xr, b0, b1, tw);
 }
 
 // With demag: inverse x FFT of the convolved beta spectrum (1/P), then the
-// cotangent of y_j for the work-group's cells.
-template <class T, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__ AP, AdjArgs<T> J0, int j) {
+// cotangent of y_j for the work-group's cells (FP as in k_adj_fx).
+template <class T, class TF, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_adj_bx(const Args<T> *__restrict__ AP, const Args<TF> *__restrict__ FP,
+                                                     AdjArgs<T> J0, int j) {
     const Args<T> &A = AP[blockIdx.y];
+    const Args<TF> &F = FP[blockIdx.y];
     constexpr bool FAST = R1 > 0;
     const AdjArgs<T> J = adj_bind(J0);
     const Method &M = c_methods[A.method];
-    C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
-    C2<T> *b1 = b0 + A.nseq_x * A.xsb;
-    C2<T> *tw = b1 + A.nseq_x * A.xsb;
+    C2<TF> *b0 = reinterpret_cast<C2<TF> *>(fm_smem);
+    C2<TF> *b1 = b0 + F.nseq_x * F.xsb;
+    C2<TF> *tw = b1 + F.nseq_x * F.xsb;
     __shared__ XRows xr;
-    x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
-    if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
+    x_rows(F, (int)xcd_block(blockIdx.x, gridDim.x) * F.rows_per_wg, xr);
+    if constexpr (R1 == 0) load_twiddles<TF>(tw, F.twx, F.px);
     __syncthreads();
-    const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
-    const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
-    const int ncl = A.rows_per_wg * nx, sb = FAST ? A.xsb : px;
+    const C2<TF> *res = x_inverse_load<TF, R1, R2>(F, xr, b0, b1, tw);
+    const int px = FAST ? R1 * R2 : F.px, nx = px >> 1;
+    const int ncl = F.rows_per_wg * nx, sb = FAST ? F.xsb : px;
     const float rnx = 1.0f / (float)nx;
-    const T scale = T(A.inv_p);
+    const double scale = F.inv_p;
     const T wf = A.wave_kind ? wave_factor(A, J.t_n + M.node[j] * J.dt) : T(0);
     double amp[3] = {0.0, 0.0, 0.0};
     for (int i = threadIdx.x; i < ncl; i += blockDim.x) {
         const int rr = small_div<FAST>(i, nx, rnx), x = i - rr * nx;
         const int c0 = xr.cell0[rr];
         if (c0 < 0) continue;
-        T hd[3];
+        double hd[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int l = 3 * rr + c;
-            const C2<T> zv = res[(l >> 1) * sb + x];
-            hd[c] = ((l & 1) ? zv.y : zv.x) * scale;
+            const C2<TF> zv = res[(l >> 1) * sb + x];
+            hd[c] = (double)(((l & 1) ? zv.y : zv.x) * TF(scale));
         }
         adj_local_cell(A, J, M, j, c0 + x, x, xr.y[rr], xr.z[rr], hd, wf, amp);
     }
@@ -3612,44 +3626,61 @@ hipError_t enqueue_trial(const Args<T> &A, int nstage, hipStream_t st) {
 }
 
 // ---- adjoint launches
-template <class T, int R1, int R2>
-static void launch_adj_x_shape(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
-    const dim3 g((unsigned)num_partials(A)), b(kThreads);
-    if (inverse) k_adj_bx<T, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, J, j);
-    else k_adj_fx<T, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, J, j);
+// F: the argument record of the field cotangent's convolution (A itself, or
+// the FP64 demag context of an FP32 system on the same grid).
+template <class T, class TF, int R1, int R2>
+static void launch_adj_x_shape(const Args<T> &A, const Args<TF> &F, const AdjArgs<T> &J, int j, bool inverse,
+                               hipStream_t st) {
+    const dim3 g((unsigned)num_partials(F)), b(kThreads);
+    if (inverse) k_adj_bx<T, TF, R1, R2><<<bgrid(A, g), b, lds_x(F), st>>>(A.self_d, F.self_d, J, j);
+    else k_adj_fx<T, TF, R1, R2><<<bgrid(A, g), b, lds_x(F), st>>>(A.self_d, F.self_d, J, j);
 }
-template <class T>
-static void launch_adj_x(const Args<T> &A, const AdjArgs<T> &J, int j, bool inverse, hipStream_t st) {
-    switch (A.xfast) {
-        case 64: launch_adj_x_shape<T, 8, 8>(A, J, j, inverse, st); break;
-        case 128: launch_adj_x_shape<T, 8, 16>(A, J, j, inverse, st); break;
-        case 256: launch_adj_x_shape<T, 16, 16>(A, J, j, inverse, st); break;
-        case 512: launch_adj_x_shape<T, 16, 32>(A, J, j, inverse, st); break;
-        case 1024: launch_adj_x_shape<T, 32, 32>(A, J, j, inverse, st); break;
-        case 2048: launch_adj_x_shape<T, 64, 32>(A, J, j, inverse, st); break;
-        default: launch_adj_x_shape<T, 0, 0>(A, J, j, inverse, st); break;
+template <class T, class TF>
+static void launch_adj_x(const Args<T> &A, const Args<TF> &F, const AdjArgs<T> &J, int j, bool inverse,
+                         hipStream_t st) {
+    switch (F.xfast) {
+        case 64: launch_adj_x_shape<T, TF, 8, 8>(A, F, J, j, inverse, st); break;
+        case 128: launch_adj_x_shape<T, TF, 8, 16>(A, F, J, j, inverse, st); break;
+        case 256: launch_adj_x_shape<T, TF, 16, 16>(A, F, J, j, inverse, st); break;
+        case 512: launch_adj_x_shape<T, TF, 16, 32>(A, F, J, j, inverse, st); break;
+        case 1024: launch_adj_x_shape<T, TF, 32, 32>(A, F, J, j, inverse, st); break;
+        case 2048: launch_adj_x_shape<T, TF, 64, 32>(A, F, J, j, inverse, st); break;
+        default: launch_adj_x_shape<T, TF, 0, 0>(A, F, J, j, inverse, st); break;
     }
 }
-template <class T, int R1, int R2>
+template <class T, class TF, int R1, int R2>
 static hipError_t set_lds_adj_shape(size_t bytes) {
-    hipError_t e = set_lds<T>((const void *)k_adj_fx<T, R1, R2>, bytes);
-    if (e == hipSuccess) e = set_lds<T>((const void *)k_adj_bx<T, R1, R2>, bytes);
+    hipError_t e = set_lds<T>((const void *)k_adj_fx<T, TF, R1, R2>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_adj_bx<T, TF, R1, R2>, bytes);
     return e;
 }
-template <class T>
+template <class T, class TF>
 static hipError_t set_lds_adj(int xfast, size_t bytes) {
     switch (xfast) {
-        case 64: return set_lds_adj_shape<T, 8, 8>(bytes);
-        case 128: return set_lds_adj_shape<T, 8, 16>(bytes);
-        case 256: return set_lds_adj_shape<T, 16, 16>(bytes);
-        case 512: return set_lds_adj_shape<T, 16, 32>(bytes);
-        case 1024: return set_lds_adj_shape<T, 32, 32>(bytes);
-        case 2048: return set_lds_adj_shape<T, 64, 32>(bytes);
-        default: return set_lds_adj_shape<T, 0, 0>(bytes);
+        case 64: return set_lds_adj_shape<T, TF, 8, 8>(bytes);
+        case 128: return set_lds_adj_shape<T, TF, 8, 16>(bytes);
+        case 256: return set_lds_adj_shape<T, TF, 16, 16>(bytes);
+        case 512: return set_lds_adj_shape<T, TF, 16, 32>(bytes);
+        case 1024: return set_lds_adj_shape<T, TF, 32, 32>(bytes);
+        case 2048: return set_lds_adj_shape<T, TF, 64, 32>(bytes);
+        default: return set_lds_adj_shape<T, TF, 0, 0>(bytes);
     }
 }
 
-// Work-groups of the adjoint per-block partial arrays (3 doubles each).
+// y/z passes of a demag context (exported per precision, so that an FP32
+// system's adjoint can run the FP64 context's passes).
+template <class T>
+hipError_t enqueue_yz(const Args<T> &A, hipStream_t st) {
+    launch_yz(A, st);
+    return hipGetLastError();
+}
+#ifdef FM_PRIMARY
+// the FP64 passes come from the FP64 translation unit
+extern template hipError_t enqueue_yz<double>(const Args<double> &, hipStream_t);
+#endif
+
+// Work-groups of the adjoint per-block partial arrays (3 doubles each);
+// F as in launch_adj_x.
 template <class T>
 long long adj_blocks(const Args<T> &A) {
     const long long cells = (A.ncell + kThreads - 1) / kThreads;
@@ -3663,15 +3694,28 @@ template <class T>
 hipError_t enqueue_adj_stage(const Args<T> &A, const AdjArgs<T> &J, int j, hipStream_t st) {
     const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
     if (A.demag) {
-        launch_adj_x(A, J, j, false, st);
+        launch_adj_x(A, A, J, j, false, st);
         launch_yz(A, st);
-        launch_adj_x(A, J, j, true, st);
+        launch_adj_x(A, A, J, j, true, st);
     } else {
         k_adj_beta<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J, j);
         k_adj_local<T><<<bgrid(A, dim3(gc)), dim3(kThreads), 0, st>>>(A.self_d, J, j);
     }
     return hipGetLastError();
 }
+#ifdef FM_PRIMARY
+// The same for an FP32 system whose field cotangent is convolved in the FP64
+// demag context F of its grid.
+hipError_t enqueue_adj_stage_f64(const Args<float> &A, const Args<double> &F, const AdjArgs<float> &J, int j,
+                                 hipStream_t st) {
+    launch_adj_x(A, F, J, j, false, st);
+    const hipError_t e = enqueue_yz<double>(F, st);
+    if (e != hipSuccess) return e;
+    launch_adj_x(A, F, J, j, true, st);
+    return hipGetLastError();
+}
+hipError_t adj_configure_f64(const Args<double> &F) { return set_lds_adj<float, double>(F.xfast, lds_x(F)); }
+#endif
 template <class T>
 hipError_t enqueue_adj_init(const Args<T> &A, const AdjArgs<T> &J, int nstage, hipStream_t st) {
     const unsigned gc = (unsigned)((A.ncell + kThreads - 1) / kThreads);
@@ -3692,7 +3736,7 @@ hipError_t enqueue_adj_objective(const Args<T> &A, const AdjArgs<T> &J, hipStrea
 }
 template <class T>
 hipError_t adj_configure(const Args<T> &A) {
-    return A.demag ? set_lds_adj<T>(A.xfast, lds_x(A)) : hipSuccess;
+    return A.demag ? set_lds_adj<T, T>(A.xfast, lds_x(A)) : hipSuccess;
 }
 
 template <class T>
@@ -3718,6 +3762,7 @@ template hipError_t enqueue_adj_stage<FM_REAL>(const Args<FM_REAL> &, const AdjA
 template hipError_t enqueue_adj_init<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, int, hipStream_t);
 template hipError_t enqueue_adj_objective<FM_REAL>(const Args<FM_REAL> &, const AdjArgs<FM_REAL> &, hipStream_t);
 template hipError_t adj_configure<FM_REAL>(const Args<FM_REAL> &);
+template hipError_t enqueue_yz<FM_REAL>(const Args<FM_REAL> &, hipStream_t);
 
 }  // namespace fm
 

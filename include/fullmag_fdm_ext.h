@@ -141,23 +141,32 @@ int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out);
  *
  * adjoint_forward runs `steps` steps of size dt from the current state and
  * time, keeping the state every checkpoint_interval steps (0: ceil(sqrt(
- * steps))), and returns J.  adjoint_backward then sweeps the steps in
- * reverse (each checkpoint segment is recomputed once with its stage
- * states and fields recorded) and returns dJ/dMs_i for every cell
+ * steps))), and returns J; the last segment runs in recording form (its
+ * stage states and fields are kept).  adjoint_backward then sweeps the
+ * steps in reverse (the last segment from the forward's record, every
+ * earlier segment recomputed once from its checkpoint with its stage states
+ * and fields recorded; checkpoint_interval = steps records the whole run
+ * and recomputes nothing) and returns dJ/dMs_i for every cell
  * (grad_ms, n values, 0 on inactive cells) and dJ/d field_amplitude_am of
  * the plan's waveform (grad_field_amplitude, 3 values).  Either output may
  * be NULL.  After either call the handle holds the final state m_steps at
  * t0 + steps*dt.  Device memory: 3N state values per checkpoint, 7 (nstage-1) N
  * per step of one segment for the record (stage states, fields and
- * pre-normalisation norms), and 27N FP64 cotangents.
+ * pre-normalisation norms), and 27N FP64 cotangents.  FP32 plans with
+ * demag convolve the field cotangent with an FP64 copy of the demag tensor
+ * and transforms (an FP64 spectrum of the grid, created on the first
+ * adjoint run) unless flags has FULLMAG_FDM_ADJOINT_BETA_STATE_PRECISION:
+ * the far field of a localised cotangent is a small difference of FFT
+ * terms that FP32 resolves only to ~1e-7 of its near field.
  */
+#define FULLMAG_FDM_ADJOINT_BETA_STATE_PRECISION 1u
 typedef struct fullmag_fdm_adjoint_desc {
     uint64_t struct_size;             /* sizeof(fullmag_fdm_adjoint_desc) */
     uint64_t steps;                   /* fixed steps of the run (>= 1) */
     double dt;                        /* step size [s] */
     uint64_t objective_first_step;    /* first step whose state enters J (1..steps) */
     uint32_t objective_component;     /* 0 x, 1 y, 2 z */
-    uint32_t reserved0;
+    uint32_t flags;                   /* FULLMAG_FDM_ADJOINT_* bits (0: defaults) */
     const double *objective_weight;   /* n per-cell weights */
     uint64_t objective_weight_len;
     uint64_t checkpoint_interval;     /* 0 = ceil(sqrt(steps)) */

@@ -162,3 +162,44 @@ def test_adjoint_rejects_adaptive_plans_and_backward_first():
             be.adjoint_backward()
         with pytest.raises(FdmError):
             be.adjoint_forward(10, 1e-13, pr.weights(), first_step=11)
+
+
+def test_fp64_field_cotangent_for_fp32_plans():
+    # FP32 plans convolve the field cotangent in FP64 by default; the FP32
+    # convolution (beta_state_precision) stays within the FP32 band of it
+    p64 = _problem("double")
+    d = _design0(p64)
+    _, gd64, ga64 = p64.value_and_grad(d, 5e-3)
+    out = {}
+    for state in (False, True):
+        pr = _problem("single")
+        pr.beta_state_precision = state
+        out[state] = pr.value_and_grad(d, 5e-3)
+    for state, (_, gd, ga) in out.items():
+        e_ms = np.abs(gd - gd64).max() / np.abs(gd64).max()
+        e_amp = abs(ga - ga64) / abs(ga64)
+        print(f"fp32 adjoint, beta in {'fp32' if state else 'fp64'}: ms {e_ms:.3e}, amplitude {e_amp:.3e}", flush=True)
+        assert e_ms <= 1e-4 and e_amp <= 1e-4
+    # the two forms differ (the FP64 context is used) but only in the FP32 band
+    _, gd_a, ga_a = out[False]
+    _, gd_b, ga_b = out[True]
+    assert ga_a != ga_b
+    assert abs(ga_a - ga_b) <= 1e-4 * abs(ga64)
+
+
+@pytest.mark.parametrize("precision,interval", [("double", 0), ("double", 160), ("single", 0), ("single", 40)])
+def test_repeated_backward_and_recorded_last_segment(precision, interval):
+    # the forward records the last segment; a second backward after the
+    # first (whose recomputes reuse those record slots) gives the same result
+    pr = _problem(precision, interval=interval)
+    d = _design0(pr)
+    j, be = pr.run(d, 5e-3)
+    try:
+        g1, a1 = be.adjoint_backward()
+        g2, a2 = be.adjoint_backward()
+    finally:
+        be.close()
+    assert np.array_equal(g1, g2) and np.array_equal(a1, a2)
+    j_ref, gd_ref, ga_ref = _problem(precision, interval=1).value_and_grad(d, 5e-3)
+    assert j == j_ref
+    assert np.abs(pr.design_grad(g1) - gd_ref).max() <= 1e-12 * np.abs(gd_ref).max()
