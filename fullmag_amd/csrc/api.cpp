@@ -20,6 +20,7 @@
 #include <limits>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -2565,10 +2566,13 @@ struct fullmag_fdm_batch {
         fm::Args<float> gf{};
         fm::Args<double> gd{};
         bool finished = false;
-        hipEvent_t ev = nullptr;
+        // two polling chunks in flight: each ends with the members' control
+        // blocks read back into its flag slot and an event
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        int next = 0;  // slot of the older chunk in flight
     };
     std::vector<Group> groups;
-    fm::Ctl *flags = nullptr;  // pinned: control blocks read back per chunk
+    fm::Ctl *flags = nullptr;  // pinned, [2][systems]: control blocks read back per chunk slot
 };
 
 namespace {
@@ -2670,12 +2674,22 @@ bool group_tune(Batch *b, Batch::Group &g) {
     return true;
 }
 
+// One polling chunk of a group into flag slot `slot`: kAttemptChunk trial
+// steps, the members' control blocks read back, the slot's event.
 template <class T>
-bool group_launch(Batch *b, Batch::Group &g) {
-    if (g.graph) return hipGraphLaunch(g.graph, g.stream) == hipSuccess || (b->err = "batch: graph launch", false);
-    for (int a = 0; a < kAttemptChunk; ++a)
-        if (fm::enqueue_trial(group_args<T>(g), g.nstage, g.stream) != hipSuccess)
-            return (b->err = "batch: launch failed", false);
+bool group_launch(Batch *b, Batch::Group &g, int slot) {
+    if (g.graph) {
+        if (hipGraphLaunch(g.graph, g.stream) != hipSuccess) return (b->err = "batch: graph launch", false);
+    } else {
+        for (int a = 0; a < kAttemptChunk; ++a)
+            if (fm::enqueue_trial(group_args<T>(g), g.nstage, g.stream) != hipSuccess)
+                return (b->err = "batch: launch failed", false);
+    }
+    fm::Ctl *f = b->flags + (size_t)slot * b->sys.size();
+    for (size_t m : g.members)
+        if (hipMemcpyAsync(&f[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream) != hipSuccess)
+            return (b->err = "batch: control readback failed", false);
+    if (hipEventRecord(g.ev[slot], g.stream) != hipSuccess) return (b->err = "batch: event record failed", false);
     return true;
 }
 
@@ -2717,7 +2731,8 @@ fullmag_fdm_batch *fullmag_fdm_batch_create(const fullmag_fdm_plan_desc *const *
     hipSetDevice(b->sys[0]->device);
     for (auto &g : b->groups) {
         if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&g.ev, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&g.ev[0], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&g.ev[1], hipEventDisableTiming) != hipSuccess) {
             b->err = "batch: stream creation failed";
             return b;
         }
@@ -2725,7 +2740,7 @@ fullmag_fdm_batch *fullmag_fdm_batch_create(const fullmag_fdm_plan_desc *const *
         const bool ok = g.fp64 ? group_tune<double>(b, g) : group_tune<float>(b, g);
         if (!ok) return b;
     }
-    if (hipHostMalloc((void **)&b->flags, b->sys.size() * sizeof(fm::Ctl), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&b->flags, 2 * b->sys.size() * sizeof(fm::Ctl), hipHostMallocDefault) != hipSuccess)
         b->err = "batch: pinned allocation failed";
     return b;
 }
@@ -2784,38 +2799,54 @@ int fullmag_fdm_batch_advance(fullmag_fdm_batch *b, double t_end, uint64_t max_s
     const auto t0 = std::chrono::steady_clock::now();
     auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
     int rc = FULLMAG_FDM_OK;
-    for (;;) {
-        bool active = false;
+    // every group progresses on its own stream: two chunks in flight, and a
+    // group whose older chunk has ended is checked and refilled at once,
+    // whatever the other groups are doing (a finished system's launches
+    // return at once; at most one chunk after a group's end is wasted)
+    auto launch = [&](Batch::Group &g, int slot) {
+        return g.fp64 ? group_launch<double>(b, g, slot) : group_launch<float>(b, g, slot);
+    };
+    for (auto &g : b->groups) {
+        g.next = 0;
+        if (!launch(g, 0) || !launch(g, 1)) {
+            rc = FULLMAG_FDM_ERR_CUDA;
+            break;
+        }
+    }
+    while (rc == FULLMAG_FDM_OK) {
+        bool active = false, progressed = false;
         for (auto &g : b->groups) {
             if (g.finished) continue;
             active = true;
-            if (!(g.fp64 ? group_launch<double>(b, g) : group_launch<float>(b, g))) rc = FULLMAG_FDM_ERR_CUDA;
-            for (size_t m : g.members)
-                if (hipMemcpyAsync(&b->flags[m], b->sys[m]->ctl_d, sizeof(fm::Ctl), hipMemcpyDeviceToHost, g.stream) !=
-                    hipSuccess)
-                    rc = FULLMAG_FDM_ERR_CUDA;
-            if (hipEventRecord(g.ev, g.stream) != hipSuccess) rc = FULLMAG_FDM_ERR_CUDA;
-            if (rc != FULLMAG_FDM_OK && b->err.empty()) b->err = "batch: control readback failed";
-        }
-        if (!active || rc != FULLMAG_FDM_OK) break;
-        for (auto &g : b->groups) {
-            if (g.finished) continue;
-            if (hipEventSynchronize(g.ev) != hipSuccess) {
-                b->err = "batch: event sync failed";
+            const int slot = g.next;
+            const hipError_t q = hipEventQuery(g.ev[slot]);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                b->err = "batch: event query failed";
                 rc = FULLMAG_FDM_ERR_CUDA;
                 break;
             }
+            progressed = true;
+            const fm::Ctl *f = b->flags + (size_t)slot * nsys;
             bool all = true;
+            const double tnow = now();
             for (size_t m : g.members) {
-                if (b->flags[m].done) {
-                    if (wall[m] < 0) wall[m] = now();
+                if (f[m].done) {
+                    if (wall[m] < 0) wall[m] = tnow;
                 } else {
                     all = false;
                 }
             }
-            g.finished = all;
+            if (all) {
+                g.finished = true;
+            } else if (!launch(g, slot)) {
+                rc = FULLMAG_FDM_ERR_CUDA;
+                break;
+            }
+            g.next = 1 - slot;
         }
-        if (rc != FULLMAG_FDM_OK) break;
+        if (!active) break;
+        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     for (auto &g : b->groups) hipStreamSynchronize(g.stream);
     const hipError_t le = hipGetLastError();
@@ -2845,7 +2876,8 @@ void fullmag_fdm_batch_destroy(fullmag_fdm_batch *b) {
         if (g.stream) hipStreamSynchronize(g.stream);
         if (g.graph) hipGraphExecDestroy(g.graph);
         if (g.args_d) hipFree(g.args_d);
-        if (g.ev) hipEventDestroy(g.ev);
+        for (hipEvent_t e : g.ev)
+            if (e) hipEventDestroy(e);
         if (g.stream) hipStreamDestroy(g.stream);
     }
     for (auto *h : b->sys) fullmag_fdm_backend_destroy(h);
