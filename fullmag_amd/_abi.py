@@ -239,6 +239,7 @@ class AdjointDesc(ctypes.Structure):
         ("objective_weight", c_double_p),
         ("objective_weight_len", ctypes.c_uint64),
         ("checkpoint_interval", ctypes.c_uint64),
+        ("record_budget_bytes", ctypes.c_uint64),
     ]
 
 

@@ -51,6 +51,7 @@ class WaveguideProblem:
     integrator: str = "rk4"
     checkpoint_interval: int = 0
     beta_state_precision: bool = False
+    record_budget_bytes: int = 0
     m0: np.ndarray | None = None
     extra: dict = field(default_factory=dict)
 
@@ -115,7 +116,7 @@ class WaveguideProblem:
         `adjoint_backward`."""
         be = backend or FdmBackend(self.plan(ms_design, amplitude_t))
         j = be.adjoint_forward(self.steps, self.dt, self.weights(), self.first_step(), self.component,
-                               self.checkpoint_interval, self.beta_state_precision)
+                               self.checkpoint_interval, self.beta_state_precision, self.record_budget_bytes)
         return j, be
 
     def value_and_grad(self, ms_design, amplitude_t):

@@ -310,11 +310,14 @@ class FdmBackend:
         return d
 
     def adjoint_forward(self, steps: int, dt: float, weight: np.ndarray, first_step: int = 1, component: int = 2,
-                        checkpoint_interval: int = 0, beta_state_precision: bool = False) -> float:
+                        checkpoint_interval: int = 0, beta_state_precision: bool = False,
+                        record_budget_bytes: int = 0) -> float:
         """Fixed-step run of `steps` steps with checkpoints; returns
         J = sum_{n>=first_step} sum_i w_i m_{c,i}(t_n)^2 / (steps - first_step + 1).
         beta_state_precision: an FP32 plan convolves the field cotangent in
-        FP32 instead of FP64."""
+        FP32 instead of FP64.  record_budget_bytes: device memory for the
+        forward's record of the trailing checkpoint segments (0: half the
+        free device memory; 1: the last segment only)."""
         w = np.ascontiguousarray(weight, dtype=np.float64).reshape(-1)
         d = abi.AdjointDesc()
         d.struct_size = ctypes.sizeof(abi.AdjointDesc)
@@ -326,6 +329,7 @@ class FdmBackend:
         d.objective_weight_len = w.size
         d.checkpoint_interval = checkpoint_interval
         d.flags = abi.ADJOINT_BETA_STATE_PRECISION if beta_state_precision else 0
+        d.record_budget_bytes = record_budget_bytes
         j = ctypes.c_double(0.0)
         self._check(self.lib.fullmag_fdm_adjoint_forward(self.h, ctypes.byref(d), ctypes.byref(j)))
         return j.value

@@ -133,6 +133,12 @@ struct fullmag_fdm_backend {
         double *grad_ms = nullptr, *grad_amp = nullptr, *obj_part = nullptr, *tn = nullptr;
         long long *step = nullptr;
         uint64_t steps = 0, seg = 0, ncp = 0, first = 0;
+        uint64_t krec = 1;      // trailing segments the forward records
+        // record ring: rec_seg = krec*seg + 1 step slots (step n in slot n mod
+        // rec_seg; a step's end state starts the next slot), so any window of
+        // krec segments and its end state, or one segment and its end state,
+        // has a slot per step
+        uint64_t rec_seg = 0;
         uint32_t comp = 0;
         double dt = 0.0, t0 = 0.0, obj_scale = 0.0;
         long long blocks = 0;
@@ -140,7 +146,7 @@ struct fullmag_fdm_backend {
         fm::Ctl ctl_end{};
         bool ready = false;
         bool beta64 = false;     // FP32 system: field cotangent convolved in the FP64 context f64
-        bool last_rec = false;   // the forward recorded the last segment (no recompute of it)
+        bool last_rec = false;   // the forward's record of the last krec segments is intact
     } adj;
     // FP64 reduced Newell spectrum of an FP32 demag plan (kept up to 64 MB),
     // and the FP64 demag context built from it on the first adjoint run
@@ -2153,7 +2159,7 @@ fm::AdjArgs<T> adj_args(Backend *h) {
     J.rec_h0 = static_cast<const T *>(D.rec_h);
     J.rec_vn0 = static_cast<const T *>(D.rec_vn);
     J.step = D.step;
-    J.seg = (long long)D.seg;
+    J.seg = (long long)D.rec_seg;
     J.sv = (long long)(ns - 1) * 3 * (long long)h->n;
     J.ss = (long long)(ns - 1) * (long long)h->n;
     J.tn = D.tn;
@@ -2183,7 +2189,7 @@ fm::Args<T> adj_record_args(Backend *h, const fm::Args<T> &A) {
     B.rec_h = static_cast<T *>(D.rec_h);
     B.rec_vn = static_cast<T *>(D.rec_vn);
     B.rec_step = D.step;
-    B.rec_seg = (long long)D.seg;
+    B.rec_seg = (long long)D.rec_seg;
     B.rec_sv = (long long)(ns - 1) * 3 * (long long)h->n;
     B.rec_ss = (long long)(ns - 1) * (long long)h->n;
     return B;
@@ -2283,6 +2289,23 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
     D.seg = d->checkpoint_interval ? d->checkpoint_interval : (uint64_t)std::ceil(std::sqrt((double)steps));
     D.seg = std::min<uint64_t>(std::max<uint64_t>(D.seg, 1), steps);
     D.ncp = (steps + D.seg - 1) / D.seg;
+    // trailing segments recorded by the forward itself (no recompute in the
+    // backward): as many as the record budget holds, at least the last one
+    {
+        const uint64_t per_step = (uint64_t)(ns - 1) * (2 * 3 * n + n) * sizeof(T);
+        uint64_t budget = 0;
+        const bool has_budget = d->struct_size >= offsetof(fullmag_fdm_adjoint_desc, record_budget_bytes) +
+                                                      sizeof(d->record_budget_bytes);
+        if (has_budget && d->record_budget_bytes) {
+            budget = d->record_budget_bytes;
+        } else {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess) budget = fr / 2;
+        }
+        const uint64_t seg_bytes = std::max<uint64_t>(per_step * D.seg, 1);
+        D.krec = std::min<uint64_t>(std::max<uint64_t>(budget / seg_bytes, 1), D.ncp);
+        D.rec_seg = D.krec * D.seg + 1;
+    }
     D.blocks = fm::adj_blocks(A);
     // FP32 systems convolve the field cotangent in FP64 unless the caller
     // asks for the state precision (or no FP64 context can be built)
@@ -2297,7 +2320,7 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
             if (D.beta64) D.blocks = std::max<long long>(D.blocks, fm::num_partials(h->f64));
         }
     }
-    const size_t v3 = 3 * n * sizeof(T), slots = (size_t)D.seg * (ns - 1) + 1;
+    const size_t v3 = 3 * n * sizeof(T), slots = (size_t)D.rec_seg * (ns - 1);
     D.cp = adj_alloc(h, D.ncp * v3);
     D.rec_y = adj_alloc(h, slots * v3);
     D.rec_h = adj_alloc(h, slots * v3);
@@ -2353,27 +2376,29 @@ int adj_forward_typed(Backend *h, fm::Args<T> &A, const fullmag_fdm_adjoint_desc
         return hip_ok(h, fm::enqueue_trial(A, ns, h->stream), "adjoint forward step") &&
                hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective");
     };
-    // the last segment runs in recording form (the arguments of the
-    // backward's recompute), so that the backward starts with its record
-    // instead of recomputing it; with checkpoint_interval = steps the whole
-    // run is recorded and nothing is recomputed
+    // the last krec segments run in recording form (the arguments of the
+    // backward's recompute, record ring of krec*seg steps), so that the
+    // backward reverses them from this record instead of recomputing them;
+    // when the budget holds every segment nothing is recomputed
     fm::Args<T> B = adj_record_args(h, A);
     if (!put_args(h, B, 2)) return FULLMAG_FDM_ERR_CUDA;
     auto rec_step = [&]() {
         return hip_ok(h, fm::enqueue_trial(B, ns, h->stream), "adjoint forward step (recording)") &&
                hip_ok(h, fm::enqueue_adj_objective(A, J, h->stream), "adjoint objective");
     };
-    const uint64_t n_last = (D.ncp - 1) * D.seg;
+    const uint64_t n_last = (D.ncp - D.krec) * D.seg;
+    const size_t sv_bytes = (size_t)(ns - 1) * v3;  // one step slot of rec_y
     hipGraphExec_t g = capture_graph(h, one_step);
     hipGraphExec_t g_rec = capture_graph(h, rec_step);
     int par = D.parity0;
     bool ok = true;
     for (uint64_t s = 1; s <= steps && ok; ++s) {
         if (s - 1 == n_last) {
-            // entering the last segment at state n_last: the record starts
-            // with that state and its k1, as the backward's restart does
+            // entering the recorded segments at state n_last: the record
+            // starts with that state and its k1, as the backward's restart does
+            char *slot0 = static_cast<char *>(D.rec_y) + (size_t)(n_last % D.rec_seg) * sv_bytes;
             ok = hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n_last, 0, h->stream), "adjoint step") &&
-                 hip_ok(h, hipMemcpyAsync(D.rec_y, A.m_all + (size_t)par * n3, v3, hipMemcpyDeviceToDevice, h->stream),
+                 hip_ok(h, hipMemcpyAsync(slot0, A.m_all + (size_t)par * n3, v3, hipMemcpyDeviceToDevice, h->stream),
                         "adjoint record start") &&
                  hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint forward (k1 record)");
             if (!ok) break;
@@ -2449,7 +2474,7 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
             ok = false;
             break;
         }
-        if (k == D.ncp - 1 && D.last_rec) {
+        if (k + D.krec >= D.ncp && D.last_rec) {
             // recorded by the forward run: reverse it directly
             ok = hip_ok(h, hipMemsetAsync(&h->ctl_d->done, 0, sizeof(int), h->stream), "adjoint loop flag") &&
                  hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n1 - 1, 0, h->stream), "adjoint step");
@@ -2468,8 +2493,9 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
         c.dt = D.dt;
         c.done = 0;
         const char *cpk = static_cast<const char *>(D.cp) + k * v3;
+        char *slot0 = static_cast<char *>(D.rec_y) + (size_t)(n0 % D.rec_seg) * (size_t)(ns - 1) * v3;
         ok = hip_ok(h, hipMemcpyAsync(A.m_all, cpk, v3, hipMemcpyDeviceToDevice, h->stream), "adjoint restart") &&
-             hip_ok(h, hipMemcpyAsync(D.rec_y, cpk, v3, hipMemcpyDeviceToDevice, h->stream), "adjoint restart") &&
+             hip_ok(h, hipMemcpyAsync(slot0, cpk, v3, hipMemcpyDeviceToDevice, h->stream), "adjoint restart") &&
              push_ctl(h) && hip_ok(h, fm::enqueue_adj_step(D.step, (long long)n0, 0, h->stream), "adjoint step") &&
              hip_ok(h, fm::enqueue_eval(B, 1, 0, h->stream), "adjoint recompute (k1)");
         for (uint64_t i = n0; i < n1 && ok; ++i) ok = run(g_rec, rec_step);
@@ -2500,8 +2526,8 @@ int adj_backward_typed(Backend *h, fm::Args<T> &A, double *grad_ms, double *grad
                 "adjoint final state") ||
         !push_ctl(h) || !eval_current(h, false))
         return FULLMAG_FDM_ERR_CUDA;
-    // the recomputed segments reused the record slots of the last one
-    if (D.ncp > 1) D.last_rec = false;
+    // the recomputed segments reused the record slots of the recorded ones
+    if (D.ncp > D.krec) D.last_rec = false;
     if (grad_ms)
         for (uint64_t i = 0; i < n; ++i) grad_ms[i] = gms[i];
     if (grad_amp) {
@@ -2519,7 +2545,7 @@ extern "C" {
 int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *h, const fullmag_fdm_adjoint_desc *d, double *out_objective) {
     if (!h || !d) return FULLMAG_FDM_ERR_INVALID;
     if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
-    if (d->struct_size < sizeof(fullmag_fdm_adjoint_desc))
+    if (d->struct_size < offsetof(fullmag_fdm_adjoint_desc, record_budget_bytes))
         return fail(h, FULLMAG_FDM_ERR_INVALID, "fullmag_fdm_adjoint_desc.struct_size too small");
     if (h->adaptive || h->multistep)
         return fail(h, FULLMAG_FDM_ERR_INVALID, "adjoint needs a one-step fixed-step integrator (rk4 or heun)");

@@ -160,10 +160,12 @@ struct Args {
     // e_q k_q (FP64, unscaled), written by the final stage combination
     double *errv;
     // trajectory record for the adjoint sweep (written only when non-NULL):
-    // for the step in flight, slot i = (*rec_step mod rec_seg), stage state
-    // y_s at rec_y[i*rec_sv + (s-1)*3N], effective field at y_s at
-    // rec_h[i*rec_sv + (s-1)*3N], pre-normalisation norm |v_s| at
-    // rec_vn[i*rec_ss + (s-2)*N]; the controller advances *rec_step
+    // a ring of rec_seg step slots; for the step in flight, slot i =
+    // (*rec_step mod rec_seg), stage state y_s at rec_y[i*rec_sv +
+    // (s-1)*3N], effective field at y_s at rec_h[i*rec_sv + (s-1)*3N],
+    // pre-normalisation norm |v_s| at rec_vn[i*rec_ss + (s-2)*N]; the final
+    // stage's state and field (the next step's stage 1) start slot
+    // (i + 1) mod rec_seg; the controller advances *rec_step
     T *rec_y, *rec_h, *rec_vn;
     long long *rec_step;
     long long rec_seg, rec_sv, rec_ss;

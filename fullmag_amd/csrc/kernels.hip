@@ -394,6 +394,27 @@ __device__ __forceinline__ size_t rec_slot(const Args<T> &A, long long stride) {
     return A.rec_step ? (size_t)((*A.rec_step) % A.rec_seg) * (size_t)stride : 0;
 }
 
+// Record offset of stage s's state y_s of the step in flight (s >= 2).  The
+// final stage's state is the next step's y_1: it goes to the next slot of
+// the ring of rec_seg step slots (wrapping at the ring's end), where that
+// step's record starts.
+template <class T>
+__device__ __forceinline__ size_t rec_y_off(const Args<T> &A, const Method &M, int s) {
+    const long long n = *A.rec_step;
+    if (s == M.nstage) return (size_t)((n + 1) % A.rec_seg) * (size_t)A.rec_sv;
+    return (size_t)(n % A.rec_seg) * (size_t)A.rec_sv + (size_t)(s - 1) * 3 * (size_t)A.ncell;
+}
+
+// Record of the field at stage s of the step in flight (NULL when not
+// recording); the final stage's field is the next step's stage-1 field.
+template <class T>
+__device__ __forceinline__ gp<T> rec_h_ptr(const Args<T> &A, const Method &M, int s) {
+    if (!A.rec_h) return nullptr;
+    const long long n = *A.rec_step;
+    if (s == M.nstage) return G(A.rec_h) + (size_t)((n + 1) % A.rec_seg) * (size_t)A.rec_sv;
+    return G(A.rec_h) + (size_t)(n % A.rec_seg) * (size_t)A.rec_sv + (size_t)(s - 1) * 3 * (size_t)A.ncell;
+}
+
 template <class T>
 __device__ __forceinline__ gp<T> stage_buf(const Args<T> &A, int s) {
     return G(A.ybuf) + (size_t)(s & 1) * 3 * (size_t)A.ncell;
@@ -474,7 +495,7 @@ __device__ __forceinline__ void stage_state(const Args<T> &A, const Method &M, i
     out[N + cell] = (T)y;
     out[2 * N + cell] = (T)z;
     if (A.rec_y) {
-        T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
+        T *ry = A.rec_y + rec_y_off(A, M, s);
         ry[cell] = (T)x;
         ry[N + cell] = (T)y;
         ry[2 * N + cell] = (T)z;
@@ -1388,7 +1409,7 @@ __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, in
                 yout[N + cell] = v[1];
                 yout[2 * N + cell] = v[2];
                 if (A.rec_y) {
-                    T *ry = A.rec_y + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * N;
+                    T *ry = A.rec_y + rec_y_off(A, M, s);
                     ry[cell] = v[0];
                     ry[N + cell] = v[1];
                     ry[2 * N + cell] = v[2];
@@ -1550,7 +1571,7 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
     const float rnx = 1.0f / (float)nx;
     const gp<T> kout = stage_rhs(A, M, s, p);
     const gp<T> kpush = stage_push(A, M, s);
-    const gp<T> hrec = A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
+    const gp<T> hrec = rec_h_ptr(A, M, s);
     const T scale = T(A.inv_p);
     const int sb = FAST ? A.xsb : px;
 #pragma unroll
@@ -1866,7 +1887,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     {
         Acc acc;
         const gp<T> kout = stage_rhs(A, M, s, p);
-        const gp<T> hrec = A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr;
+        const gp<T> hrec = rec_h_ptr(A, M, s);
         const T scale = T(A.inv_p);
         constexpr int NCL = kXpfRows * NX;
         for (int base = 0; base < NCL; base += CH * kThreads) {
@@ -2627,7 +2648,8 @@ __global__ void __launch_bounds__(kThreads) k_adj_init(const Args<T> *__restrict
     const AdjArgs<T> J = adj_bind(J0);
     const bool with_obj = *J0.step + 1 >= J0.first;
     const Method &M = c_methods[A.method];
-    const T *y = J.rec_y + (size_t)(nstage - 1) * 3 * N;
+    // the step's end state m_{n+1} starts the next slot of the record ring
+    const T *y = J.rec_y0 + (size_t)((*J0.step + 1) % J0.seg) * (size_t)J0.sv;
     const double m[3] = {(double)y[cell], (double)y[N + cell], (double)y[2 * N + cell]};
     double lam[3] = {J.cot_m[cell], J.cot_m[N + cell], J.cot_m[2 * N + cell]};
     if (with_obj) lam[J.obj_comp] += 2.0 * J.obj_scale * (double)J.obj_w[cell] * m[J.obj_comp];
@@ -2821,7 +2843,7 @@ __global__ void __launch_bounds__(kThreads) k_local(const Args<T> *__restrict__ 
         CellIn<T> g;
         gather_cell<T, FT>(A, M, p, (int)cell, x, y, z, stage_input(A, M, s, p), err, g);
         compute_cell<T, FT>(A, M, (int)cell, dt, g, stage_rhs(A, M, s, p), hd, tf, stats != 0, err, acc,
-                             A.rec_h ? G(A.rec_h) + rec_slot(A, A.rec_sv) + (size_t)(s - 1) * 3 * A.ncell : nullptr,
+                             rec_h_ptr(A, M, s),
                              nullptr, stage_push(A, M, s));
     }
     if (stats || err) block_reduce_global(acc, A.partials + (long long)blockIdx.x * kStatSlots);

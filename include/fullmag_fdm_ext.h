@@ -141,18 +141,19 @@ int fullmag_fdm_random_unit_vectors(uint64_t seed, uint64_t count, double *out);
  *
  * adjoint_forward runs `steps` steps of size dt from the current state and
  * time, keeping the state every checkpoint_interval steps (0: ceil(sqrt(
- * steps))), and returns J; the last segment runs in recording form (its
- * stage states and fields are kept).  adjoint_backward then sweeps the
- * steps in reverse (the last segment from the forward's record, every
- * earlier segment recomputed once from its checkpoint with its stage states
- * and fields recorded; checkpoint_interval = steps records the whole run
- * and recomputes nothing) and returns dJ/dMs_i for every cell
+ * steps))), and returns J; the trailing segments that fit the record budget
+ * (at least the last one) run in recording form (their stage states and
+ * fields are kept).  adjoint_backward then sweeps the steps in reverse (the
+ * recorded segments from the forward's record, every earlier segment
+ * recomputed once from its checkpoint with its stage states and fields
+ * recorded; when the budget holds the whole run nothing is recomputed) and
+ * returns dJ/dMs_i for every cell
  * (grad_ms, n values, 0 on inactive cells) and dJ/d field_amplitude_am of
  * the plan's waveform (grad_field_amplitude, 3 values).  Either output may
  * be NULL.  After either call the handle holds the final state m_steps at
  * t0 + steps*dt.  Device memory: 3N state values per checkpoint, 7 (nstage-1) N
- * per step of one segment for the record (stage states, fields and
- * pre-normalisation norms), and 27N FP64 cotangents.  FP32 plans with
+ * per recorded step (stage states, fields and pre-normalisation norms), and
+ * 27N FP64 cotangents.  FP32 plans with
  * demag convolve the field cotangent with an FP64 copy of the demag tensor
  * and transforms (an FP64 spectrum of the grid, created on the first
  * adjoint run) unless flags has FULLMAG_FDM_ADJOINT_BETA_STATE_PRECISION:
@@ -170,6 +171,11 @@ typedef struct fullmag_fdm_adjoint_desc {
     const double *objective_weight;   /* n per-cell weights */
     uint64_t objective_weight_len;
     uint64_t checkpoint_interval;     /* 0 = ceil(sqrt(steps)) */
+    /* device memory for the forward's record of the trailing segments
+       (0: half the free device memory); a budget below one segment's
+       record keeps only the last segment.  Absent (struct_size without
+       it): 0. */
+    uint64_t record_budget_bytes;
 } fullmag_fdm_adjoint_desc;
 
 int fullmag_fdm_adjoint_forward(fullmag_fdm_backend *handle, const fullmag_fdm_adjoint_desc *desc,

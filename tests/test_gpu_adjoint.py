@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 MU0 = 4e-7 * np.pi
 
 
-def _problem(precision="double", integrator="rk4", demag=True, nz=2, steps=160, interval=0):
+def _problem(precision="double", integrator="rk4", demag=True, nz=2, steps=160, interval=0, budget=0):
     from fullmag_amd.adjoint import WaveguideProblem
     rng = np.random.default_rng(5)
     nx, ny = 24, 8
@@ -25,7 +25,7 @@ def _problem(precision="double", integrator="rk4", demag=True, nz=2, steps=160, 
     m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
     return WaveguideProblem(nx=nx, ny=ny, nz=nz, design_x=(8, 16), design_y=(0, 8), pulse_cols=3, probe_cols=3,
                             t0=4e-12, dt=50e-15, steps=steps, window=4e-12, precision=precision,
-                            integrator=integrator, checkpoint_interval=interval, m0=m0,
+                            integrator=integrator, checkpoint_interval=interval, record_budget_bytes=budget, m0=m0,
                             extra={} if demag else {"demag": False})
 
 
@@ -106,11 +106,17 @@ def test_fp64_adjoint_matches_oracle_central_differences(integrator, demag):
     assert err.max() <= 1.0
 
 
+# record budgets: 1 byte keeps only the last segment (every other segment is
+# recomputed), 3 segments of 7 steps (86 KB per FP64 step of this problem),
+# and the default (the whole run)
+BUDGETS = (1, 3 * 7 * 4 * 7 * 384 * 8, 0)
+
+
 def test_checkpoint_interval_does_not_change_the_gradient():
     d = None
     ref = None
-    for interval in (0, 1, 7, 160):
-        pr = _problem(interval=interval)
+    for interval, budget in [(i, b) for i in (0, 1, 7, 160) for b in BUDGETS]:
+        pr = _problem(interval=interval, budget=budget)
         if d is None:
             d = _design0(pr)
         j, gd, ga = pr.value_and_grad(d, 5e-3)
@@ -187,11 +193,12 @@ def test_fp64_field_cotangent_for_fp32_plans():
     assert abs(ga_a - ga_b) <= 1e-4 * abs(ga64)
 
 
-@pytest.mark.parametrize("precision,interval", [("double", 0), ("double", 160), ("single", 0), ("single", 40)])
-def test_repeated_backward_and_recorded_last_segment(precision, interval):
-    # the forward records the last segment; a second backward after the
+@pytest.mark.parametrize("precision,interval,budget", [("double", 0, 1), ("double", 7, BUDGETS[1]), ("double", 160, 1),
+                                                      ("double", 0, 0), ("single", 0, 1), ("single", 40, 1)])
+def test_repeated_backward_and_recorded_last_segment(precision, interval, budget):
+    # the forward records the trailing segments; a second backward after the
     # first (whose recomputes reuse those record slots) gives the same result
-    pr = _problem(precision, interval=interval)
+    pr = _problem(precision, interval=interval, budget=budget)
     d = _design0(pr)
     j, be = pr.run(d, 5e-3)
     try:
@@ -200,6 +207,6 @@ def test_repeated_backward_and_recorded_last_segment(precision, interval):
     finally:
         be.close()
     assert np.array_equal(g1, g2) and np.array_equal(a1, a2)
-    j_ref, gd_ref, ga_ref = _problem(precision, interval=1).value_and_grad(d, 5e-3)
+    j_ref, gd_ref, ga_ref = _problem(precision, interval=1, budget=1).value_and_grad(d, 5e-3)
     assert j == j_ref
     assert np.abs(pr.design_grad(g1) - gd_ref).max() <= 1e-12 * np.abs(gd_ref).max()
