@@ -235,6 +235,7 @@ std::vector<double> cached_newell(int nx, int ny, int nz, double dx, double dy, 
 struct TuneEntry {
     std::vector<int> key;
     int rows, xfast;
+    int yl = 0, zf = 0;  // 3-D y-line threads / z-pass ky per work-group (0: default)
 };
 std::mutex g_tune_mu;
 std::vector<TuneEntry> g_tune;
@@ -248,26 +249,32 @@ int tune_feat(const fm::Args<T> &A) {
            (A.layered ? 32 : 0) | (A.pbc << 6) | (A.gtens ? 512 : 0);
 }
 
-void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int feat, int *rows, int *xfast) {
+void tuned_lookup(bool fp64, int nx, int ny, int nz, int method, int feat, int *rows, int *xfast, int *yl = nullptr,
+                  int *zf = nullptr) {
     const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method, feat};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const auto &e : g_tune)
         if (e.key == key) {
             *rows = e.rows;
             *xfast = e.xfast;
+            if (yl) *yl = e.yl;
+            if (zf) *zf = e.zf;
         }
 }
 
-void tuned_store(bool fp64, int nx, int ny, int nz, int method, int feat, int rows, int xfast) {
+void tuned_store(bool fp64, int nx, int ny, int nz, int method, int feat, int rows, int xfast, int yl = 0,
+                 int zf = 0) {
     const std::vector<int> key = {fp64 ? 1 : 0, nx, ny, nz, method, feat};
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (auto &e : g_tune)
         if (e.key == key) {
             e.rows = rows;
             e.xfast = xfast;
+            e.yl = yl;
+            e.zf = zf;
             return;
         }
-    g_tune.push_back({key, rows, xfast});
+    g_tune.push_back({key, rows, xfast, yl, zf});
 }
 
 // Publishes B as the kernel arguments in device slot `slot` (the kernels
@@ -1160,8 +1167,19 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device) != hipSuccess || lds <= 0)
             lds = 65536;
         h->lds_limit = (size_t)lds - 1024;
-        int rows_req = 0, xfast_req = -1;
-        tuned_lookup(h->fp64, nx, ny, nz, A.method, tune_feat(A), &rows_req, &xfast_req);
+        int rows_req = 0, xfast_req = -1, yl_req = 0, zf_req = 0;
+        tuned_lookup(h->fp64, nx, ny, nz, A.method, tune_feat(A), &rows_req, &xfast_req, &yl_req, &zf_req);
+        A.yl_threads = yl_req;
+        A.zf_ub = zf_req;
+        // FULLMAG_FDM_YZ_TILE="threads,ky" forces the 3-D y-line and z-pass
+        // tiles (invalid values fall back to the defaults in the launches)
+        if (const char *yz = std::getenv("FULLMAG_FDM_YZ_TILE")) {
+            int y = 0, z = 0;
+            if (std::sscanf(yz, "%d,%d", &y, &z) == 2) {
+                A.yl_threads = y;
+                A.zf_ub = z;
+            }
+        }
         const char *ce = fm::configure(A, h->lds_limit, rows_req, xfast_req);
         if (rows_req && ce && *ce) ce = fm::configure(A, h->lds_limit);
         if (ce && *ce) {
@@ -2061,6 +2079,65 @@ double time_x_passes(Backend *h, const fm::Args<T> &A, int reps, hipStream_t st)
     return total;
 }
 
+// Mean time of the 3-D y/z passes of one stage (forward y-lines, z pass,
+// inverse y-lines), reps back-to-back launches of each on stream st.
+template <class T>
+double time_yz_passes(const fm::Args<T> &A, int reps, hipStream_t st) {
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double total = 0.0;
+    for (int kind = 1; kind <= 3; ++kind) {
+        if (fm::enqueue_kind(A, kind, 2, st) != hipSuccess) continue;
+        hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) fm::enqueue_kind(A, kind, 2, st);
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        total += ms / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return total;
+}
+
+// Chooses the 3-D y-line tile (threads per work-group: 128 or 256) and the
+// z-pass tile (ky per work-group: the full, half or quarter count) by timing
+// `apply`'d candidates with `time` (seconds); leaves the winner applied.
+// Films and the Stockham y/z forms have no such tiles (false: nothing done).
+template <class T, class Apply, class Time>
+bool tune_yz(const fm::Args<T> &A0, Apply &&apply, Time &&time, int *yl_out, int *zf_out) {
+    if (A0.nz == 1 || (!A0.ylfast && !A0.zfast)) return false;
+    int zfull = 0;
+    {
+        int r1 = 0, r2 = 0;
+        if (A0.zfast && fm::zfast_factors(A0.pz, &r1, &r2)) zfull = 256 / std::max(r1, r2);
+    }
+    double best = std::numeric_limits<double>::infinity();
+    int best_yl = 0, best_zf = 0;
+    for (int yl : {256, 128}) {
+        if (!A0.ylfast && yl != 256) continue;
+        for (int div : {1, 2, 4}) {
+            const int zf = zfull ? zfull / div : 0;
+            if (zfull && zf < 1) continue;
+            if (!zfull && div != 1) continue;
+            if (!apply(yl, zf)) return false;
+            time(2);
+            const double t = time(5);
+            if (t < best) {
+                best = t;
+                best_yl = yl;
+                best_zf = zf;
+            }
+        }
+    }
+    if (!apply(best_yl, best_zf)) return false;
+    *yl_out = best_yl;
+    *zf_out = best_zf;
+    return true;
+}
+
 template <class T>
 int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out_fs) {
     if (!A.demag) {
@@ -2094,6 +2171,20 @@ int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out
     const char *ce = fm::configure(A, h->lds_limit, A.rows_per_wg, A.xfast ? 1 : 0);  // kernel LDS attributes
     if (ce && *ce) return fail(h, FULLMAG_FDM_ERR_INTERNAL, ce);
     if (!put_args(h, A, 0)) return FULLMAG_FDM_ERR_CUDA;
+    // 3-D: the y-line and z-pass tiles, timed on the y/z passes
+    int yl = A.yl_threads, zf = A.zf_ub;
+    {
+        auto apply = [&](int y, int z) {
+            A.yl_threads = y;
+            A.zf_ub = z;
+            return put_args(h, A, 0);
+        };
+        auto time = [&](int r) { return time_yz_passes(A, r, h->stream); };
+        if (!tune_yz(A, apply, time, &yl, &zf) && !h->err.empty()) return FULLMAG_FDM_ERR_CUDA;
+        A.yl_threads = yl;
+        A.zf_ub = zf;
+        if (!put_args(h, A, 0)) return FULLMAG_FDM_ERR_CUDA;
+    }
     // graphs were captured with the old launch geometry
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->graph_chunk) hipGraphExecDestroy(h->graph_chunk);
@@ -2103,7 +2194,8 @@ int autotune_typed(Backend *h, fm::Args<T> &A, int reps, int *out_rows, int *out
     h->graph_tried = false;
     *h->ctl_h = saved;
     if (!push_ctl(h) || !eval_current(h, false)) return FULLMAG_FDM_ERR_CUDA;
-    tuned_store(h->fp64, (int)h->nx, (int)h->ny, (int)h->nz, A.method, tune_feat(A), A.rows_per_wg, A.xfast ? 1 : 0);
+    tuned_store(h->fp64, (int)h->nx, (int)h->ny, (int)h->nz, A.method, tune_feat(A), A.rows_per_wg, A.xfast ? 1 : 0,
+                A.yl_threads, A.zf_ub);
     if (out_rows) *out_rows = A.rows_per_wg;
     if (out_fs) *out_fs = A.xfast ? 1 : 0;
     return FULLMAG_FDM_OK;
@@ -2613,7 +2705,7 @@ std::vector<long long> group_key(const fm::Args<T> &A, bool fp64) {
     // must agree as well
     return {fp64, A.nx, A.ny, A.nz, A.demag, A.method, A.region != nullptr, A.rows_per_wg, A.xfast, A.yfast,
             A.ylfast, A.zfast, A.ty, A.nseq_x, A.xt, A.has_th, A.lmix != nullptr, A.layered, A.pbc,
-            A.gtens != nullptr, A.xpf};
+            A.gtens != nullptr, A.xpf, A.yl_threads, A.zf_ub};
 }
 
 template <class T>
@@ -2693,10 +2785,27 @@ bool group_tune(Batch *b, Batch::Group &g) {
             }
         }
     if (!apply(best_rows, best_fs)) return (b->err = "batch: tile configuration failed", false);
-    // k1 of the initial states with the chosen tile (as a create with it gives)
+    // 3-D: the y-line and z-pass tiles, timed on the group's y/z launches
+    int yl = A0.yl_threads, zf = A0.zf_ub;
+    {
+        auto apply_yz = [&](int y, int z) {
+            for (size_t m : g.members) {
+                Backend *h = b->sys[m];
+                fm::Args<T> &A = *reinterpret_cast<fm::Args<T> *>(h->fp64 ? (void *)&h->ad : (void *)&h->af);
+                A.yl_threads = y;
+                A.zf_ub = z;
+                if (!put_args(h, A, 0)) return false;
+            }
+            return group_prepare<T>(b, g, false);
+        };
+        auto time = [&](int r) { return time_yz_passes(group_args<T>(g), r, g.stream); };
+        tune_yz(A0, apply_yz, time, &yl, &zf);
+        if (!apply_yz(yl, zf)) return (b->err = "batch: tile configuration failed", false);
+    }
+    // k1 of the initial states with the chosen tiles (as a create with them gives)
     for (size_t m : g.members)
         if (!eval_current(b->sys[m], false)) return (b->err = "batch: evaluation failed", false);
-    tuned_store(h0->fp64, (int)h0->nx, (int)h0->ny, (int)h0->nz, A0.method, tune_feat(A0), best_rows, best_fs);
+    tuned_store(h0->fp64, (int)h0->nx, (int)h0->ny, (int)h0->nz, A0.method, tune_feat(A0), best_rows, best_fs, yl, zf);
     return true;
 }
 

@@ -145,6 +145,8 @@ struct Args {
     int xpf;                // 1 = film joined x pass with its inputs prefetched into LDS (k_xxp)
     int xpf_off;            // byte offset of the prefetch area in the x passes' dynamic LDS
     int ylfast;             // 3-D: 0 = Stockham y passes, else py of the register four-step y-line passes
+    int yl_threads;         // 3-D four-step y-line passes: threads per work-group (lines = yl_threads / max(R1,R2))
+    int zf_ub;              // 3-D four-step z pass: consecutive ky per work-group
     const void *tw4z;       // four-step twiddles of the 3-D z pass (W_pz^{t1 k2} at [k2*R1 + t1])
     int zfast;              // 3-D: 0 = Stockham z pass, else pz of the register four-step z pass
     const uint8_t *mask;    // NULL = all active

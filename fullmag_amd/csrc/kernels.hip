@@ -2271,7 +2271,9 @@ template <class T, int R1, int R2, bool INV>
 __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
     if (yz_exit_check(A) && ctl_ro(A)->done) return;
-    constexpr int TS = R1 > R2 ? R1 : R2, LPB = kThreads / TS;
+    // lines per work-group from the launch (Args::yl_threads, tuned)
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    const int LPB = (int)blockDim.x / TS;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
     const int nlines = 3 * A.nzs * A.nkx;
@@ -2349,18 +2351,20 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
     const int done = yz_exit_check(A) ? ctl_ro(A)->done : 0;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     const int nz = A.nz, py = A.py, ny = A.ny, ly = A.ly, nkx = A.nkx, nzs = A.nzs;
-    const int ntile = (py + UB - 1) / UB;
+    // ky per work-group from the launch (Args::zf_ub, tuned; at most UB)
+    const int ub = (int)blockDim.x / (R1 > R2 ? R1 : R2);
+    const int ntile = (py + ub - 1) / ub;
     // (pairing tiles j and ntile-1-j, which read the same tensor records,
     // was measured slower than consecutive tiles)
-    const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x - kx * ntile) * UB;
-    const int u = threadIdx.x % UB, t = threadIdx.x / UB;
+    const int kx = blockIdx.x / ntile, ky0 = (blockIdx.x - kx * ntile) * ub;
+    const int u = threadIdx.x % ub, t = threadIdx.x / ub;
     const int ky = ky0 + u;
     const bool on = ky < py;
     const size_t zs = (size_t)nkx * ly;  // stride between z planes (complex elements)
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + (size_t)kx * ly + (on ? ky : py - 1);
     const C2<T> *tw4 = reinterpret_cast<const C2<T> *>(A.tw4z);
     C2<T> *tb = reinterpret_cast<C2<T> *>(fm_smem);
-    auto L = [&](int i) -> C2<T> & { return tb[i * UB + u]; };
+    auto L = [&](int i) -> C2<T> & { return tb[i * ub + u]; };
     // every z-point load of the three components is issued first; thread
     // t < R1 holds z = t + R1 n2 (n2 < R2/2 non-zero)
     C2<T> raw[3][R2 / 2];
@@ -3122,18 +3126,31 @@ constexpr int zf_units() {
     return kThreads / (R1 > R2 ? R1 : R2);
 }
 
+// Tile sizes of the 3-D y-line and z passes: the tuned value when it is a
+// valid tile of the shape (a multiple of the line's threads, at most the
+// compiled maximum), else the default (kThreads threads, zf_units ky).
+template <int R1, int R2>
+int yl_threads_of(int req) {
+    constexpr int TS = R1 > R2 ? R1 : R2;
+    return (req >= TS && req <= kThreads && req % TS == 0 && req % 64 == 0) ? req : kThreads;
+}
+template <int R1, int R2>
+int zf_ub_of(int req) {
+    return (req >= 1 && req <= zf_units<R1, R2>()) ? req : zf_units<R1, R2>();
+}
+
 template <class T, int R1, int R2>
-static size_t lds_yl() {
+static size_t lds_yl(int threads = kThreads) {
     constexpr int TS = R1 > R2 ? R1 : R2;
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
-    return (size_t)(kThreads / TS) * TB * sizeof(C2<T>);
+    return (size_t)(threads / TS) * TB * sizeof(C2<T>);
 }
 template <class T, int R1, int R2>
-static size_t lds_zf() {
+static size_t lds_zf(int ub = zf_units<R1, R2>()) {
     constexpr int P1 = R1 + 1, P2 = R2 + 1;
     constexpr int TB = (R2 * P1 > R1 * P2 ? R2 * P1 : R1 * P2);
-    return 1ull * TB * zf_units<R1, R2>() * sizeof(C2<T>);
+    return 1ull * TB * ub * sizeof(C2<T>);
 }
 
 template <class T>
@@ -3492,9 +3509,10 @@ static void launch_inv_x(const Args<T> &A, int s, int stats, hipStream_t st) {
 template <class T, int R1, int R2>
 static void launch_yl_shape(const Args<T> &A, bool inv, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
-    const unsigned g = (unsigned)((3ll * A.nzs * A.nkx + kThreads / TS - 1) / (kThreads / TS));
-    if (inv) k_yl<T, R1, R2, true><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
-    else k_yl<T, R2, R1, false><<<bgrid(A, dim3(g)), dim3(kThreads), lds_yl<T, R1, R2>(), st>>>(A.self_d);
+    const int nt = yl_threads_of<R1, R2>(A.yl_threads), lpb = nt / TS;
+    const unsigned g = (unsigned)((3ll * A.nzs * A.nkx + lpb - 1) / lpb);
+    if (inv) k_yl<T, R1, R2, true><<<bgrid(A, dim3(g)), dim3(nt), lds_yl<T, R1, R2>(nt), st>>>(A.self_d);
+    else k_yl<T, R2, R1, false><<<bgrid(A, dim3(g)), dim3(nt), lds_yl<T, R1, R2>(nt), st>>>(A.self_d);
 }
 // 3-D y pass (forward: inv = false; inverse: inv = true).
 template <class T>
@@ -3515,8 +3533,9 @@ template <class T, int R1, int R2>
 static void launch_zf_shape(const Args<T> &A, hipStream_t st) {
     constexpr int TS = R1 > R2 ? R1 : R2;
     constexpr int UB = zf_units<R1, R2>();
-    const unsigned g = (unsigned)(A.nkx * ((A.py + UB - 1) / UB));
-    k_zf<T, R1, R2, UB><<<bgrid(A, dim3(g)), dim3(UB * TS), lds_zf<T, R1, R2>(), st>>>(A.self_d);
+    const int ub = zf_ub_of<R1, R2>(A.zf_ub);
+    const unsigned g = (unsigned)(A.nkx * ((A.py + ub - 1) / ub));
+    k_zf<T, R1, R2, UB><<<bgrid(A, dim3(g)), dim3(ub * TS), lds_zf<T, R1, R2>(ub), st>>>(A.self_d);
 }
 // 3-D z pass (forward FFT, tensor multiply, inverse FFT).
 template <class T>

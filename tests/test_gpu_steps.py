@@ -367,3 +367,32 @@ def test_state_and_history_survive_tuning_calls(integrator, call):
         d = float(np.abs(a.field("m") - b.field("m")).max())
         # the tuner may pick another tile shape (other FFT rounding)
         assert d <= (0.0 if call == "time_kernels" else 1e-13), d
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("tile", ["128,0", "256,8", "128,4", "256,1", "64,2"])
+def test_yz_tiles_give_identical_fields(precision, tile):
+    # the 3-D y-line tile (threads per work-group) and z-pass tile (ky per
+    # work-group) only regroup independent lines: the demag field and a step
+    # are bit-identical to the default tiles (FULLMAG_FDM_YZ_TILE forces them)
+    import os
+    from fullmag_amd.backend import FdmBackend
+    case = random_case(24, 32, 8, seed=3)
+    plan = case.plan(precision=precision)
+    out = []
+    for t in (None, tile):
+        old = os.environ.pop("FULLMAG_FDM_YZ_TILE", None)
+        if t:
+            os.environ["FULLMAG_FDM_YZ_TILE"] = t
+        try:
+            be = FdmBackend(plan)
+        finally:
+            os.environ.pop("FULLMAG_FDM_YZ_TILE", None)
+            if old is not None:
+                os.environ["FULLMAG_FDM_YZ_TILE"] = old
+        with be:
+            h = be.field("h_demag")
+            be.step(1e-13)
+            out.append((h, be.field("m")))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
