@@ -1300,9 +1300,8 @@ __device__ __forceinline__ void x_rows(const Args<T> &A, int r0, XRows &xr) {
 // into b0.  NK = k-vectors of the combination (0: stage 1, y = m0).  Cells
 // are taken in chunks of CH per thread (cell i = base + j*blockDim + tid);
 // the loads of a chunk are all issued before any is consumed.
-template <class T, int NK>
+template <class T, int NK, int CH = Chunk<T>::value>
 struct PackIn {
-    static constexpr int CH = Chunk<T>::value;
     T mv[CH][3], kv[CH][NK > 0 ? NK : 1][3], msv[CH];
 };
 
@@ -1340,11 +1339,10 @@ __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, i
 }
 
 // Combination, normalisation, store and packing of one loaded chunk.
-template <class T, int R1, int R2, int NK>
+template <class T, int R1, int R2, int NK, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_pack_put(const Args<T> &A, const Method &M, int s, int p, double dt,
-                                           const XRows &xr, int base, const PackIn<T, NK> &in, C2<T> *b0) {
+                                           const XRows &xr, int base, const PackIn<T, NK, CH> &in, C2<T> *b0) {
     constexpr bool FAST = R1 > 0;
-    constexpr int CH = Chunk<T>::value;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx;
     const float rnx = 1.0f / (float)nx;
@@ -1719,6 +1717,10 @@ __global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(co
 //   TW [2nx] complex  the four-step twiddles of the x FFTs
 //   NM [2][nx] bytes  activity bytes of the two rows (with a mask)
 constexpr int kXpfRows = 2;
+// threads of a k_xxp work-group: eight waves, two per SIMD of the CU that
+// holds the work-group (its LDS fits one work-group per CU), so that one
+// wave's LDS and memory waits overlap the other's work
+constexpr int kXpfThreads = 512;
 
 template <class T>
 __host__ __device__ constexpr size_t xpf_floats(int nx, int nk) {
@@ -1777,13 +1779,14 @@ __device__ __forceinline__ void gather_film_lds(const Args<T> &A, int cell, int 
 }
 
 template <class T, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP, int s) {
+__global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__ AP, int s) {
     static_assert(sizeof(T) == 4, "FP32 only");
-    constexpr int CH = Chunk<T>::value;
     constexpr int PX = R1 * R2, NX = PX / 2, NKX = NX + 1;
+    // cells per thread of the local pass and the combination
+    constexpr int CH = kXpfRows * NX > kXpfThreads ? kXpfRows * NX / kXpfThreads : 1;
     constexpr int NLP = 2 * ((3 * kXpfRows + 1) / 2);           // packed lines (lane pairs)
     constexpr int TOT = NLP * NKX;                              // spectrum values of the work-group
-    constexpr int NV = (TOT + kThreads - 1) / kThreads;         // per thread
+    constexpr int NV = (TOT + kXpfThreads - 1) / kXpfThreads;   // per thread
     const Args<T> &A = AP[blockIdx.y];
     const int N = (int)A.ncell, sb = A.xsb, ly = A.ly, nkx = A.nkx;
     C2<T> *b0 = reinterpret_cast<C2<T> *>(fm_smem);
@@ -1808,7 +1811,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     C2<T> v[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        const int i = j * kThreads + (int)threadIdx.x;
+        const int i = j * kXpfThreads + (int)threadIdx.x;
         const int ic = i < TOT ? i : TOT - 1;
         const int kk = small_div<false>(ic, NLP, rnl), l = ic - kk * NLP;
         const int rr = l / 3, c = l - 3 * rr;  // line l = 3 rr + c, as XRows::spec
@@ -1817,7 +1820,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
         // and the compiler then re-reads the arguments inside each branch
         v[j] = ldg(S + (c * nkx) * ly + y0 + rr + kk * ly);
     }
-    const int wv = threadIdx.x >> 6, nw = kThreads / 64;
+    const int wv = threadIdx.x >> 6, nw = kXpfThreads / 64;
     if (wv == nw - 1) dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
     if (A.nmask && wv == nw - 2) dma_bytes(A.nmask + c0, NM, kXpfRows * NX);
     // the control block through the scalar cache (this kernel does not
@@ -1866,7 +1869,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     // 3. Hermitian rebuild of the packed sequences, inverse FFT
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        const int i = j * kThreads + (int)threadIdx.x;
+        const int i = j * kXpfThreads + (int)threadIdx.x;
         const C2<T> b = {__shfl_xor(v[j].x, 1, 64), __shfl_xor(v[j].y, 1, 64)};
         if (i >= TOT || (threadIdx.x & 1)) continue;
         const int kk = small_div<false>(i, NLP, rnl), l = i - kk * NLP;
@@ -1890,18 +1893,18 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
         const gp<T> hrec = rec_h_ptr(A, M, s);
         const T scale = T(A.inv_p);
         constexpr int NCL = kXpfRows * NX;
-        for (int base = 0; base < NCL; base += CH * kThreads) {
+        for (int base = 0; base < NCL; base += CH * kXpfThreads) {
             CellIn<T> g[CH];
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
-                const int i = base + j * kThreads + (int)threadIdx.x;
+                const int i = base + j * kXpfThreads + (int)threadIdx.x;
                 const int ic = i < NCL ? i : NCL - 1;
                 const int rr = ic / NX, x = ic - rr * NX;
                 gather_film_lds(A, c0 + ic, x, rr, Y, NM, g[j]);
             }
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
-                const int i = base + j * kThreads + (int)threadIdx.x;
+                const int i = base + j * kXpfThreads + (int)threadIdx.x;
                 if (i >= NCL) continue;
                 const int rr = i / NX, x = i - rr * NX;
                 T hd[3];
@@ -1923,11 +1926,11 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
     // 6. combination of stage s+1 from LDS, Ms*y packed, forward FFT, store
     {
         constexpr int NCL = kXpfRows * NX;
-        for (int base = 0; base < NCL; base += CH * kThreads) {
-            PackIn<T, NK> in;
+        for (int base = 0; base < NCL; base += CH * kXpfThreads) {
+            PackIn<T, NK, CH> in;
 #pragma unroll
             for (int j = 0; j < CH; ++j) {
-                const int i = base + j * kThreads + (int)threadIdx.x;
+                const int i = base + j * kXpfThreads + (int)threadIdx.x;
                 const int ic = i < NCL ? i : NCL - 1;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) in.mv[j][c] = Mz[c * NCL + ic];
@@ -1937,7 +1940,7 @@ __global__ void __launch_bounds__(kThreads) k_xxp(const Args<T> *__restrict__ AP
                     for (int c = 0; c < 3; ++c) in.kv[j][q][c] = K[(q * 3 + c) * NCL + ic];
                 in.msv[j] = A.ms_pack ? G(A.ms_pack)[c0 + ic] : A.ms;
             }
-            x_pack_put<T, R1, R2, NK>(A, M, s + 1, p, dt, xr, base, in, b0);
+            x_pack_put<T, R1, R2, NK, CH>(A, M, s + 1, p, dt, xr, base, in, b0);
         }
         x_pack_tail<T, R1, R2>(A, b0);
     }
@@ -3452,7 +3455,7 @@ static void launch_xx_shape(const Args<T> &A, int s, hipStream_t st) {
 }
 template <class T, int R1, int R2>
 static void launch_xxp_shape(const Args<T> &A, int s, hipStream_t st) {
-    const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    const dim3 g((unsigned)num_partials(A)), b(kXpfThreads);
     const size_t l = (size_t)A.xpf_off + xpf_floats<T>(A.nx, kMaxK) * sizeof(T) + (size_t)kXpfRows * A.nx;
     switch (A.stage_nk[s + 1]) {
         case 1: k_xxp<T, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
