@@ -524,7 +524,7 @@ __device__ __forceinline__ bool slot_is_max(int i) {
 // space explicit: per-work-group partials go to global memory, the
 // controller's totals to LDS.
 __device__ double block_reduce_slot(Acc &a) {
-    __shared__ double red[kStatSlots][kThreads / 64];
+    __shared__ double red[kStatSlots][16];  // up to 1024 threads
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < kStatSlots; ++i) {
@@ -1307,11 +1307,10 @@ struct PackIn {
 
 // Issues the loads of one chunk; the k-vector at row position `skip` (-1:
 // none) is not loaded (the caller has it in registers).
-template <class T, int R1, int R2, int NK>
+template <class T, int R1, int R2, int NK, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_pack_load(const Args<T> &A, const Method &M, int s, int p, const XRows &xr,
-                                            int base, int skip, PackIn<T, NK> &in) {
+                                            int base, int skip, PackIn<T, NK, CH> &in) {
     constexpr bool FAST = R1 > 0;
-    constexpr int CH = Chunk<T>::value;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx;
     const float rnx = 1.0f / (float)nx;
@@ -1434,15 +1433,14 @@ __device__ __forceinline__ void x_pack_tail(const Args<T> &A, C2<T> *b0) {
         for (int x = threadIdx.x; x < nx; x += blockDim.x) b0[(nl >> 1) * sb + x].y = T(0);
 }
 
-template <class T, int R1, int R2, int NK>
+template <class T, int R1, int R2, int NK, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, int s, int p, double dt,
                                              const XRows &xr, C2<T> *b0) {
     const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
-    constexpr int CH = Chunk<T>::value;
     for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
-        PackIn<T, NK> in;
-        x_pack_load<T, R1, R2, NK>(A, M, s, p, xr, base, -1, in);
-        x_pack_put<T, R1, R2, NK>(A, M, s, p, dt, xr, base, in, b0);
+        PackIn<T, NK, CH> in;
+        x_pack_load<T, R1, R2, NK, CH>(A, M, s, p, xr, base, -1, in);
+        x_pack_put<T, R1, R2, NK, CH>(A, M, s, p, dt, xr, base, in, b0);
     }
     x_pack_tail<T, R1, R2>(A, b0);
 }
@@ -1540,11 +1538,10 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
 // inverse x FFT (1/P scale), exchange, Zeeman, anisotropy, LLG torque,
 // optional statistics and error norm.  x_local_load issues the loads of one
 // chunk of cells, x_local_put consumes it (and returns the chunk's k).
-template <class T, int FT, int R1, int R2>
+template <class T, int FT, int R1, int R2, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, int p, gp<const T> yin,
                                              const XRows &xr, int base, bool err, CellIn<T> *g) {
     constexpr bool FAST = R1 > 0;
-    constexpr int CH = Chunk<T>::value;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
     const float rnx = 1.0f / (float)nx;
@@ -1558,12 +1555,11 @@ __device__ __forceinline__ void x_local_load(const Args<T> &A, const Method &M, 
     }
 }
 
-template <class T, int FT, int R1, int R2>
+template <class T, int FT, int R1, int R2, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, int s, int p, double dt, TimeF<T> tf,
                                             const XRows &xr, int base, const CellIn<T> *g, const C2<T> *res,
                                             bool stats, bool err, Acc &acc, T (*kk)[3]) {
     constexpr bool FAST = R1 > 0;
-    constexpr int CH = Chunk<T>::value;
     const int px = FAST ? R1 * R2 : A.px, nx = px >> 1;
     const int ncl = A.rows_per_wg * nx, bd = blockDim.x;
     const float rnx = 1.0f / (float)nx;
@@ -1591,22 +1587,24 @@ __device__ __forceinline__ void x_local_put(const Args<T> &A, const Method &M, i
     }
 }
 
-template <class T, int FT, int R1, int R2>
+template <class T, int FT, int R1, int R2, int CH = Chunk<T>::value>
 __device__ __forceinline__ void x_local(const Args<T> &A, const Method &M, int s, int p, double dt, TimeF<T> tf,
                                         const XRows &xr, const C2<T> *res, bool stats, bool err, Acc &acc) {
-    constexpr int CH = Chunk<T>::value;
     const int px = R1 > 0 ? R1 * R2 : A.px, ncl = A.rows_per_wg * (px >> 1);
     const gp<const T> yin = stage_input(A, M, s, p);
     for (int base = 0; base < ncl; base += CH * (int)blockDim.x) {
         CellIn<T> g[CH];
-        x_local_load<T, FT, R1, R2>(A, M, p, yin, xr, base, err, g);
-        x_local_put<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, base, g, res, stats, err, acc, nullptr);
+        x_local_load<T, FT, R1, R2, CH>(A, M, p, yin, xr, base, err, g);
+        x_local_put<T, FT, R1, R2, CH>(A, M, s, p, dt, tf, xr, base, g, res, stats, err, acc, nullptr);
     }
 }
 
 // Forward x pass of stage s: combination, Ms*y packing, R2C FFT along x.
-template <class T, int R1, int R2, int NK>
-__global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ AP, int s) {
+// TPB: work-group size (kThreads, or kXpfThreads for the FP32 film, whose
+// work-groups are one per CU: two waves per SIMD); CH: cells per thread of
+// a chunk.
+template <class T, int R1, int R2, int NK, int TPB = kThreads, int CH = Chunk<T>::value>
+__global__ void __launch_bounds__(TPB) k_fwd_x(const Args<T> *__restrict__ AP, int s) {
     const Args<T> &A = AP[blockIdx.y];
     const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
@@ -1621,14 +1619,15 @@ __global__ void __launch_bounds__(kThreads) k_fwd_x(const Args<T> *__restrict__ 
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
-    x_pack_stage<T, R1, R2, NK>(A, M, s, p, dt, xr, b0);
+    x_pack_stage<T, R1, R2, NK, CH>(A, M, s, p, dt, xr, b0);
     x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
 }
 
 // Inverse x pass and local pass of stage s, with the work-group partials of
-// the statistics / error norm when asked (k_control reduces them).
-template <class T, int FT, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
+// the statistics / error norm when asked (k_control reduces them).  TPB and
+// CH as in k_fwd_x.
+template <class T, int FT, int R1, int R2, int TPB = kThreads, int CH = Chunk<T>::value>
+__global__ void __launch_bounds__(TPB) k_inv_x(const Args<T> *__restrict__ AP, int s, int stats) {
     const Args<T> &A = AP[blockIdx.y];
     const CtlR ctl = ctl_ro(A);
     // the control fields in one round trip, then the early exit
@@ -1648,7 +1647,7 @@ __global__ void __launch_bounds__(kThreads) k_inv_x(const Args<T> *__restrict__ 
     const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     Acc acc;
     acc.init();
-    x_local<T, FT, R1, R2>(A, M, s, p, dt, tf, xr, res, stats != 0, err, acc);
+    x_local<T, FT, R1, R2, CH>(A, M, s, p, dt, tf, xr, res, stats != 0, err, acc);
     if (stats || err) block_reduce_global(acc, A.partials + (long long)blockIdx.x * kStatSlots);
 }
 
@@ -1721,6 +1720,17 @@ constexpr int kXpfRows = 2;
 // holds the work-group (its LDS fits one work-group per CU), so that one
 // wave's LDS and memory waits overlap the other's work
 constexpr int kXpfThreads = 512;
+
+// The FP32 film's x passes (Args::xpf: one work-group per CU) run
+// kXpfThreads threads per work-group, xpf_chunk cells per thread of a chunk.
+template <int R1, int R2>
+constexpr int xpf_chunk() {
+    return kXpfRows * (R1 * R2 / 2) > kXpfThreads ? kXpfRows * (R1 * R2 / 2) / kXpfThreads : 1;
+}
+template <class T, int R1, int R2>
+constexpr bool xpf_shape() {
+    return sizeof(T) == 4 && (R1 * R2 == 512 || R1 * R2 == 1024);
+}
 
 template <class T>
 __host__ __device__ constexpr size_t xpf_floats(int nx, int nk) {
@@ -3273,7 +3283,16 @@ static hipError_t set_lds_zf(int pz, size_t limit) {
 
 template <class T, int R1, int R2>
 static hipError_t set_lds_xxp_shape(size_t bytes) {
-    hipError_t e = set_lds<T>((const void *)k_xxp<T, R1, R2, 1>, bytes);
+    // the film's first and final x passes run with kXpfThreads too
+    constexpr int TP = kXpfThreads, CF = xpf_chunk<R1, R2>();
+    hipError_t e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 0, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 1, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 2, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 3, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 4, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_fwd_x<T, R1, R2, 5, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_inv_x<T, 0, R1, R2, TP, CF>, bytes);
+    if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 1>, bytes);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 2>, bytes);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 3>, bytes);
     if (e == hipSuccess) e = set_lds<T>((const void *)k_xxp<T, R1, R2, 4>, bytes);
@@ -3407,6 +3426,21 @@ template <class T, int R1, int R2>
 static void launch_fwd_x_shape(const Args<T> &A, int s, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
     const size_t l = lds_x(A);
+    if constexpr (xpf_shape<T, R1, R2>()) {
+        if (A.xpf) {
+            constexpr int TP = kXpfThreads, CF = xpf_chunk<R1, R2>();
+            const dim3 bf(TP);
+            switch (A.stage_nk[s]) {
+                case 0: k_fwd_x<T, R1, R2, 0, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+                case 1: k_fwd_x<T, R1, R2, 1, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+                case 2: k_fwd_x<T, R1, R2, 2, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+                case 3: k_fwd_x<T, R1, R2, 3, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+                case 4: k_fwd_x<T, R1, R2, 4, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+                default: k_fwd_x<T, R1, R2, 5, TP, CF><<<bgrid(A, g), bf, l, st>>>(A.self_d, s); break;
+            }
+            return;
+        }
+    }
     switch (A.stage_nk[s]) {
         case 0: k_fwd_x<T, R1, R2, 0><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
         case 1: k_fwd_x<T, R1, R2, 1><<<bgrid(A, g), b, l, st>>>(A.self_d, s); break;
@@ -3425,6 +3459,14 @@ static int feature_set(const Args<T> &A) {
 template <class T, int R1, int R2>
 static void launch_inv_x_shape(const Args<T> &A, int s, int stats, hipStream_t st) {
     const dim3 g((unsigned)num_partials(A)), b(kThreads);
+    if constexpr (xpf_shape<T, R1, R2>()) {
+        if (A.xpf) {
+            // Args::xpf: uniform exchange, no further terms (feature set 0)
+            constexpr int TP = kXpfThreads, CF = xpf_chunk<R1, R2>();
+            k_inv_x<T, 0, R1, R2, TP, CF><<<bgrid(A, g), dim3(TP), lds_x(A), st>>>(A.self_d, s, stats);
+            return;
+        }
+    }
     switch (feature_set(A)) {
         case 0: k_inv_x<T, 0, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
         case 1: k_inv_x<T, 1, R1, R2><<<bgrid(A, g), b, lds_x(A), st>>>(A.self_d, s, stats); break;
