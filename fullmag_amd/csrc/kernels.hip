@@ -1034,6 +1034,20 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned g) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// Global-memory index of the four-step twiddle W_N^{t k2} (stored at
+// [k2*R1 + t]) that the forward step's thread t applies to its value k2.
+// For R1 = R2 the table is symmetric and [t*R1 + k2] holds the same value
+// with a thread's k2 consecutive, so its loads merge into 16-byte ones
+// (x passes 512 x 512 film: first pass 10.7 -> 9.8 us FP32; the film y
+// pass, whose lanes then read 32 lines per load, measured slower: 10.8 ->
+// 13.8 us, and keeps [k2*R1 + t]).  LDS reads keep [k2*R1 + t]:
+// consecutive lanes there hit different banks.
+template <int R1, int R2>
+__device__ __forceinline__ int tw_fwd_gidx(int t, int k2) {
+    if constexpr (R1 == R2) return t * R1 + k2;
+    else return k2 * R1 + t;
+}
+
 // Register four-step FFT of `nseq` sequences of length N = R1*R2 held in
 // natural order in LDS `buf` (sequence stride sb >= N), in place.  Thread t1 < R1 of a
 // sequence holds x[t1 + R1 n2] (inputs n >= n_in are zero), does the
@@ -1103,7 +1117,8 @@ __device__ void fs_forward1(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4
             }
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], TL ? tw4[k2 * R1 + t] : ldg(tw4 + (k2 * R1 + t)));
+            for (int k2 = 1; k2 < R2; ++k2)
+                v[k2] = cmul(v[k2], TL ? tw4[k2 * R1 + t] : ldg(tw4 + tw_fwd_gidx<R1, R2>(t, k2)));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) tt[k2 * F::P1 + t] = v[k2];
         }
@@ -1190,7 +1205,7 @@ __device__ void fs_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
 #pragma unroll
             for (int k = 0; k < R2 / 2; ++k) {
                 const int k2 = k + e * (R2 / 2);
-                const C2<T> w = TL ? tw4[k2 * R1 + t] : ldg(tw4 + (k2 * R1 + t));
+                const C2<T> w = TL ? tw4[k2 * R1 + t] : ldg(tw4 + tw_fwd_gidx<R1, R2>(t, k2));
                 tt[k2 * F::P1 + t] = k2 ? cmul(v[k], w) : v[k];
             }
         }
@@ -2041,8 +2056,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     // lanes: the table must be a multiple of that, and the work-group at
     // least one full wave)
     // (measured: FP64 26.6 us against 27.7 on the 512 x 512 film; FP32 is
-    // faster with the exit check first and the twiddles from global memory,
-    // 11.4 us against 13.1)
+    // faster with the twiddles from global memory, 10.8 us against 12.5)
     constexpr int TWF = 2 * N * (int)sizeof(T) / 4;
     constexpr bool TWL = sizeof(T) == 8 && TWF % 256 == 0 && 3 * TS >= 64;
     if constexpr (!TWL) {
@@ -2303,7 +2317,7 @@ __global__ void __launch_bounds__(kThreads) k_yl(const Args<T> *__restrict__ AP)
             for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? ldg(S + t + R1 * n2) : C2<T>{T(0), T(0)};
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + tw_fwd_gidx<R1, R2>(t, k2)));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
         }
@@ -2399,7 +2413,7 @@ __global__ void __launch_bounds__(UB * (R1 > R2 ? R1 : R2)) k_zf(const Args<T> *
             for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? raw[c][n2] : C2<T>{T(0), T(0)};
             DftReg<T, R2, false>::run(v);
 #pragma unroll
-            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + k2 * R1 + t));
+            for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], ldg(tw4 + tw_fwd_gidx<R1, R2>(t, k2)));
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) L(k2 * P1 + t) = v[k2];
         }
@@ -2874,6 +2888,14 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     // allocations: global accesses, which do not share the LDS wait counter
     // (a generic store there made every later LDS read wait for it)
     const gp<Ctl> c = G(A.ctl);
+    // every control field the decision reads, and the method's, loaded with
+    // the partials (one round trip instead of a chain after the reduction)
+    const Method &M = method_for(A);
+    const double dt = c->dt, t0 = c->t, abm_dt = c->abm_dt, t_end = c->t_end, dt_carry = c->dt_carry;
+    const unsigned long long attempts = c->attempts, rejected = c->rejected, step = c->step, step_end = c->step_end;
+    const int parity = c->parity, abm_head = c->abm_head, abm_count = c->abm_count, loop = c->loop;
+    const int adaptive = M.adaptive, abm = M.abm;
+    const double ctrl_exp = M.ctrl_exp;
     Acc acc;
     acc.init();
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
@@ -2889,56 +2911,61 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     block_reduce_lds(acc, tot);
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const Method &M = method_for(A);
     if (observe) {
         for (int k = 0; k < kStatSlots; ++k) c->stats[k] = tot[k];
         return;
     }
-    const double dt = c->dt;
-    c->attempts += 1;
+    c->attempts = attempts + 1;
     bool accept = true;
-    if (M.adaptive) {
+    double dt_next = 0.0;
+    if (adaptive) {
         const double err = tot[ST_ERR];
         c->err = err;
         const double thr = A.rtol > 0.0 ? 1.0 : A.atol;
         accept = err <= thr || dt <= A.dt_min;
         const double ratio =
-            fmax(fmin(A.headroom * pow(thr / (accept ? fmax(err, 1e-30) : err), M.ctrl_exp), A.growth), A.shrink);
+            fmax(fmin(A.headroom * pow(thr / (accept ? fmax(err, 1e-30) : err), ctrl_exp), A.growth), A.shrink);
         const double dtn = fmin(fmax(dt * ratio, A.dt_min), A.dt_max);
-        if (accept) c->dt_next = dtn;
+        if (accept) c->dt_next = dt_next = dtn;
         else c->dt = dtn;
     } else {
         c->dt_next = 0.0;
     }
     c->k1_redo = accept ? 0 : 1;
     if (!accept) {
-        c->rejected += 1;
+        c->rejected = rejected + 1;
         c->accepted = 0;
         return;
     }
     for (int k = 0; k < kStatSlots; ++k) c->stats[k] = tot[k];
     if (A.rec_step) *G(A.rec_step) += 1;
-    c->t += dt;
-    c->step += 1;
-    c->parity ^= 1;
+    const double t = t0 + dt;
+    const unsigned long long stepn = step + 1;
+    c->t = t;
+    c->step = stepn;
+    c->parity = parity ^ 1;
     c->dt_used = dt;
     c->accepted = 1;
-    if (M.abm) {
+    if (abm) {
         // AbmHistory::push (fdm_state.rs:247-257): the kernels stored the
         // pushed value in the slot after the head; a dt change above 10%
         // restarts the history first
-        if (c->abm_dt > 0.0 && fabs(dt - c->abm_dt) / c->abm_dt > 0.1) c->abm_count = 0;
-        c->abm_head = (c->abm_head + 1) % 3;
-        c->abm_count = c->abm_count < 3 ? c->abm_count + 1 : 3;
+        const int cnt = (abm_dt > 0.0 && fabs(dt - abm_dt) / abm_dt > 0.1) ? 0 : abm_count;
+        c->abm_head = (abm_head + 1) % 3;
+        c->abm_count = cnt < 3 ? cnt + 1 : 3;
         c->abm_dt = dt;
     }
-    if (c->loop) {
-        if (M.adaptive) c->dt_carry = c->dt_next;
-        if (c->t >= c->t_end || c->step >= c->step_end) {
+    if (loop) {
+        double carry = dt_carry;
+        if (adaptive) {
+            carry = dt_next;
+            c->dt_carry = carry;
+        }
+        if (t >= t_end || stepn >= step_end) {
             c->done = 1;
         } else {
-            double d = fmin(c->dt_carry, c->t_end - c->t);
-            if (M.adaptive) d = fmin(fmax(d, A.dt_min), A.dt_max);
+            double d = fmin(carry, t_end - t);
+            if (adaptive) d = fmin(fmax(d, A.dt_min), A.dt_max);
             c->dt = d;
         }
     }
