@@ -2099,6 +2099,19 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) v[n2] = n2 < R2 / 2 ? ldg(S + t + R1 * n2) : C2<T>{T(0), T(0)};
     }
+    // R1 = R2 with global twiddles: W_N^{t k} is symmetric in (t, k), so the
+    // forward twiddles of thread t (t = t1, k = k2) are also its inverse ones
+    // (t = k2, k = t1, conjugated); they are loaded here with the launch's
+    // other loads (lanes consecutive) and kept in registers, instead of two
+    // dependent table reads after the forward and the inverse DFTs
+    constexpr bool TWR = !TWL && R1 == R2;
+    C2<T> twr[TWR ? R2 : 1];
+    if constexpr (TWR) {
+        if (t < R1) {
+#pragma unroll
+            for (int k = 1; k < R2; ++k) twr[k] = ldg(tw4 + k * R1 + t);
+        }
+    }
     if constexpr (TWL) {
         if (threadIdx.x < 64)
             dma_floats(reinterpret_cast<const float *>(tw4), reinterpret_cast<float *>(twl), TWF);
@@ -2118,7 +2131,8 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     }
     if (t < R1) {
 #pragma unroll
-        for (int k2 = 1; k2 < R2; ++k2) v[k2] = cmul(v[k2], TWL ? twl[k2 * R1 + t] : ldg(tw4 + k2 * R1 + t));
+        for (int k2 = 1; k2 < R2; ++k2)
+            v[k2] = cmul(v[k2], TWL ? twl[k2 * R1 + t] : TWR ? twr[k2] : ldg(tw4 + k2 * R1 + t));
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) tb[k2 * P1 + t] = v[k2];
     }
@@ -2153,7 +2167,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
         DftReg<T, R1, true>::run(u);
 #pragma unroll
         for (int t1 = 1; t1 < R1; ++t1) {
-            C2<T> w = TWL ? twl[t * R1 + t1] : ldg(tw4 + t * R1 + t1);
+            C2<T> w = TWL ? twl[t * R1 + t1] : TWR ? twr[t1] : ldg(tw4 + t * R1 + t1);
             w.y = -w.y;
             u[t1] = cmul(u[t1], w);
         }
@@ -2651,7 +2665,7 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
 
 // Deterministic block sum of 3 doubles added to out[0..3) by thread 0.
 __device__ void block_add3(double v[3], double *out) {
-    __shared__ double red3[3][kThreads / 64];
+    __shared__ double red3[3][16];  // up to 1024 threads
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {

@@ -113,9 +113,12 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *handle, int reps, doub
 /*
  * Tile tuner (no reference counterpart): times the x-pass launches of a
  * trial step for each x tile (rows per work-group 1, 2, 4; register
- * four-step or LDS Stockham FFT) at `reps` launches each, keeps the fastest
- * for this handle and records it for later creates of the same precision,
- * grid and integrator in this process.  The state is not changed.  Outputs
+ * four-step or LDS Stockham FFT) at `reps` launches each and, on 3-D grids,
+ * the y-line and z passes for each of their tiles (128 or 256 threads per
+ * y-line group; all, half or a quarter of the ky lines per z group), keeps
+ * the fastest for this handle and records it for later creates of the same
+ * precision, grid and integrator in this process.  Every tile gives
+ * bit-identical fields.  The state is not changed.  Outputs
  * (optional): the chosen rows per work-group and 1 if the four-step FFT is
  * used (both 0 without demag).
  */
