@@ -365,6 +365,33 @@ bool four_step_twiddles(Backend *h, int n, const void **dst, bool zshape = false
     return true;
 }
 
+// Per-lane twiddle bases of the film's three-step y pass at py = 1024
+// (k_yf8): [q][lane] (q < 6, lane < 128) holds W_1024^{x} (complex<T>, W =
+// exp(-2 pi i / 1024)) with x = 8 t1' e, 16 t1' (steps A), t1 k2', 16 t1
+// (B), k2 (C'), 8 k2' (B') for the lane's roles e = lane & 1, t1' = lane >> 4
+// in steps A/A', t1 = lane & 7, k2' = lane >> 3 in steps B/B', k2 = lane in
+// steps C/C'; the kernel forms the other twiddles as powers of these.  A
+// wave's loads of one q are contiguous.
+template <class T>
+bool yf8_twiddles(Backend *h, const void **dst) {
+    const int n = 1024;
+    std::vector<T> t(2 * 6 * 128);
+    for (int q = 0; q < 6; ++q)
+        for (int l = 0; l < 128; ++l) {
+            const int e = l & 1, at1p = l >> 4, bt1 = l & 7, bk2p = l >> 3, k2 = l;
+            const int xs[6] = {8 * at1p * e, 16 * at1p, bt1 * bk2p, 16 * bt1, k2, 8 * bk2p};
+            const double a = 2.0 * 3.14159265358979323846 * (double)(xs[q] % n) / (double)n;
+            const size_t i = 2 * ((size_t)q * 128 + l);
+            t[i] = (T)std::cos(a);
+            t[i + 1] = (T)(-std::sin(a));
+        }
+    void *d = dalloc(h, t.size() * sizeof(T));
+    if (!d || hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return hip_ok(h, hipErrorOutOfMemory, "y-pass exponent table");
+    *dst = d;
+    return true;
+}
+
 int select_device(Backend *h) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(h, FULLMAG_FDM_ERR_CUDA, "no HIP device");
@@ -1159,6 +1186,13 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
         if (!four_step_twiddles<T>(h, A.py, &A.tw4y)) return false;
+        // film at py = 1024: three-step y pass (FULLMAG_FDM_YPASS8=0 keeps
+        // the 32 x 32 four-step one)
+        {
+            const char *e8 = std::getenv("FULLMAG_FDM_YPASS8");
+            if (nz == 1 && A.py == 1024 && !(e8 && e8[0] == '0') && !yf8_twiddles<T>(h, &A.tw1y))
+                return false;
+        }
         if (nz > 1 && !four_step_twiddles<T>(h, A.py, &A.tw4y_f, false, true)) return false;
         if (!four_step_twiddles<T>(h, A.px, &A.tw4x)) return false;
         if (nz > 1 && !four_step_twiddles<T>(h, A.pz, &A.tw4z, true)) return false;

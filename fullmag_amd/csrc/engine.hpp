@@ -140,6 +140,8 @@ struct Args {
     const void *tw4y;       // four-step twiddles W_py^{t1 k2} at [k2*R1 + t1] (thin-film fast y pass)
     const void *tw4y_f;     // 3-D forward y-lines: the same for the exchanged factors (R2, R1)
     int yfast;              // 0 = Stockham y pass, else py of the register four-step y pass
+    const void *tw1y;       // thin film at py = 1024: per-lane twiddles of the three-step y pass (k_yf8), or null
+    int yf8;                // 1 = the film y pass runs k_yf8
     const void *tw4x;       // four-step twiddles of the x passes (W_px^{t1 k2} at [k2*R1 + t1])
     int xfast;              // 0 = Stockham x passes, else px of the register four-step x passes
     int xpf;                // 1 = film joined x pass with its inputs prefetched into LDS (k_xxp)

@@ -2185,6 +2185,170 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
     }
 }
 
+// Thin-film y pass at py = 1024 (ny = 512) as three steps of 8-point
+// register DFTs, 128 lanes per component (384 threads; one work-group per kx
+// column).  Index maps: n = b + 64 m with b = t1 + 8 t1' (t1, t1' < 8,
+// m < 16; n < 512 <=> m < 8) and k = k2 + 128 k1 with k2 = k2' + 16 k1'.
+// Forward: X[k2 + 128 k1] = sum_t1 W_8^{t1 k1} W_1024^{t1 k2}
+//   sum_t1' W_8^{t1' k1'} W_128^{t1' k2'} sum_m x[b + 64 m] W_16^{m k2'}.
+// Step A (lane pair b, e): the 16-point DFT over m of 8 nonzero inputs is
+// two 8-point DFTs, lane e taking outputs k2' = 2j + e of x[m] W_16^{m e};
+// step B (lane t1, k2'): 8-point DFT over t1'; step C (lane k2): 8-point DFT
+// over t1.  Lane k2 then holds X at its 8 frequencies, multiplies by -N
+// (the other in-plane component through LDS) and runs the steps backwards;
+// the inverse's 16-point step keeps outputs m < 8 as A + W_16^{-m} B of the
+// lane pair's two 8-point DFTs.  Per lane ~560 VALU operations against
+// ~2,100 for the 32 x 32 four-step form (k_yf): the film's y pass is one
+// work-group per column, so its time is the lane's dependent chain.
+// Twiddles are powers of per-lane bases from a table (Args::tw1y,
+// [q][lane]: a wave's loads of one entry are contiguous), loaded with the
+// inputs.
+constexpr int kYf8Ta = 8 * 132, kYf8Tb = 128 * 9;   // LDS regions per component (complex values)
+template <class T>
+__host__ __device__ constexpr size_t lds_yf8() {
+    return 3ull * (kYf8Ta + kYf8Tb) * sizeof(C2<T>);
+}
+template <class T>
+__global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
+    const Args<T> &A = AP[blockIdx.y];
+    if (yz_exit_check(A) && ctl_ro(A)->done) return;
+    constexpr int NY = 512, N = 1024;
+    constexpr int ST1 = 132, ST2 = 9;   // padded LDS strides (bank spread)
+    const int nkx = A.nkx, kx = blockIdx.x;
+    const int c = (int)threadIdx.x >> 7, l = (int)threadIdx.x & 127;
+    C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + ((long long)c * A.nzs * nkx + kx) * A.ly;
+    const C2<T> *W = reinterpret_cast<const C2<T> *>(A.tw1y);
+    C2<T> *TA = reinterpret_cast<C2<T> *>(fm_smem) + c * kYf8Ta;
+    C2<T> *TB = reinterpret_cast<C2<T> *>(fm_smem) + 3 * kYf8Ta + c * kYf8Tb;
+    const int other = c == 2 ? 2 : (c ^ 1);
+    const C2<T> *TAo = reinterpret_cast<C2<T> *>(fm_smem) + other * kYf8Ta;
+    // lane roles: step A/A' (b, e), b = t1 + 8 t1'; step B/B' (t1 = l & 7,
+    // k2' = l >> 3); step C/C' k2 = l
+    const int e = l & 1, b = l >> 1, at1 = b & 7, at1p = b >> 3;
+    const int bt1 = l & 7, bk2p = l >> 3;
+    const int k2 = l;
+
+    // ---- the launch's loads: inputs (m = 2i + e), tensor values of the
+    // lane's frequencies k = k2 + 128 k1, twiddles of every step
+    C2<T> in4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) in4[i] = ldg(S + b + 64 * (2 * i + e));
+    const long long cs = (long long)nkx * (NY + 1);
+    const T *NT = A.ntens + (long long)kx * (NY + 1);
+    T na[8], nb[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) {
+        const int k = k2 + 128 * k1;
+        const int kr = k <= NY ? k : N - k;
+        na[k1] = ldg(NT + c * cs + kr);
+        const T bv = ldg(NT + 3 * cs + kr);
+        nb[k1] = c == 2 ? T(0) : bv;
+    }
+    // twiddles as powers of per-lane bases (one table row each):
+    // wa[j] = W^{8 t1' e} (W^{16 t1'})^j, wb[j] = W^{t1 k2'} (W^{16 t1})^j,
+    // wc[j] = (W^{k2})^j, wd[j] = (W^{8 k2'})^j, W = W_1024
+    C2<T> wa[8], wb[8], wc[8], wd[8];
+    {
+        const C2<T> a0 = ldg(W + 0 * 128 + l), a1 = ldg(W + 1 * 128 + l);
+        const C2<T> b0 = ldg(W + 2 * 128 + l), b1 = ldg(W + 3 * 128 + l);
+        const C2<T> c1 = ldg(W + 4 * 128 + l), d1 = ldg(W + 5 * 128 + l);
+        wa[0] = a0;
+        wb[0] = b0;
+        wc[0] = {T(1), T(0)};
+        wd[0] = {T(1), T(0)};
+        wc[1] = c1;
+        wd[1] = d1;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            wa[j] = cmul(wa[j - 1], a1);
+            wb[j] = cmul(wb[j - 1], b1);
+            if (j > 1) {
+                wc[j] = cmul(wc[j - 1], c1);
+                wd[j] = cmul(wd[j - 1], d1);
+            }
+        }
+    }
+
+    // ---- forward step A
+    C2<T> v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const C2<T> p = {__shfl_xor(in4[i].x, 1, 64), __shfl_xor(in4[i].y, 1, 64)};
+        v[2 * i] = e ? p : in4[i];
+        v[2 * i + 1] = e ? in4[i] : p;
+    }
+#pragma unroll
+    for (int m = 1; m < 8; ++m) {
+        const C2<T> w16 = {T(TwHolder<16>::tw.c[m]), T(-TwHolder<16>::tw.s[m])};
+        const C2<T> r = cmul(v[m], w16);
+        v[m] = e ? r : v[m];
+    }
+    DftReg<T, 8, false>::run(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) TA[at1 * ST1 + (2 * j + e) * 8 + at1p] = cmul(v[j], wa[j]);
+    __syncthreads();
+    // ---- forward step B
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = TA[bt1 * ST1 + bk2p * 8 + j];
+    DftReg<T, 8, false>::run(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) TB[(bk2p + 16 * j) * ST2 + bt1] = cmul(v[j], wb[j]);
+    __syncthreads();
+    // ---- forward step C, then X at k2 + 128 k1 to LDS for the exchange
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = TB[k2 * ST2 + j];
+    DftReg<T, 8, false>::run(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) TA[k2 + 128 * j] = v[j];
+    __syncthreads();
+    // ---- tensor multiply H = -N M (N_xy's sign flips in ky's upper half)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = k2 + 128 * j;
+        const C2<T> mo = TAo[k];
+        const T bv = k <= NY ? nb[j] : -nb[j];
+        v[j] = {-(na[j] * v[j].x + bv * mo.x), -(na[j] * v[j].y + bv * mo.y)};
+    }
+    // ---- inverse step C': 8-point over k1 -> t1, conj W_1024^{t1 k2}
+    DftReg<T, 8, true>::run(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const C2<T> w = {wc[j].x, -wc[j].y};
+        TB[j * ST1 + k2] = j ? cmul(v[j], w) : v[j];
+    }
+    __syncthreads();
+    // ---- inverse step B': 8-point over k1' -> t1', conj W_128^{t1' k2'}
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = TB[bt1 * ST1 + bk2p + 16 * j];
+    DftReg<T, 8, true>::run(v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const C2<T> w = {wd[j].x, -wd[j].y};
+        TA[bt1 * ST1 + j * 16 + bk2p] = j ? cmul(v[j], w) : v[j];
+    }
+    __syncthreads();
+    // ---- inverse step A': lane e transforms k2' = 2j + e; the pair adds
+    // A + W_16^{-m} B for the outputs m < 8 and stores m = 2i + e
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = TA[at1 * ST1 + at1p * 16 + 2 * j + e];
+    DftReg<T, 8, true>::run(v);
+#pragma unroll
+    for (int m = 1; m < 8; ++m) {
+        const C2<T> w16 = {T(TwHolder<16>::tw.c[m]), T(TwHolder<16>::tw.s[m])};
+        const C2<T> r = cmul(v[m], w16);
+        v[m] = e ? r : v[m];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const C2<T> a0 = v[2 * i], a1 = v[2 * i + 1];
+        const C2<T> p0 = {__shfl_xor(a0.x, 1, 64), __shfl_xor(a0.y, 1, 64)};
+        const C2<T> p1 = {__shfl_xor(a1.x, 1, 64), __shfl_xor(a1.y, 1, 64)};
+        // even lane: x[2i] = own + partner's; odd lane: x[2i + 1]
+        const C2<T> o = e ? a1 : a0, q = e ? p1 : p0;
+        stg(S + b + 64 * (2 * i + e), e ? cadd(q, o) : cadd(o, q));
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_z(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
@@ -3410,6 +3574,8 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
     if (A.yfast == 2048 && (lds_yf<T, 64, 32>() > lds_limit ||
                             set_lds<T>((const void *)k_yf<T, 64, 32>, lds_yf<T, 64, 32>()) != hipSuccess))
         return "cannot raise LDS limit (four-step y)";
+    A.yf8 = A.yfast == 1024 && A.tw1y && lds_yf8<T>() <= lds_limit && (sizeof(T) == 4 || std::getenv("FULLMAG_FDM_YPASS8_FP64")) &&
+            set_lds<T>((const void *)k_yf8<T>, lds_yf8<T>()) == hipSuccess;
     if (A.yfast == 1024 && set_lds<T>((const void *)k_yf<T, 32, 32>, lds_yf<T, 32, 32>()) != hipSuccess)
         return "cannot raise LDS limit (four-step y)";
     if (A.nz > 1 && !A.ylfast && set_lds<T>((const void *)k_y<T, 1>, lds_y(A)) != hipSuccess)
@@ -3451,7 +3617,10 @@ static void launch_yf(const Args<T> &A, hipStream_t st) {
         case 128: launch_yf_shape<T, 8, 16>(A, st); break;
         case 256: launch_yf_shape<T, 16, 16>(A, st); break;
         case 512: launch_yf_shape<T, 16, 32>(A, st); break;
-        case 1024: launch_yf_shape<T, 32, 32>(A, st); break;
+        case 1024:
+            if (A.yf8) k_yf8<T><<<bgrid(A, dim3(A.nkx)), dim3(384), lds_yf8<T>(), st>>>(A.self_d);
+            else launch_yf_shape<T, 32, 32>(A, st);
+            break;
         case 2048: launch_yf_shape<T, 64, 32>(A, st); break;
         default: break;
     }
