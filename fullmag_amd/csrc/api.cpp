@@ -1190,7 +1190,7 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         // the 32 x 32 four-step one)
         {
             const char *e8 = std::getenv("FULLMAG_FDM_YPASS8");
-            if (nz == 1 && A.py == 1024 && !(e8 && e8[0] == '0') && !yf8_twiddles<T>(h, &A.tw1y))
+            if (!h->fp64 && nz == 1 && A.py == 1024 && !(e8 && e8[0] == '0') && !yf8_twiddles<T>(h, &A.tw1y))
                 return false;
         }
         if (nz > 1 && !four_step_twiddles<T>(h, A.py, &A.tw4y_f, false, true)) return false;

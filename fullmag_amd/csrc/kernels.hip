@@ -2203,7 +2203,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
 // Twiddles are powers of per-lane bases from a table (Args::tw1y,
 // [q][lane]: a wave's loads of one entry are contiguous), loaded with the
 // inputs.
-constexpr int kYf8Ta = 8 * 132, kYf8Tb = 128 * 9;   // LDS regions per component (complex values)
+constexpr int kYf8Ta = 1040, kYf8Tb = 1056;   // LDS regions per component (complex values)
 template <class T>
 __host__ __device__ constexpr size_t lds_yf8() {
     return 3ull * (kYf8Ta + kYf8Tb) * sizeof(C2<T>);
@@ -2213,7 +2213,12 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
     const Args<T> &A = AP[blockIdx.y];
     if (yz_exit_check(A) && ctl_ro(A)->done) return;
     constexpr int NY = 512, N = 1024;
-    constexpr int ST1 = 132, ST2 = 9;   // padded LDS strides (bank spread)
+    // LDS layouts, both directions (complex values): step A/B transposes
+    // at ta(t1, t1', k2') = 4 t1 + (k2' & 3) + 32 (k2' >> 2) + 130 t1' and
+    // step B/C ones at tb(k2, t1) = k2 + 132 t1; both make the 32 lanes of
+    // each half-wave hit 32 different bank pairs on either side
+    constexpr int TBS = 132;
+    auto ta = [](int t1, int t1p, int k2p) { return 4 * t1 + (k2p & 3) + 32 * (k2p >> 2) + 130 * t1p; };
     const int nkx = A.nkx, kx = blockIdx.x;
     const int c = (int)threadIdx.x >> 7, l = (int)threadIdx.x & 127;
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + ((long long)c * A.nzs * nkx + kx) * A.ly;
@@ -2273,7 +2278,7 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
     C2<T> v[8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const C2<T> p = {__shfl_xor(in4[i].x, 1, 64), __shfl_xor(in4[i].y, 1, 64)};
+        const C2<T> p = {pair_swap(in4[i].x), pair_swap(in4[i].y)};
         v[2 * i] = e ? p : in4[i];
         v[2 * i + 1] = e ? in4[i] : p;
     }
@@ -2285,18 +2290,18 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
     }
     DftReg<T, 8, false>::run(v);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) TA[at1 * ST1 + (2 * j + e) * 8 + at1p] = cmul(v[j], wa[j]);
+    for (int j = 0; j < 8; ++j) TA[ta(at1, at1p, 2 * j + e)] = cmul(v[j], wa[j]);
     __syncthreads();
     // ---- forward step B
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = TA[bt1 * ST1 + bk2p * 8 + j];
+    for (int j = 0; j < 8; ++j) v[j] = TA[ta(bt1, j, bk2p)];
     DftReg<T, 8, false>::run(v);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) TB[(bk2p + 16 * j) * ST2 + bt1] = cmul(v[j], wb[j]);
+    for (int j = 0; j < 8; ++j) TB[bk2p + 16 * j + TBS * bt1] = cmul(v[j], wb[j]);
     __syncthreads();
     // ---- forward step C, then X at k2 + 128 k1 to LDS for the exchange
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = TB[k2 * ST2 + j];
+    for (int j = 0; j < 8; ++j) v[j] = TB[k2 + TBS * j];
     DftReg<T, 8, false>::run(v);
 #pragma unroll
     for (int j = 0; j < 8; ++j) TA[k2 + 128 * j] = v[j];
@@ -2314,23 +2319,23 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const C2<T> w = {wc[j].x, -wc[j].y};
-        TB[j * ST1 + k2] = j ? cmul(v[j], w) : v[j];
+        TB[k2 + TBS * j] = j ? cmul(v[j], w) : v[j];
     }
     __syncthreads();
     // ---- inverse step B': 8-point over k1' -> t1', conj W_128^{t1' k2'}
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = TB[bt1 * ST1 + bk2p + 16 * j];
+    for (int j = 0; j < 8; ++j) v[j] = TB[bk2p + 16 * j + TBS * bt1];
     DftReg<T, 8, true>::run(v);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const C2<T> w = {wd[j].x, -wd[j].y};
-        TA[bt1 * ST1 + j * 16 + bk2p] = j ? cmul(v[j], w) : v[j];
+        TA[ta(bt1, j, bk2p)] = j ? cmul(v[j], w) : v[j];
     }
     __syncthreads();
     // ---- inverse step A': lane e transforms k2' = 2j + e; the pair adds
     // A + W_16^{-m} B for the outputs m < 8 and stores m = 2i + e
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = TA[at1 * ST1 + at1p * 16 + 2 * j + e];
+    for (int j = 0; j < 8; ++j) v[j] = TA[ta(at1, at1p, 2 * j + e)];
     DftReg<T, 8, true>::run(v);
 #pragma unroll
     for (int m = 1; m < 8; ++m) {
@@ -2341,8 +2346,8 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const C2<T> a0 = v[2 * i], a1 = v[2 * i + 1];
-        const C2<T> p0 = {__shfl_xor(a0.x, 1, 64), __shfl_xor(a0.y, 1, 64)};
-        const C2<T> p1 = {__shfl_xor(a1.x, 1, 64), __shfl_xor(a1.y, 1, 64)};
+        const C2<T> p0 = {pair_swap(a0.x), pair_swap(a0.y)};
+        const C2<T> p1 = {pair_swap(a1.x), pair_swap(a1.y)};
         // even lane: x[2i] = own + partner's; odd lane: x[2i + 1]
         const C2<T> o = e ? a1 : a0, q = e ? p1 : p0;
         stg(S + b + 64 * (2 * i + e), e ? cadd(q, o) : cadd(o, q));
@@ -3574,7 +3579,8 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
     if (A.yfast == 2048 && (lds_yf<T, 64, 32>() > lds_limit ||
                             set_lds<T>((const void *)k_yf<T, 64, 32>, lds_yf<T, 64, 32>()) != hipSuccess))
         return "cannot raise LDS limit (four-step y)";
-    A.yf8 = A.yfast == 1024 && A.tw1y && lds_yf8<T>() <= lds_limit && (sizeof(T) == 4 || std::getenv("FULLMAG_FDM_YPASS8_FP64")) &&
+    // (FP32 only: the FP64 form measured 2x slower over the whole film step)
+    A.yf8 = sizeof(T) == 4 && A.yfast == 1024 && A.tw1y && lds_yf8<T>() <= lds_limit &&
             set_lds<T>((const void *)k_yf8<T>, lds_yf8<T>()) == hipSuccess;
     if (A.yfast == 1024 && set_lds<T>((const void *)k_yf<T, 32, 32>, lds_yf<T, 32, 32>()) != hipSuccess)
         return "cannot raise LDS limit (four-step y)";
