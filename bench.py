@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # recorded when those profiles were taken; bench.py cannot read hardware
 # counters itself.
 TRAFFIC_PROFILES = {"film512": "film512_dp45_hbm_traffic.md", "granular3d": "granular3d_dp45_profile.md"}
-TRAFFIC_ROWS = {0: [("k_fwd_x<float", ", 1>`")], 1: [("k_yf<float", ">`")], 2: [("k_zf<float", ">`")],
+TRAFFIC_ROWS = {0: [("k_fwd_x<float", ", 1>`")], 1: [("k_yf8<float", ">`"), ("k_yf<float", ">`")], 2: [("k_zf<float", ">`")],
                 4: [("k_inv_x<float", ">`")], 6: [("k_xxp<float", ", 2>`"), ("k_xx<float", ", 2>`")]}
 
 
@@ -71,7 +71,8 @@ def recorded_traffic(workload, kind):
     return None, f"no row for {rows[0][0]} in profiles/{name}"
 
 
-NAMES_FILM = {0: "k_fwd_x", 1: "k_yf", 4: "k_inv_x", 6: "k_xxp (FP32 film joined x pass; k_xx in FP64)"}
+NAMES_FILM = {0: "k_fwd_x", 1: "k_yf8 (FP32 film y pass; k_yf in FP64)", 4: "k_inv_x",
+              6: "k_xxp (FP32 film joined x pass; k_xx in FP64)"}
 NAMES_3D = {0: "k_fwd_x", 1: "k_y (forward)", 2: "k_z", 3: "k_y (inverse)", 4: "k_inv_x", 6: "k_xx"}
 
 
