@@ -97,6 +97,25 @@ def test_fields_full_size_film_512(precision):
     _compare(thin_film_512(), precision)
 
 
+def test_fp32_film_y_pass_forms_agree(monkeypatch):
+    # at py = 1024 FP32 plans take the three-step y pass (k_yf8); the 32 x 32
+    # four-step pass (FULLMAG_FDM_YPASS8=0) gives the same demag field to
+    # FP32 roundoff, and both the same DP45 steps within the FP32 band
+    from fullmag_amd.backend import FdmBackend
+    case = random_case(12, 512, 1, seed=7, hole_radius_cells=3)
+    out = {}
+    for form in ("1", "0"):
+        monkeypatch.setenv("FULLMAG_FDM_YPASS8", form)
+        with FdmBackend(case.plan(precision="single")) as be:
+            h = be.field("h_demag")
+            be.step(1e-13)
+            out[form] = (h, be.field("m"))
+    scale = np.abs(out["0"][0]).max()
+    assert scale > 0
+    assert np.abs(out["1"][0] - out["0"][0]).max() <= 2e-6 * scale
+    assert np.abs(out["1"][1] - out["0"][1]).max() <= 1e-6
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
 @pytest.mark.parametrize("nx,ny", [(32, 20), (64, 12), (128, 8), (256, 6), (512, 5), (1024, 3)])
 def test_fields_register_four_step_x_shapes(precision, nx, ny):
