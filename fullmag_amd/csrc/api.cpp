@@ -365,15 +365,16 @@ bool four_step_twiddles(Backend *h, int n, const void **dst, bool zshape = false
     return true;
 }
 
-// Per-lane twiddle bases of the film's three-step y pass at py = 1024
-// (k_yf8): [q][lane] (q < 6, lane < 128) holds W_1024^{x} (complex<T>, W =
+// Per-lane twiddle bases of the three-step length-1024 FFTs (the FP32
+// film's y pass k_yf8 and x passes at px = 1024): [q][lane] (q < 6,
+// lane < 128) holds W_1024^{x} (complex<T>, W =
 // exp(-2 pi i / 1024)) with x = 8 t1' e, 16 t1' (steps A), t1 k2', 16 t1
 // (B), k2 (C'), 8 k2' (B') for the lane's roles e = lane & 1, t1' = lane >> 4
 // in steps A/A', t1 = lane & 7, k2' = lane >> 3 in steps B/B', k2 = lane in
 // steps C/C'; the kernel forms the other twiddles as powers of these.  A
 // wave's loads of one q are contiguous.
 template <class T>
-bool yf8_twiddles(Backend *h, const void **dst) {
+bool f8_bases(Backend *h, const void **dst) {
     const int n = 1024;
     std::vector<T> t(2 * 6 * 128);
     for (int q = 0; q < 6; ++q)
@@ -387,7 +388,7 @@ bool yf8_twiddles(Backend *h, const void **dst) {
         }
     void *d = dalloc(h, t.size() * sizeof(T));
     if (!d || hipMemcpy(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-        return hip_ok(h, hipErrorOutOfMemory, "y-pass exponent table");
+        return hip_ok(h, hipErrorOutOfMemory, "three-step FFT twiddles");
     *dst = d;
     return true;
 }
@@ -1186,12 +1187,12 @@ bool create_typed(Backend *h, const fullmag_fdm_plan_desc *p, const fullmag_fdm_
         A.twy = upload_twiddles<T>(h, A.py);
         A.twz = A.pz > 1 ? upload_twiddles<T>(h, A.pz) : A.twy;
         if (!four_step_twiddles<T>(h, A.py, &A.tw4y)) return false;
-        // film at py = 1024: three-step y pass (FULLMAG_FDM_YPASS8=0 keeps
-        // the 32 x 32 four-step one)
+        // FP32 film at px or py = 1024: three-step FFTs (the y pass's
+        // FULLMAG_FDM_YPASS8=0 keeps the 32 x 32 four-step one)
+        if (!h->fp64 && nz == 1 && (A.py == 1024 || A.px == 1024) && !f8_bases<T>(h, &A.tw8)) return false;
         {
             const char *e8 = std::getenv("FULLMAG_FDM_YPASS8");
-            if (!h->fp64 && nz == 1 && A.py == 1024 && !(e8 && e8[0] == '0') && !yf8_twiddles<T>(h, &A.tw1y))
-                return false;
+            A.yf8_allowed = !(e8 && e8[0] == '0');
         }
         if (nz > 1 && !four_step_twiddles<T>(h, A.py, &A.tw4y_f, false, true)) return false;
         if (!four_step_twiddles<T>(h, A.px, &A.tw4x)) return false;

@@ -140,7 +140,8 @@ struct Args {
     const void *tw4y;       // four-step twiddles W_py^{t1 k2} at [k2*R1 + t1] (thin-film fast y pass)
     const void *tw4y_f;     // 3-D forward y-lines: the same for the exchanged factors (R2, R1)
     int yfast;              // 0 = Stockham y pass, else py of the register four-step y pass
-    const void *tw1y;       // thin film at py = 1024: per-lane twiddles of the three-step y pass (k_yf8), or null
+    const void *tw8;        // FP32 film at px or py = 1024: twiddle bases of the three-step FFTs ([6][128] complex), or null
+    int yf8_allowed;        // 0 = keep the four-step y pass (FULLMAG_FDM_YPASS8=0)
     int yf8;                // 1 = the film y pass runs k_yf8
     const void *tw4x;       // four-step twiddles of the x passes (W_px^{t1 k2} at [k2*R1 + t1])
     int xfast;              // 0 = Stockham x passes, else px of the register four-step x passes

@@ -1266,6 +1266,185 @@ __device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
     }
 }
 
+// ---------------------------------------- three-step length-1024 FFTs --
+// Length 1024 with 512 nonzero inputs / 512 kept outputs as three steps of
+// 8-point register DFTs over 128 lanes: n = b + 64 m (b = t1 + 8 t1',
+// t1, t1' < 8, m < 16), k = k2 + 128 k1 (k2 = k2' + 16 k1' < 128, k1 < 8).
+// Forward: X[k2 + 128 k1] = sum_t1 W_8^{t1 k1} W_1024^{t1 k2} sum_t1'
+// W_8^{t1' k1'} W_128^{t1' k2'} sum_m x[b + 64 m] W_16^{m k2'}; step A (lane
+// pair b, e = lane & 1) takes the 16-point DFT over the 8 nonzero m as two
+// 8-point DFTs, lane e the outputs k2' = 2j + e of x[m] W_16^{m e}; step B
+// (lane t1 = lane & 7, k2' = lane >> 3) the 8-point DFT over t1'; step C
+// (lane k2) over t1.  The inverse runs the steps backwards; its 16-point
+// step keeps m < 8 as A + W_16^{-m} B of the pair's two 8-point DFTs.
+
+// Transpose layout of steps A/B (and A'/B'): 32 lanes of a half-wave hit
+// 32 distinct bank pairs on both sides (checked exhaustively on the host).
+__device__ __forceinline__ int f8_ta(int t1, int t1p, int k2p) {
+    return 4 * t1 + (k2p & 3) + 32 * (k2p >> 2) + 130 * t1p;
+}
+constexpr int kF8Ta = 1040;    // extent of f8_ta (complex values)
+constexpr int kF8Tb = 1056;    // steps B/C transpose region: k2 + 132 t1
+constexpr int kF8Tbs = 132;
+
+// The lane's twiddles from its six bases (Args::tw8, [q][lane]):
+// wa[j] = W^{8 t1' (2j + e)} = a0 a1^j, wb[j] = W^{t1 (k2' + 16 j)} = b0 b1^j,
+// wc[j] = W^{j k2} = c1^j, wd[j] = W^{8 j k2'} = d1^j (W = W_1024).
+template <class T>
+__device__ __forceinline__ void f8_twiddles(const C2<T> (&q)[6], C2<T> (&wa)[8], C2<T> (&wb)[8], C2<T> (&wc)[8],
+                                            C2<T> (&wd)[8]) {
+    wa[0] = q[0];
+    wb[0] = q[2];
+    wc[0] = {T(1), T(0)};
+    wd[0] = {T(1), T(0)};
+    wc[1] = q[4];
+    wd[1] = q[5];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        wa[j] = cmul(wa[j - 1], q[1]);
+        wb[j] = cmul(wb[j - 1], q[3]);
+        if (j > 1) {
+            wc[j] = cmul(wc[j - 1], q[4]);
+            wd[j] = cmul(wd[j - 1], q[5]);
+        }
+    }
+}
+
+// Forward three-step FFT of `nseq` sequences of LDS `buf` (stride sb >=
+// kF8Ta), in place, natural order, inputs n < 512 (the rest is taken as
+// zero and not read), 128 lanes per sequence; the step A/B transpose
+// reuses the sequence's own buffer, tb holds one kF8Tb region per sequence
+// of a round.  bases: the [6][128] twiddle bases, in LDS (TL) or global
+// memory.
+template <class T, bool TL>
+__device__ void fs8_forward(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ bases, int nseq, int sb) {
+    const int spb = blockDim.x / 128;
+    const int l = threadIdx.x & 127;
+    const int e = l & 1, b = l >> 1, at1 = b & 7, at1p = b >> 3, bt1 = l & 7, bk2p = l >> 3, k2 = l;
+    C2<T> q[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) q[i] = TL ? bases[i * 128 + l] : ldg(bases + i * 128 + l);
+    C2<T> wa[8], wb[8], wc[8], wd[8];
+    f8_twiddles(q, wa, wb, wc, wd);
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + (int)threadIdx.x / 128;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * 128;
+        C2<T> *B = buf + sq * sb;
+        C2<T> *TB = tb + (sq - base) * kF8Tb;
+        C2<T> v[8];
+        if (on) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) v[m] = B[b + 64 * m];
+#pragma unroll
+            for (int m = 1; m < 8; ++m) {
+                const C2<T> w16 = {T(TwHolder<16>::tw.c[m]), T(-TwHolder<16>::tw.s[m])};
+                const C2<T> r = cmul(v[m], w16);
+                v[m] = e ? r : v[m];
+            }
+            DftReg<T, 8, false>::run(v);
+        }
+        __syncthreads();  // every input read before the transpose overwrites them
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) B[f8_ta(at1, at1p, 2 * j + e)] = cmul(v[j], wa[j]);
+        }
+        __syncthreads();
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = B[f8_ta(bt1, j, bk2p)];
+            DftReg<T, 8, false>::run(v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) TB[bk2p + 16 * j + kF8Tbs * bt1] = cmul(v[j], wb[j]);
+        }
+        __syncthreads();
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = TB[k2 + kF8Tbs * j];
+            DftReg<T, 8, false>::run(v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) B[k2 + 128 * j] = v[j];
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse of fs8_forward (unnormalised): all 1024 inputs in natural order,
+// outputs n < 512 written in natural order.
+template <class T, bool TL>
+__device__ void fs8_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ bases, int nseq, int sb) {
+    const int spb = blockDim.x / 128;
+    const int l = threadIdx.x & 127;
+    const int e = l & 1, b = l >> 1, at1 = b & 7, at1p = b >> 3, bt1 = l & 7, bk2p = l >> 3, k2 = l;
+    C2<T> q[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) q[i] = TL ? bases[i * 128 + l] : ldg(bases + i * 128 + l);
+    C2<T> wa[8], wb[8], wc[8], wd[8];
+    f8_twiddles(q, wa, wb, wc, wd);
+    for (int base = 0; base < nseq; base += spb) {
+        const int sq = base + (int)threadIdx.x / 128;
+        const bool on = sq < nseq && (int)threadIdx.x < spb * 128;
+        C2<T> *B = buf + sq * sb;
+        C2<T> *TB = tb + (sq - base) * kF8Tb;
+        C2<T> v[8];
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = B[k2 + 128 * j];
+            DftReg<T, 8, true>::run(v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const C2<T> w = {wc[j].x, -wc[j].y};
+                TB[k2 + kF8Tbs * j] = j ? cmul(v[j], w) : v[j];
+            }
+        }
+        __syncthreads();
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = TB[bk2p + 16 * j + kF8Tbs * bt1];
+            DftReg<T, 8, true>::run(v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const C2<T> w = {wd[j].x, -wd[j].y};
+                B[f8_ta(bt1, j, bk2p)] = j ? cmul(v[j], w) : v[j];
+            }
+        }
+        __syncthreads();
+        C2<T> o[4];
+        if (on) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = B[f8_ta(at1, at1p, 2 * j + e)];
+            DftReg<T, 8, true>::run(v);
+#pragma unroll
+            for (int m = 1; m < 8; ++m) {
+                const C2<T> w16 = {T(TwHolder<16>::tw.c[m]), T(TwHolder<16>::tw.s[m])};
+                const C2<T> r = cmul(v[m], w16);
+                v[m] = e ? r : v[m];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const C2<T> a0 = v[2 * i], a1 = v[2 * i + 1];
+                const C2<T> p0 = {pair_swap(a0.x), pair_swap(a0.y)};
+                const C2<T> p1 = {pair_swap(a1.x), pair_swap(a1.y)};
+                // even lane: x[2i] = own + partner's; odd lane: x[2i + 1]
+                const C2<T> mine = e ? a1 : a0, theirs = e ? p1 : p0;
+                o[i] = e ? cadd(theirs, mine) : cadd(mine, theirs);
+            }
+        }
+        __syncthreads();  // every transpose value read before the outputs overwrite them
+        if (on) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) B[b + 64 * (2 * i + e)] = o[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Rows per work-group of the FP32 film's x passes with their inputs
+// prefetched into LDS (Args::xpf), and their threads: eight waves, two per
+// SIMD of the CU that holds the work-group (its LDS fits one work-group per
+// CU), so that one wave's LDS and memory waits overlap the other's work.
+constexpr int kXpfRows = 2;
+constexpr int kXpfThreads = 512;
+
 // ------------------------------------------------------------- x passes --
 // An x-pass work-group owns `rows_per_wg` consecutive x-rows of the grid
 // (row = y + ny z) and every cell and half-spectrum element of those rows.
@@ -1463,7 +1642,9 @@ __device__ __forceinline__ void x_pack_stage(const Args<T> &A, const Method &M, 
 // Zero-padded forward FFT of the packed sequences in b0 and the unpacking of
 // each sequence into the half spectra of its two real lines,
 // X = (Z[k] + conj Z[-k])/2 and (Z[k] - conj Z[-k])/(2i), y-lines contiguous.
-template <class T, int R1, int R2, bool TL = false>
+// F8: the three-step FFT (fs8_forward; R1 = R2 = 32, FP32), `tw` then
+// holding its twiddle bases when TL.
+template <class T, int R1, int R2, bool TL = false, bool F8 = false>
 __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
                                                 const C2<T> *tw) {
     constexpr bool FAST = R1 > 0;
@@ -1473,7 +1654,10 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
     const int sb = FAST ? A.xsb : px;
     if constexpr (FAST) {
         // TL: `tw` holds the four-step twiddles in LDS
-        fs_forward<T, R1, R2, TL>(b0, b1, TL ? tw : reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
+        if constexpr (F8)
+            fs8_forward<T, TL>(b0, b1, TL ? tw : reinterpret_cast<const C2<T> *>(A.tw8), A.nseq_x, sb);
+        else
+            fs_forward<T, R1, R2, TL>(b0, b1, TL ? tw : reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
         res = b0;
     } else {
         res = lds_fft<T, false>(b0, b1, A.fx, tw, A.nseq_x, nx, px);
@@ -1503,7 +1687,7 @@ __device__ __forceinline__ void x_forward_store(const Args<T> &A, const XRows &x
 // that the rows of a component are consecutive y-points of one load); each
 // half-spectrum value is loaded once, and the even lane writes Z = A + iB at
 // kx and, from the conjugates, at px - kx.
-template <class T, int R1, int R2>
+template <class T, int R1, int R2, bool F8 = false>
 __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const XRows &xr, C2<T> *b0, C2<T> *b1,
                                                        const C2<T> *tw) {
     constexpr bool FAST = R1 > 0;
@@ -1542,7 +1726,10 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
     }
     __syncthreads();
     if constexpr (FAST) {
-        fs_inverse<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
+        if constexpr (F8)
+            fs8_inverse<T, false>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw8), A.nseq_x, sb);
+        else
+            fs_inverse<T, R1, R2>(b0, b1, reinterpret_cast<const C2<T> *>(A.tw4x), A.nseq_x, nx, sb);
         return b0;
     } else {
         return lds_fft<T, true>(b0, b1, A.fx, tw, A.nseq_x, px, nx);
@@ -1635,7 +1822,9 @@ __global__ void __launch_bounds__(TPB) k_fwd_x(const Args<T> *__restrict__ AP, i
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
     x_pack_stage<T, R1, R2, NK, CH>(A, M, s, p, dt, xr, b0);
-    x_forward_store<T, R1, R2>(A, xr, b0, b1, tw);
+    // the FP32 film's px = 1024 with kXpfThreads: three-step FFT
+    constexpr bool F8 = sizeof(T) == 4 && R1 == 32 && R2 == 32 && TPB == kXpfThreads;
+    x_forward_store<T, R1, R2, false, F8>(A, xr, b0, b1, tw);
 }
 
 // Inverse x pass and local pass of stage s, with the work-group partials of
@@ -1657,7 +1846,8 @@ __global__ void __launch_bounds__(TPB) k_inv_x(const Args<T> *__restrict__ AP, i
     x_rows(A, (int)xcd_block(blockIdx.x, gridDim.x) * A.rows_per_wg, xr);
     if constexpr (R1 == 0) load_twiddles<T>(tw, A.twx, A.px);
     __syncthreads();
-    const C2<T> *res = x_inverse_load<T, R1, R2>(A, xr, b0, b1, tw);
+    constexpr bool F8 = sizeof(T) == 4 && R1 == 32 && R2 == 32 && TPB == kXpfThreads;
+    const C2<T> *res = x_inverse_load<T, R1, R2, F8>(A, xr, b0, b1, tw);
     const bool err = M.adaptive && s == M.nstage;
     const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     Acc acc;
@@ -1730,11 +1920,6 @@ __global__ void __launch_bounds__(kThreads, (xx_waves<T, FT, R1, R2>())) k_xx(co
 //                  k_s is filled by the local pass)
 //   TW [2nx] complex  the four-step twiddles of the x FFTs
 //   NM [2][nx] bytes  activity bytes of the two rows (with a mask)
-constexpr int kXpfRows = 2;
-// threads of a k_xxp work-group: eight waves, two per SIMD of the CU that
-// holds the work-group (its LDS fits one work-group per CU), so that one
-// wave's LDS and memory waits overlap the other's work
-constexpr int kXpfThreads = 512;
 
 // The FP32 film's x passes (Args::xpf: one work-group per CU) run
 // kXpfThreads threads per work-group, xpf_chunk cells per thread of a chunk.
@@ -1846,7 +2031,12 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
         v[j] = ldg(S + (c * nkx) * ly + y0 + rr + kk * ly);
     }
     const int wv = threadIdx.x >> 6, nw = kXpfThreads / 64;
-    if (wv == nw - 1) dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
+    // px = 1024: three-step FFTs, TW holds their [6][128] twiddle bases
+    constexpr bool F8 = R1 == 32 && R2 == 32;
+    if (wv == nw - 1) {
+        if constexpr (F8) dma_floats(reinterpret_cast<const T *>(A.tw8), TW, 2 * 6 * 128);
+        else dma_floats(reinterpret_cast<const T *>(A.tw4x), TW, 2 * PX);
+    }
     if (A.nmask && wv == nw - 2) dma_bytes(A.nmask + c0, NM, kXpfRows * NX);
     // the control block through the scalar cache (this kernel does not
     // write it; the scalar cache is invalidated at every launch): its round
@@ -1909,7 +2099,8 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const C2<T> *tw = reinterpret_cast<const C2<T> *>(TW);
-    fs_inverse<T, R1, R2, true>(b0, b1, tw, A.nseq_x, NX, sb);
+    if constexpr (F8) fs8_inverse<T, true>(b0, b1, tw, A.nseq_x, sb);
+    else fs_inverse<T, R1, R2, true>(b0, b1, tw, A.nseq_x, NX, sb);
     // 5. local pass of stage s from the LDS rows; k_s also into its K slot
     const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     {
@@ -1969,7 +2160,7 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
         }
         x_pack_tail<T, R1, R2>(A, b0);
     }
-    x_forward_store<T, R1, R2, true>(A, xr, b0, b1, tw);
+    x_forward_store<T, R1, R2, true, F8>(A, xr, b0, b1, tw);
 }
 
 // MODE 0: thin-film fused forward FFT + tensor multiply + inverse FFT.
@@ -2200,7 +2391,7 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
 // lane pair's two 8-point DFTs.  Per lane ~560 VALU operations against
 // ~2,100 for the 32 x 32 four-step form (k_yf): the film's y pass is one
 // work-group per column, so its time is the lane's dependent chain.
-// Twiddles are powers of per-lane bases from a table (Args::tw1y,
+// Twiddles are powers of per-lane bases from a table (Args::tw8,
 // [q][lane]: a wave's loads of one entry are contiguous), loaded with the
 // inputs.
 constexpr int kYf8Ta = 1040, kYf8Tb = 1056;   // LDS regions per component (complex values)
@@ -2222,7 +2413,7 @@ __global__ void __launch_bounds__(384) k_yf8(const Args<T> *__restrict__ AP) {
     const int nkx = A.nkx, kx = blockIdx.x;
     const int c = (int)threadIdx.x >> 7, l = (int)threadIdx.x & 127;
     C2<T> *S = reinterpret_cast<C2<T> *>(A.spec) + ((long long)c * A.nzs * nkx + kx) * A.ly;
-    const C2<T> *W = reinterpret_cast<const C2<T> *>(A.tw1y);
+    const C2<T> *W = reinterpret_cast<const C2<T> *>(A.tw8);
     C2<T> *TA = reinterpret_cast<C2<T> *>(fm_smem) + c * kYf8Ta;
     C2<T> *TB = reinterpret_cast<C2<T> *>(fm_smem) + 3 * kYf8Ta + c * kYf8Tb;
     const int other = c == 2 ? 2 : (c ^ 1);
@@ -3580,7 +3771,7 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
                             set_lds<T>((const void *)k_yf<T, 64, 32>, lds_yf<T, 64, 32>()) != hipSuccess))
         return "cannot raise LDS limit (four-step y)";
     // (FP32 only: the FP64 form measured 2x slower over the whole film step)
-    A.yf8 = sizeof(T) == 4 && A.yfast == 1024 && A.tw1y && lds_yf8<T>() <= lds_limit &&
+    A.yf8 = sizeof(T) == 4 && A.yfast == 1024 && A.tw8 && A.yf8_allowed && lds_yf8<T>() <= lds_limit &&
             set_lds<T>((const void *)k_yf8<T>, lds_yf8<T>()) == hipSuccess;
     if (A.yfast == 1024 && set_lds<T>((const void *)k_yf<T, 32, 32>, lds_yf<T, 32, 32>()) != hipSuccess)
         return "cannot raise LDS limit (four-step y)";
@@ -3597,13 +3788,22 @@ const char *configure(Args<T> &A, size_t lds_limit, int rows_req, int xfast_req)
     if constexpr (sizeof(T) == 4) {
         const char *env = std::getenv("FULLMAG_FDM_XPASS_PREFETCH");
         const bool allowed = !(env && env[0] == '0');
+        // px = 1024: the three-step FFTs transpose inside each sequence's
+        // buffer (kF8Ta values); 32 more keeps the stride's bank offset
+        const int xsb0 = A.xsb;
+        if (A.xfast == 1024 && A.tw8) A.xsb = std::max(A.xsb, A.px + 32 + (32 + A.nseq_x - 1) / A.nseq_x);
         const int off = (int)((lds_x(A) + 15) / 16 * 16);
         const size_t bytes = (size_t)off + xpf_floats<T>(A.nx, kMaxK) * sizeof(T) + (size_t)kXpfRows * A.nx;
-        if (allowed && A.nz == 1 && (A.xfast == 512 || A.xfast == 1024) && A.rows_per_wg == kXpfRows &&
+        if (allowed && A.nz == 1 && (A.xfast == 512 || (A.xfast == 1024 && A.tw8)) && A.rows_per_wg == kXpfRows &&
             A.nrows % kXpfRows == 0 && !A.region && !A.xt && A.method != M_ABM3 && bytes <= lds_limit &&
             set_lds_xxp<T>(A.xfast, bytes) == hipSuccess) {
             A.xpf = 1;
             A.xpf_off = off;
+            // the other x-pass kernels see the wider stride too
+            if (A.xsb != xsb0 && set_lds_x<T>(A.xfast, lds_x(A)) != hipSuccess)
+                return "cannot raise LDS limit (x passes)";
+        } else {
+            A.xsb = xsb0;
         }
     }
     return "";
