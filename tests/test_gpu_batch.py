@@ -140,3 +140,25 @@ def test_stack_and_plain_grid_of_one_shape_form_two_groups():
             r = be.advance(1e-12, 100000, 5e-14)
             assert r["steps_taken"] == steps[i], i
             assert np.array_equal(be.field("m"), got[i]), i
+
+
+def test_fp32_three_step_fft_shapes_batched():
+    # FP32 films with px = 1024 (x passes with the three-step FFT and the
+    # prefetching joined pass) and py = 1024 (the three-step y pass), two
+    # systems each with their own Ms and m0: the batch equals each alone
+    from fullmag_amd.backend import FdmBackend, FdmBatch, FdmPlan
+    from fullmag_amd.workloads import batch_sweep
+    shapes = ((512, 8, 1), (16, 512, 1))
+    plans = [FdmPlan(**kw) for kw in batch_sweep(count=4, seed=5, shapes=shapes, precision="single")]
+    t_end = 5e-13
+    assert sorted((p.nx, p.ny) for p in plans) == [(16, 512), (16, 512), (512, 8), (512, 8)]
+    with FdmBatch(plans) as b:
+        assert b.groups() == 2
+        _, steps = b.advance(t_end, 100000, 1e-13)
+        got = [s.field("m") for s in b.systems]
+    assert np.all(steps > 0)
+    for i, p in enumerate(plans):
+        with FdmBackend(p) as be:
+            r = be.advance(t_end, 100000, 1e-13)
+            assert r["steps_taken"] == steps[i], i
+            assert np.array_equal(be.field("m"), got[i]), i
