@@ -1216,6 +1216,27 @@ int orc_fft(double *data, int n, int inverse) {
     return 0;
 }
 
+// One step of the planner's xorshift64 (util.rs:128-130: shifts 13, 7, 17,
+// left-right-left, Marsaglia's xor64).
+static inline uint64_t xorshift64_next(uint64_t &st) {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return st;
+}
+
+// The first n states after `seed` (the values util.rs maps to components).
+int orc_xorshift64_states(uint64_t seed, uint64_t n, uint64_t *out) {
+    if (!out) return -1;
+    uint64_t st = seed;
+    for (uint64_t i = 0; i < n; ++i) out[i] = xorshift64_next(st);
+    return 0;
+}
+
+// util.rs:133: component in [-1, 1] of a state (Rust `as f64` rounds to
+// nearest, and u64::MAX as f64 is 2^64).
+double orc_state_component(uint64_t st) { return ((double)st / (double)UINT64_MAX) * 2.0 - 1.0; }
+
 // util.rs:118-147.  Seed 0 never leaves the rejection loop, so it is refused.
 int orc_random_unit_vectors(uint64_t seed, uint64_t count, double *out) {
     if (seed == 0 || !out) return -1;
@@ -1223,12 +1244,7 @@ int orc_random_unit_vectors(uint64_t seed, uint64_t count, double *out) {
     for (uint64_t i = 0; i < count; ++i) {
         double c[3];
         for (;;) {
-            for (int k = 0; k < 3; ++k) {
-                st ^= st << 13;
-                st ^= st >> 7;
-                st ^= st << 17;
-                c[k] = ((double)st / (double)UINT64_MAX) * 2.0 - 1.0;
-            }
+            for (int k = 0; k < 3; ++k) c[k] = orc_state_component(xorshift64_next(st));
             const double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
             if (nrm > 1e-10) {
                 out[3 * i + 0] = c[0] / nrm;

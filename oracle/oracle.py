@@ -105,6 +105,9 @@ def lib():
         L.orc_newell_kernels.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_double] * 3 + [_dp]
         L.orc_fft.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
         L.orc_random_unit_vectors.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _dp]
+        L.orc_xorshift64_states.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_state_component.restype = ctypes.c_double
+        L.orc_state_component.argtypes = [ctypes.c_uint64]
         L.orc_hole_mask.restype = ctypes.c_uint64
         L.orc_hole_mask.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_uint8)]
         L.orc_create.restype = ctypes.c_void_p
@@ -168,6 +171,19 @@ def random_unit_vectors(seed, count):
     out = np.zeros((count, 3))
     assert lib().orc_random_unit_vectors(seed, count, _ptr(out)) == 0
     return out
+
+
+def xorshift64_states(seed, n):
+    """The first n states of the planner's xorshift64 after `seed`
+    (util.rs:128-130)."""
+    out = np.zeros(n, dtype=np.uint64)
+    assert lib().orc_xorshift64_states(seed, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))) == 0
+    return [int(v) for v in out]
+
+
+def state_component(state):
+    """util.rs:133: the [-1, 1] component of one state."""
+    return lib().orc_state_component(state)
 
 
 def hole_mask(nx, ny, nz, dx, dy, radius):
