@@ -287,6 +287,7 @@ EXT_SYMBOLS = {
         ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                        ctypes.POINTER(ctypes.c_double)]),
     "fullmag_fdm_backend_time_kernels": (ctypes.c_int, [_H, ctypes.c_int, c_double_p, ctypes.c_int]),
+    "fullmag_fdm_backend_time_trial_kernels": (ctypes.c_int, [_H, ctypes.c_int, c_double_p, ctypes.c_int]),
     "fullmag_fdm_random_unit_vectors": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, c_double_p]),
     "fullmag_fdm_backend_create_ext": (_H, [ctypes.POINTER(PlanDesc), ctypes.POINTER(PlanExt)]),
     "fullmag_fdm_adjoint_forward": (ctypes.c_int, [_H, ctypes.POINTER(AdjointDesc), c_double_p]),

@@ -445,6 +445,16 @@ class FdmBackend:
         self._check(self.lib.fullmag_fdm_backend_time_kernels(self.h, reps, _dp(out), 8))
         return out[:7].tolist()
 
+    def time_trial_kernels(self, trials: int = 20) -> list:
+        """Mean µs of the first launch of each kernel kind inside whole trial
+        steps (each kernel reading what the one before it wrote; stage 2,
+        the final x pass at the last stage), less the per-launch cost of the
+        timing events; [7] whole trial step, [8] that event cost.  State
+        unchanged."""
+        out = np.zeros(9)
+        self._check(self.lib.fullmag_fdm_backend_time_trial_kernels(self.h, trials, _dp(out), 9))
+        return out.tolist()
+
 
 # ------------------------------------------------------------ input builders
 def random_unit_vectors(seed: int, count: int) -> np.ndarray:

@@ -2084,6 +2084,89 @@ int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *h, int reps, double *o
     return rc;
 }
 
+int fullmag_fdm_backend_time_trial_kernels(fullmag_fdm_backend *h, int trials, double *out_us, int n_out) {
+    if (!h || !out_us || n_out < 7 || trials < 1) return FULLMAG_FDM_ERR_INVALID;
+    if (!h->ok) return fail(h, FULLMAG_FDM_ERR_INVALID, "backend not initialised");
+    hipSetDevice(h->device);
+    for (int i = 0; i < n_out; ++i) out_us[i] = 0.0;
+    const bool demag = with_args(h, [&](auto &A) { return A.demag != 0 && !A.lmix; });
+    if (!demag) return fail(h, FULLMAG_FDM_ERR_INVALID, "trial-step timing needs a demag plan without layer stacks");
+    if (!pull_ctl(h)) return FULLMAG_FDM_ERR_CUDA;
+    // the launches of a trial step, as the captured graph has them:
+    // (kind, stage)
+    const bool film = h->nz == 1;
+    const int ns = h->nstage;
+    std::vector<std::pair<int, int>> seq{{0, 2}};
+    for (int s = 2; s <= ns; ++s) {
+        seq.push_back({1, s});
+        if (!film) {
+            seq.push_back({2, s});
+            seq.push_back({3, s});
+        }
+        seq.push_back({s < ns ? 6 : 4, s});
+    }
+    seq.push_back({5, ns});
+    // a trial step leaves m, k1 (and with the history guard the ABM3 ring)
+    // as they were; the controller's decisions live in the control block,
+    // which is restored after each trial step
+    const fm::Ctl saved = *h->ctl_h;
+    HistoryGuard keep(h);
+    if (!keep.ok()) return fail(h, FULLMAG_FDM_ERR_CUDA, "history copy allocation failed");
+    std::vector<hipEvent_t> ev(seq.size() + 1);
+    for (auto &e : ev) hipEventCreate(&e);
+    double sum[7] = {0.0};
+    int cnt[7] = {0};
+    double with_events = 0.0, plain = 0.0;
+    int rc = FULLMAG_FDM_OK;
+    // trials with an event after every launch, then as many with events
+    // only around the whole trial step: the difference is the events' own
+    // cost, taken off each launch's figure in equal parts
+    for (int t = 0; t <= 2 * trials && rc == FULLMAG_FDM_OK; ++t) {
+        const bool inner = t <= trials;
+        hipEventRecord(ev[0], h->stream);
+        for (size_t i = 0; i < seq.size(); ++i) {
+            with_args(h, [&](auto &A) { return fm::enqueue_kind(A, seq[i].first, seq[i].second, h->stream); });
+            if (inner || i + 1 == seq.size()) hipEventRecord(ev[i + 1], h->stream);
+        }
+        if (hipEventSynchronize(ev.back()) != hipSuccess) {
+            rc = fail(h, FULLMAG_FDM_ERR_CUDA, "timing sync");
+            break;
+        }
+        if (!inner) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[0], ev.back());
+            plain += 1000.0 * ms;
+        } else if (t > 0) {
+            float all = 0.f;
+            hipEventElapsedTime(&all, ev[0], ev.back());
+            with_events += 1000.0 * all;
+            // the first launch of each kind in the trial step (stage 2; the
+            // final x pass at the last stage)
+            bool seen[7] = {false};
+            for (size_t i = 0; i < seq.size(); ++i) {
+                const int k = seq[i].first;
+                if (seen[k]) continue;
+                seen[k] = true;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+                sum[k] += 1000.0 * ms;
+                ++cnt[k];
+            }
+        }
+        *h->ctl_h = saved;
+        if (!push_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    const double overhead = std::max(0.0, (with_events - plain) / trials / (double)seq.size());
+    for (int k = 0; k < 7; ++k) out_us[k] = cnt[k] ? std::max(0.0, sum[k] / cnt[k] - overhead) : 0.0;
+    if (n_out > 7) out_us[7] = plain / trials;
+    if (n_out > 8) out_us[8] = overhead;
+    *h->ctl_h = saved;
+    if (!push_ctl(h) || !pull_ctl(h)) rc = FULLMAG_FDM_ERR_CUDA;
+    h->obs_fresh = false;
+    return rc;
+}
+
 }  // extern "C"
 
 

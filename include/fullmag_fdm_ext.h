@@ -13,7 +13,7 @@ extern "C" {
 #endif
 
 /* ABI revision of this library (bumped when an entry point is added). */
-#define FULLMAG_FDM_EXT_VERSION 8
+#define FULLMAG_FDM_EXT_VERSION 9
 int fullmag_fdm_ext_version(void);
 
 /* Time-dependent Zeeman waveforms: H_ext(t) = H0 + f(t) * H1 on the cells
@@ -109,6 +109,25 @@ int fullmag_fdm_backend_get_counters(fullmag_fdm_backend *handle, uint64_t *acce
  */
 int fullmag_fdm_backend_time_kernels(fullmag_fdm_backend *handle, int reps, double *out_us,
                                      int n_out);
+
+/*
+ * Kernel timing inside whole trial steps (no reference counterpart): runs
+ * `trials` trial steps of the configured method (after one untimed) as
+ * separate launches in the order of the captured trial step, an event
+ * after each, so that every kernel reads what the kernel before it wrote,
+ * as in the stepping loop; out_us[i] is the mean time between the events
+ * around the first launch of kind i in a trial step (kinds as
+ * fullmag_fdm_backend_time_kernels, at stage 2; kind 4, the final x pass,
+ * at the last stage with the error norm; 0 for kinds a trial step does not
+ * have), less the events' own cost per launch (the same number of trial
+ * steps with events only around each step measures it).  With n_out > 7,
+ * out_us[7] is the time of a whole trial step without the inner events and
+ * out_us[8] (n_out > 8) that per-launch event cost.  The control block is restored
+ * after every trial step, so the state does not change.  Demag plans
+ * without layer stacks; n_out at least 7.
+ */
+int fullmag_fdm_backend_time_trial_kernels(fullmag_fdm_backend *handle, int trials, double *out_us,
+                                           int n_out);
 
 /*
  * Tile tuner (no reference counterpart): times the x-pass launches of a
