@@ -3268,6 +3268,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
     const double dt = c->dt, t0 = c->t, abm_dt = c->abm_dt, t_end = c->t_end, dt_carry = c->dt_carry;
     const unsigned long long attempts = c->attempts, rejected = c->rejected, step = c->step, step_end = c->step_end;
     const int parity = c->parity, abm_head = c->abm_head, abm_count = c->abm_count, loop = c->loop;
+    const int done = c->done;
     const int adaptive = M.adaptive, abm = M.abm;
     const double ctrl_exp = M.ctrl_exp;
     Acc acc;
@@ -3281,6 +3282,9 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
             acc.v[k] = slot_is_max(k) ? fmax(acc.v[k], v) : acc.v[k] + v;
         }
     }
+    // the device loop has finished: nothing to decide (checked here, so
+    // that its round trip overlaps the loads above; uniform per work-group)
+    if (done) return;
     __shared__ double tot[kStatSlots];
     block_reduce_lds(acc, tot);
     __syncthreads();
@@ -3353,9 +3357,7 @@ __device__ void control_block(const Args<T> &A, int nparts, int observe) {
 // each time; measured 11% of a 256x256x64 step.)
 template <class T>
 __global__ void __launch_bounds__(kThreads) k_control(const Args<T> *__restrict__ AP, int nparts, int observe) {
-    const Args<T> &A = AP[blockIdx.y];
-    if (ctl_ro(A)->done) return;
-    control_block(A, nparts, observe);
+    control_block(AP[blockIdx.y], nparts, observe);
 }
 
 // ------------------------------------------------------------ host side --
