@@ -2388,9 +2388,10 @@ __global__ void __launch_bounds__(3 * (R1 > R2 ? R1 : R2)) k_yf(const Args<T> *_
 // over t1.  Lane k2 then holds X at its 8 frequencies, multiplies by -N
 // (the other in-plane component through LDS) and runs the steps backwards;
 // the inverse's 16-point step keeps outputs m < 8 as A + W_16^{-m} B of the
-// lane pair's two 8-point DFTs.  Per lane ~560 VALU operations against
-// ~2,100 for the 32 x 32 four-step form (k_yf): the film's y pass is one
-// work-group per column, so its time is the lane's dependent chain.
+// lane pair's two 8-point DFTs.  About 1,600 VALU instructions per wave
+// against 4,500 for the 32 x 32 four-step form (k_yf; the same total over
+// three times the waves): the film's y pass is one work-group per column,
+// so its time is a wave's dependent chain.
 // Twiddles are powers of per-lane bases from a table (Args::tw8,
 // [q][lane]: a wave's loads of one entry are contiguous), loaded with the
 // inputs.
