@@ -526,14 +526,21 @@ __device__ __forceinline__ bool slot_is_max(int i) {
 __device__ double block_reduce_slot(Acc &a) {
     __shared__ double red[kStatSlots][16];  // up to 1024 threads
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // wave butterfly, all slots per exchange distance: the slots' lane
+    // exchanges are independent, so each distance costs one exchange
+    // latency instead of one per slot (the per-slot order of operations,
+    // and so the result, is that of a slot-at-a-time butterfly)
 #pragma unroll
-    for (int i = 0; i < kStatSlots; ++i) {
-        double x = a.v[i];
-        for (int off = 32; off > 0; off >>= 1) {
-            const double y = __shfl_xor(x, off, 64);
-            x = slot_is_max(i) ? fmax(x, y) : x + y;
-        }
-        if (lane == 0) red[i][wid] = x;
+    for (int off = 32; off > 0; off >>= 1) {
+        double y[kStatSlots];
+#pragma unroll
+        for (int i = 0; i < kStatSlots; ++i) y[i] = __shfl_xor(a.v[i], off, 64);
+#pragma unroll
+        for (int i = 0; i < kStatSlots; ++i) a.v[i] = slot_is_max(i) ? fmax(a.v[i], y[i]) : a.v[i] + y[i];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kStatSlots; ++i) red[i][wid] = a.v[i];
     }
     __syncthreads();
     double x = 0.0;
