@@ -1722,7 +1722,7 @@ __device__ __forceinline__ const C2<T> *x_inverse_load(const Args<T> &A, const X
         for (int j = 0; j < CZ; ++j) {
             const int i = base + j * bd + (int)threadIdx.x;
             // every lane takes part in the exchange (wave-uniform control flow)
-            const C2<T> b = {__shfl_xor(v[j].x, 1, 64), __shfl_xor(v[j].y, 1, 64)};
+            const C2<T> b = {pair_swap(v[j].x), pair_swap(v[j].y)};
             if (i >= tot || (threadIdx.x & 1)) continue;
             const int kk = small_div<false>(i, nlp, rnl), l = i - kk * nlp;
             const C2<T> a = v[j];
@@ -2092,7 +2092,7 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         const int i = j * kXpfThreads + (int)threadIdx.x;
-        const C2<T> b = {__shfl_xor(v[j].x, 1, 64), __shfl_xor(v[j].y, 1, 64)};
+        const C2<T> b = {pair_swap(v[j].x), pair_swap(v[j].y)};
         if (i >= TOT || (threadIdx.x & 1)) continue;
         const int kk = small_div<false>(i, NLP, rnl), l = i - kk * NLP;
         const C2<T> a = v[j];
@@ -3035,11 +3035,20 @@ __device__ __forceinline__ void adj_local_cell(const Args<T> &A, const AdjArgs<T
 __device__ void block_add3(double v[3], double *out) {
     __shared__ double red3[3][16];  // up to 1024 threads
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // the three components' exchanges per distance together (one exchange
+    // latency per distance; per-component order unchanged)
+    double x[3] = {v[0], v[1], v[2]};
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        double x = v[i];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if (lane == 0) red3[i][wid] = x;
+    for (int off = 32; off > 0; off >>= 1) {
+        double y[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) y[i] = __shfl_xor(x[i], off, 64);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x[i] += y[i];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) red3[i][wid] = x[i];
     }
     __syncthreads();
     if (threadIdx.x < 3) {
