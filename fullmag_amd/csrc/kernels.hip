@@ -1286,7 +1286,8 @@ __device__ void fs_inverse(C2<T> *buf, C2<T> *tb, const C2<T> *__restrict__ tw4,
 // step keeps m < 8 as A + W_16^{-m} B of the pair's two 8-point DFTs.
 
 // Transpose layout of steps A/B (and A'/B'): 32 lanes of a half-wave hit
-// 32 distinct bank pairs on both sides (checked exhaustively on the host).
+// 32 distinct bank pairs on both sides (tests/test_three_step_fft_cpu.py
+// checks it for every step and lane, and emulates the whole transform).
 __device__ __forceinline__ int f8_ta(int t1, int t1p, int k2p) {
     return 4 * t1 + (k2p & 3) + 32 * (k2p >> 2) + 130 * t1p;
 }
