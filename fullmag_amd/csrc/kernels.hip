@@ -2073,20 +2073,24 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
     const T *ysrc = (const T *)stage_input(A, M, s, p);
     const T *msrc = A.m_all + (size_t)p * 3 * N + c0;
     // 2. the rest of the launch's inputs into LDS, spread over the waves
+    // (the twiddle wave takes no rows: its only global->LDS loads are the
+    // twiddles, so that the inverse FFT waits for those alone and runs
+    // while the rows, needed from the local pass on, still arrive)
     {
+        const int nwr = nw - 1;
         int item = 0;
         for (int k = 0; k < 4; ++k) {
             const int row = min(max(y0 - 1 + k, 0), A.ny - 1);
             for (int c = 0; c < 3; ++c, ++item)
-                if (item % nw == wv) dma_floats(ysrc + c * N + row * NX, Y + (k * 3 + c) * NX, NX);
+                if (item % nwr == wv) dma_floats(ysrc + c * N + row * NX, Y + (k * 3 + c) * NX, NX);
         }
         for (int c = 0; c < 3; ++c, ++item)
-            if (item % nw == wv) dma_floats(msrc + c * N, Mz + c * kXpfRows * NX, kXpfRows * NX);
+            if (item % nwr == wv) dma_floats(msrc + c * N, Mz + c * kXpfRows * NX, kXpfRows * NX);
 #pragma unroll
         for (int q = 0; q < NK; ++q) {
             if (q == qs) continue;
             for (int c = 0; c < 3; ++c, ++item)
-                if (item % nw == wv) dma_floats(ksrc[q] + c * N, K + (q * 3 + c) * kXpfRows * NX, kXpfRows * NX);
+                if (item % nwr == wv) dma_floats(ksrc[q] + c * N, K + (q * 3 + c) * kXpfRows * NX, kXpfRows * NX);
         }
     }
     // 3. Hermitian rebuild of the packed sequences, inverse FFT
@@ -2101,14 +2105,17 @@ __global__ void __launch_bounds__(kXpfThreads) k_xxp(const Args<T> *__restrict__
         q[kk] = {a.x - b.y, a.y + b.x};
         if (kk > 0 && kk < NX) q[PX - kk] = {a.x + b.y, b.x - a.y};
     }
-    // the prefetched rows and twiddles have arrived (every wave waits for
-    // its own global->LDS loads; the barrier publishes them to the
-    // work-group)
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // the twiddles have arrived (the twiddle wave waits for its
+    // global->LDS loads; the barrier publishes them)
+    if (wv == nw - 1) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const C2<T> *tw = reinterpret_cast<const C2<T> *>(TW);
     if constexpr (F8) fs8_inverse<T, true>(b0, b1, tw, A.nseq_x, sb);
     else fs_inverse<T, R1, R2, true>(b0, b1, tw, A.nseq_x, NX, sb);
+    // the prefetched rows have arrived (every wave waits for its own
+    // global->LDS loads, the barrier publishes them)
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
     // 5. local pass of stage s from the LDS rows; k_s also into its K slot
     const TimeF<T> tf = time_factors(A, stage_time(A, M, s, ctl, dt));
     {
